@@ -1,0 +1,164 @@
+"""Benchmark of the fused D3Q19 step on one MI355X.
+
+Workload (BASELINE config "flow-only throughput"): 256^3 lattice, FP32
+populations, MRT (d'Humieres moment basis, s_nu = 1/0.55, all other
+non-conserved rates 1.2), Bouzidi links on a random sequential-addition pack
+of non-overlapping spheres (radii uniform 4-10, porosity 0.6, seed 0), fully
+periodic, Guo body force 1e-6 along x, macros written every step.
+
+Timing: W warm-up steps, then K steps bracketed by synchronisation and HIP
+events on the stream the kernels run on. value = fluid-node MLUPS (solid
+nodes move no populations); roofline.achieved = 152 B x fluid nodes per
+launch / average launch time. cpu_baseline: the FP32 oracle (C + OpenMP
+restatement of the reference step) on the host cores over a few steps of the
+same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "D3Q19 MLUPS (%HBM roofline) + thermal solve s to 1e-8 resid, NN-PCG vs GMG-PCG"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=50)
+    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--collision", default="mrt", choices=["bgk", "trt", "mrt", "mrt_smag"])
+    p.add_argument("--storage", default="fp32", choices=["fp32", "fp16", "bf16"])
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--macro-every-step", type=int, default=1)
+    return p.parse_args()
+
+
+def build_workload(n, seed, collision, storage, device):
+    import torch
+
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.pre_post.sdf_tracing import bfl_q_and_weights, build_links_from_mask_gpu
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    from fluxcore3d_amd.solver.d3q19 import W
+
+    t0 = time.perf_counter()
+    phi, solid, info = geometry.rsa_spheres(n, porosity=0.6, rmin=4.0, rmax=10.0, seed=seed)
+    per = (True, True, True)
+    solid_t = torch.as_tensor(solid, device=device)
+    ff, di, do, fff, ok = build_links_from_mask_gpu(solid_t, periodic=per)
+    q, ge, w1, w2 = bfl_q_and_weights(torch.as_tensor(phi, device=device), (0.0, 0.0, 0.0), 1.0, ff, di)
+    links = {k: v.cpu().numpy() for k, v in dict(fluid_flat=ff, dir_i=di, ff_flat=fff, ff_valid=ok, w1=w1, w2=w2,
+                                                   q_ge_half=ge).items()}
+    omega = 1.0 / 0.55
+    S = np.full(19, 1.2, np.float32)
+    S[[0, 3, 5, 7]] = 0.0
+    S[[9, 11, 13, 14, 15]] = omega
+    eng = AAEngine(n, n, n, solid, periodic=per, collision=collision, omega=omega, omega_minus=1.2, S=S,
+                   force=(1e-6, 0.0, 0.0), links=links, storage=storage, device=device)
+    f0 = torch.as_tensor(np.broadcast_to(W[:, None], (19, n ** 3)).copy(), device=device)
+    eng.load(f0)
+    setup_s = time.perf_counter() - t0
+    return eng, solid, links, S, omega, info, setup_s
+
+
+def cpu_baseline(solid, links, S, omega, collision, steps):
+    """FP32 oracle step on the host (kind "port")."""
+    from oracle import lbm as olbm
+
+    n = solid.shape[0]
+    N = solid.size
+    f = np.broadcast_to(olbm.W32[:, None], (19, N)).copy()
+    threads = int(olbm._lib(np.float32).orc_num_threads())
+    f, *_ = olbm.step(f, solid, n, n, n, (1, 1, 1), collision, omega, 1.2, S, 0.0, links)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f, *_ = olbm.step(f, solid, n, n, n, (1, 1, 1), collision, omega, 1.2, S, 0.0, links)
+    dt = time.perf_counter() - t0
+    n_fluid = int((~solid).sum())
+    return {"value": round(n_fluid * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps (+1 warm-up) of the same {n}^3 {collision} workload, FP32 oracle "
+                      f"(C/OpenMP restatement of the reference step, no body force), {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("bench.py needs a HIP device")
+    dev = torch.device("cuda:0")
+    eng, solid, links, S, omega, info, setup_s = build_workload(args.n, args.seed, args.collision, args.storage, dev)
+    N = args.n ** 3
+    n_fluid = int((~solid).sum())
+    macro = [torch.empty(N, device=dev) for _ in range(4)]
+    for _ in range(args.warmup):
+        eng.step(*macro)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    ev0.record()
+    for s in range(args.steps):
+        if args.macro_every_step and s % args.macro_every_step == 0:
+            eng.step(*macro)
+        else:
+            eng.step()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = ev0.elapsed_time(ev1)
+    ms_step = ms / args.steps
+    mlups = n_fluid * args.steps / (ms * 1e-3) / 1e6
+    bpu = eng.bytes_per_update()
+    achieved = bpu * n_fluid / (ms_step * 1e-3) / 1e9
+    ok = bool(torch.isfinite(macro[1]).all()) and bool(torch.isfinite(macro[0]).all())
+    ux_mean = float(macro[1][torch.as_tensor(~solid.reshape(-1), device=dev)].mean())
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(solid, links, S, omega, args.collision, args.cpu_steps)
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.storage,
+        "data": "synthetic",
+        "config": {"workload": f"flow-only throughput: {args.n}^3 {args.collision.upper()} + Bouzidi, periodic RSA "
+                               f"sphere pack porosity {info['porosity']:.4f} (seed {args.seed}), body force 1e-6 x",
+                   "grid": [args.n] * 3, "fluid_nodes": n_fluid, "n_links": int(eng.n_links),
+                   "storage": args.storage, "collision": args.collision, "ibb": "reference Bouzidi (periodic links)",
+                   "macro_every_step": args.macro_every_step},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_update": bpu},
+        "cpu_baseline": cpu,
+        "mlups_all_nodes": round(N * args.steps / (ms * 1e-3) / 1e6, 2),
+        "wall_s_timed": round(wall, 4),
+        "setup_s": round(setup_s, 2),
+        "finite": ok,
+        "ux_mean": ux_mean,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

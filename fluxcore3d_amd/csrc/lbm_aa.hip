@@ -1,0 +1,767 @@
+// lbm_aa.hip — fused D3Q19 collide + stream + Bouzidi step, AA pattern, in place.
+//
+// One population buffer pop[19][Ns] (SoA over a padded grid), one thread per
+// lattice node, one launch per time step:
+//   even step: node x reads slot (i, x) for every i, collides, writes f*_i to
+//              slot (opp i, x);
+//   odd step:  node x reads slot (opp i, x - c_i), collides, writes f*_i to
+//              slot (i, x + c_i).
+// Every slot is read and written by exactly one node in a step, so the update
+// is race-free in place and moves 19 loads + 19 stores per node (152 B FP32,
+// 76 B for 16-bit storage).
+//
+// The step reproduces the reference step of lbm_cuda_ext.py:445-517
+// (collide -> [sponge] -> stream_pull -> ibb -> reset_solid -> flow BCs):
+//  * an input population whose upstream node is solid is the rest weight w_i
+//    (the reference streams f_post = W out of solid nodes, lbm_kernels.cu:277);
+//    it is substituted in registers, solid slots are never read;
+//  * a non-periodic face keeps the node's own post-collision value
+//    (lbm_kernels.cu:472-491) through a one-cell ghost layer;
+//  * Bouzidi links are handled per node. For a link (x, t) with weights
+//    (w1, w2) the overridden input at the next step is
+//       w1 f*_{opp t}(x) + w2 f*_t(x)            (no second node), or
+//       w1 f*_{opp t}(x) + w2 f*_{opp t}(x_ff)   (second fluid node x_ff).
+//    The own-node part is computed by x at the end of its collision and kept
+//    in A[l] (read back by the same thread next step). The second-node part
+//    is published by x_ff into B[parity][m] and read by x next step (double
+//    buffered by step parity). t = q for the reference's link direction
+//    choice (lbm_kernels.cu:567-587); the host passes t = opp(q) for textbook
+//    Bouzidi. Either way the kernel is the same.
+//  * planes with inlet/outlet conditions are rebuilt by plane kernels between
+//    steps (the nodes then take their inputs from plain slots).
+#include "fc_common.hpp"
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+using namespace fc;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// node flag bits
+constexpr uint8_t F_SOLID = 1;    // flow-solid node: no populations, macros (1,0,0,0)
+constexpr uint8_t F_SPECIAL = 2;  // has a record in the special-node table
+constexpr int F_BC_SHIFT = 3;     // bits 3..5: 1 + index of the first BC plane holding the node
+
+struct Grid {
+  int nx, ny, nz;
+  int gx, gy, gz;  // ghost width per axis (0 periodic, 1 non-periodic)
+  int PY, PZ;      // padded extents along y and z
+  int64_t Ns;      // padded node count = stride between population arrays
+  int64_t N;       // interior node count
+};
+
+struct Tables {
+  const uint8_t* nflag;      // [N]
+  const int32_t* wave_base;  // [ceil(N/64)] special nodes before each 64-node group
+  const int4* sp_mask;       // [n_special] {L, S, V, F}
+  const int4* sp_base;       // [n_special] {link_base, v_base, ff_base, 0}
+  const float* w1;           // [n_links]
+  const float* w2;           // [n_links]
+  float* A;                  // [n_links]
+  float* B;                  // [2][n_v]
+  const int32_t* ff_idx;     // [n_ff]
+  int64_t n_v;
+};
+
+struct Collision {
+  int kind;          // 0 BGK, 1 TRT, 2 MRT, 3 MRT + Smagorinsky
+  float om1, om2;    // BGK omega | TRT omega+, omega- | Smagorinsky base omega0
+  float S[19];       // MRT rates by moment row
+  float cs_dx2;      // (Cs dx)^2 for Smagorinsky
+  float F[3];        // body force (lattice units), zero when unused
+  int use_force;
+  const float* sponge;  // sigma profile along sponge_axis, or null
+  int sponge_axis;
+};
+
+// ---- storage types: FP32 values, or 16-bit deviations f - w_i (rho0 = 1) --
+template <typename ST> struct Store;
+template <> struct Store<float> {
+  static __device__ __forceinline__ float ld(const float* p, int) { return *p; }
+  static __device__ __forceinline__ void st(float* p, float v, int) { *p = v; }
+};
+template <> struct Store<__half> {
+  static __device__ __forceinline__ float ld(const __half* p, int q) { return __half2float(*p) + WQ(q); }
+  static __device__ __forceinline__ void st(__half* p, float v, int q) { *p = __float2half(v - WQ(q)); }
+};
+template <> struct Store<__hip_bfloat16> {
+  static __device__ __forceinline__ float ld(const __hip_bfloat16* p, int q) {
+    return __bfloat162float(*p) + WQ(q);
+  }
+  static __device__ __forceinline__ void st(__hip_bfloat16* p, float v, int q) {
+    *p = __float2bfloat16(v - WQ(q));
+  }
+};
+
+__device__ __forceinline__ int wrap_or_pad(int c, int d, int n, int g) {
+  int v = c + d;
+  if (g) return v + 1;  // ghost layer: -1 -> 0, n -> n+1
+  return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+__device__ __forceinline__ int64_t slot_node(const Grid& g, int i, int j, int k, int dx, int dy, int dz) {
+  const int I = wrap_or_pad(i, dx, g.nx, g.gx);
+  const int J = wrap_or_pad(j, dy, g.ny, g.gy);
+  const int K = wrap_or_pad(k, dz, g.nz, g.gz);
+  return ((int64_t)I * g.PY + J) * g.PZ + K;
+}
+// true when x - c_q leaves the domain through a non-periodic face
+__device__ __forceinline__ bool oob_source(const Grid& g, int i, int j, int k, int q) {
+  return (g.gx && (i - CX(q) < 0 || i - CX(q) >= g.nx)) || (g.gy && (j - CY(q) < 0 || j - CY(q) >= g.ny)) ||
+         (g.gz && (k - CZ(q) < 0 || k - CZ(q) >= g.nz));
+}
+
+template <int BIT>
+__device__ __forceinline__ int rank_below(uint32_t mask) {
+  return __popc(mask & ((1u << BIT) - 1u));
+}
+
+// Plain slot that holds input population q of node (i,j,k) at this parity.
+template <typename ST>
+__device__ __forceinline__ float load_plain(const ST* pop, const Grid& g, int parity, int i, int j, int k, int q) {
+  if (parity == 0) return Store<ST>::ld(pop + (int64_t)q * g.Ns + slot_node(g, i, j, k, 0, 0, 0), q);
+  return Store<ST>::ld(pop + (int64_t)OPP(q) * g.Ns + slot_node(g, i, j, k, -CX(q), -CY(q), -CZ(q)), OPP(q));
+}
+template <typename ST>
+__device__ __forceinline__ void store_plain(ST* pop, const Grid& g, int parity, int i, int j, int k, int q, float v) {
+  if (parity == 0) Store<ST>::st(pop + (int64_t)q * g.Ns + slot_node(g, i, j, k, 0, 0, 0), v, q);
+  else Store<ST>::st(pop + (int64_t)OPP(q) * g.Ns + slot_node(g, i, j, k, -CX(q), -CY(q), -CZ(q)), v, OPP(q));
+}
+
+// All 19 inputs of a node: plain slots, rest weights for solid sources and
+// Bouzidi values for links. `first` marks the step right after loading, when
+// A holds the loaded populations themselves.
+template <typename ST>
+__device__ __forceinline__ void compose_inputs(const ST* __restrict__ pop, const Grid& g, const Tables& t,
+                                               int parity, int first, int i, int j, int k, bool special,
+                                               bool plain, int4 m, int4 b, float f[19]) {
+  const uint32_t L = plain ? 0u : (uint32_t)m.x, S = plain ? 0u : (uint32_t)m.y, V = (uint32_t)m.z;
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    if (special && ((L >> q) & 1u)) {
+      const int64_t l = b.x + __popc(L & ((1u << q) - 1u));
+      float v = t.A[l];
+      if (!first && ((V >> q) & 1u)) {
+        const int64_t mv = b.y + __popc(V & ((1u << q) - 1u));
+        v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + mv];
+      }
+      f[q] = v;
+    } else if (special && ((S >> q) & 1u)) {
+      f[q] = WQ(q);
+    } else {
+      f[q] = load_plain(pop, g, parity, i, j, k, q);
+    }
+  }
+}
+
+// Collision in registers; returns the macros of the incoming state.
+// Optional Guo body force (u = (j + F/2)/rho, source (1 - s/2) F_i with the
+// relaxation rate of each mode) and sponge relaxation towards feq.
+__device__ __forceinline__ void guo_source(const Collision& c, float ux, float uy, float uz, float src[19]) {
+  const float uf = ux * c.F[0] + uy * c.F[1] + uz * c.F[2];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const float cx = (float)CX(q), cy = (float)CY(q), cz = (float)CZ(q);
+    const float cu = cx * ux + cy * uy + cz * uz;
+    const float cf = cx * c.F[0] + cy * c.F[1] + cz * c.F[2];
+    src[q] = WQ(q) * (3.f * (cf - uf) + 9.f * cu * cf);
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ void collide(const Collision& c, float f[19], float& rho, float& ux, float& uy, float& uz,
+                                        float sigma) {
+  float jx, jy, jz;
+  moments0(f, rho, jx, jy, jz);
+  if (c.use_force) {
+    jx += 0.5f * c.F[0]; jy += 0.5f * c.F[1]; jz += 0.5f * c.F[2];
+  }
+  const float inv = 1.f / (rho + 1e-12f);
+  ux = jx * inv; uy = jy * inv; uz = jz * inv;
+  float feq[19];
+  feq19(rho, ux, uy, uz, feq);
+  if constexpr (KIND == 0) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = f[q] + c.om1 * (feq[q] - f[q]);
+    if (c.use_force) {
+      float src[19];
+      guo_source(c, ux, uy, uz, src);
+#pragma unroll
+      for (int q = 0; q < 19; ++q) f[q] += (1.f - 0.5f * c.om1) * src[q];
+    }
+  } else if constexpr (KIND == 1) {
+    float o[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      const int p = OPP(q);
+      const float fp = 0.5f * (f[q] + f[p]), fm = 0.5f * (f[q] - f[p]);
+      const float ep = 0.5f * (feq[q] + feq[p]), em = 0.5f * (feq[q] - feq[p]);
+      o[q] = (fp + c.om1 * (ep - fp)) + (fm + c.om2 * (em - fm));
+    }
+    if (c.use_force) {
+      float src[19];
+      guo_source(c, ux, uy, uz, src);
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        const int p = OPP(q);
+        o[q] += (1.f - 0.5f * c.om1) * (0.5f * (src[q] + src[p])) + (1.f - 0.5f * c.om2) * (0.5f * (src[q] - src[p]));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = o[q];
+  } else {
+    float S[19];
+#pragma unroll
+    for (int k = 0; k < 19; ++k) S[k] = c.S[k];
+    float fneq[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fneq[q] = f[q] - feq[q];
+    if constexpr (KIND == 3) {
+      float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        const float x = (float)CX(q), y = (float)CY(q), z = (float)CZ(q);
+        pxx += fneq[q] * x * x; pyy += fneq[q] * y * y; pzz += fneq[q] * z * z;
+        pxy += fneq[q] * x * y; pxz += fneq[q] * x * z; pyz += fneq[q] * y * z;
+      }
+      const float pif = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.f * (pxy * pxy + pxz * pxz + pyz * pyz));
+      const float nu0 = (1.f / c.om1 - 0.5f) / 3.f;
+      const float disc = nu0 * nu0 + 4.f * c.cs_dx2 * (pif * inv) * 4.5f;
+      const float nu_eff = 0.5f * (nu0 + sqrtf(fmaxf(disc, nu0 * nu0)));
+      const float om = fminf(1.99f, 1.f / (3.f * nu_eff + 0.5f));
+      S[9] = om; S[11] = om; S[13] = om; S[14] = om; S[15] = om;
+    }
+    float df[19];
+    mrt_df(fneq, S, df);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = f[q] - df[q];
+    if (c.use_force) {
+      // f += M^-1 (I - S/2) M src: non-conserved rows through mrt_df with
+      // rates -(1 - S_k/2); the momentum rows (S = 0, |M_k|^2 = 10) carry F.
+      float src[19], Sm[19];
+      guo_source(c, ux, uy, uz, src);
+#pragma unroll
+      for (int k = 0; k < 19; ++k) Sm[k] = 0.5f * S[k] - 1.f;
+      mrt_df(src, Sm, df);
+#pragma unroll
+      for (int q = 0; q < 19; ++q)
+        f[q] = f[q] - df[q] + 0.1f * ((float)CX(q) * c.F[0] + (float)CY(q) * c.F[1] + (float)CZ(q) * c.F[2]);
+    }
+  }
+  if (sigma > 0.f) {  // sponge (lbm_kernels.cu:505-528) on the post-collision state
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = f[q] - sigma * (f[q] - feq[q]);
+  }
+}
+
+__device__ __forceinline__ unsigned logical_block(unsigned bid, unsigned nblk) {
+  // Consecutive node blocks on the same XCD (blocks are dealt round-robin to
+  // the 8 XCDs), so stencil neighbours share that XCD's L2.
+  const unsigned xcd = bid & 7u, local = bid >> 3;
+  const unsigned base = nblk >> 3, rem = nblk & 7u;
+  return xcd * base + min(xcd, rem) + local;
+}
+
+template <typename ST, int KIND>
+__global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
+                                                         int parity, int first, float* __restrict__ rho_o,
+                                                         float* __restrict__ u_o, float* __restrict__ v_o,
+                                                         float* __restrict__ w_o) {
+  const unsigned lb = logical_block(blockIdx.x, gridDim.x);
+  const int64_t x = (int64_t)lb * kBlock + threadIdx.x;
+  const bool live = x < g.N;
+  const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
+  const bool special = (fl & F_SPECIAL) != 0;
+  // rank among the special nodes of this 64-node group
+  const uint64_t bal = __ballot(special);
+  const int lane = threadIdx.x & 63;
+  int4 m = make_int4(0, 0, 0, 0), b = make_int4(0, 0, 0, 0);
+  if (special) {
+    const int64_t sidx = t.wave_base[x >> 6] + __popcll(bal & ((1ull << lane) - 1ull));
+    m = t.sp_mask[sidx];
+    b = t.sp_base[sidx];
+  }
+  if (!live) return;
+  if (fl & F_SOLID) {
+    if (rho_o) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
+    return;
+  }
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
+  const bool plain = (fl >> F_BC_SHIFT) != 0;
+
+  float f[19];
+  compose_inputs(pop, g, t, parity, first, i, j, k, special, plain, m, b, f);
+
+  float sigma = 0.f;
+  if (c.sponge) sigma = c.sponge[c.sponge_axis == 0 ? i : (c.sponge_axis == 1 ? j : k)];
+  float rho, ux, uy, uz;
+  collide<KIND>(c, f, rho, ux, uy, uz, sigma);
+  if (rho_o) { rho_o[x] = rho; u_o[x] = ux; v_o[x] = uy; w_o[x] = uz; }
+
+  if (parity == 0) {
+    const int64_t self = slot_node(g, i, j, k, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      Store<ST>::st(pop + (int64_t)OPP(q) * g.Ns + self, f[q], OPP(q));
+      if (oob_source(g, i, j, k, q))
+        Store<ST>::st(pop + (int64_t)OPP(q) * g.Ns + slot_node(g, i, j, k, -CX(q), -CY(q), -CZ(q)), f[q], OPP(q));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      Store<ST>::st(pop + (int64_t)q * g.Ns + slot_node(g, i, j, k, CX(q), CY(q), CZ(q)), f[q], q);
+      if (oob_source(g, i, j, k, q))
+        Store<ST>::st(pop + (int64_t)q * g.Ns + slot_node(g, i, j, k, 0, 0, 0), f[q], q);
+    }
+  }
+
+  if (special) {
+    const uint32_t L = (uint32_t)m.x, V = (uint32_t)m.z, F = (uint32_t)m.w;
+    int64_t l = b.x;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      if ((L >> q) & 1u) {
+        float a = t.w1[l] * f[OPP(q)];
+        if (!((V >> q) & 1u)) a = a + t.w2[l] * f[q];
+        t.A[l] = a;
+        ++l;
+      }
+    }
+    int64_t e = b.z;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      if ((F >> q) & 1u) {
+        t.B[(int64_t)parity * t.n_v + t.ff_idx[e]] = f[q];
+        ++e;
+      }
+    }
+  }
+}
+
+// ---- helpers for kernels that visit arbitrary nodes -------------------------
+__device__ __forceinline__ int64_t special_index_scan(const Tables& t, int64_t x) {
+  int64_t s = t.wave_base[x >> 6];
+  for (int64_t p = x & ~(int64_t)63; p < x; ++p) s += (t.nflag[p] & F_SPECIAL) != 0;
+  return s;
+}
+
+// Inputs of node x for a plane kernel running as BC number `bc` of this step:
+// nodes already rebuilt by an earlier BC of the same step hold plain inputs.
+template <typename ST>
+__device__ __forceinline__ void node_inputs(const ST* pop, const Grid& g, const Tables& t, int parity, int first,
+                                            int64_t x, int bc, float f[19]) {
+  const uint8_t fl = t.nflag[x];
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
+  const int fb = fl >> F_BC_SHIFT;
+  const bool plain = fb != 0 && fb - 1 < bc;
+  const bool special = (fl & F_SPECIAL) != 0;
+  int4 m = make_int4(0, 0, 0, 0), b = make_int4(0, 0, 0, 0);
+  if (special) {
+    const int64_t s = special_index_scan(t, x);
+    m = t.sp_mask[s];
+    b = t.sp_base[s];
+  }
+  compose_inputs(pop, g, t, parity, first, i, j, k, special, plain, m, b, f);
+}
+
+template <typename ST>
+__device__ __forceinline__ void store_inputs(ST* pop, const Grid& g, int parity, int64_t x, const float f[19]) {
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) store_plain(pop, g, parity, i, j, k, q, f[q]);
+}
+
+// Reference-layout copy of the populations entering the next step
+// (solid nodes hold W, as after lbm_kernels.cu:597-609).
+template <typename ST>
+__global__ __launch_bounds__(kBlock) void aa_materialize_kernel(const ST* __restrict__ pop, Grid g, Tables t,
+                                                                int parity, int first, float* __restrict__ out) {
+  const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool live = x < g.N;
+  const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
+  const bool special = (fl & F_SPECIAL) != 0;
+  const uint64_t bal = __ballot(special);
+  const int lane = threadIdx.x & 63;
+  int4 m = make_int4(0, 0, 0, 0), b = make_int4(0, 0, 0, 0);
+  if (special) {
+    const int64_t sidx = t.wave_base[x >> 6] + __popcll(bal & ((1ull << lane) - 1ull));
+    m = t.sp_mask[sidx];
+    b = t.sp_base[sidx];
+  }
+  if (!live) return;
+  if (fl & F_SOLID) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) out[q * g.N + x] = WQ(q);
+    return;
+  }
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
+  float f[19];
+  compose_inputs(pop, g, t, parity, first, i, j, k, special, (fl >> F_BC_SHIFT) != 0, m, b, f);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) out[q * g.N + x] = f[q];
+}
+
+// Load reference-layout populations as the input of an even step; link
+// inputs go to A (used alone on the first step).
+template <typename ST>
+__global__ __launch_bounds__(kBlock) void aa_load_kernel(ST* __restrict__ pop, Grid g, Tables t,
+                                                         const float* __restrict__ in) {
+  const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool live = x < g.N;
+  const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
+  const bool special = (fl & F_SPECIAL) != 0;
+  const uint64_t bal = __ballot(special);
+  const int lane = threadIdx.x & 63;
+  if (!live) return;
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
+  const int64_t self = slot_node(g, i, j, k, 0, 0, 0);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (int64_t)q * g.Ns + self, in[q * g.N + x], q);
+  if (special) {
+    const int64_t sidx = t.wave_base[x >> 6] + __popcll(bal & ((1ull << lane) - 1ull));
+    const int4 m = t.sp_mask[sidx], b = t.sp_base[sidx];
+    const uint32_t L = (uint32_t)m.x;
+    int64_t l = b.x;
+    for (int q = 0; q < 19; ++q)
+      if ((L >> q) & 1u) t.A[l++] = in[q * g.N + x];
+  }
+}
+
+// ---- inlet / outlet planes on the AA state ---------------------------------
+__device__ __forceinline__ int64_t plane_flat(const Grid& g, int axis, int plane, int pa, int pb) {
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  if (axis == 0) return plane * nynz + (int64_t)pa * g.nz + pb;
+  if (axis == 1) return pa * nynz + (int64_t)plane * g.nz + pb;
+  return pa * nynz + (int64_t)pb * g.nz + plane;
+}
+
+__device__ __forceinline__ float qdiag(int c) { return c != 0 ? 0.66666667f : -0.33333333f; }
+
+// Regularised Zou/He on the AA state (same node update as zou_he_kernel in
+// lbm_ref.hip, lbm_kernels.cu:791-969).
+template <typename ST>
+__global__ __launch_bounds__(kBlock) void aa_zou_he_kernel(ST* __restrict__ pop, Grid g, Tables t, int parity,
+                                                           int first, int bc, int axis, int plane, int interior,
+                                                           int bc_type, float u_in, float rho_out, int sign, int n1,
+                                                           int n2) {
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  if (tid >= n1 * n2) return;
+  const int64_t fb = plane_flat(g, axis, plane, tid / n2, tid % n2);
+  if (t.nflag[fb] & F_SOLID) return;
+  float fi[19];
+  node_inputs(pop, g, t, parity, first, fb, bc, fi);
+  const int unknown = plane == 0 ? 1 : -1, known = -unknown;
+  float s0 = 0.f, sk = 0.f;
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const int cq = C_AX(q, axis);
+    if (cq == 0) s0 += fi[q];
+    if (cq == known) sk += fi[q];
+  }
+  float rho_bc, uvec[3] = {0.f, 0.f, 0.f};
+  if (bc_type == 0) {
+    rho_bc = fminf(fmaxf((s0 + 2.f * sk) / (1.f - fabsf(u_in) + 1e-12f), 0.2f), 5.0f);
+    uvec[axis] = u_in;
+  } else {
+    rho_bc = rho_out;
+    const float uax = (float)known * ((s0 + 2.f * sk) / (rho_bc + 1e-12f) - 1.f);
+    const int64_t fin = plane_flat(g, axis, interior, tid / n2, tid % n2);
+    float gi[19];
+    node_inputs(pop, g, t, parity, first, fin, bc, gi);
+    if (uax * (float)sign < 0.f) {
+      store_inputs(pop, g, parity, fb, gi);
+      return;
+    }
+    float r, jx, jy, jz;
+    moments0(gi, r, jx, jy, jz);
+    const float inv = 1.f / (r + 1e-12f);
+    uvec[0] = jx * inv; uvec[1] = jy * inv; uvec[2] = jz * inv;
+    uvec[axis] = uax;
+  }
+  float feq[19];
+  feq19(rho_bc, uvec[0], uvec[1], uvec[2], feq);
+#pragma unroll
+  for (int q = 0; q < 19; ++q)
+    if (C_AX(q, axis) == unknown) fi[q] = fi[OPP(q)] + feq[q] - feq[OPP(q)];
+  float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const float fn = fi[q] - feq[q];
+    pxx += fn * qdiag(CX(q)); pyy += fn * qdiag(CY(q)); pzz += fn * qdiag(CZ(q));
+    pxy += fn * (float)(CX(q) * CY(q)); pxz += fn * (float)(CX(q) * CZ(q)); pyz += fn * (float)(CY(q) * CZ(q));
+  }
+  float out[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const float s = qdiag(CX(q)) * pxx + qdiag(CY(q)) * pyy + qdiag(CZ(q)) * pzz +
+                    2.f * ((float)(CX(q) * CY(q)) * pxy + (float)(CX(q) * CZ(q)) * pxz + (float)(CY(q) * CZ(q)) * pyz);
+    out[q] = feq[q] + WQ(q) * s * 4.5f;
+  }
+  store_inputs(pop, g, parity, fb, out);
+}
+
+// Convective outlet on the AA state, pass 1 (copy + partial sums) and pass 2
+// (mass correction); same semantics as convective_pass1/2 in lbm_ref.hip.
+template <typename ST>
+__global__ __launch_bounds__(kBlock) void aa_convective_pass1(ST* __restrict__ pop, Grid g, Tables t, int parity,
+                                                              int first, int bc, int axis, int outlet, int interior,
+                                                              int unknown_cax, int n1, int n2,
+                                                              double* __restrict__ partial) {
+  __shared__ double s_r[kBlock / 64], s_c[kBlock / 64];
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  double r = 0.0, cnt = 0.0;
+  if (tid < n1 * n2) {
+    const int64_t fo = plane_flat(g, axis, outlet, tid / n2, tid % n2);
+    if (!(t.nflag[fo] & F_SOLID)) {
+      const int64_t fi = plane_flat(g, axis, interior, tid / n2, tid % n2);
+      float a[19], b2[19];
+      node_inputs(pop, g, t, parity, first, fo, bc, a);
+      node_inputs(pop, g, t, parity, first, fi, bc, b2);
+      float rho = 0.f;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        if (C_AX(q, axis) == unknown_cax) a[q] = b2[q];
+        rho += a[q];
+      }
+      store_inputs(pop, g, parity, fo, a);
+      r = rho; cnt = 1.0;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    r += __shfl_down(r, off, 64);
+    cnt += __shfl_down(cnt, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { s_r[threadIdx.x >> 6] = r; s_c[threadIdx.x >> 6] = cnt; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) { a += s_r[w]; b += s_c[w]; }
+    partial[2 * blockIdx.x] = a;
+    partial[2 * blockIdx.x + 1] = b;
+  }
+}
+
+template <typename ST>
+__global__ __launch_bounds__(kBlock) void aa_convective_pass2(ST* __restrict__ pop, Grid g, Tables t, int parity,
+                                                              int axis, int outlet, int n1, int n2, float rho_out,
+                                                              const double* __restrict__ partial, int nparts) {
+  __shared__ float s_corr;
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < nparts; ++w) { a += partial[2 * w]; b += partial[2 * w + 1]; }
+    const float cnt = (float)b, sum = (float)a;
+    s_skip = (cnt < 0.5f || sum < 1e-10f);
+    s_corr = s_skip ? 1.f : rho_out * cnt / sum;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  if (tid >= n1 * n2) return;
+  const int64_t fo = plane_flat(g, axis, outlet, tid / n2, tid % n2);
+  if (t.nflag[fo] & F_SOLID) return;
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  const int i = (int)(fo / nynz), j = (int)((fo / g.nz) % g.ny), k = (int)(fo % g.nz);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const float v = load_plain(pop, g, parity, i, j, k, q) * s_corr;
+    store_plain(pop, g, parity, i, j, k, q, v);
+  }
+}
+
+// ---- host-side argument packing ----------------------------------------------
+int make_grid(Grid& g, int nx, int ny, int nz, int px, int py, int pz) {
+  FC_REQUIRE(nx > 0 && ny > 0 && nz > 0, "aa: grid extents must be positive");
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.gx = px ? 0 : 1; g.gy = py ? 0 : 1; g.gz = pz ? 0 : 1;
+  g.PY = ny + 2 * g.gy; g.PZ = nz + 2 * g.gz;
+  g.Ns = (int64_t)(nx + 2 * g.gx) * g.PY * g.PZ;
+  g.N = (int64_t)nx * ny * nz;
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI. Storage type code: 0 FP32, 1 FP16 deviation, 2 BF16 deviation.
+// ===========================================================================
+extern "C" {
+
+struct fc_aa_tables {
+  const uint8_t* nflag;
+  const int32_t* wave_base;
+  const int32_t* sp_mask;  // [n_special][4]
+  const int32_t* sp_base;  // [n_special][4]
+  const float* w1;
+  const float* w2;
+  float* A;
+  float* B;
+  const int32_t* ff_idx;
+  int64_t n_v;
+};
+
+struct fc_aa_collision {
+  int kind;
+  float om1, om2;
+  float S[19];
+  float cs_dx2;
+  float force[3];
+  int use_force;
+  const float* sponge;
+  int sponge_axis;
+};
+
+int64_t fc_aa_slot_count(int nx, int ny, int nz, int px, int py, int pz) {
+  Grid g;
+  if (make_grid(g, nx, ny, nz, px, py, pz)) return -1;
+  return g.Ns;
+}
+
+static int pack(Grid& g, Tables& t, const fc_aa_tables* tb, int nx, int ny, int nz, int px, int py, int pz) {
+  if (int rc = make_grid(g, nx, ny, nz, px, py, pz)) return rc;
+  FC_REQUIRE(tb && tb->nflag && tb->wave_base, "aa: node flag tables are required");
+  t.nflag = tb->nflag; t.wave_base = tb->wave_base;
+  t.sp_mask = (const int4*)tb->sp_mask; t.sp_base = (const int4*)tb->sp_base;
+  t.w1 = tb->w1; t.w2 = tb->w2; t.A = tb->A; t.B = tb->B; t.ff_idx = tb->ff_idx; t.n_v = tb->n_v;
+  return 0;
+}
+
+int fc_aa_step(void* pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables* tb,
+               const fc_aa_collision* col, int parity, int first, float* rho, float* u, float* v, float* w,
+               hipStream_t stream) {
+  Grid g; Tables t;
+  if (int rc = pack(g, t, tb, nx, ny, nz, px, py, pz)) return rc;
+  FC_REQUIRE(pop, "aa_step: null population buffer");
+  FC_REQUIRE(col && col->kind >= 0 && col->kind <= 3, "aa_step: collision kind must be 0..3");
+  FC_REQUIRE(parity == 0 || parity == 1, "aa_step: parity must be 0 or 1");
+  FC_REQUIRE((rho && u && v && w) || (!rho && !u && !v && !w), "aa_step: macro outputs must be all set or all null");
+  Collision c;
+  c.kind = col->kind; c.om1 = col->om1; c.om2 = col->om2;
+  for (int k = 0; k < 19; ++k) c.S[k] = col->S[k];
+  c.cs_dx2 = col->cs_dx2;
+  for (int d = 0; d < 3; ++d) c.F[d] = col->force[d];
+  c.use_force = col->use_force; c.sponge = col->sponge; c.sponge_axis = col->sponge_axis;
+  const unsigned grid = grid_for(g.N, kBlock);
+#define FC_STEP(T, K)                                                                                          \
+  hipLaunchKernelGGL((aa_step_kernel<T, K>), dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, c, parity, first, \
+                     rho, u, v, w)
+#define FC_STEP_KIND(T)                      \
+  switch (c.kind) {                          \
+    case 0: FC_STEP(T, 0); break;            \
+    case 1: FC_STEP(T, 1); break;            \
+    case 2: FC_STEP(T, 2); break;            \
+    default: FC_STEP(T, 3); break;           \
+  }
+  switch (storage) {
+    case 0: FC_STEP_KIND(float); break;
+    case 1: FC_STEP_KIND(__half); break;
+    case 2: FC_STEP_KIND(__hip_bfloat16); break;
+    default: return fc::set_error(1, "aa_step: storage code must be 0 (fp32), 1 (fp16) or 2 (bf16)");
+  }
+#undef FC_STEP_KIND
+#undef FC_STEP
+  return check_launch("aa_step");
+}
+
+int fc_aa_materialize(const void* pop, int storage, int nx, int ny, int nz, int px, int py, int pz,
+                      const fc_aa_tables* tb, int parity, int first, float* out, hipStream_t stream) {
+  Grid g; Tables t;
+  if (int rc = pack(g, t, tb, nx, ny, nz, px, py, pz)) return rc;
+  FC_REQUIRE(pop && out, "aa_materialize: null pointer argument");
+  const unsigned grid = grid_for(g.N, kBlock);
+  switch (storage) {
+    case 0: hipLaunchKernelGGL(aa_materialize_kernel<float>, dim3(grid), dim3(kBlock), 0, stream, (const float*)pop,
+                               g, t, parity, first, out); break;
+    case 1: hipLaunchKernelGGL(aa_materialize_kernel<__half>, dim3(grid), dim3(kBlock), 0, stream,
+                               (const __half*)pop, g, t, parity, first, out); break;
+    case 2: hipLaunchKernelGGL(aa_materialize_kernel<__hip_bfloat16>, dim3(grid), dim3(kBlock), 0, stream,
+                               (const __hip_bfloat16*)pop, g, t, parity, first, out); break;
+    default: return fc::set_error(1, "aa_materialize: bad storage code");
+  }
+  return check_launch("aa_materialize");
+}
+
+int fc_aa_load(void* pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables* tb,
+               const float* in, hipStream_t stream) {
+  Grid g; Tables t;
+  if (int rc = pack(g, t, tb, nx, ny, nz, px, py, pz)) return rc;
+  FC_REQUIRE(pop && in, "aa_load: null pointer argument");
+  const unsigned grid = grid_for(g.N, kBlock);
+  switch (storage) {
+    case 0: hipLaunchKernelGGL(aa_load_kernel<float>, dim3(grid), dim3(kBlock), 0, stream, (float*)pop, g, t, in);
+      break;
+    case 1: hipLaunchKernelGGL(aa_load_kernel<__half>, dim3(grid), dim3(kBlock), 0, stream, (__half*)pop, g, t, in);
+      break;
+    case 2: hipLaunchKernelGGL(aa_load_kernel<__hip_bfloat16>, dim3(grid), dim3(kBlock), 0, stream,
+                               (__hip_bfloat16*)pop, g, t, in); break;
+    default: return fc::set_error(1, "aa_load: bad storage code");
+  }
+  return check_launch("aa_load");
+}
+
+int fc_aa_zou_he(void* pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables* tb,
+                 int parity, int first, int bc_order, int axis, int plane_idx, int interior_idx, int bc_type,
+                 float u_in_lat, float rho_out_lat, int sign, hipStream_t stream) {
+  Grid g; Tables t;
+  if (int rc = pack(g, t, tb, nx, ny, nz, px, py, pz)) return rc;
+  FC_REQUIRE(axis >= 0 && axis <= 2 && (bc_type == 0 || bc_type == 1), "aa_zou_he: bad axis or bc_type");
+  const int n_ax = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  FC_REQUIRE(plane_idx >= 0 && plane_idx < n_ax && interior_idx >= 0 && interior_idx < n_ax,
+             "aa_zou_he: plane index out of range");
+  const int n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
+  const unsigned grid = grid_for((int64_t)n1 * n2, kBlock);
+#define FC_ZH(T)                                                                                                  \
+  hipLaunchKernelGGL(aa_zou_he_kernel<T>, dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, parity, first,      \
+                     bc_order, axis, plane_idx, interior_idx, bc_type, u_in_lat, rho_out_lat, sign, n1, n2)
+  switch (storage) {
+    case 0: FC_ZH(float); break;
+    case 1: FC_ZH(__half); break;
+    case 2: FC_ZH(__hip_bfloat16); break;
+    default: return fc::set_error(1, "aa_zou_he: bad storage code");
+  }
+#undef FC_ZH
+  return check_launch("aa_zou_he");
+}
+
+size_t fc_aa_convective_workspace_bytes(int n1, int n2) {
+  if (n1 <= 0 || n2 <= 0) return 0;
+  return sizeof(double) * 2 * (size_t)grid_for((int64_t)n1 * n2, kBlock);
+}
+
+int fc_aa_convective(void* pop, int storage, int nx, int ny, int nz, int px, int py, int pz,
+                     const fc_aa_tables* tb, int parity, int first, int bc_order, int axis, int outlet_idx,
+                     int interior_idx, int unknown_cax, float rho_out_lat, void* workspace, size_t ws_bytes,
+                     hipStream_t stream) {
+  Grid g; Tables t;
+  if (int rc = pack(g, t, tb, nx, ny, nz, px, py, pz)) return rc;
+  FC_REQUIRE(axis >= 0 && axis <= 2 && (unknown_cax == 1 || unknown_cax == -1), "aa_convective: bad axis/unknown_cax");
+  const int n_ax = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  FC_REQUIRE(outlet_idx >= 0 && outlet_idx < n_ax && interior_idx >= 0 && interior_idx < n_ax,
+             "aa_convective: plane index out of range");
+  const int n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
+  const size_t need = fc_aa_convective_workspace_bytes(n1, n2);
+  FC_REQUIRE(workspace && ws_bytes >= need, "aa_convective: workspace of %zu bytes required", need);
+  const unsigned grid = grid_for((int64_t)n1 * n2, kBlock);
+#define FC_CV(T)                                                                                                 \
+  hipLaunchKernelGGL(aa_convective_pass1<T>, dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, parity, first,  \
+                     bc_order, axis, outlet_idx, interior_idx, unknown_cax, n1, n2, (double*)workspace);          \
+  if (int rc = check_launch("aa_convective/pass1")) return rc;                                                    \
+  hipLaunchKernelGGL(aa_convective_pass2<T>, dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, parity, axis,    \
+                     outlet_idx, n1, n2, rho_out_lat, (const double*)workspace, (int)grid)
+  switch (storage) {
+    case 0: FC_CV(float); break;
+    case 1: FC_CV(__half); break;
+    case 2: FC_CV(__hip_bfloat16); break;
+    default: return fc::set_error(1, "aa_convective: bad storage code");
+  }
+#undef FC_CV
+  return check_launch("aa_convective/pass2");
+}
+
+}  // extern "C"

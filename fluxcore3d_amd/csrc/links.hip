@@ -1,0 +1,169 @@
+// links.hip — Bouzidi link classification and wall distances on the GPU.
+//
+// Behaviour of pre_post/sdf_tracing.py:20-259:
+//  * link_mask: for every fluid node, bit q (1..18) when the neighbour x + c_q
+//    is solid (in range, or wrapped on axes flagged periodic — the reference
+//    never wraps), and bit q of a second mask when x_ff = x - c_q is in range
+//    (or wrapped) and fluid. The caller compacts the set bits per direction
+//    in node order, which gives the reference's link order (q outer, C order
+//    inner).
+//  * link_q: coarse sampling of the trilinear SDF along the link, first
+//    +/- crossing, bisection, q = hi / (L + 1e-30) (0.5 without a crossing),
+//    and the BFL weights. Every float operation is rounded on its own
+//    (contraction off), matching the eager FP32 operation sequence.
+#include "fc_common.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace fc;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ bool nb(int c, int d, int n, int per, int& out) {
+  int v = c + d;
+  if (v < 0 || v >= n) {
+    if (!per) return false;
+    v = v < 0 ? v + n : v - n;
+  }
+  out = v;
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void link_mask_kernel(const uint8_t* __restrict__ solid, int nx, int ny, int nz,
+                                                           int px, int py, int pz, uint32_t* __restrict__ mask,
+                                                           uint32_t* __restrict__ ffok) {
+  const int64_t N = (int64_t)nx * ny * nz;
+  const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (x >= N) return;
+  uint32_t m = 0, ok = 0;
+  if (!solid[x]) {
+    const int i = (int)(x / ((int64_t)ny * nz)), j = (int)((x / nz) % ny), k = (int)(x % nz);
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+      int a, b, c;
+      if (nb(i, CX(q), nx, px, a) && nb(j, CY(q), ny, py, b) && nb(k, CZ(q), nz, pz, c) &&
+          solid[((int64_t)a * ny + b) * nz + c]) {
+        m |= 1u << q;
+        int a2, b2, c2;
+        if (nb(i, -CX(q), nx, px, a2) && nb(j, -CY(q), ny, py, b2) && nb(k, -CZ(q), nz, pz, c2) &&
+            !solid[((int64_t)a2 * ny + b2) * nz + c2])
+          ok |= 1u << q;
+      }
+    }
+  }
+  mask[x] = m;
+  ffok[x] = ok;
+}
+
+struct Sdf {
+  const float* phi;
+  int nx, ny, nz;
+  float ox, oy, oz, dx;
+  float hx, hy, hz;  // extent - 1.001 (clamp limits)
+};
+
+__device__ float sdf_tri(const Sdf& s, float x, float y, float z) {
+  float gx = (x - s.ox) / s.dx, gy = (y - s.oy) / s.dx, gz = (z - s.oz) / s.dx;
+  gx = gx < 0.0f ? 0.0f : (gx > s.hx ? s.hx : gx);
+  gy = gy < 0.0f ? 0.0f : (gy > s.hy ? s.hy : gy);
+  gz = gz < 0.0f ? 0.0f : (gz > s.hz ? s.hz : gz);
+  const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
+  const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
+  const int i1 = min(i0 + 1, s.nx - 1), j1 = min(j0 + 1, s.ny - 1), k1 = min(k0 + 1, s.nz - 1);
+  auto P = [&](int a, int b, int c) { return s.phi[((int64_t)a * s.ny + b) * s.nz + c]; };
+  const float c000 = P(i0, j0, k0), c100 = P(i1, j0, k0), c010 = P(i0, j1, k0), c110 = P(i1, j1, k0);
+  const float c001 = P(i0, j0, k1), c101 = P(i1, j0, k1), c011 = P(i0, j1, k1), c111 = P(i1, j1, k1);
+  const float cx00 = c000 + fx * (c100 - c000);
+  const float cx10 = c010 + fx * (c110 - c010);
+  const float cx01 = c001 + fx * (c101 - c001);
+  const float cx11 = c011 + fx * (c111 - c011);
+  const float cxy0 = cx00 + fy * (cx10 - cx00);
+  const float cxy1 = cx01 + fy * (cx11 - cx01);
+  return cxy0 + fz * (cxy1 - cxy0);
+}
+
+__global__ __launch_bounds__(kBlock) void link_q_kernel(Sdf s, const int64_t* __restrict__ fluid_flat,
+                                                        const int64_t* __restrict__ dir_i, int64_t n, int n_coarse,
+                                                        int n_bisect, float* __restrict__ q_out,
+                                                        float* __restrict__ w1, float* __restrict__ w2,
+                                                        uint8_t* __restrict__ ge) {
+  const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (l >= n) return;
+  // torch.linspace(0, 1, n_coarse + 1) in FP32
+  const int steps = n_coarse + 1, half = steps / 2;
+  const float step = 1.0f / (float)(steps - 1);
+  const int64_t f = fluid_flat[l];
+  const float fi = (float)(f / ((int64_t)s.ny * s.nz)), fj = (float)((f / s.nz) % s.ny), fk = (float)(f % s.nz);
+  const float x0 = fi * s.dx + s.ox, y0 = fj * s.dx + s.oy, z0 = fk * s.dx + s.oz;
+  const int q = (int)dir_i[l];
+  const float cx = (float)CX(q), cy = (float)CY(q), cz = (float)CZ(q);
+  const float cn = sqrtf(cx * cx + cy * cy + cz * cz);
+  const float den = cn + 1e-30f;
+  const float dx = cx / den, dy = cy / den, dz = cz / den;
+  const float L = s.dx * cn;
+  float t_prev = 0.f, p_prev = 0.f, lo = 0.f, hi = 0.f;
+  bool found = false;
+  for (int k = 0; k < steps; ++k) {
+    const float tv = k < half ? 0.0f + step * (float)k : 1.0f - step * (float)(steps - k - 1);
+    const float t = tv * L;
+    const float p = sdf_tri(s, x0 + t * dx, y0 + t * dy, z0 + t * dz);
+    if (k > 0 && !found && p_prev > 0.0f && !(p > 0.0f)) {
+      found = true;
+      lo = t_prev;
+      hi = t;
+    }
+    t_prev = t;
+    p_prev = p;
+  }
+  float qv = 0.5f;
+  if (found) {
+    for (int it = 0; it < n_bisect; ++it) {
+      const float mid = 0.5f * (lo + hi);
+      const float pm = sdf_tri(s, x0 + mid * dx, y0 + mid * dy, z0 + mid * dz);
+      if (pm > 0.0f) lo = mid; else hi = mid;
+    }
+    qv = hi / (L + 1e-30f);
+  }
+  q_out[l] = qv;
+  if (qv >= 0.5f) {
+    const float a = 1.0f / (2.0f * qv);
+    w1[l] = a; w2[l] = 1.0f - a; ge[l] = 1;
+  } else {
+    const float a = 2.0f * qv;
+    w1[l] = a; w2[l] = 1.0f - a; ge[l] = 0;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fc_links_mask(const uint8_t* solid, int nx, int ny, int nz, int px, int py, int pz, uint32_t* mask,
+                  uint32_t* ffok, hipStream_t stream) {
+  FC_REQUIRE(nx > 0 && ny > 0 && nz > 0, "links_mask: grid extents must be positive");
+  FC_REQUIRE(solid && mask && ffok, "links_mask: null pointer argument");
+  const int64_t N = (int64_t)nx * ny * nz;
+  hipLaunchKernelGGL(link_mask_kernel, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, stream, solid, nx, ny, nz,
+                     px != 0, py != 0, pz != 0, mask, ffok);
+  return check_launch("links_mask");
+}
+
+int fc_links_q(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch,
+               const int64_t* fluid_flat, const int64_t* dir_i, int64_t n, int n_coarse, int n_bisect, float* q,
+               float* w1, float* w2, uint8_t* ge, hipStream_t stream) {
+  FC_REQUIRE(nx > 1 && ny > 1 && nz > 1, "links_q: phi extents must be > 1");
+  FC_REQUIRE(n_coarse >= 1 && n_bisect >= 0, "links_q: n_coarse >= 1 and n_bisect >= 0 required");
+  FC_REQUIRE(pitch > 0.f, "links_q: pitch must be > 0");
+  if (n == 0) return 0;
+  FC_REQUIRE(phi && fluid_flat && dir_i && q && w1 && w2 && ge, "links_q: null pointer argument");
+  Sdf s;
+  s.phi = phi; s.nx = nx; s.ny = ny; s.nz = nz; s.ox = ox; s.oy = oy; s.oz = oz; s.dx = pitch;
+  s.hx = (float)((double)nx - 1.001); s.hy = (float)((double)ny - 1.001); s.hz = (float)((double)nz - 1.001);
+  hipLaunchKernelGGL(link_q_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, s, fluid_flat, dir_i, n,
+                     n_coarse, n_bisect, q, w1, w2, ge);
+  return check_launch("links_q");
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+"""Synthetic geometries of the benchmark workloads (seeded, reproducible).
+
+* ``sphere_channel``: 64x32x32 channel, sphere r=6 at (20,16,16), analytic SDF
+  |x - c| - r sampled at the nodes (origin 0, pitch 1), walls on +-Y and +-Z.
+* ``rsa_spheres``: periodic random sequential addition of non-overlapping
+  spheres with radii uniform in [rmin, rmax] until the target solid fraction
+  (attempts capped; the achieved porosity is returned), union SDF with the
+  minimum-image distance.
+
+Each returns float32 ``phi`` (negative in solid) and the boolean ``solid``.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def sphere_channel(nx=64, ny=32, nz=32, center=(20.0, 16.0, 16.0), radius=6.0, walls=("+Y", "-Y", "+Z", "-Z")):
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    phi = (np.sqrt((i - center[0]) ** 2 + (j - center[1]) ** 2 + (k - center[2]) ** 2) - radius).astype(np.float32)
+    solid = phi < 0
+    solid_flow = solid.copy()
+    sl = {"+X": (slice(-1, None), slice(None), slice(None)), "-X": (slice(0, 1), slice(None), slice(None)),
+          "+Y": (slice(None), slice(-1, None), slice(None)), "-Y": (slice(None), slice(0, 1), slice(None)),
+          "+Z": (slice(None), slice(None), slice(-1, None)), "-Z": (slice(None), slice(None), slice(0, 1))}
+    for face in walls:
+        solid_flow[sl[face]] = True
+    return phi, solid, solid_flow
+
+
+def rsa_spheres(n=256, porosity=0.6, rmin=4.0, rmax=10.0, seed=0, tries_per_sphere=64):
+    """Random sequential addition of non-overlapping spheres in a periodic box.
+
+    Radii are drawn uniform in [rmin, rmax] until their volume reaches the
+    target solid fraction (plus 25 % slack for rejections), then inserted
+    largest first at random positions (``tries_per_sphere`` attempts each;
+    a sphere that does not fit is dropped). Plain RSA in random order jams
+    near a solid fraction of 0.34 for this radius range; inserting by size
+    reaches 0.40. Returns phi, solid, info (achieved porosity, counts, seed).
+    """
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    L = float(n)
+    target = (1.0 - porosity) * L ** 3
+    radii_all = []
+    v = 0.0
+    while v < 1.25 * target:
+        r = rng.uniform(rmin, rmax)
+        radii_all.append(r)
+        v += 4.0 / 3.0 * math.pi * r ** 3
+    radii_all = np.sort(np.asarray(radii_all))[::-1]
+    centers = np.zeros((0, 3))
+    radii = np.zeros(0)
+    tree = None
+    tree_n = 0
+    vol = 0.0
+    attempts = 0
+    for r in radii_all:
+        if vol >= target:
+            break
+        cand = rng.uniform(0.0, L, size=(tries_per_sphere, 3))
+        attempts += tries_per_sphere
+        ok = np.ones(tries_per_sphere, bool)
+        if tree is not None:
+            hits = tree.query_ball_point(cand, r + rmax)
+            for t, h in enumerate(hits):
+                if h:
+                    d = centers[h] - cand[t]
+                    d -= L * np.round(d / L)
+                    if np.any(np.einsum("ij,ij->i", d, d) < (r + radii[h]) ** 2):
+                        ok[t] = False
+        recent = slice(tree_n, len(radii))
+        if len(radii) > tree_n:
+            d = centers[recent][None, :, :] - cand[:, None, :]
+            d -= L * np.round(d / L)
+            ok &= ~np.any(np.einsum("tij,tij->ti", d, d) < (r + radii[recent][None, :]) ** 2, axis=1)
+        good = np.nonzero(ok)[0]
+        if good.size == 0:
+            continue
+        centers = np.vstack([centers, cand[good[0]]])
+        radii = np.append(radii, r)
+        vol += 4.0 / 3.0 * math.pi * r ** 3
+        if len(radii) - tree_n >= 256:
+            tree = cKDTree(np.mod(centers, L), boxsize=L)
+            tree_n = len(radii)
+    phi = np.full((n, n, n), 1e3, dtype=np.float32)
+    ar = np.arange(n)
+    for c, r in zip(centers, radii):
+        m = int(math.ceil(r + 2))
+        idx = [np.arange(int(math.floor(c[a])) - m, int(math.floor(c[a])) + m + 2) for a in range(3)]
+        dist = np.sqrt((idx[0] - c[0])[:, None, None] ** 2 + (idx[1] - c[1])[None, :, None] ** 2
+                       + (idx[2] - c[2])[None, None, :] ** 2) - r
+        sel = np.ix_(ar[idx[0] % n], ar[idx[1] % n], ar[idx[2] % n])
+        phi[sel] = np.minimum(phi[sel], dist.astype(np.float32))
+    solid = phi < 0
+    info = {"porosity": float(1.0 - solid.mean()), "n_spheres": int(len(radii)), "attempts": int(attempts),
+            "seed": int(seed), "target_porosity": float(porosity)}
+    return phi, solid, info

@@ -1,0 +1,288 @@
+"""Fused in-place D3Q19 step (AA pattern) — host side.
+
+``AAEngine`` owns the population buffer of the fused kernel in
+``csrc/lbm_aa.hip`` and the small node/link tables it reads. It is built from
+the same inputs as the reference step (lbm_cuda_ext.py:445-517): the flow
+solid mask, the Bouzidi link set (``IBBData``, sdf_tracing.py:8-18), the
+streaming periodicity and the list of inlet/outlet planes. One ``step()`` is
+one launch of the fused kernel plus the plane kernels of the flow BCs; the
+state it leaves equals the reference's ``sim.f`` after ``_step_cuda``.
+
+Tables (built once, in node order):
+  nflag[N]       uint8: 1 solid, 2 special, bits 3..5 first BC plane + 1
+  wave_base[G]   special nodes before each group of 64 nodes
+  sp_mask[S,4]   per special node: link mask L, solid-source mask S,
+                 second-node mask V, publish mask F
+  sp_base[S,4]   first link, first V link, first publish entry
+  w1, w2, A      per link (sorted by node, then direction)
+  B[2, n_v]      second-node values, double-buffered by step parity
+  ff_idx[n_ff]   V-link slot each publishing node writes
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .d3q19 import C as C_NP, OPP as OPP_NP, W as W_NP
+
+STORAGE_CODES = {"fp32": 0, "fp16": 1, "bf16": 2}
+STORAGE_DTYPES = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}
+COLLISION_CODES = {"bgk": 0, "trt": 1, "mrt": 2, "mrt_smag": 3}
+
+F_SOLID, F_SPECIAL, F_BC_SHIFT = 1, 2, 3
+
+
+def _neighbor_solid_masks(solid: np.ndarray, periodic) -> np.ndarray:
+    """bit q set where the upstream node x - c_q is solid (wrapped on periodic
+    axes, out-of-domain counts as fluid)."""
+    nx, ny, nz = solid.shape
+    S = np.zeros(solid.shape, dtype=np.uint32)
+    pad = np.pad(solid, 1, mode="constant", constant_values=False)
+    for ax, per in enumerate(periodic):
+        if per:
+            # wrap the halo on periodic axes
+            idx = [slice(None)] * 3
+            idx[ax] = 0
+            src = [slice(None)] * 3
+            src[ax] = -2
+            pad[tuple(idx)] = pad[tuple(src)]
+            idx[ax] = -1
+            src[ax] = 1
+            pad[tuple(idx)] = pad[tuple(src)]
+    for q in range(1, 19):
+        cx, cy, cz = C_NP[q]
+        up = pad[1 - cx:1 - cx + nx, 1 - cy:1 - cy + ny, 1 - cz:1 - cz + nz]
+        S |= up.astype(np.uint32) << np.uint32(q)
+    return S
+
+
+class AAEngine:
+    """In-place D3Q19 state + tables of the fused kernel."""
+
+    def __init__(self, nx, ny, nz, solid_flow, *, periodic=(False, False, False), collision="bgk",
+                 omega=1.0, omega_minus=1.0, S=None, cs_dx2=0.0, force=None, sponge=None, sponge_axis=0,
+                 links=None, ibb_mode="reference", bcs=(), storage="fp32", device="cuda"):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.N = self.nx * self.ny * self.nz
+        self.periodic = tuple(bool(p) for p in periodic)
+        if storage not in STORAGE_CODES:
+            raise ValueError(f"storage must be one of {sorted(STORAGE_CODES)}")
+        if collision not in COLLISION_CODES:
+            raise ValueError(f"collision must be one of {sorted(COLLISION_CODES)}")
+        if ibb_mode not in ("reference", "textbook"):
+            raise ValueError("ibb_mode must be 'reference' or 'textbook'")
+        self.storage = storage
+        self.device = torch.device(device)
+        self.ibb_mode = ibb_mode
+        self.bcs = [dict(b) for b in bcs]
+        if len(self.bcs) > 6:
+            raise ValueError("at most 6 inlet/outlet planes")
+        solid = np.ascontiguousarray(np.asarray(solid_flow, dtype=bool).reshape(self.nx, self.ny, self.nz))
+        self._build_tables(solid, links)
+        self._collision = self._make_collision(collision, omega, omega_minus, S, cs_dx2, force, sponge, sponge_axis)
+        self.Ns = int(_lib.call("fc_aa_slot_count", self.nx, self.ny, self.nz, *[int(p) for p in self.periodic]))
+        self.pop = torch.zeros(19 * self.Ns, dtype=STORAGE_DTYPES[storage], device=self.device)
+        self.t = 0
+        self.t_loaded = 0
+        self._ws = None
+
+    # ------------------------------------------------------------------ tables
+    def _build_tables(self, solid: np.ndarray, links) -> None:
+        nx, ny, nz, N = self.nx, self.ny, self.nz, self.N
+        flat_solid = solid.reshape(-1)
+        Smask = _neighbor_solid_masks(solid, self.periodic).reshape(-1)
+        Smask[flat_solid] = 0
+
+        Lmask = np.zeros(N, dtype=np.uint32)
+        Vmask = np.zeros(N, dtype=np.uint32)
+        Fmask = np.zeros(N, dtype=np.uint32)
+        w1s = np.zeros(0, np.float32)
+        w2s = np.zeros(0, np.float32)
+        link_node = np.zeros(0, np.int64)
+        v_node = np.zeros(0, np.int64)
+        ff_node = np.zeros(0, np.int64)
+        ff_slot = np.zeros(0, np.int32)
+        B_init = np.zeros(0, np.float32)
+        self.n_links = 0
+        if links is not None and int(links["fluid_flat"].size) > 0:
+            xf = np.asarray(links["fluid_flat"], np.int64)
+            q = np.asarray(links["dir_i"], np.int64)
+            w1 = np.asarray(links["w1"], np.float32).copy()
+            w2 = np.asarray(links["w2"], np.float32).copy()
+            ge = np.asarray(links["q_ge_half"], bool)
+            ffv = np.asarray(links["ff_valid"], bool)
+            ff = np.asarray(links["ff_flat"], np.int64)
+            if xf.min() < 0 or xf.max() >= N or q.min() < 1 or q.max() > 18:
+                raise ValueError("link table out of range")
+            keep = ~flat_solid[xf]  # links at flow-solid nodes are reset by the reference
+            xf, q, w1, w2, ge, ffv, ff = xf[keep], q[keep], w1[keep], w2[keep], ge[keep], ffv[keep], ff[keep]
+            t = q if self.ibb_mode == "reference" else OPP_NP[q]
+            # q < 1/2 without a second node: the reference takes f*_opp alone
+            bad = (~ge) & (~ffv)
+            w1[bad] = 1.0
+            w2[bad] = 0.0
+            v = (~ge) & ffv
+            order = np.lexsort((t, xf))
+            xf, t, w1, w2, v, ff = xf[order], t[order], w1[order], w2[order], v[order], ff[order]
+            key = xf * 19 + t
+            if np.any(key[1:] == key[:-1]):
+                raise ValueError("duplicate Bouzidi link (node, direction)")
+            np.bitwise_or.at(Lmask, xf, (np.uint32(1) << t.astype(np.uint32)))
+            # a link overrides any solid-source substitution of the same input
+            Smask[xf] &= ~Lmask[xf]
+            np.bitwise_or.at(Vmask, xf[v], (np.uint32(1) << t[v].astype(np.uint32)))
+            w1s, w2s, link_node = w1, w2, xf
+            self.n_links = int(xf.size)
+            # second-node (V) links in (node, direction) order
+            vx, vt, vff = xf[v], t[v], ff[v]
+            v_node = vx
+            n_v = int(vx.size)
+            ff_solid = flat_solid[vff]
+            p = OPP_NP[vt]  # population the second node publishes
+            B_init = np.where(ff_solid, W_NP[p], 0.0).astype(np.float32)
+            pub = ~ff_solid
+            pm = np.arange(n_v, dtype=np.int64)[pub]
+            pn, pp = vff[pub], p[pub]
+            o2 = np.lexsort((pp, pn))
+            pn, pp, pm = pn[o2], pp[o2], pm[o2]
+            if np.any((pn * 19 + pp)[1:] == (pn * 19 + pp)[:-1]):
+                raise ValueError("duplicate second-node publication")
+            np.bitwise_or.at(Fmask, pn, (np.uint32(1) << pp.astype(np.uint32)))
+            ff_node, ff_slot = pn, pm.astype(np.int32)
+        self.n_v = int(v_node.size)
+
+        special = (~flat_solid) & ((Lmask | Smask | Fmask) != 0)
+        sp_idx = np.nonzero(special)[0]
+        self.n_special = int(sp_idx.size)
+
+        def first_index(owner: np.ndarray) -> np.ndarray:
+            # index of the first entry owned by each special node (entries sorted by owner)
+            return np.searchsorted(owner, sp_idx).astype(np.int32)
+
+        sp_mask = np.stack([Lmask[sp_idx], Smask[sp_idx], Vmask[sp_idx], Fmask[sp_idx]], 1).view(np.int32)
+        sp_base = np.stack([first_index(link_node), first_index(v_node), first_index(ff_node),
+                            np.zeros(self.n_special, np.int32)], 1).astype(np.int32)
+
+        nflag = flat_solid.astype(np.uint8) * F_SOLID | special.astype(np.uint8) * F_SPECIAL
+        for b, bc in enumerate(self.bcs):
+            ax, pl = int(bc["axis"]), int(bc["plane"])
+            sel = [slice(None)] * 3
+            sel[ax] = pl
+            plane = np.zeros((nx, ny, nz), bool)
+            plane[tuple(sel)] = True
+            plane = plane.reshape(-1) & ~flat_solid & ((nflag >> F_BC_SHIFT) == 0)
+            nflag[plane] |= np.uint8((b + 1) << F_BC_SHIFT)
+
+        groups = (N + 63) // 64
+        cnt = np.zeros(groups * 64, np.int64)
+        cnt[:N] = special
+        wave_base = np.concatenate([[0], np.cumsum(cnt.reshape(groups, 64).sum(1))[:-1]]).astype(np.int32)
+
+        dev = self.device
+
+        def dt(a, dtype):
+            a = np.ascontiguousarray(a)
+            if a.size == 0:
+                a = np.zeros(1, a.dtype)
+            return torch.from_numpy(a).to(device=dev, dtype=dtype)
+
+        self.nflag = dt(nflag, torch.uint8)
+        self.wave_base = dt(wave_base, torch.int32)
+        self.sp_mask = dt(sp_mask, torch.int32)
+        self.sp_base = dt(sp_base, torch.int32)
+        self.w1 = dt(w1s, torch.float32)
+        self.w2 = dt(w2s, torch.float32)
+        self.A = torch.zeros(max(self.n_links, 1), dtype=torch.float32, device=dev)
+        self.B_init = dt(np.concatenate([B_init, B_init]) if self.n_v else np.zeros(2, np.float32), torch.float32)
+        self.B = self.B_init.clone()
+        self.ff_idx = dt(ff_slot, torch.int32)
+        self._tables = _lib.AATables(
+            self.nflag.data_ptr(), self.wave_base.data_ptr(), self.sp_mask.data_ptr(), self.sp_base.data_ptr(),
+            self.w1.data_ptr(), self.w2.data_ptr(), self.A.data_ptr(), self.B.data_ptr(), self.ff_idx.data_ptr(),
+            max(self.n_v, 1))
+
+    def _make_collision(self, collision, omega, omega_minus, S, cs_dx2, force, sponge, sponge_axis):
+        c = _lib.AACollision()
+        c.kind = COLLISION_CODES[collision]
+        c.om1 = float(omega)
+        c.om2 = float(omega_minus)
+        if c.kind >= 2:
+            if S is None:
+                raise ValueError("MRT collision needs the 19 relaxation rates S")
+            s = np.asarray(S, np.float32).reshape(-1)
+            if s.size != 19:
+                raise ValueError("S must have 19 elements")
+            for k in range(19):
+                c.S[k] = float(s[k])
+        c.cs_dx2 = float(cs_dx2)
+        if force is not None and any(float(v) != 0.0 for v in force):
+            for d in range(3):
+                c.force[d] = float(force[d])
+            c.use_force = 1
+        self._sponge = None
+        if sponge is not None:
+            prof = torch.as_tensor(np.asarray(sponge, np.float32)).to(self.device)
+            if prof.numel() != (self.nx, self.ny, self.nz)[sponge_axis]:
+                raise ValueError("sponge profile length must match the sponge axis")
+            self._sponge = prof.contiguous()
+            c.sponge = self._sponge.data_ptr()
+            c.sponge_axis = int(sponge_axis)
+        return c
+
+    # ------------------------------------------------------------------ state
+    @property
+    def _dims(self):
+        return (self.nx, self.ny, self.nz, *[int(p) for p in self.periodic])
+
+    def load(self, f: torch.Tensor) -> None:
+        """Start from reference-layout populations f[19, nx, ny, nz] (FP32)."""
+        if f.dtype != torch.float32 or f.numel() != 19 * self.N:
+            raise RuntimeError("f must be float32 with 19*nx*ny*nz elements")
+        f = f.to(self.device).contiguous()
+        self.B.copy_(self.B_init)
+        _lib.call("fc_aa_load", self.pop.data_ptr(), STORAGE_CODES[self.storage], *self._dims, self._tables,
+                  f.data_ptr(), _lib.stream_handle(self.device))
+        self.t_loaded = self.t
+
+    def materialize(self, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Populations entering the next step, reference layout [19, nx, ny, nz]."""
+        if out is None:
+            out = torch.empty((19, self.nx, self.ny, self.nz), dtype=torch.float32, device=self.device)
+        _lib.call("fc_aa_materialize", self.pop.data_ptr(), STORAGE_CODES[self.storage], *self._dims,
+                  self._tables, self.t & 1, int(self.t == self.t_loaded), out.data_ptr(),
+                  _lib.stream_handle(self.device))
+        return out
+
+    def step(self, rho=None, u=None, v=None, w=None) -> None:
+        """One time step; writes pre-collision macros if the four outputs are given."""
+        stream = _lib.stream_handle(self.device)
+        _lib.call("fc_aa_step", self.pop.data_ptr(), STORAGE_CODES[self.storage], *self._dims, self._tables,
+                  self._collision, self.t & 1, int(self.t == self.t_loaded), _lib.ptr(rho), _lib.ptr(u),
+                  _lib.ptr(v), _lib.ptr(w), stream)
+        self.t += 1
+        self._apply_bcs(stream)
+
+    def _apply_bcs(self, stream) -> None:
+        parity = self.t & 1
+        for b, bc in enumerate(self.bcs):
+            if bc["kind"] == "zou_he":
+                _lib.call("fc_aa_zou_he", self.pop.data_ptr(), STORAGE_CODES[self.storage], *self._dims,
+                          self._tables, parity, 0, b, int(bc["axis"]), int(bc["plane"]), int(bc["interior"]),
+                          int(bc["bc_type"]), float(bc["u_in"]), float(bc["rho_out"]), int(bc["sign"]), stream)
+            elif bc["kind"] == "convective":
+                n1, n2 = {0: (self.ny, self.nz), 1: (self.nx, self.nz), 2: (self.nx, self.ny)}[int(bc["axis"])]
+                need = int(_lib.call("fc_aa_convective_workspace_bytes", n1, n2))
+                if self._ws is None or self._ws.numel() < need:
+                    self._ws = torch.empty(max(need, 16), dtype=torch.uint8, device=self.device)
+                _lib.call("fc_aa_convective", self.pop.data_ptr(), STORAGE_CODES[self.storage], *self._dims,
+                          self._tables, parity, 0, b, int(bc["axis"]), int(bc["plane"]), int(bc["interior"]),
+                          int(bc["unknown_cax"]), float(bc["rho_out"]), self._ws.data_ptr(), self._ws.numel(),
+                          stream)
+            else:
+                raise ValueError(f"unknown BC kind {bc['kind']!r}")
+
+    # ------------------------------------------------------------------ info
+    def bytes_per_update(self) -> int:
+        """Algorithmic population bytes per lattice update (19 loads + 19 stores)."""
+        return 2 * 19 * self.pop.element_size()

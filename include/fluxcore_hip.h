@@ -1,0 +1,133 @@
+/* fluxcore_hip.h — C ABI of libfluxcore_hip.so (MI355X / gfx950).
+ *
+ * Conventions
+ *  - All pointers are device pointers (hipMalloc / torch CUDA tensors) unless
+ *    stated otherwise; sizes are element counts.
+ *  - Populations are SoA f[q*N + flat], flat = (i*ny + j)*nz + k, the layout of
+ *    the reference's f[19, nx, ny, nz] tensor (solver/lbm_kernels.cu:6-9).
+ *  - Every int-returning entry point returns 0 on success; on failure it
+ *    returns non-zero and fc_last_error() holds the message (per thread).
+ *  - Work is enqueued on `stream` and returns without synchronising.
+ *  - No entry point keeps a reference to its arguments; scratch memory is
+ *    passed in by the caller (workspace arguments).
+ *
+ * Each declaration names the reference interface it replaces (path:line in
+ * the fluxcore3D tree).
+ */
+#ifndef FLUXCORE_HIP_H
+#define FLUXCORE_HIP_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *fc_last_error(void);
+int fc_version(void);
+
+/* ---- D3Q19 kernels, one per pybind function of lbm_kernels.cu:1347-1386 ---- */
+
+/* macro_collide_bgk (lbm_kernels.cu:628-647): f -> f_post, pre-collision rho,u,v,w;
+ * solid nodes get rest weights and (1,0,0,0). */
+int fc_lbm_macro_collide_bgk(const float *f, float *f_post, float *rho, float *u, float *v, float *w,
+                             const uint8_t *solid, float omega, int64_t N, hipStream_t stream);
+/* macro_collide_trt (lbm_kernels.cu:649-669) */
+int fc_lbm_macro_collide_trt(const float *f, float *f_post, float *rho, float *u, float *v, float *w,
+                             const uint8_t *solid, float omega_plus, float omega_minus, int64_t N,
+                             hipStream_t stream);
+/* macro_collide_mrt (lbm_kernels.cu:671-692); S_rates: 19 device floats by moment row */
+int fc_lbm_macro_collide_mrt(const float *f, float *f_post, float *rho, float *u, float *v, float *w,
+                             const uint8_t *solid, const float *S_rates, int64_t N, hipStream_t stream);
+/* macro_collide_mrt_smag (lbm_kernels.cu:1120-1145) */
+int fc_lbm_macro_collide_mrt_smag(const float *f, float *f_post, float *rho, float *u, float *v, float *w,
+                                  const uint8_t *solid, const float *S_rates, float Cs_dx2, float omega0,
+                                  int64_t N, hipStream_t stream);
+/* stream_pull (lbm_kernels.cu:695-708): pull with periodic wrap / copy-self */
+int fc_lbm_stream_pull(const float *f_in, float *f_out, int nx, int ny, int nz, int periodic_x, int periodic_y,
+                       int periodic_z, hipStream_t stream);
+/* sponge (lbm_kernels.cu:711-727) */
+int fc_lbm_sponge(float *f, const float *rho, const float *u, const float *v, const float *w, const float *sigma,
+                  int64_t N, hipStream_t stream);
+/* ibb (lbm_kernels.cu:730-754): Bouzidi per link, writes f_stream[dir_i][fluid_flat] */
+int fc_lbm_ibb(const float *f_post, float *f_stream, const int32_t *fluid_flat, const int32_t *dir_i,
+               const int32_t *dir_opp, const float *w1, const float *w2, const uint8_t *q_ge_half,
+               const int32_t *ff_flat, const uint8_t *ff_valid, int64_t n_links, int64_t N, hipStream_t stream);
+/* reset_solid (lbm_kernels.cu:757-764) */
+int fc_lbm_reset_solid(float *f, const uint8_t *solid, int64_t N, hipStream_t stream);
+/* zou_he_bc (lbm_kernels.cu:975-996): regularised Zou/He on one plane */
+int fc_lbm_zou_he_bc(float *f, const uint8_t *solid, int nx, int ny, int nz, int axis, int plane_idx,
+                     int interior_idx, int bc_type, float u_in_lat, float rho_out_lat, int sign,
+                     hipStream_t stream);
+/* convective_bc (lbm_kernels.cu:1299-1344): zero-gradient copy + mass correction.
+ * The reference keeps two static device scalars; here the caller passes a
+ * workspace of fc_lbm_convective_bc_workspace_bytes(n1, n2) bytes. */
+size_t fc_lbm_convective_bc_workspace_bytes(int n1, int n2);
+int fc_lbm_convective_bc(float *f, const uint8_t *solid, int nx, int ny, int nz, int axis, int outlet_idx,
+                         int interior_idx, int unknown_cax, float rho_out_lat, int n1, int n2, void *workspace,
+                         size_t workspace_bytes, hipStream_t stream);
+
+/* ---- fused in-place step (AA pattern): replaces the kernel sequence of
+ *      LBMCUDAUpgrade._step_cuda (lbm_cuda_ext.py:445-517) ---- */
+typedef struct fc_aa_tables {
+  const uint8_t *nflag;     /* [N] 1 solid, 2 special, bits 3..5 first BC plane + 1 */
+  const int32_t *wave_base; /* [ceil(N/64)] special nodes before each 64-node group */
+  const int32_t *sp_mask;   /* [n_special][4] link, solid-source, second-node, publish masks */
+  const int32_t *sp_base;   /* [n_special][4] first link, first second-node link, first publish entry */
+  const float *w1;          /* [n_links] Bouzidi weights */
+  const float *w2;          /* [n_links] */
+  float *A;                 /* [n_links] own-node part of each link value */
+  float *B;                 /* [2][n_v] second-node part, by step parity */
+  const int32_t *ff_idx;    /* [n_publish] slot in B written by each publishing node */
+  int64_t n_v;
+} fc_aa_tables;
+
+typedef struct fc_aa_collision {
+  int kind;            /* 0 BGK, 1 TRT, 2 MRT, 3 MRT + Smagorinsky */
+  float om1, om2;      /* BGK omega | TRT omega+, omega- | Smagorinsky omega0 */
+  float S[19];         /* MRT rates by moment row */
+  float cs_dx2;        /* (Cs dx)^2 */
+  float force[3];      /* Guo body force (lattice units) */
+  int use_force;
+  const float *sponge; /* device sigma profile along sponge_axis, or NULL */
+  int sponge_axis;
+} fc_aa_collision;
+
+/* storage: 0 FP32 populations, 1 FP16 deviations f - w_i, 2 BF16 deviations.
+ * px/py/pz: periodic axes (non-periodic axes carry a one-cell ghost layer). */
+int64_t fc_aa_slot_count(int nx, int ny, int nz, int px, int py, int pz);
+int fc_aa_step(void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables *tables,
+               const fc_aa_collision *collision, int parity, int first, float *rho, float *u, float *v, float *w,
+               hipStream_t stream);
+int fc_aa_materialize(const void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz,
+                      const fc_aa_tables *tables, int parity, int first, float *f_out, hipStream_t stream);
+int fc_aa_load(void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables *tables,
+               const float *f_in, hipStream_t stream);
+/* inlet/outlet planes on the AA state (zou_he_bc / convective_bc semantics) */
+int fc_aa_zou_he(void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables *tables,
+                 int parity, int first, int bc_order, int axis, int plane_idx, int interior_idx, int bc_type,
+                 float u_in_lat, float rho_out_lat, int sign, hipStream_t stream);
+size_t fc_aa_convective_workspace_bytes(int n1, int n2);
+int fc_aa_convective(void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz,
+                     const fc_aa_tables *tables, int parity, int first, int bc_order, int axis, int outlet_idx,
+                     int interior_idx, int unknown_cax, float rho_out_lat, void *workspace, size_t workspace_bytes,
+                     hipStream_t stream);
+
+/* ---- Bouzidi link set and wall distances (pre_post/sdf_tracing.py:20-259) ---- */
+/* build_links_from_mask_gpu (sdf_tracing.py:20-85): per node, bit q of mask when
+ * x + c_q is solid, bit q of ffok when x - c_q is a valid fluid node. */
+int fc_links_mask(const uint8_t *solid, int nx, int ny, int nz, int px, int py, int pz, uint32_t *mask,
+                  uint32_t *ffok, hipStream_t stream);
+/* bfl_q_from_sdf + bfl_weights_from_q (sdf_tracing.py:127-205) with x0, d, L of
+ * build_ibb_from_sdf_gpu (:208-247) */
+int fc_links_q(const float *phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch,
+               const int64_t *fluid_flat, const int64_t *dir_i, int64_t n, int n_coarse, int n_bisect, float *q,
+               float *w1, float *w2, uint8_t *q_ge_half, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUXCORE_HIP_H */

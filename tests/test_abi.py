@@ -1,0 +1,47 @@
+"""The C ABI library loads and exports every symbol declared in include/*.h (no GPU calls)."""
+import ctypes
+import glob
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(fc_[A-Za-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    from fluxcore3d_amd import _lib
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert len(names) > 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_knows_every_declared_symbol():
+    from fluxcore3d_amd import _lib
+    import fluxcore3d_amd.pre_post.sdf_tracing  # noqa: F401  (registers its entry points)
+    import fluxcore3d_amd.solver.lbm_cuda_ext  # noqa: F401
+    for mod in ("fluxcore3d_amd.solver.mg_cuda_ext", "fluxcore3d_amd.solver.thermal_cuda_ext",
+                "fluxcore3d_amd.learned.conv"):
+        try:
+            __import__(mod)
+        except ModuleNotFoundError:
+            pass
+    missing = [n for n in _declared() if n not in _lib.exported_symbols()]
+    assert not missing, missing
+
+
+def test_error_channel_without_gpu():
+    from fluxcore3d_amd import _lib
+    # argument validation fails before any device call
+    rc = _lib.lib().fc_lbm_stream_pull(None, None, 0, 1, 1, 0, 0, 0, None)
+    assert rc != 0
+    assert b"positive" in _lib.lib().fc_last_error()

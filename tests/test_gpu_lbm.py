@@ -1,0 +1,217 @@
+"""GPU parity of the D3Q19 kernels against the FP32 oracle, through the C ABI.
+
+Tolerances: the reference kernels are FP32 with fast-math contraction, the
+HIP kernels FP32 with contraction and the oracle FP32 without it, so values
+agree to a few ulp per operation; fields are compared as
+max|a - b| / max|b| <= 1e-5 (velocity, density, populations), streaming and
+solid resets bit-exactly.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import lbm as olbm  # noqa: E402
+from tests import cases  # noqa: E402
+
+DEV = "cuda"
+S_TEST = np.array([0, 1.19, 1.4, 0, 1.4, 0, 1.4, 0, 1.4, 1.8, 1.8, 1.8, 1.8, 1.8, 1.8, 1.8, 1.2, 1.2, 1.2], np.float32)
+
+
+@pytest.fixture(scope="module")
+def K():
+    from fluxcore3d_amd.solver.lbm_cuda_ext import LBM_CUDA, LBM_CUDA_AVAILABLE
+    assert LBM_CUDA_AVAILABLE, "HIP device required"
+    return LBM_CUDA
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _rand_f(n, seed=42):
+    rng = np.random.default_rng(seed)
+    f = rng.random((19, n)).astype(np.float32) * 0.1
+    return (f / f.sum(0, keepdims=True)).astype(np.float32)
+
+
+def T(a, dtype=None):
+    t = torch.as_tensor(np.ascontiguousarray(a), device=DEV)
+    return t if dtype is None else t.to(dtype)
+
+
+@pytest.mark.parametrize("kind", ["bgk", "trt", "mrt", "mrt_smag"])
+def test_macro_collide(K, kind):
+    nx, ny, nz = 8, 10, 6
+    N = nx * ny * nz
+    f = _rand_f(N)
+    solid = np.zeros(N, np.uint8)
+    solid[::17] = 1
+    om1, om2 = (1.6, 1.1) if kind != "mrt_smag" else (1.7, 0.0)
+    want = olbm.macro_collide(kind, f, solid, om1, om2, S_TEST, 0.01)
+    ft, fp = T(f), torch.empty(19, N, device=DEV)
+    macro = [torch.empty(N, device=DEV) for _ in range(4)]
+    st = T(solid)
+    if kind == "bgk":
+        K.macro_collide_bgk(ft, fp, *macro, st, om1)
+    elif kind == "trt":
+        K.macro_collide_trt(ft, fp, *macro, st, om1, om2)
+    elif kind == "mrt":
+        K.macro_collide_mrt(ft, fp, *macro, st, T(S_TEST))
+    else:
+        K.macro_collide_mrt_smag(ft, fp, *macro, st, T(S_TEST), 0.01, om1)
+    got = [fp.cpu().numpy()] + [m.cpu().numpy() for m in macro]
+    for g, w in zip(got, want):
+        assert rel(g, w) < 1e-5
+    np.testing.assert_array_equal(got[0][:, solid.astype(bool)], want[0][:, solid.astype(bool)])
+
+
+@pytest.mark.parametrize("per", [(1, 1, 1), (0, 0, 0), (0, 1, 0)])
+def test_stream_pull_bit_exact(K, per):
+    nx, ny, nz = 9, 7, 11
+    f = _rand_f(nx * ny * nz, seed=3)
+    out = torch.empty(19, nx * ny * nz, device=DEV)
+    K.stream_pull(T(f), out, nx, ny, nz, *per)
+    np.testing.assert_array_equal(out.cpu().numpy(), olbm.stream_pull(f, nx, ny, nz, *per))
+
+
+def test_ibb_reset_sponge(K):
+    c = cases.config1()
+    N = c["nx"] * c["ny"] * c["nz"]
+    rng = np.random.default_rng(5)
+    fp = rng.random((19, N)).astype(np.float32)
+    fs = rng.random((19, N)).astype(np.float32)
+    L = c["links"]
+    want = olbm.ibb(fp, fs, L)
+    fs_t = T(fs)
+    i32 = lambda a: T(a.astype(np.int32))  # noqa: E731
+    K.ibb(T(fp), fs_t, i32(L["fluid_flat"]), i32(L["dir_i"]), i32(olbm.OPP[L["dir_i"]]), T(L["w1"]), T(L["w2"]),
+          T(L["q_ge_half"].astype(np.uint8)), i32(L["ff_flat"]), T(L["ff_valid"].astype(np.uint8)))
+    assert rel(fs_t.cpu().numpy(), want) < 1e-6
+    solid = c["solid_flow"].reshape(-1).astype(np.uint8)
+    K.reset_solid(fs_t, T(solid))
+    np.testing.assert_array_equal(fs_t.cpu().numpy(), olbm.reset_solid(fs_t.cpu().numpy(), solid))
+    rho = 1 + 0.01 * rng.random(N).astype(np.float32)
+    u, v, w = (0.02 * rng.standard_normal(N).astype(np.float32) for _ in range(3))
+    sigma = np.where(rng.random(N) < 0.3, 0.2, 0.0).astype(np.float32)
+    fsp = T(fp)
+    K.sponge(fsp, T(rho), T(u), T(v), T(w), T(sigma))
+    assert rel(fsp.cpu().numpy(), olbm.sponge(fp, rho, u, v, w, sigma)) < 1e-6
+
+
+@pytest.mark.parametrize("bc", ["inlet", "outlet", "convective"])
+def test_plane_bcs(K, bc):
+    nx, ny, nz = 10, 8, 6
+    N = nx * ny * nz
+    rng = np.random.default_rng(2)
+    rho = 1 + 0.01 * rng.random(N)
+    u = 0.03 * rng.standard_normal((3, N))
+    f = (olbm.feq(rho, *u, np.float64) * (1 + 0.01 * rng.standard_normal((19, N)))).astype(np.float32)
+    solid = np.zeros(N, np.uint8)
+    solid[rng.random(N) < 0.1] = 1
+    ft = T(f)
+    if bc == "inlet":
+        K.zou_he_bc(ft, T(solid), nx, ny, nz, 0, 0, 1, 0, 0.05, 1.0, 1)
+        want = olbm.zou_he(f, solid, nx, ny, nz, 0, 0, 1, 0, 0.05, 1.0, 1)
+    elif bc == "outlet":
+        K.zou_he_bc(ft, T(solid), nx, ny, nz, 1, ny - 1, ny - 2, 1, 0.05, 1.0, 1)
+        want = olbm.zou_he(f, solid, nx, ny, nz, 1, ny - 1, ny - 2, 1, 0.05, 1.0, 1)
+    else:
+        K.convective_bc(ft, T(solid), nx, ny, nz, 0, nx - 1, nx - 2, -1, 1.0, ny, nz)
+        want = olbm.convective(f, solid, nx, ny, nz, 0, nx - 1, nx - 2, -1, 1.0)
+    assert rel(ft.cpu().numpy(), want) < 1e-5
+
+
+def _oracle_run(c, steps, kind="bgk", om2=1.0, S=None, periodic=(0, 0, 0), bcs=()):
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    f = np.broadcast_to(olbm.W32[:, None], (19, nx * ny * nz)).copy()
+    macro = None
+    for _ in range(steps):
+        f, *macro = olbm.step(f, c["solid_flow"], nx, ny, nz, periodic, kind, c["omega"], om2, S, 0.0, c["links"],
+                              None, bcs)
+    return f, macro
+
+
+def _sim_for(c, kind, engine, S=None, om2=None):
+    cfg = cases.Cfg(**vars(c["cfg"]))
+    cfg.collision = kind
+    cfg.fc_engine = engine
+    from fluxcore3d_amd.pre_post.sdf_tracing import IBBData
+    L = c["links"]
+    ibb = IBBData(n_links=int(L["fluid_flat"].size), fluid_flat=torch.as_tensor(L["fluid_flat"]),
+                  dir_i=torch.as_tensor(L["dir_i"]), dir_opp=torch.as_tensor(olbm.OPP[L["dir_i"]]),
+                  q_ge_half=torch.as_tensor(L["q_ge_half"]), w_i=torch.as_tensor(L["w1"]),
+                  w_second=torch.as_tensor(L["w2"]), ff_flat=torch.as_tensor(L["ff_flat"]),
+                  ff_valid=torch.as_tensor(L["ff_valid"]))
+    return cases.MiniSim(c["nx"], c["ny"], c["nz"], cfg, c["solid"], c["solid_flow"], c["phi"], c["omega"], DEV,
+                         S=S, links=ibb, omega_minus=om2)
+
+
+@pytest.mark.parametrize("engine", ["aa", "reference"])
+@pytest.mark.parametrize("kind", ["bgk", "trt", "mrt"])
+def test_config1_step_sequence(engine, kind):
+    """64x32x32 sphere channel (inlet, convective outlet, walls, Bouzidi links),
+    150 steps of the drop-in step against the oracle's reference step."""
+    from fluxcore3d_amd.solver.lbm_cuda_ext import LBMCUDAUpgrade, flow_bc_plan
+    c = cases.config1()
+    om2 = 1.0 / (0.5 + 0.1875 / (0.6 - 0.5)) if kind == "trt" else None
+    S = cases.mrt_S(c["omega"]) if kind == "mrt" else None
+    sim = _sim_for(c, kind, engine, S=S, om2=om2)
+    LBMCUDAUpgrade.patch(sim)
+    steps = 150
+    for _ in range(steps):
+        sim.step(do_thermal=False)
+    torch.cuda.synchronize()
+    f_want, (rho_w, u_w, v_w, w_w) = _oracle_run(c, steps, kind, om2 or 1.0, S, (0, 0, 0), flow_bc_plan(sim))
+    assert rel(sim.u.cpu().numpy().reshape(-1), u_w) < 1e-5
+    assert rel(sim.v.cpu().numpy().reshape(-1), v_w) < 1e-5
+    assert rel(sim.wv.cpu().numpy().reshape(-1), w_w) < 1e-5
+    assert rel(sim.rho.cpu().numpy().reshape(-1), rho_w) < 1e-5
+    assert rel(sim.f.cpu().numpy().reshape(19, -1), f_want) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["bgk", "mrt"])
+def test_aa_periodic_pack(kind):
+    """Fully periodic sphere pack (48^3, porosity 0.6) with reference Bouzidi links."""
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    c = cases.periodic_pack()
+    c["omega"] = 1.0 / 0.55
+    S = cases.mrt_S(c["omega"]) if kind == "mrt" else None
+    eng = AAEngine(c["nx"], c["ny"], c["nz"], c["solid_flow"], periodic=(1, 1, 1), collision=kind, omega=c["omega"],
+                   S=S, links=c["links"], device=DEV)
+    n = c["nx"] * c["ny"] * c["nz"]
+    rng = np.random.default_rng(0)
+    rho = 1 + 0.01 * rng.random(n)
+    u = 0.02 * rng.standard_normal((3, n))
+    f0 = olbm.feq(rho, *u, np.float32).astype(np.float32)
+    eng.load(torch.as_tensor(f0, device=DEV))
+    macro = [torch.empty(n, device=DEV) for _ in range(4)]
+    steps = 60
+    for _ in range(steps):
+        eng.step(*macro)
+    f = f0.copy()
+    for _ in range(steps):
+        f, *mw = olbm.step(f, c["solid_flow"], c["nx"], c["ny"], c["nz"], (1, 1, 1), kind, c["omega"], 1.0, S, 0.0,
+                           c["links"])
+    for g, w in zip(macro, mw):
+        assert rel(g.cpu().numpy(), w) < 1e-5
+    assert rel(eng.materialize().cpu().numpy().reshape(19, -1), f) < 1e-5
+
+
+def test_links_and_q_bit_exact():
+    from fluxcore3d_amd.pre_post.sdf_tracing import bfl_q_and_weights, build_links_from_mask_gpu
+    c = cases.config1()
+    ff, di, do, fff, ok = build_links_from_mask_gpu(torch.as_tensor(c["solid"], device=DEV))
+    L = c["links"]
+    np.testing.assert_array_equal(ff.cpu().numpy(), L["fluid_flat"])
+    np.testing.assert_array_equal(di.cpu().numpy(), L["dir_i"])
+    np.testing.assert_array_equal(fff.cpu().numpy(), L["ff_flat"])
+    np.testing.assert_array_equal(ok.cpu().numpy(), L["ff_valid"])
+    q, ge, w1, w2 = bfl_q_and_weights(torch.as_tensor(c["phi"], device=DEV), (0, 0, 0), 1.0, ff, di)
+    np.testing.assert_array_equal(q.cpu().numpy(), L["q"])
+    np.testing.assert_array_equal(ge.cpu().numpy(), L["q_ge_half"])
+    np.testing.assert_array_equal(w1.cpu().numpy(), L["w1"])
+    np.testing.assert_array_equal(w2.cpu().numpy(), L["w2"])
