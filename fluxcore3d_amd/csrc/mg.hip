@@ -1,0 +1,616 @@
+// mg.hip — geometric multigrid for the implicit thermal operator A = I + dt L.
+//
+// L is the 7-point finite-volume operator of nlevel_mg_cuda.py:418-507
+// (harmonic-mean diffusion + first-order upwind), with the boundary ghosts of
+// mg_kernels.cu:37-44: the high face of every axis mirrors to n-2, the low
+// face mirrors to 1 except on the flow axis with the inlet at index 0, where
+// the ghost is the node itself.
+//
+// Entry points:
+//  * fc_mg_mop_fused / fc_mg_jacobi_smooth — the two reference kernels
+//    (mg_kernels.cu:55-124), same arguments;
+//  * fc_mg_build_coeffs — the coefficient fields of one level, every float
+//    operation rounded on its own (same values as the eager tensor ops);
+//  * fc_mg_restrict_fields — 2x2x2 means for the hierarchy build;
+//  * fc_mg_sweep — fused weighted-Jacobi sweep x_out = x + D^-1 (b - A x);
+//  * fc_mg_residual_restrict — coarse rhs = mean_8((b - A x) * mask);
+//  * fc_mg_prolong_correct — x = (x + 0.75 * trilinear(x_c)) * mask;
+//  * fc_mg_coarse_solve — all coarsest-level sweeps in one workgroup;
+//  * FP64 vector kernels of the flexible CG outer loop, with wavefront DPP
+//    reductions and device-resident scalars.
+#include "fc_common.hpp"
+
+using namespace fc;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Level {
+  int nx, ny, nz;
+  int ax, out_idx;
+};
+
+struct Coef {
+  const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *invD;
+};
+
+__device__ __forceinline__ int lo_ghost(const Level& g, int this_ax) {
+  return (g.ax != this_ax || g.out_idx == 0) ? 1 : 0;
+}
+
+// Lo = T + dt L(T) at node (i,j,k) with the reference's ghosts and operand order.
+template <typename TT>
+__device__ __forceinline__ float stencil_lo(const TT* __restrict__ T, const Coef& c, const Level& g, float dt,
+                                            int i, int j, int k, int64_t idx) {
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  const int iE = i < g.nx - 1 ? i + 1 : i - 1;
+  const int iW = i > 0 ? i - 1 : lo_ghost(g, 0);
+  const int jN = j < g.ny - 1 ? j + 1 : j - 1;
+  const int jS = j > 0 ? j - 1 : lo_ghost(g, 1);
+  const int kU = k < g.nz - 1 ? k + 1 : k - 1;
+  const int kD = k > 0 ? k - 1 : lo_ghost(g, 2);
+  const float TC = (float)T[idx];
+  const float TE = (float)T[iE * nynz + (int64_t)j * g.nz + k];
+  const float TW = (float)T[iW * nynz + (int64_t)j * g.nz + k];
+  const float TN = (float)T[i * nynz + (int64_t)jN * g.nz + k];
+  const float TS = (float)T[i * nynz + (int64_t)jS * g.nz + k];
+  const float TU = (float)T[i * nynz + (int64_t)j * g.nz + kU];
+  const float TD = (float)T[i * nynz + (int64_t)j * g.nz + kD];
+  const float LT = c.diagL[idx] * TC - c.aE[idx] * TE - c.aW[idx] * TW - c.aN[idx] * TN - c.aS[idx] * TS -
+                   c.aU[idx] * TU - c.aD[idx] * TD;
+  return TC + dt * LT;
+}
+
+__device__ __forceinline__ void unflat(const Level& g, int64_t idx, int& i, int& j, int& k) {
+  const int64_t nynz = (int64_t)g.ny * g.nz;
+  i = (int)(idx / nynz);
+  j = (int)((idx / g.nz) % g.ny);
+  k = (int)(idx % g.nz);
+}
+
+__global__ __launch_bounds__(kBlock) void mop_kernel(const float* __restrict__ T, Coef c, Level g, float dt,
+                                                     float* __restrict__ Lo) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (int64_t)g.nx * g.ny * g.nz) return;
+  int i, j, k;
+  unflat(g, idx, i, j, k);
+  Lo[idx] = stencil_lo(T, c, g, dt, i, j, k, idx);
+}
+
+__global__ __launch_bounds__(kBlock) void jacobi_kernel(float* __restrict__ x, const float* __restrict__ b,
+                                                        const float* __restrict__ Lo,
+                                                        const float* __restrict__ invD, int64_t n) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  x[idx] += invD[idx] * (b[idx] - Lo[idx]);
+}
+
+// x_out = x + invD (b - A x): the two reference kernels in one pass.
+__global__ __launch_bounds__(kBlock) void sweep_kernel(const float* __restrict__ x, float* __restrict__ x_out,
+                                                       const float* __restrict__ b, Coef c, Level g, float dt,
+                                                       int zero_guess) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (int64_t)g.nx * g.ny * g.nz) return;
+  if (zero_guess) {  // x = 0: A x = 0
+    x_out[idx] = 0.f + c.invD[idx] * (b[idx] - 0.f);
+    return;
+  }
+  int i, j, k;
+  unflat(g, idx, i, j, k);
+  const float lo = stencil_lo(x, c, g, dt, i, j, k, idx);
+  x_out[idx] = x[idx] + c.invD[idx] * (b[idx] - lo);
+}
+
+// coarse rhs: mean of the 8 fine residuals (b - A x) * mask (odd extents truncated).
+__global__ __launch_bounds__(kBlock) void residual_restrict_kernel(const float* __restrict__ x,
+                                                                   const float* __restrict__ b,
+                                                                   const float* __restrict__ mask, Coef c, Level g,
+                                                                   float dt, float* __restrict__ bc, int cx, int cy,
+                                                                   int cz) {
+  const int64_t cidx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (cidx >= (int64_t)cx * cy * cz) return;
+  const int I = (int)(cidx / ((int64_t)cy * cz)), J = (int)((cidx / cz) % cy), K = (int)(cidx % cz);
+  float s = 0.f;
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    const int i = 2 * I + (d >> 2), j = 2 * J + ((d >> 1) & 1), k = 2 * K + (d & 1);
+    const int64_t idx = ((int64_t)i * g.ny + j) * g.nz + k;
+    const float lo = stencil_lo(x, c, g, dt, i, j, k, idx);
+    s += (b[idx] - lo) * mask[idx];
+  }
+  bc[cidx] = s / 8.f;
+}
+
+// linear interpolation source (align_corners = False) along one axis
+__device__ __forceinline__ void lin_src(int o, int n_out, int n_in, int& i0, int& i1, float& l0, float& l1) {
+  const float scale = (float)n_in / (float)n_out;
+  float src = scale * ((float)o + 0.5f) - 0.5f;
+  src = src < 0.f ? 0.f : src;
+  i0 = min((int)src, n_in - 1);
+  i1 = min(i0 + 1, n_in - 1);
+  l1 = src - (float)i0;
+  l0 = 1.f - l1;
+}
+
+__global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __restrict__ xc, float* __restrict__ x,
+                                                                 const float* __restrict__ mask, int nx, int ny,
+                                                                 int nz, int cx, int cy, int cz, float alpha) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (int64_t)nx * ny * nz) return;
+  const int i = (int)(idx / ((int64_t)ny * nz)), j = (int)((idx / nz) % ny), k = (int)(idx % nz);
+  int i0, i1, j0, j1, k0, k1;
+  float a0, a1, b0, b1, c0, c1;
+  lin_src(i, nx, cx, i0, i1, a0, a1);
+  lin_src(j, ny, cy, j0, j1, b0, b1);
+  lin_src(k, nz, cz, k0, k1, c0, c1);
+  auto X = [&](int a, int b2, int c2) { return xc[((int64_t)a * cy + b2) * cz + c2]; };
+  const float e = a0 * (b0 * (c0 * X(i0, j0, k0) + c1 * X(i0, j0, k1)) + b1 * (c0 * X(i0, j1, k0) + c1 * X(i0, j1, k1))) +
+                  a1 * (b0 * (c0 * X(i1, j0, k0) + c1 * X(i1, j0, k1)) + b1 * (c0 * X(i1, j1, k0) + c1 * X(i1, j1, k1)));
+  x[idx] = (x[idx] + alpha * e) * mask[idx];
+}
+
+// All sweeps of the coarsest level in one workgroup (global ping-pong, the
+// level is small enough to stay in L2); x holds the result on return.
+__global__ __launch_bounds__(1024) void coarse_solve_kernel(float* __restrict__ x, float* __restrict__ tmp,
+                                                            const float* __restrict__ b, Coef c, Level g, float dt,
+                                                            int sweeps, int zero_guess) {
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  float* src = x;
+  float* dst = tmp;
+  for (int s = 0; s < sweeps; ++s) {
+    for (int64_t idx = threadIdx.x; idx < n; idx += blockDim.x) {
+      if (s == 0 && zero_guess) {
+        dst[idx] = 0.f + c.invD[idx] * (b[idx] - 0.f);
+      } else {
+        int i, j, k;
+        unflat(g, idx, i, j, k);
+        const float lo = stencil_lo(src, c, g, dt, i, j, k, idx);
+        dst[idx] = src[idx] + c.invD[idx] * (b[idx] - lo);
+      }
+    }
+    __syncthreads();
+    float* t = src; src = dst; dst = t;
+  }
+  if (src != x) {
+    for (int64_t idx = threadIdx.x; idx < n; idx += blockDim.x) x[idx] = src[idx];
+  }
+}
+
+// ---- coefficient fields (nlevel_mg_cuda.py:418-507), contraction off -------
+#pragma clang fp contract(off)
+__device__ __forceinline__ float nb_field(const float* A, const Level& g, int i, int j, int k, int dim, int step) {
+  int a[3] = {i, j, k};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  if (step > 0) a[dim] = a[dim] < n[dim] - 1 ? a[dim] + 1 : a[dim] - 1;
+  else a[dim] = a[dim] > 0 ? a[dim] - 1 : lo_ghost(g, dim);
+  return A[((int64_t)a[0] * g.ny + a[1]) * g.nz + a[2]];
+}
+
+__global__ __launch_bounds__(kBlock) void build_coeffs_kernel(const float* __restrict__ kf, const float* __restrict__ rcp,
+                                                              const float* __restrict__ u, const float* __restrict__ v,
+                                                              const float* __restrict__ w, Level g, float cd, float ca,
+                                                              float* aE, float* aW, float* aN, float* aS, float* aU,
+                                                              float* aD, float* diagL) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (int64_t)g.nx * g.ny * g.nz) return;
+  int i, j, k;
+  unflat(g, idx, i, j, k);
+  const float kc = kf[idx];
+  const float inv_rcp = 1.0f / (rcp[idx] + 1e-30f);
+  float out[6];
+  const float vel[3] = {u[idx], v[idx], w[idx]};
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int dim = f >> 1, step = (f & 1) ? -1 : +1;  // E W N S U D
+    const float kn = nb_field(kf, g, i, j, k, dim, step);
+    const float num = (2.0f * kc) * kn;
+    const float den = (kc + kn) + 1e-30f;
+    const float harm = num / den;
+    const float diff = (cd * harm) * inv_rcp;
+    // upwind: W/S/D take max(u,0), E/N/U take max(-u,0)
+    const float up = step > 0 ? fmaxf(-vel[dim], 0.0f) : fmaxf(vel[dim], 0.0f);
+    out[f] = diff + ca * up;
+  }
+  aE[idx] = out[0]; aW[idx] = out[1]; aN[idx] = out[2]; aS[idx] = out[3]; aU[idx] = out[4]; aD[idx] = out[5];
+  diagL[idx] = ((((out[0] + out[1]) + out[2]) + out[3]) + out[4]) + out[5];
+}
+
+// inv_diagM = omega * (1 / max(1 + dt diagL, 1e-12)) * mask (nlevel_mg_cuda.py:262-267)
+__global__ __launch_bounds__(kBlock) void inv_diag_kernel(const float* __restrict__ diagL,
+                                                          const float* __restrict__ mask, float dt, float omega,
+                                                          float* __restrict__ invD, int64_t n) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  float d = 1.0f + dt * diagL[idx];
+  d = d < 1e-12f ? 1e-12f : d;
+  const float r = 1.0f / d;
+  invD[idx] = (r * omega) * mask[idx];
+}
+
+// 2x2x2 means of up to 6 fields (hierarchy build); mask_mode 1 thresholds
+// the mean at 0.999 (nlevel_mg_cuda.py:255).
+__global__ __launch_bounds__(kBlock) void restrict_field_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                                int nx, int ny, int nz, int cx, int cy, int cz,
+                                                                int threshold) {
+  const int64_t cidx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (cidx >= (int64_t)cx * cy * cz) return;
+  const int I = (int)(cidx / ((int64_t)cy * cz)), J = (int)((cidx / cz) % cy), K = (int)(cidx % cz);
+  float s = 0.f;
+  for (int d = 0; d < 8; ++d) {
+    const int i = 2 * I + (d >> 2), j = 2 * J + ((d >> 1) & 1), k = 2 * K + (d & 1);
+    s += src[((int64_t)i * ny + j) * nz + k];
+  }
+  const float m = s / 8.f;
+  dst[cidx] = threshold ? (m >= 0.999f ? 1.f : 0.f) : m;
+}
+#pragma clang fp contract(on)
+
+// ---- FP64 outer loop ---------------------------------------------------------
+// Sum over a 64-wide wavefront with DPP lane moves (quad permutes, half-row and
+// row mirrors), rows combined through readlane.
+__device__ __forceinline__ double dpp_move(double v, int ctrl_sel) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  switch (ctrl_sel) {
+    case 0:  // quad_perm [1,0,3,2]
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+      break;
+    case 1:  // quad_perm [2,3,0,1]
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+      break;
+    case 2:  // row_half_mirror
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xF, 0xF, false);
+      break;
+    default:  // row_mirror
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xF, 0xF, false);
+      break;
+  }
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_move(v, 0);
+  v += dpp_move(v, 1);
+  v += dpp_move(v, 2);
+  v += dpp_move(v, 3);
+  // every lane of each 16-lane row now holds that row's sum
+  const int lo0 = __builtin_amdgcn_readlane(__double2loint(v), 0), hi0 = __builtin_amdgcn_readlane(__double2hiint(v), 0);
+  const int lo1 = __builtin_amdgcn_readlane(__double2loint(v), 16), hi1 = __builtin_amdgcn_readlane(__double2hiint(v), 16);
+  const int lo2 = __builtin_amdgcn_readlane(__double2loint(v), 32), hi2 = __builtin_amdgcn_readlane(__double2hiint(v), 32);
+  const int lo3 = __builtin_amdgcn_readlane(__double2loint(v), 48), hi3 = __builtin_amdgcn_readlane(__double2hiint(v), 48);
+  return (__hiloint2double(hi0, lo0) + __hiloint2double(hi1, lo1)) +
+         (__hiloint2double(hi2, lo2) + __hiloint2double(hi3, lo3));
+}
+
+// block sum of up to 3 values into partial[blockIdx][0..2]
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partial) {
+  __shared__ double sh[kBlock / 64][3];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    const double s = wave_sum(v[a]);
+    if (lane == 0) sh[wid][a] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) s += sh[w][a];
+      partial[(int64_t)blockIdx.x * 3 + a] = s;
+    }
+  }
+}
+
+// Device scalar slots used by the FCG kernels (FP64):
+// S_PAP/S_PR (A p, one fold), S_RR/S_BB (true residual, one fold), S_ZAP, alpha, beta.
+enum { S_PAP = 0, S_PR = 1, S_RR = 2, S_BB = 3, S_ZAP = 4, S_ALPHA = 5, S_BETA = 6, S_N = 8 };
+
+// fold the partials of `nparts` blocks into scal[slot0..slot0+nv-1] (one block)
+__global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__ partial, int nparts, int nv,
+                                                      double* __restrict__ scal, int slot0, int update) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int p = threadIdx.x; p < nparts; p += kBlock)
+    for (int a = 0; a < nv; ++a) v[a] += partial[(int64_t)p * 3 + a];
+  __shared__ double sh[kBlock / 64][3];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int a = 0; a < 3; ++a) {
+    const double s = wave_sum(v[a]);
+    if (lane == 0) sh[wid][a] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < nv; ++a) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) s += sh[w][a];
+      scal[slot0 + a] = s;
+    }
+    if (update == 1) {  // alpha = (p.r) / (p.Ap)
+      scal[S_ALPHA] = scal[S_PR] / scal[S_PAP];
+    } else if (update == 2) {  // beta = (z.Ap_old) / (p_old.Ap_old); S_PAP still holds p_old.Ap_old
+      scal[S_BETA] = scal[S_PAP] != 0.0 ? scal[S_ZAP] / scal[S_PAP] : 0.0;
+    }
+  }
+}
+
+// Ap = (p + dt L p) * mask in FP64 (FP32 coefficients); partials of p.Ap, p.r
+__global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restrict__ p, double* __restrict__ Ap,
+                                                           const double* __restrict__ r, const float* __restrict__ mask,
+                                                           Coef c, Level g, double dt, double* __restrict__ partial) {
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (idx < n) {
+    int i, j, k;
+    unflat(g, idx, i, j, k);
+    const int64_t nynz = (int64_t)g.ny * g.nz;
+    const int iE = i < g.nx - 1 ? i + 1 : i - 1, iW = i > 0 ? i - 1 : lo_ghost(g, 0);
+    const int jN = j < g.ny - 1 ? j + 1 : j - 1, jS = j > 0 ? j - 1 : lo_ghost(g, 1);
+    const int kU = k < g.nz - 1 ? k + 1 : k - 1, kD = k > 0 ? k - 1 : lo_ghost(g, 2);
+    const double pc = p[idx];
+    const double L = (double)c.diagL[idx] * pc - (double)c.aE[idx] * p[iE * nynz + (int64_t)j * g.nz + k] -
+                     (double)c.aW[idx] * p[iW * nynz + (int64_t)j * g.nz + k] -
+                     (double)c.aN[idx] * p[i * nynz + (int64_t)jN * g.nz + k] -
+                     (double)c.aS[idx] * p[i * nynz + (int64_t)jS * g.nz + k] -
+                     (double)c.aU[idx] * p[i * nynz + (int64_t)j * g.nz + kU] -
+                     (double)c.aD[idx] * p[i * nynz + (int64_t)j * g.nz + kD];
+    const double a = (pc + dt * L) * (double)mask[idx];
+    Ap[idx] = a;
+    v[0] = pc * a;
+    v[1] = r ? pc * r[idx] : 0.0;
+  }
+  block_partials<2>(v, partial);
+}
+
+// x += alpha p; r -= alpha Ap; r32 = (float) r; partial of r.r
+__global__ __launch_bounds__(kBlock) void update_xr_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                           float* __restrict__ r32, const double* __restrict__ p,
+                                                           const double* __restrict__ Ap,
+                                                           const double* __restrict__ scal, int64_t n,
+                                                           double* __restrict__ partial) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (idx < n) {
+    const double alpha = scal[S_ALPHA];
+    x[idx] += alpha * p[idx];
+    const double rn = r[idx] - alpha * Ap[idx];
+    r[idx] = rn;
+    r32[idx] = (float)rn;
+    v[0] = rn * rn;
+  }
+  block_partials<1>(v, partial);
+}
+
+// partial of z.Ap_old (z is the FP32 V-cycle output)
+__global__ __launch_bounds__(kBlock) void dot_z_kernel(const float* __restrict__ z, const double* __restrict__ Ap_old,
+                                                       int64_t n, double* __restrict__ partial) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (idx < n) v[0] = (double)z[idx] * Ap_old[idx];
+  block_partials<1>(v, partial);
+}
+
+// p = z - beta p_old (in place on p)
+__global__ __launch_bounds__(kBlock) void new_dir_kernel(double* __restrict__ p, const float* __restrict__ z,
+                                                         const double* __restrict__ scal, int64_t n, int first) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  p[idx] = first ? (double)z[idx] : (double)z[idx] - scal[S_BETA] * p[idx];
+}
+
+// true residual r = (b - A x) * mask in FP64, r32 copy, partials of r.r
+__global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __restrict__ x,
+                                                               const double* __restrict__ b, double* __restrict__ r,
+                                                               float* __restrict__ r32,
+                                                               const float* __restrict__ mask, Coef c, Level g,
+                                                               double dt, double* __restrict__ partial) {
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (idx < n) {
+    int i, j, k;
+    unflat(g, idx, i, j, k);
+    const int64_t nynz = (int64_t)g.ny * g.nz;
+    const int iE = i < g.nx - 1 ? i + 1 : i - 1, iW = i > 0 ? i - 1 : lo_ghost(g, 0);
+    const int jN = j < g.ny - 1 ? j + 1 : j - 1, jS = j > 0 ? j - 1 : lo_ghost(g, 1);
+    const int kU = k < g.nz - 1 ? k + 1 : k - 1, kD = k > 0 ? k - 1 : lo_ghost(g, 2);
+    const double xc = x[idx];
+    const double L = (double)c.diagL[idx] * xc - (double)c.aE[idx] * x[iE * nynz + (int64_t)j * g.nz + k] -
+                     (double)c.aW[idx] * x[iW * nynz + (int64_t)j * g.nz + k] -
+                     (double)c.aN[idx] * x[i * nynz + (int64_t)jN * g.nz + k] -
+                     (double)c.aS[idx] * x[i * nynz + (int64_t)jS * g.nz + k] -
+                     (double)c.aU[idx] * x[i * nynz + (int64_t)j * g.nz + kU] -
+                     (double)c.aD[idx] * x[i * nynz + (int64_t)j * g.nz + kD];
+    const double m = (double)mask[idx];
+    const double bb = b[idx] * m;
+    const double rr = (bb - (xc + dt * L)) * m;
+    r[idx] = rr;
+    r32[idx] = (float)rr;
+    v[0] = rr * rr;
+    v[1] = bb * bb;
+  }
+  block_partials<2>(v, partial);
+}
+
+int make_level(Level& g, int nx, int ny, int nz, int ax, int out_idx) {
+  FC_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "mg: every extent must be >= 2 (got %d,%d,%d)", nx, ny, nz);
+  FC_REQUIRE(ax >= 0 && ax <= 2, "mg: flow axis must be 0, 1 or 2");
+  g.nx = nx; g.ny = ny; g.nz = nz; g.ax = ax; g.out_idx = out_idx;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+struct fc_mg_coef {
+  const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *invD;
+};
+
+static Coef to_coef(const fc_mg_coef* c) {
+  Coef o;
+  o.aE = c->aE; o.aW = c->aW; o.aN = c->aN; o.aS = c->aS; o.aU = c->aU; o.aD = c->aD; o.diagL = c->diagL;
+  o.invD = c->invD;
+  return o;
+}
+
+int fc_mg_mop_fused(const float* T, const float* aE, const float* aW, const float* aN, const float* aS,
+                    const float* aU, const float* aD, const float* diagL, float* Lo, double dt, int nx, int ny, int nz,
+                    int ax, int out_idx, hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(T && aE && aW && aN && aS && aU && aD && diagL && Lo, "mop_fused: null pointer argument");
+  Coef c{aE, aW, aN, aS, aU, aD, diagL, nullptr};
+  const int64_t n = (int64_t)nx * ny * nz;
+  hipLaunchKernelGGL(mop_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, T, c, g, (float)dt, Lo);
+  return check_launch("mop_fused");
+}
+
+int fc_mg_jacobi_smooth(float* x, const float* b, const float* Lo, const float* inv_diagM, int64_t n,
+                        hipStream_t stream) {
+  FC_REQUIRE(n >= 0, "jacobi_smooth: n must be >= 0");
+  if (n == 0) return 0;
+  FC_REQUIRE(x && b && Lo && inv_diagM, "jacobi_smooth: null pointer argument");
+  hipLaunchKernelGGL(jacobi_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x, b, Lo, inv_diagM, n);
+  return check_launch("jacobi_smooth");
+}
+
+int fc_mg_build_coeffs(const float* k, const float* rho_cp, const float* u, const float* v, const float* w, int nx,
+                       int ny, int nz, int ax, int out_idx, double dx, double s, float* aE, float* aW, float* aN,
+                       float* aS, float* aU, float* aD, float* diagL, hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(k && rho_cp && u && v && w && aE && aW && aN && aS && aU && aD && diagL,
+             "build_coeffs: null pointer argument");
+  const int64_t n = (int64_t)nx * ny * nz;
+  hipLaunchKernelGGL(build_coeffs_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, k, rho_cp, u, v, w, g,
+                     (float)(s / (dx * dx)), (float)(s / dx), aE, aW, aN, aS, aU, aD, diagL);
+  return check_launch("build_coeffs");
+}
+
+int fc_mg_inv_diag(const float* diagL, const float* mask, double dt, double omega, float* inv_diagM, int64_t n,
+                   hipStream_t stream) {
+  if (n == 0) return 0;
+  FC_REQUIRE(diagL && mask && inv_diagM, "inv_diag: null pointer argument");
+  hipLaunchKernelGGL(inv_diag_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, diagL, mask, (float)dt,
+                     (float)omega, inv_diagM, n);
+  return check_launch("inv_diag");
+}
+
+int fc_mg_restrict_field(const float* src, float* dst, int nx, int ny, int nz, int threshold, hipStream_t stream) {
+  FC_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "restrict_field: extents must be >= 2");
+  FC_REQUIRE(src && dst, "restrict_field: null pointer argument");
+  const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
+  hipLaunchKernelGGL(restrict_field_kernel, dim3(grid_for((int64_t)cx * cy * cz, kBlock)), dim3(kBlock), 0, stream,
+                     src, dst, nx, ny, nz, cx, cy, cz, threshold);
+  return check_launch("restrict_field");
+}
+
+int fc_mg_sweep(const float* x, float* x_out, const float* b, const fc_mg_coef* coef, double dt, int nx, int ny,
+                int nz, int ax, int out_idx, int zero_guess, hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(coef && coef->invD && x && x_out && b && x != x_out, "sweep: bad arguments (x and x_out must differ)");
+  const int64_t n = (int64_t)nx * ny * nz;
+  hipLaunchKernelGGL(sweep_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x, x_out, b, to_coef(coef),
+                     g, (float)dt, zero_guess);
+  return check_launch("sweep");
+}
+
+int fc_mg_residual_restrict(const float* x, const float* b, const float* mask, const fc_mg_coef* coef, double dt,
+                            int nx, int ny, int nz, int ax, int out_idx, float* b_coarse, hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(coef && x && b && mask && b_coarse, "residual_restrict: null pointer argument");
+  const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
+  hipLaunchKernelGGL(residual_restrict_kernel, dim3(grid_for((int64_t)cx * cy * cz, kBlock)), dim3(kBlock), 0,
+                     stream, x, b, mask, to_coef(coef), g, (float)dt, b_coarse, cx, cy, cz);
+  return check_launch("residual_restrict");
+}
+
+int fc_mg_prolong_correct(const float* x_coarse, float* x, const float* mask, int nx, int ny, int nz, int cx, int cy,
+                          int cz, float alpha, hipStream_t stream) {
+  FC_REQUIRE(nx > 0 && ny > 0 && nz > 0 && cx > 0 && cy > 0 && cz > 0, "prolong_correct: extents must be positive");
+  FC_REQUIRE(x_coarse && x && mask, "prolong_correct: null pointer argument");
+  const int64_t n = (int64_t)nx * ny * nz;
+  hipLaunchKernelGGL(prolong_correct_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x_coarse, x, mask,
+                     nx, ny, nz, cx, cy, cz, alpha);
+  return check_launch("prolong_correct");
+}
+
+int fc_mg_coarse_solve(float* x, float* tmp, const float* b, const fc_mg_coef* coef, double dt, int nx, int ny,
+                       int nz, int ax, int out_idx, int sweeps, int zero_guess, hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(coef && coef->invD && x && tmp && b && x != tmp, "coarse_solve: bad arguments");
+  FC_REQUIRE(sweeps >= 0, "coarse_solve: sweeps must be >= 0");
+  hipLaunchKernelGGL(coarse_solve_kernel, dim3(1), dim3(1024), 0, stream, x, tmp, b, to_coef(coef), g, (float)dt,
+                     sweeps, zero_guess);
+  return check_launch("coarse_solve");
+}
+
+// ---- FP64 flexible-CG kernels -------------------------------------------------
+int fc_mg_partials_count(int64_t n) { return (int)grid_for(n, kBlock); }
+
+int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* mask, const fc_mg_coef* coef,
+                    double dt, int nx, int ny, int nz, int ax, int out_idx, double* partial, double* scal,
+                    hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(p && Ap && mask && coef && partial && scal, "apply_A64: null pointer argument");
+  const int64_t n = (int64_t)nx * ny * nz;
+  const unsigned gsz = grid_for(n, kBlock);
+  hipLaunchKernelGGL(apply_A64_kernel, dim3(gsz), dim3(kBlock), 0, stream, p, Ap, r, mask, to_coef(coef), g, dt,
+                     partial);
+  if (int rc = check_launch("apply_A64")) return rc;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 2, scal, (int)S_PAP,
+                     r ? 1 : 0);
+  return check_launch("apply_A64/fold");
+}
+
+int fc_mg_update_xr(double* x, double* r, float* r32, const double* p, const double* Ap, int64_t n, double* partial,
+                    double* scal, hipStream_t stream) {
+  FC_REQUIRE(x && r && r32 && p && Ap && partial && scal, "update_xr: null pointer argument");
+  const unsigned gsz = grid_for(n, kBlock);
+  hipLaunchKernelGGL(update_xr_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, r, r32, p, Ap, scal, n, partial);
+  if (int rc = check_launch("update_xr")) return rc;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, (int)S_RR, 0);
+  return check_launch("update_xr/fold");
+}
+
+int fc_mg_new_direction(double* p, const float* z, const double* Ap_old, int64_t n, int first, double* partial,
+                        double* scal, hipStream_t stream) {
+  FC_REQUIRE(p && z && partial && scal, "new_direction: null pointer argument");
+  const unsigned gsz = grid_for(n, kBlock);
+  if (!first) {
+    FC_REQUIRE(Ap_old, "new_direction: Ap_old required after the first iteration");
+    hipLaunchKernelGGL(dot_z_kernel, dim3(gsz), dim3(kBlock), 0, stream, z, Ap_old, n, partial);
+    if (int rc = check_launch("new_direction/dot")) return rc;
+    hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, (int)S_ZAP, 2);
+    if (int rc = check_launch("new_direction/fold")) return rc;
+  }
+  hipLaunchKernelGGL(new_dir_kernel, dim3(gsz), dim3(kBlock), 0, stream, p, z, scal, n, first);
+  return check_launch("new_direction");
+}
+
+int fc_mg_true_residual(const double* x, const double* b, double* r, float* r32, const float* mask,
+                        const fc_mg_coef* coef, double dt, int nx, int ny, int nz, int ax, int out_idx,
+                        double* partial, double* scal, hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(x && b && r && r32 && mask && coef && partial && scal, "true_residual: null pointer argument");
+  const int64_t n = (int64_t)nx * ny * nz;
+  const unsigned gsz = grid_for(n, kBlock);
+  hipLaunchKernelGGL(true_residual_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, b, r, r32, mask, to_coef(coef), g,
+                     dt, partial);
+  if (int rc = check_launch("true_residual")) return rc;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 2, scal, (int)S_RR, 0);
+  return check_launch("true_residual/fold");
+}
+
+}  // extern "C"
