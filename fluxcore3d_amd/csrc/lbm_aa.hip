@@ -263,6 +263,89 @@ __device__ __forceinline__ unsigned logical_block(unsigned bid, unsigned nblk) {
   return xcd * base + min(xcd, rem) + local;
 }
 
+// MRT in moment space: moments m = M f, equilibrium moments in closed form
+// for the second-order equilibrium (m_eq of compute_feq's polynomial), relax
+// the 15 non-conserved rows, add the Guo source moments (1 - s/2) M F_i, and
+// return f* = M^T diag(1/|M_k|^2) m*. Only 19 moments are live between the
+// two transforms.
+__device__ __forceinline__ void mrt_moment_collide(const Collision& c, const float S[19], float f[19], float rho,
+                                                   float ux, float uy, float uz, bool force) {
+  float m[19];
+#pragma unroll
+  for (int k = 0; k < 19; ++k) {
+    float a = 0.f;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      const int cf = MROW(k, q);
+      if (cf == 1) a += f[q];
+      else if (cf == -1) a -= f[q];
+      else if (cf != 0) a += (float)cf * f[q];
+    }
+    m[k] = a;
+  }
+  const float u2 = ux * ux + uy * uy + uz * uz;
+  const float pxx = 3.f * ux * ux - u2, pww = uy * uy - uz * uz;
+  float meq[19];
+  meq[0] = rho; meq[1] = rho * (-11.f + 19.f * u2); meq[2] = rho * (3.f - 5.5f * u2);
+  meq[3] = rho * ux; meq[4] = (-2.f / 3.f) * rho * ux; meq[5] = rho * uy; meq[6] = (-2.f / 3.f) * rho * uy;
+  meq[7] = rho * uz; meq[8] = (-2.f / 3.f) * rho * uz;
+  meq[9] = rho * pxx; meq[10] = -0.5f * rho * pxx; meq[11] = rho * pww; meq[12] = -0.5f * rho * pww;
+  meq[13] = rho * ux * uy; meq[14] = rho * uy * uz; meq[15] = rho * ux * uz;
+  meq[16] = 0.f; meq[17] = 0.f; meq[18] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 19; ++k)
+    if (!(k == 0 || k == 3 || k == 5 || k == 7)) m[k] = m[k] - S[k] * (m[k] - meq[k]);
+  if (force) {
+    const float Fx = c.F[0], Fy = c.F[1], Fz = c.F[2];
+    const float uf = ux * Fx + uy * Fy + uz * Fz;
+    const float fxx = 2.f * (2.f * ux * Fx - uy * Fy - uz * Fz), fww = 2.f * (uy * Fy - uz * Fz);
+    float Fm[19];
+    Fm[0] = 0.f; Fm[1] = 38.f * uf; Fm[2] = -11.f * uf;
+    Fm[3] = Fx; Fm[4] = (-2.f / 3.f) * Fx; Fm[5] = Fy; Fm[6] = (-2.f / 3.f) * Fy; Fm[7] = Fz; Fm[8] = (-2.f / 3.f) * Fz;
+    Fm[9] = fxx; Fm[10] = -0.5f * fxx; Fm[11] = fww; Fm[12] = -0.5f * fww;
+    Fm[13] = ux * Fy + uy * Fx; Fm[14] = uy * Fz + uz * Fy; Fm[15] = ux * Fz + uz * Fx;
+    Fm[16] = 0.f; Fm[17] = 0.f; Fm[18] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 19; ++k) m[k] += (1.f - 0.5f * S[k]) * Fm[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 19; ++k) m[k] *= 1.0f / (float)MNORM2(k);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    float a = 0.f;
+#pragma unroll
+    for (int k = 0; k < 19; ++k) {
+      const int cf = MROW(k, q);
+      if (cf == 1) a += m[k];
+      else if (cf == -1) a -= m[k];
+      else if (cf != 0) a += (float)cf * m[k];
+    }
+    f[q] = a;
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ void collide_fast(const Collision& c, float f[19], float& rho, float& ux, float& uy,
+                                             float& uz, float sigma) {
+  if constexpr (KIND == 2) {
+    float jx, jy, jz;
+    moments0(f, rho, jx, jy, jz);
+    const bool force = c.use_force != 0;
+    if (force) { jx += 0.5f * c.F[0]; jy += 0.5f * c.F[1]; jz += 0.5f * c.F[2]; }
+    const float inv = 1.f / (rho + 1e-12f);
+    ux = jx * inv; uy = jy * inv; uz = jz * inv;
+    float feq_s[19];
+    if (sigma > 0.f) feq19(rho, ux, uy, uz, feq_s);
+    mrt_moment_collide(c, c.S, f, rho, ux, uy, uz, force);
+    if (sigma > 0.f) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) f[q] = f[q] - sigma * (f[q] - feq_s[q]);
+    }
+  } else {
+    collide<KIND>(c, f, rho, ux, uy, uz, sigma);
+  }
+}
+
 template <typename ST, int KIND>
 __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
                                                          int parity, int first, float* __restrict__ rho_o,
@@ -273,7 +356,6 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
   const bool live = x < g.N;
   const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
   const bool special = (fl & F_SPECIAL) != 0;
-  // rank among the special nodes of this 64-node group
   const uint64_t bal = __ballot(special);
   const int lane = threadIdx.x & 63;
   int4 m = make_int4(0, 0, 0, 0), b = make_int4(0, 0, 0, 0);
@@ -287,33 +369,78 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     if (rho_o) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
     return;
   }
-  const int64_t nynz = (int64_t)g.ny * g.nz;
-  const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
+  const uint32_t xu = (uint32_t)x;
+  const uint32_t nynz = (uint32_t)g.ny * (uint32_t)g.nz;
+  const int i = (int)(xu / nynz);
+  const uint32_t rem = xu - (uint32_t)i * nynz;
+  const int j = (int)(rem / (uint32_t)g.nz);
+  const int k = (int)(rem - (uint32_t)j * (uint32_t)g.nz);
   const bool plain = (fl >> F_BC_SHIFT) != 0;
+  const uint32_t PZ = (uint32_t)g.PZ, PYZ = (uint32_t)g.PY * (uint32_t)g.PZ;
 
+  // padded coordinates of x - 1, x, x + 1 along each axis
+  const int Im = wrap_or_pad(i, -1, g.nx, g.gx), I0 = i + g.gx, Ip = wrap_or_pad(i, 1, g.nx, g.gx);
+  const int Jm = wrap_or_pad(j, -1, g.ny, g.gy), J0 = j + g.gy, Jp = wrap_or_pad(j, 1, g.ny, g.gy);
+  const int Km = wrap_or_pad(k, -1, g.nz, g.gz), K0 = k + g.gz, Kp = wrap_or_pad(k, 1, g.nz, g.gz);
+  auto rowb = [&](int dx, int dy) -> uint32_t {
+    const int I = dx < 0 ? Im : (dx > 0 ? Ip : I0);
+    const int J = dy < 0 ? Jm : (dy > 0 ? Jp : J0);
+    return (uint32_t)I * PYZ + (uint32_t)J * PZ;
+  };
+  auto slot = [&](int dx, int dy, int dz) -> uint32_t {
+    return rowb(dx, dy) + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
+  };
+  const uint32_t self = (uint32_t)I0 * PYZ + (uint32_t)J0 * PZ + (uint32_t)K0;
+
+  // all 19 plain loads first (branch-free), then special overrides
   float f[19];
-  compose_inputs(pop, g, t, parity, first, i, j, k, special, plain, m, b, f);
+  if (parity == 0) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = Store<ST>::ld(pop + (size_t)q * (size_t)g.Ns + self, q);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 19; ++q)
+      f[q] = Store<ST>::ld(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), OPP(q));
+  }
+  if (special && !plain) {
+    const uint32_t L = (uint32_t)m.x, Sm = (uint32_t)m.y, V = (uint32_t)m.z;
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+      if ((L >> q) & 1u) {
+        const int64_t l = b.x + __popc(L & ((1u << q) - 1u));
+        float v = t.A[l];
+        if (!first && ((V >> q) & 1u))
+          v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + b.y + __popc(V & ((1u << q) - 1u))];
+        f[q] = v;
+      } else if ((Sm >> q) & 1u) {
+        f[q] = WQ(q);
+      }
+    }
+  }
 
   float sigma = 0.f;
   if (c.sponge) sigma = c.sponge[c.sponge_axis == 0 ? i : (c.sponge_axis == 1 ? j : k)];
   float rho, ux, uy, uz;
-  collide<KIND>(c, f, rho, ux, uy, uz, sigma);
+  collide_fast<KIND>(c, f, rho, ux, uy, uz, sigma);
   if (rho_o) { rho_o[x] = rho; u_o[x] = ux; v_o[x] = uy; w_o[x] = uz; }
 
   if (parity == 0) {
-    const int64_t self = slot_node(g, i, j, k, 0, 0, 0);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      Store<ST>::st(pop + (int64_t)OPP(q) * g.Ns + self, f[q], OPP(q));
-      if (oob_source(g, i, j, k, q))
-        Store<ST>::st(pop + (int64_t)OPP(q) * g.Ns + slot_node(g, i, j, k, -CX(q), -CY(q), -CZ(q)), f[q], OPP(q));
-    }
+    for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + self, f[q], OPP(q));
   } else {
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      Store<ST>::st(pop + (int64_t)q * g.Ns + slot_node(g, i, j, k, CX(q), CY(q), CZ(q)), f[q], q);
-      if (oob_source(g, i, j, k, q))
-        Store<ST>::st(pop + (int64_t)q * g.Ns + slot_node(g, i, j, k, 0, 0, 0), f[q], q);
+    for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + slot(CX(q), CY(q), CZ(q)), f[q], q);
+  }
+  // copy-self on non-periodic faces through the ghost layer
+  if ((g.gx && (i == 0 || i == g.nx - 1)) || (g.gy && (j == 0 || j == g.ny - 1)) ||
+      (g.gz && (k == 0 || k == g.nz - 1))) {
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+      if (!oob_source(g, i, j, k, q)) continue;
+      if (parity == 0)
+        Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), f[q], OPP(q));
+      else
+        Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + self, f[q], q);
     }
   }
 
@@ -321,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     const uint32_t L = (uint32_t)m.x, V = (uint32_t)m.z, F = (uint32_t)m.w;
     int64_t l = b.x;
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
+    for (int q = 1; q < 19; ++q) {
       if ((L >> q) & 1u) {
         float a = t.w1[l] * f[OPP(q)];
         if (!((V >> q) & 1u)) a = a + t.w2[l] * f[q];
@@ -331,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     }
     int64_t e = b.z;
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
+    for (int q = 1; q < 19; ++q) {
       if ((F >> q) & 1u) {
         t.B[(int64_t)parity * t.n_v + t.ff_idx[e]] = f[q];
         ++e;
@@ -638,6 +765,7 @@ int fc_aa_step(void* pop, int storage, int nx, int ny, int nz, int px, int py, i
   Grid g; Tables t;
   if (int rc = pack(g, t, tb, nx, ny, nz, px, py, pz)) return rc;
   FC_REQUIRE(pop, "aa_step: null population buffer");
+  FC_REQUIRE(g.Ns < ((int64_t)1 << 31), "aa_step: padded grid of %lld nodes exceeds 2^31", (long long)g.Ns);
   FC_REQUIRE(col && col->kind >= 0 && col->kind <= 3, "aa_step: collision kind must be 0..3");
   FC_REQUIRE(parity == 0 || parity == 1, "aa_step: parity must be 0 or 1");
   FC_REQUIRE((rho && u && v && w) || (!rho && !u && !v && !w), "aa_step: macro outputs must be all set or all null");

@@ -437,6 +437,32 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
   block_partials<2>(v, partial);
 }
 
+// partial of sum(x^2) for an FP32 vector, FP64 accumulation
+__global__ __launch_bounds__(kBlock) void norm2_f32_kernel(const float* __restrict__ x, int64_t n,
+                                                           double* __restrict__ partial) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (idx < n) {
+    const double a = (double)x[idx];
+    v[0] = a * a;
+  }
+  block_partials<1>(v, partial);
+}
+
+// r = (b - Lo) * mask, partial of r.r (the residual of one V-cycle iteration)
+__global__ __launch_bounds__(kBlock) void residual_f32_kernel(const float* __restrict__ b, const float* __restrict__ Lo,
+                                                              const float* __restrict__ mask, float* __restrict__ r,
+                                                              int64_t n, double* __restrict__ partial) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (idx < n) {
+    const float a = mask ? (b[idx] - Lo[idx]) * mask[idx] : b[idx] - Lo[idx];
+    r[idx] = a;
+    v[0] = (double)a * (double)a;
+  }
+  block_partials<1>(v, partial);
+}
+
 int make_level(Level& g, int nx, int ny, int nz, int ax, int out_idx) {
   FC_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "mg: every extent must be >= 2 (got %d,%d,%d)", nx, ny, nz);
   FC_REQUIRE(ax >= 0 && ax <= 2, "mg: flow axis must be 0, 1 or 2");
@@ -612,5 +638,28 @@ int fc_mg_true_residual(const double* x, const double* b, double* r, float* r32,
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 2, scal, (int)S_RR, 0);
   return check_launch("true_residual/fold");
 }
+
+int fc_mg_norm2_f32(const float* x, int64_t n, double* partial, double* scal, int slot, hipStream_t stream) {
+  FC_REQUIRE(x && partial && scal && slot >= 0 && slot < S_N, "norm2_f32: bad arguments");
+  if (n == 0) return 0;
+  const unsigned gsz = grid_for(n, kBlock);
+  hipLaunchKernelGGL(norm2_f32_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, n, partial);
+  if (int rc = check_launch("norm2_f32")) return rc;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, slot, 0);
+  return check_launch("norm2_f32/fold");
+}
+
+int fc_mg_residual_f32(const float* b, const float* Lo, const float* mask, float* r, int64_t n, double* partial,
+                       double* scal, int slot, hipStream_t stream) {
+  FC_REQUIRE(b && Lo && r && partial && scal && slot >= 0 && slot < S_N, "residual_f32: bad arguments");
+  if (n == 0) return 0;
+  const unsigned gsz = grid_for(n, kBlock);
+  hipLaunchKernelGGL(residual_f32_kernel, dim3(gsz), dim3(kBlock), 0, stream, b, Lo, mask, r, n, partial);
+  if (int rc = check_launch("residual_f32")) return rc;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, slot, 0);
+  return check_launch("residual_f32/fold");
+}
+
+int fc_mg_scalar_slots(void) { return S_N; }
 
 }  // extern "C"

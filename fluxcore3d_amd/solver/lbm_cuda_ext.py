@@ -360,7 +360,7 @@ class LBMCUDAUpgrade:
         if has_sponge:
             sim._apply_sponge_zone = types.MethodType(LBMCUDAUpgrade._sponge_cuda, sim)
         sim._apply_flow_bcs = types.MethodType(_apply_flow_bcs_cuda, sim)
-        sim._original_step_thermal = sim.step
+        sim._original_step_thermal = getattr(sim, "step", None)
         sim.step = types.MethodType(LBMCUDAUpgrade._step_cuda, sim)
 
     @staticmethod

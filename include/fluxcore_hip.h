@@ -126,6 +126,74 @@ int fc_links_q(const float *phi, int nx, int ny, int nz, float ox, float oy, flo
                const int64_t *fluid_flat, const int64_t *dir_i, int64_t n, int n_coarse, int n_bisect, float *q,
                float *w1, float *w2, uint8_t *q_ge_half, hipStream_t stream);
 
+
+/* ---- geometric multigrid (mg_kernels.cu / nlevel_mg_cuda.py) ---- */
+/* mop_fused (mg_kernels.cu:133-174): Lo = T + dt L(T) with the reference ghosts */
+int fc_mg_mop_fused(const float *T, const float *aE, const float *aW, const float *aN, const float *aS,
+                    const float *aU, const float *aD, const float *diagL, float *Lo, double dt, int nx, int ny, int nz,
+                    int ax, int out_idx, hipStream_t stream);
+/* jacobi_smooth (mg_kernels.cu:177-197): x += inv_diagM (b - Lo) */
+int fc_mg_jacobi_smooth(float *x, const float *b, const float *Lo, const float *inv_diagM, int64_t n,
+                        hipStream_t stream);
+/* _build_coeffs_full (nlevel_mg_cuda.py:418-507): aE..aD, diagL of one level */
+int fc_mg_build_coeffs(const float *k, const float *rho_cp, const float *u, const float *v, const float *w, int nx,
+                       int ny, int nz, int ax, int out_idx, double dx, double s, float *aE, float *aW, float *aN,
+                       float *aS, float *aU, float *aD, float *diagL, hipStream_t stream);
+/* _update_inv_diag (nlevel_mg_cuda.py:262-267) */
+int fc_mg_inv_diag(const float *diagL, const float *mask, double dt, double omega, float *inv_diagM, int64_t n,
+                   hipStream_t stream);
+/* _restrict_2x (nlevel_mg_cuda.py:512-519); threshold=1 gives the coarse mask (>= 0.999) */
+int fc_mg_restrict_field(const float *src, float *dst, int nx, int ny, int nz, int threshold, hipStream_t stream);
+
+typedef struct fc_mg_coef {
+  const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *invD;
+} fc_mg_coef;
+
+/* fused _Mop + _smooth_step (nlevel_mg_cuda.py:311-372): x_out = x + invD (b - A x) */
+int fc_mg_sweep(const float *x, float *x_out, const float *b, const fc_mg_coef *coef, double dt, int nx, int ny,
+                int nz, int ax, int out_idx, int zero_guess, hipStream_t stream);
+/* residual + _restrict_into of _vcycle (nlevel_mg_cuda.py:289-295) */
+int fc_mg_residual_restrict(const float *x, const float *b, const float *mask, const fc_mg_coef *coef, double dt,
+                            int nx, int ny, int nz, int ax, int out_idx, float *b_coarse, hipStream_t stream);
+/* _prolong_2x + correction of _vcycle (nlevel_mg_cuda.py:300-303,531-537) */
+int fc_mg_prolong_correct(const float *x_coarse, float *x, const float *mask, int nx, int ny, int nz, int cx, int cy,
+                          int cz, float alpha, hipStream_t stream);
+/* coarsest-level sweeps of _vcycle (nlevel_mg_cuda.py:279-283) in one workgroup */
+int fc_mg_coarse_solve(float *x, float *tmp, const float *b, const fc_mg_coef *coef, double dt, int nx, int ny,
+                       int nz, int ax, int out_idx, int sweeps, int zero_guess, hipStream_t stream);
+/* residual norm of solve() (nlevel_mg_cuda.py:133-135), FP64 accumulation */
+int fc_mg_norm2_f32(const float *x, int64_t n, double *partial, double *scal, int slot, hipStream_t stream);
+int fc_mg_residual_f32(const float *b, const float *Lo, const float *mask, float *r, int64_t n, double *partial,
+                       double *scal, int slot, hipStream_t stream);
+
+/* FP64 flexible-CG outer loop replacing pcg_mg (mg_pcg.py:33-97); partial holds
+ * 3 * fc_mg_partials_count(n) doubles, scal fc_mg_scalar_slots() doubles */
+int fc_mg_partials_count(int64_t n);
+int fc_mg_scalar_slots(void);
+int fc_mg_apply_A64(const double *p, double *Ap, const double *r, const float *mask, const fc_mg_coef *coef,
+                    double dt, int nx, int ny, int nz, int ax, int out_idx, double *partial, double *scal,
+                    hipStream_t stream);
+int fc_mg_update_xr(double *x, double *r, float *r32, const double *p, const double *Ap, int64_t n, double *partial,
+                    double *scal, hipStream_t stream);
+int fc_mg_new_direction(double *p, const float *z, const double *Ap_old, int64_t n, int first, double *partial,
+                        double *scal, hipStream_t stream);
+int fc_mg_true_residual(const double *x, const double *b, double *r, float *r32, const float *mask,
+                        const fc_mg_coef *coef, double dt, int nx, int ny, int nz, int ax, int out_idx,
+                        double *partial, double *scal, hipStream_t stream);
+
+/* ---- thermal step kernels (thermal_kernels.cu:198-263) ---- */
+int fc_thermal_rhs(const float *T_old, const float *T0, const float *src, const float *LT0, const float *mask,
+                   float *rhs, float *theta0, double dt, int64_t n, hipStream_t stream);
+int fc_thermal_reconstruct(const float *T0, const float *theta, float *T_new, int nx, int ny, int nz, int ax,
+                           int out_idx, hipStream_t stream);
+size_t fc_thermal_metrics_workspace_bytes(int64_t n);
+int fc_thermal_convergence_metrics(const float *T_new, const float *T_old, const uint8_t *solid, int64_t n,
+                                   float *out_solid, float *out_fluid, void *workspace, size_t workspace_bytes,
+                                   hipStream_t stream);
+size_t fc_thermal_check_workspace_bytes(int64_t n);
+int fc_thermal_physics_check(const float *T, double T_min, double T_max, int64_t n, int *flag, void *workspace,
+                             size_t workspace_bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,105 @@
+"""Analytic pins of the build extensions without a reference counterpart
+(parity unpinned against the reference, pinned by physics):
+
+* Guo body force + textbook Bouzidi + fully periodic AA stepping: plane
+  Poiseuille flow between walls at arbitrary positions (q = 0.3 and 0.7),
+  u(y) = F / (2 nu) (y - y1)(y2 - y).
+* FP16 / BF16 population storage (deviations from w_i): flow fields against
+  the FP64 oracle of the same reference step.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from fluxcore3d_amd.solver.d3q19 import mrt_rates  # noqa: E402
+from oracle import lbm as olbm  # noqa: E402
+from tests import cases  # noqa: E402
+
+DEV = "cuda"
+
+
+def _plates(ny=24, y1=0.7, y2=None):
+    nx, nz = 4, 4
+    y2 = ny - 1 - 0.3 if y2 is None else y2
+    j = np.arange(ny, dtype=np.float64)
+    d = np.minimum(j - y1, y2 - j)
+    phi = np.broadcast_to(d[None, :, None], (nx, ny, nz)).astype(np.float32).copy()
+    return phi, phi < 0, (nx, ny, nz), y1, y2
+
+
+@pytest.mark.parametrize("kind", ["bgk", "trt", "mrt"])
+def test_poiseuille_textbook_bouzidi_guo_force(kind):
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    phi, solid, (nx, ny, nz), y1, y2 = _plates()
+    links = olbm.ibb_from_sdf(solid, phi)  # walls inside the domain: no wrap needed
+    tau = 0.8
+    om = 1.0 / tau
+    nu = (tau - 0.5) / 3.0
+    Fx = 1e-6
+    om_minus = 1.0 / (0.5 + 0.1875 / (tau - 0.5))
+    eng = AAEngine(nx, ny, nz, solid, periodic=(1, 1, 1), collision=kind, omega=om, omega_minus=om_minus,
+                   S=mrt_rates(om) if kind == "mrt" else None, force=(Fx, 0, 0), links=links, ibb_mode="textbook",
+                   device=DEV)
+    f0 = np.broadcast_to(olbm.W32[:, None], (19, nx * ny * nz)).copy()
+    eng.load(torch.as_tensor(f0, device=DEV))
+    N = nx * ny * nz
+    macro = [torch.empty(N, device=DEV) for _ in range(4)]
+    for _ in range(12000):
+        eng.step(*macro)
+    u = macro[1].cpu().numpy().reshape(nx, ny, nz)[1, :, 1]
+    y = np.arange(ny, dtype=np.float64)
+    fluid = ~solid[1, :, 1]
+    exact = Fx / (2 * nu) * (y - y1) * (y2 - y)
+    err = np.linalg.norm(u[fluid] - exact[fluid]) / np.linalg.norm(exact[fluid])
+    assert err < 2e-2, err
+    assert np.abs(macro[2].cpu().numpy()).max() < 1e-3 * exact.max()
+
+
+def test_reference_and_textbook_bouzidi_differ_as_documented():
+    """At q = 1/2 textbook BFL is halfway bounce-back (no-slip); the reference's
+    direction choice (lbm_kernels.cu:567-587) is not, so its wall slips."""
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    phi, solid, (nx, ny, nz), _, _ = _plates(y1=0.5, y2=22.5)
+    links = olbm.ibb_from_sdf(solid, phi)
+    out = {}
+    for mode in ("reference", "textbook"):
+        eng = AAEngine(nx, ny, nz, solid, periodic=(1, 1, 1), collision="bgk", omega=1.0, force=(1e-6, 0, 0),
+                       links=links, ibb_mode=mode, device=DEV)
+        eng.load(torch.as_tensor(np.broadcast_to(olbm.W32[:, None], (19, nx * ny * nz)).copy(), device=DEV))
+        macro = [torch.empty(nx * ny * nz, device=DEV) for _ in range(4)]
+        for _ in range(3000):
+            eng.step(*macro)
+        out[mode] = macro[1].cpu().numpy().reshape(nx, ny, nz)[1, :, 1]
+    near_wall = out["textbook"][1] / out["textbook"][ny // 2]
+    assert near_wall < 0.15  # first fluid node of a no-slip channel
+    assert np.abs(out["reference"] - out["textbook"]).max() > 0.05 * out["textbook"].max()
+
+
+@pytest.mark.parametrize("storage,tol", [("fp32", 1e-5), ("fp16", 1e-3), ("bf16", 1e-2)])
+def test_storage_precision_vs_fp64_oracle(storage, tol):
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    c = cases.periodic_pack(n=40, seed=5)
+    n = c["nx"]
+    om = 1.0 / 0.55
+    S = mrt_rates(om, s_e=1.2, s_bulk=1.2, s_ghost=1.2)
+    rng = np.random.default_rng(0)
+    N = n ** 3
+    rho = 1 + 0.005 * rng.random(N)
+    u = 0.02 * rng.standard_normal((3, N))
+    f0 = olbm.feq(rho, *u, np.float64)
+    eng = AAEngine(n, n, n, c["solid"], periodic=(1, 1, 1), collision="mrt", omega=om, S=S, links=c["links"],
+                   storage=storage, device=DEV)
+    eng.load(torch.as_tensor(f0.astype(np.float32), device=DEV))
+    macro = [torch.empty(N, device=DEV) for _ in range(4)]
+    steps = 100
+    for _ in range(steps):
+        eng.step(*macro)
+    f = f0.copy()
+    for _ in range(steps):
+        f, rho_w, u_w, v_w, w_w = olbm.step(f, c["solid"], n, n, n, (1, 1, 1), "mrt", om, 1.0, S.astype(np.float64),
+                                            0.0, c["links"])
+    umax = np.abs(np.stack([u_w, v_w, w_w])).max()
+    err = max(np.abs(macro[a + 1].cpu().numpy() - (u_w, v_w, w_w)[a]).max() for a in range(3)) / umax
+    assert err < tol, (storage, err)

@@ -1,0 +1,116 @@
+"""Pins of the multigrid oracle (oracle/mg.py) — CPU only."""
+import numpy as np
+import pytest
+
+from oracle import mg
+from tests import thermal_cases
+
+
+def _rand_fields(shape, seed=0, dtype=np.float64, flow=True):
+    rng = np.random.default_rng(seed)
+    k = (0.1 + rng.random(shape)).astype(dtype)
+    rcp = (0.5 + rng.random(shape)).astype(dtype)
+    u, v, w = ((0.05 * rng.standard_normal(shape)) if flow else np.zeros(shape) for _ in range(3))
+    return k, rcp, u.astype(dtype), v.astype(dtype), w.astype(dtype)
+
+
+def _brute_operator(k, rcp, u, v, w, ax, out_idx, dx, s, dt):
+    """Dense A = I + dt L assembled face by face from the definition
+    (harmonic face conductivity, first-order upwind, ghost rules)."""
+    shape = k.shape
+    n = k.size
+    A = np.eye(n)
+    idx = np.arange(n).reshape(shape)
+    for c in np.ndindex(shape):
+        row = idx[c]
+        for dim in range(3):
+            for step in (+1, -1):
+                nb = list(c)
+                if step > 0:
+                    nb[dim] = c[dim] + 1 if c[dim] < shape[dim] - 1 else c[dim] - 1
+                else:
+                    if c[dim] > 0:
+                        nb[dim] = c[dim] - 1
+                    else:
+                        nb[dim] = 0 if (ax == dim and out_idx != 0) else 1
+                nb = tuple(nb)
+                kh = 2 * k[c] * k[nb] / (k[c] + k[nb] + 1e-30)
+                vel = (u, v, w)[dim][c]
+                up = max(-vel, 0.0) if step > 0 else max(vel, 0.0)
+                a = s / dx**2 * kh / (rcp[c] + 1e-30) + s / dx * up
+                A[row, row] += dt * a
+                A[row, idx[nb]] -= dt * a
+    return A
+
+
+@pytest.mark.parametrize("ax,sign", [(0, 1), (1, -1), (2, 1)])
+def test_operator_against_brute_force(ax, sign):
+    shape = (4, 3, 5)
+    k, rcp, u, v, w = _rand_fields(shape, seed=ax)
+    out_idx = shape[ax] - 1 if sign == 1 else 0
+    co = mg.build_coeffs_full(k, rcp, u, v, w, ax, out_idx, 2.0, 1.5)
+    A = mg.dense_operator(co, 0.3, ax, out_idx)
+    B = _brute_operator(k, rcp, u, v, w, ax, out_idx, 2.0, 1.5, 0.3)
+    assert np.abs(A - B).max() < 1e-12
+
+
+def test_prolong_matches_torch_interpolate():
+    torch = pytest.importorskip("torch")
+    import torch.nn.functional as F
+    rng = np.random.default_rng(2)
+    for cs, fs in (((4, 3, 5), (8, 6, 10)), ((4, 3, 5), (9, 7, 11)), ((8, 4, 4), (16, 8, 8))):
+        xc = rng.standard_normal(cs).astype(np.float32)
+        got = mg.prolong_2x(xc, *fs)
+        want = F.interpolate(torch.from_numpy(xc)[None, None], size=fs, mode="trilinear",
+                             align_corners=False)[0, 0].numpy()
+        assert np.abs(got - want).max() < 2e-6
+
+
+def test_restrict_is_block_mean():
+    x = np.random.default_rng(3).standard_normal((5, 4, 7))
+    r = mg.restrict_2x(x)
+    assert r.shape == (2, 2, 3)
+    for c in np.ndindex(r.shape):
+        blk = x[2 * c[0]:2 * c[0] + 2, 2 * c[1]:2 * c[1] + 2, 2 * c[2]:2 * c[2] + 2]
+        assert abs(r[c] - blk.mean()) < 1e-14
+
+
+def _spsolve_masked(h, b):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    lv = h.levels[0]
+    A = mg.dense_operator(lv["co"], h.dt, h.ax, lv["out_idx"])
+    m = lv["mask"].ravel().astype(bool)
+    x = np.zeros(b.size)
+    x[m] = spla.spsolve(sp.csr_matrix(A[np.ix_(m, m)]), (b.ravel() * m)[m])
+    return x.reshape(b.shape)
+
+
+def test_vcycle_solve_and_fcg_reach_direct_solution():
+    c = thermal_cases.sphere_conduction(16, 8, 8, dt=50.0)
+    h = mg.Hierarchy(c["k"].astype(np.float64), c["rcp"].astype(np.float64), c["u"], c["v"], c["w"], c["mask"], 0, 1,
+                     1.0, omega=0.6)
+    h.set_dt(c["dt"])
+    b = c["b"].astype(np.float64)
+    x_direct = _spsolve_masked(h, b)
+    # solve() measures ||b - Lo|| without the mask (nlevel_mg_cuda.py:133-135),
+    # so Dirichlet rows leave a floor; the masked residual and the solution converge
+    x, hist = h.solve(b, max_cycles=300, tol=1e-10)
+    rm = (b - h.mop(x, 0)) * h.levels[0]["mask"]
+    assert np.linalg.norm(rm) / np.linalg.norm(b) < 1e-9
+    assert np.abs(x - x_direct).max() / np.abs(x_direct).max() < 1e-8
+    xf, info = mg.fcg_mg_fp64(h, b, tol=1e-10, max_it=100)
+    assert info["converged"] and info["it"] < len(hist)
+    assert np.abs(xf - x_direct).max() / np.abs(x_direct).max() < 1e-8
+    xp, pinfo = mg.pcg_mg(h, b, tol=1e-8, max_it=100)
+    assert pinfo["res_rel"] < 1e-6
+
+
+def test_hierarchy_levels_and_masks():
+    c = thermal_cases.sphere_conduction(64, 32, 32)
+    h = mg.Hierarchy(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], 0, 1, 1.0)
+    assert [lv["shape"] for lv in h.levels] == [(64, 32, 32), (32, 16, 16), (16, 8, 8)]
+    # a coarse cell is an unknown only when all 8 children are
+    m1 = h.levels[1]["mask"]
+    m0 = c["mask"].astype(np.float32)
+    assert np.array_equal(m1 > 0, mg.restrict_2x(m0) >= 0.999)
