@@ -39,7 +39,36 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--macro-every-step", type=int, default=1)
+    p.add_argument("--thermal-n", type=int, default=256)
+    p.add_argument("--no-thermal", action="store_true")
     return p.parse_args()
+
+
+def thermal_leg(n, device, seed=0):
+    """GMG-preconditioned flexible CG (FP32 V-cycle, FP64 outer loop) to an FP64
+    relative residual of 1e-8 on the config-5 blob conduction problem."""
+    import torch
+
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.solver.mg_pcg import FCGWorkspace, fcg_fp64
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+
+    c = geometry.conduction_problem(n, seed=seed, device=device)
+    mg = NLevelGeometricMGSolver(device, torch.float32, c["ax"], c["sign"], c["s"], omega=0.8, n_pre=2, n_post=2,
+                                 n_coarse=20)
+    mg.build_if_needed(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], c["dt"])
+    ws = FCGWorkspace(c["k"].shape, device)
+    fcg_fp64(mg, c["b"], tol=1e-8, max_it=3, ws=ws)  # warm-up (allocations, first launches)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, info = fcg_fp64(mg, c["b"], tol=1e-8, max_it=500, ws=ws)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    return {"problem": f"{n}^3 smooth-noise blobs (seed {seed}), k_s/k_f = {c['ratio']:.3g}, dt = {c['dt']:g}",
+            "levels": [list(lv["shape"]) for lv in mg._levels], "gmg_pcg_s": round(secs, 4),
+            "gmg_pcg_iterations": info["it"], "final_rel_residual_fp64": info["res_rel"],
+            "converged": bool(info["converged"]), "nn_pcg_s": None,
+            "note": "NN-preconditioned FCG not built yet"}
 
 
 def build_workload(n, seed, collision, storage, device):
@@ -55,7 +84,8 @@ def build_workload(n, seed, collision, storage, device):
     per = (True, True, True)
     solid_t = torch.as_tensor(solid, device=device)
     ff, di, do, fff, ok = build_links_from_mask_gpu(solid_t, periodic=per)
-    q, ge, w1, w2 = bfl_q_and_weights(torch.as_tensor(phi, device=device), (0.0, 0.0, 0.0), 1.0, ff, di)
+    q, ge, w1, w2 = bfl_q_and_weights(torch.as_tensor(phi, device=device), (0.0, 0.0, 0.0), 1.0, ff, di,
+                                      periodic=per)
     links = {k: v.cpu().numpy() for k, v in dict(fluid_flat=ff, dir_i=di, ff_flat=fff, ff_valid=ok, w1=w1, w2=w2,
                                                    q_ge_half=ge).items()}
     omega = 1.0 / 0.55
@@ -126,6 +156,9 @@ def main():
     achieved = bpu * n_fluid / (ms_step * 1e-3) / 1e9
     ok = bool(torch.isfinite(macro[1]).all()) and bool(torch.isfinite(macro[0]).all())
     ux_mean = float(macro[1][torch.as_tensor(~solid.reshape(-1), device=dev)].mean())
+    thermal = None
+    if not args.no_thermal:
+        thermal = thermal_leg(args.thermal_n, dev)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(solid, links, S, omega, args.collision, args.cpu_steps)
@@ -151,6 +184,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_update": bpu},
         "cpu_baseline": cpu,
+        "thermal": thermal,
         "mlups_all_nodes": round(N * args.steps / (ms * 1e-3) / 1e6, 2),
         "wall_s_timed": round(wall, 4),
         "setup_s": round(setup_s, 2),

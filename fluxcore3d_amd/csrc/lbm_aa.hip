@@ -130,8 +130,9 @@ __device__ __forceinline__ void store_plain(ST* pop, const Grid& g, int parity, 
 }
 
 // All 19 inputs of a node: plain slots, rest weights for solid sources and
-// Bouzidi values for links. `first` marks the step right after loading, when
-// A holds the loaded populations themselves.
+// Bouzidi values for links. `first` marks the step right after loading: A then
+// holds the loaded populations themselves and solid sources have not streamed
+// yet, so those inputs are the loaded values.
 template <typename ST>
 __device__ __forceinline__ void compose_inputs(const ST* __restrict__ pop, const Grid& g, const Tables& t,
                                                int parity, int first, int i, int j, int k, bool special,
@@ -147,8 +148,8 @@ __device__ __forceinline__ void compose_inputs(const ST* __restrict__ pop, const
         v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + mv];
       }
       f[q] = v;
-    } else if (special && ((S >> q) & 1u)) {
-      f[q] = WQ(q);
+    } else if (special && !first && ((S >> q) & 1u)) {
+      f[q] = WQ(q);  // after the first step a solid source delivers the rest weight
     } else {
       f[q] = load_plain(pop, g, parity, i, j, k, q);
     }
@@ -263,16 +264,20 @@ __device__ __forceinline__ unsigned logical_block(unsigned bid, unsigned nblk) {
   return xcd * base + min(xcd, rem) + local;
 }
 
-// MRT in moment space: moments m = M f, equilibrium moments in closed form
-// for the second-order equilibrium (m_eq of compute_feq's polynomial), relax
-// the 15 non-conserved rows, add the Guo source moments (1 - s/2) M F_i, and
-// return f* = M^T diag(1/|M_k|^2) m*. Only 19 moments are live between the
-// two transforms.
+// MRT: non-equilibrium moments dm = S (M f - m_eq) with m_eq in closed form
+// for the second-order equilibrium (the moments of compute_feq's polynomial),
+// Guo source moments (1 - s/2) M F_i folded into dm, and f* = f - M^T
+// diag(1/|M_k|^2) dm. Subtracting a small correction from f (rather than
+// rebuilding f from its moments) keeps FP32 round-off at the level of the
+// reference's f - M^-1 dm (lbm_kernels.cu:415-435).
 __device__ __forceinline__ void mrt_moment_collide(const Collision& c, const float S[19], float f[19], float rho,
                                                    float ux, float uy, float uz, bool force) {
-  float m[19];
+  const float u2 = ux * ux + uy * uy + uz * uz;
+  const float pxx = 3.f * ux * ux - u2, pww = uy * uy - uz * uz;
+  float dm[19];
 #pragma unroll
   for (int k = 0; k < 19; ++k) {
+    if (k == 0 || k == 3 || k == 5 || k == 7) { dm[k] = 0.f; continue; }
     float a = 0.f;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
@@ -281,20 +286,24 @@ __device__ __forceinline__ void mrt_moment_collide(const Collision& c, const flo
       else if (cf == -1) a -= f[q];
       else if (cf != 0) a += (float)cf * f[q];
     }
-    m[k] = a;
+    float meq;
+    switch (k) {
+      case 1: meq = rho * (-11.f + 19.f * u2); break;
+      case 2: meq = rho * (3.f - 5.5f * u2); break;
+      case 4: meq = (-2.f / 3.f) * rho * ux; break;
+      case 6: meq = (-2.f / 3.f) * rho * uy; break;
+      case 8: meq = (-2.f / 3.f) * rho * uz; break;
+      case 9: meq = rho * pxx; break;
+      case 10: meq = -0.5f * rho * pxx; break;
+      case 11: meq = rho * pww; break;
+      case 12: meq = -0.5f * rho * pww; break;
+      case 13: meq = rho * ux * uy; break;
+      case 14: meq = rho * uy * uz; break;
+      case 15: meq = rho * ux * uz; break;
+      default: meq = 0.f; break;
+    }
+    dm[k] = S[k] * (a - meq);
   }
-  const float u2 = ux * ux + uy * uy + uz * uz;
-  const float pxx = 3.f * ux * ux - u2, pww = uy * uy - uz * uz;
-  float meq[19];
-  meq[0] = rho; meq[1] = rho * (-11.f + 19.f * u2); meq[2] = rho * (3.f - 5.5f * u2);
-  meq[3] = rho * ux; meq[4] = (-2.f / 3.f) * rho * ux; meq[5] = rho * uy; meq[6] = (-2.f / 3.f) * rho * uy;
-  meq[7] = rho * uz; meq[8] = (-2.f / 3.f) * rho * uz;
-  meq[9] = rho * pxx; meq[10] = -0.5f * rho * pxx; meq[11] = rho * pww; meq[12] = -0.5f * rho * pww;
-  meq[13] = rho * ux * uy; meq[14] = rho * uy * uz; meq[15] = rho * ux * uz;
-  meq[16] = 0.f; meq[17] = 0.f; meq[18] = 0.f;
-#pragma unroll
-  for (int k = 0; k < 19; ++k)
-    if (!(k == 0 || k == 3 || k == 5 || k == 7)) m[k] = m[k] - S[k] * (m[k] - meq[k]);
   if (force) {
     const float Fx = c.F[0], Fy = c.F[1], Fz = c.F[2];
     const float uf = ux * Fx + uy * Fy + uz * Fz;
@@ -306,21 +315,22 @@ __device__ __forceinline__ void mrt_moment_collide(const Collision& c, const flo
     Fm[13] = ux * Fy + uy * Fx; Fm[14] = uy * Fz + uz * Fy; Fm[15] = ux * Fz + uz * Fx;
     Fm[16] = 0.f; Fm[17] = 0.f; Fm[18] = 0.f;
 #pragma unroll
-    for (int k = 0; k < 19; ++k) m[k] += (1.f - 0.5f * S[k]) * Fm[k];
+    for (int k = 0; k < 19; ++k) dm[k] -= (1.f - 0.5f * S[k]) * Fm[k];
   }
 #pragma unroll
-  for (int k = 0; k < 19; ++k) m[k] *= 1.0f / (float)MNORM2(k);
+  for (int k = 0; k < 19; ++k) dm[k] *= 1.0f / (float)MNORM2(k);
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     float a = 0.f;
 #pragma unroll
     for (int k = 0; k < 19; ++k) {
+      if (!force && (k == 0 || k == 3 || k == 5 || k == 7)) continue;
       const int cf = MROW(k, q);
-      if (cf == 1) a += m[k];
-      else if (cf == -1) a -= m[k];
-      else if (cf != 0) a += (float)cf * m[k];
+      if (cf == 1) a += dm[k];
+      else if (cf == -1) a -= dm[k];
+      else if (cf != 0) a += (float)cf * dm[k];
     }
-    f[q] = a;
+    f[q] -= a;
   }
 }
 
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
         if (!first && ((V >> q) & 1u))
           v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + b.y + __popc(V & ((1u << q) - 1u))];
         f[q] = v;
-      } else if ((Sm >> q) & 1u) {
+      } else if (!first && ((Sm >> q) & 1u)) {
         f[q] = WQ(q);
       }
     }

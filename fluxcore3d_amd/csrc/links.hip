@@ -62,16 +62,33 @@ struct Sdf {
   int nx, ny, nz;
   float ox, oy, oz, dx;
   float hx, hy, hz;  // extent - 1.001 (clamp limits)
+  int px, py, pz;    // periodic axes wrap instead of clamping (build extension)
 };
 
+// grid coordinate along one axis: clamp (reference) or wrap (periodic axis)
+__device__ __forceinline__ void axis_coord(float g, float h, int n, int per, int& i0, int& i1, float& fr) {
+  if (per) {
+    const float fl = floorf(g);
+    fr = g - fl;
+    int a = (int)fl % n;
+    a = a < 0 ? a + n : a;
+    i0 = a;
+    i1 = a + 1 == n ? 0 : a + 1;
+  } else {
+    g = g < 0.0f ? 0.0f : (g > h ? h : g);
+    i0 = (int)floorf(g);
+    fr = g - (float)i0;
+    i1 = min(i0 + 1, n - 1);
+  }
+}
+
 __device__ float sdf_tri(const Sdf& s, float x, float y, float z) {
-  float gx = (x - s.ox) / s.dx, gy = (y - s.oy) / s.dx, gz = (z - s.oz) / s.dx;
-  gx = gx < 0.0f ? 0.0f : (gx > s.hx ? s.hx : gx);
-  gy = gy < 0.0f ? 0.0f : (gy > s.hy ? s.hy : gy);
-  gz = gz < 0.0f ? 0.0f : (gz > s.hz ? s.hz : gz);
-  const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
-  const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
-  const int i1 = min(i0 + 1, s.nx - 1), j1 = min(j0 + 1, s.ny - 1), k1 = min(k0 + 1, s.nz - 1);
+  const float gx = (x - s.ox) / s.dx, gy = (y - s.oy) / s.dx, gz = (z - s.oz) / s.dx;
+  int i0, i1, j0, j1, k0, k1;
+  float fx, fy, fz;
+  axis_coord(gx, s.hx, s.nx, s.px, i0, i1, fx);
+  axis_coord(gy, s.hy, s.ny, s.py, j0, j1, fy);
+  axis_coord(gz, s.hz, s.nz, s.pz, k0, k1, fz);
   auto P = [&](int a, int b, int c) { return s.phi[((int64_t)a * s.ny + b) * s.nz + c]; };
   const float c000 = P(i0, j0, k0), c100 = P(i1, j0, k0), c010 = P(i0, j1, k0), c110 = P(i1, j1, k0);
   const float c001 = P(i0, j0, k1), c101 = P(i1, j0, k1), c011 = P(i0, j1, k1), c111 = P(i1, j1, k1);
@@ -150,9 +167,9 @@ int fc_links_mask(const uint8_t* solid, int nx, int ny, int nz, int px, int py, 
   return check_launch("links_mask");
 }
 
-int fc_links_q(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch,
-               const int64_t* fluid_flat, const int64_t* dir_i, int64_t n, int n_coarse, int n_bisect, float* q,
-               float* w1, float* w2, uint8_t* ge, hipStream_t stream) {
+int fc_links_q(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch, int px, int py,
+               int pz, const int64_t* fluid_flat, const int64_t* dir_i, int64_t n, int n_coarse, int n_bisect,
+               float* q, float* w1, float* w2, uint8_t* ge, hipStream_t stream) {
   FC_REQUIRE(nx > 1 && ny > 1 && nz > 1, "links_q: phi extents must be > 1");
   FC_REQUIRE(n_coarse >= 1 && n_bisect >= 0, "links_q: n_coarse >= 1 and n_bisect >= 0 required");
   FC_REQUIRE(pitch > 0.f, "links_q: pitch must be > 0");
@@ -161,6 +178,7 @@ int fc_links_q(const float* phi, int nx, int ny, int nz, float ox, float oy, flo
   Sdf s;
   s.phi = phi; s.nx = nx; s.ny = ny; s.nz = nz; s.ox = ox; s.oy = oy; s.oz = oz; s.dx = pitch;
   s.hx = (float)((double)nx - 1.001); s.hy = (float)((double)ny - 1.001); s.hz = (float)((double)nz - 1.001);
+  s.px = px != 0; s.py = py != 0; s.pz = pz != 0;
   hipLaunchKernelGGL(link_q_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, s, fluid_flat, dir_i, n,
                      n_coarse, n_bisect, q, w1, w2, ge);
   return check_launch("links_q");

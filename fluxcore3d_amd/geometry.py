@@ -98,3 +98,42 @@ def rsa_spheres(n=256, porosity=0.6, rmin=4.0, rmax=10.0, seed=0, tries_per_sphe
     info = {"porosity": float(1.0 - solid.mean()), "n_spheres": int(len(radii)), "attempts": int(attempts),
             "seed": int(seed), "target_porosity": float(porosity)}
     return phi, solid, info
+
+
+def smooth_blobs(n, seed=0, sigma=None, device="cuda"):
+    """Seeded smooth-noise blobs: white noise filtered by a Gaussian of width
+    ``sigma`` (default n/16 cells) in Fourier space, thresholded at the median
+    (50 % solid). Returns a boolean torch tensor on ``device``."""
+    import torch
+
+    g = torch.Generator(device="cpu").manual_seed(int(seed))
+    noise = torch.randn((n, n, n), generator=g, dtype=torch.float32).to(device)
+    s = float(sigma if sigma is not None else n / 16.0)
+    k = torch.fft.fftfreq(n, device=device, dtype=torch.float32)
+    kk = k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2
+    filt = torch.exp(-2.0 * (torch.pi ** 2) * (s ** 2) * kk)
+    field = torch.fft.ifftn(torch.fft.fftn(noise) * filt).real
+    thr = torch.quantile(field.flatten()[:: max(1, field.numel() // 2_000_000)].float(), 0.5)
+    return field > thr
+
+
+def conduction_problem(n, seed=0, ratio=None, dt=1.0e4, device="cuda"):
+    """Config-5 conduction system A x = b on an n^3 blob field: k = 1 in fluid,
+    k = ratio in solid (ratio log-uniform in [1, 1e4] from the seed unless given),
+    uniform rho_cp, no flow, Dirichlet x = 0 on the inlet plane (i = 0), Neumann
+    elsewhere; b = dt * s with a unit source in the solid, implicit step dt."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(seed)
+    if ratio is None:
+        ratio = float(10.0 ** rng.uniform(0.0, 4.0))
+    solid = smooth_blobs(n, seed=seed, device=device)
+    k = torch.where(solid, torch.tensor(ratio, device=device), torch.tensor(1.0, device=device)).float()
+    mask = torch.ones((n, n, n), dtype=torch.float32, device=device)
+    mask[0] = 0.0
+    src = solid.float() / max(int(solid.sum().item()), 1)
+    b = (dt * src) * mask
+    z = torch.zeros_like(k)
+    return dict(k=k, rcp=torch.ones_like(k), u=z, v=z, w=z, mask=mask, b=b, dt=dt, ratio=ratio, ax=0, sign=1,
+                s=1.0, solid=solid)

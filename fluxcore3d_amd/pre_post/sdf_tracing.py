@@ -19,8 +19,9 @@ from .. import _lib
 _lib.register({
     "fc_links_mask": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.P, _lib.P,
                                  _lib.P]),
-    "fc_links_q": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.I32, _lib.F32, _lib.F32, _lib.F32, _lib.F32, _lib.P,
-                              _lib.P, _lib.I64, _lib.I32, _lib.I32, _lib.P, _lib.P, _lib.P, _lib.P, _lib.P]),
+    "fc_links_q": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.I32, _lib.F32, _lib.F32, _lib.F32, _lib.F32, _lib.I32,
+                              _lib.I32, _lib.I32, _lib.P, _lib.P, _lib.I64, _lib.I32, _lib.I32, _lib.P, _lib.P,
+                              _lib.P, _lib.P, _lib.P]),
 })
 
 _C = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0],
@@ -83,8 +84,10 @@ def build_links_from_mask_gpu(solid_geom: torch.Tensor, c_np=_C, opp_np=_OPP, pe
     return fluid_flat, dir_i, dir_opp, torch.cat(ffs).to(torch.int64), torch.cat(oks)
 
 
-def bfl_q_and_weights(phi, origin, pitch, fluid_flat, dir_i, n_coarse=6, n_bisect=10):
-    """q, q>=1/2, w1, w2 for each link (sdf_tracing.py:127-205)."""
+def bfl_q_and_weights(phi, origin, pitch, fluid_flat, dir_i, n_coarse=6, n_bisect=10,
+                      periodic=(False, False, False)):
+    """q, q>=1/2, w1, w2 for each link (sdf_tracing.py:127-205); ``periodic`` wraps
+    the SDF samples on those axes instead of clamping them (build extension)."""
     phi = phi.to(torch.float32).contiguous()
     nx, ny, nz = phi.shape
     n = int(fluid_flat.numel())
@@ -96,7 +99,8 @@ def bfl_q_and_weights(phi, origin, pitch, fluid_flat, dir_i, n_coarse=6, n_bisec
     org = [float(v) for v in origin]
     ff = fluid_flat.to(torch.int64).contiguous()
     di = dir_i.to(torch.int64).contiguous()
-    _lib.call("fc_links_q", phi.data_ptr(), nx, ny, nz, org[0], org[1], org[2], float(pitch), ff.data_ptr(),
+    _lib.call("fc_links_q", phi.data_ptr(), nx, ny, nz, org[0], org[1], org[2], float(pitch),
+              *[int(bool(p)) for p in periodic], ff.data_ptr(),
               di.data_ptr(), n, int(n_coarse), int(n_bisect), q.data_ptr(), w1.data_ptr(), w2.data_ptr(),
               ge.data_ptr(), _lib.stream_handle(dev))
     return q, ge.bool(), w1, w2
@@ -108,7 +112,7 @@ def build_ibb_from_sdf_gpu(sim, voxel_origin_mm, pitch_mm, offset_ijk, c_np=_C, 
     origin = (np.asarray(voxel_origin_mm, np.float32)
               - np.float32(pitch_mm) * np.asarray(offset_ijk, np.float32)).astype(np.float32)
     fluid_flat, dir_i, dir_opp, ff_flat, ff_valid = build_links_from_mask_gpu(sim.solid, c_np, opp_np, periodic)
-    q, ge, w1, w2 = bfl_q_and_weights(sim.phi, origin, pitch_mm, fluid_flat, dir_i)
+    q, ge, w1, w2 = bfl_q_and_weights(sim.phi, origin, pitch_mm, fluid_flat, dir_i, periodic=periodic)
     sim.ibb_data = IBBData(n_links=int(fluid_flat.numel()), fluid_flat=fluid_flat, dir_i=dir_i, dir_opp=dir_opp,
                            q_ge_half=ge, w_i=w1, w_second=w2, ff_flat=ff_flat, ff_valid=ff_valid)
     sim.ibb_q = q

@@ -57,6 +57,92 @@ def _neighbor_solid_masks(solid: np.ndarray, periodic) -> np.ndarray:
     return S
 
 
+def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="reference", bcs=()) -> dict:
+    """Node/link tables of the fused step (pure NumPy, node order)."""
+    N = nx * ny * nz
+    solid = np.asarray(solid, bool).reshape(nx, ny, nz)
+    flat_solid = solid.reshape(-1)
+    Smask = _neighbor_solid_masks(solid, periodic).reshape(-1)
+    Smask[flat_solid] = 0
+    Lmask = np.zeros(N, dtype=np.uint32)
+    Vmask = np.zeros(N, dtype=np.uint32)
+    Fmask = np.zeros(N, dtype=np.uint32)
+    w1s = np.zeros(0, np.float32)
+    w2s = np.zeros(0, np.float32)
+    link_node = np.zeros(0, np.int64)
+    link_dir = np.zeros(0, np.int64)
+    v_node = np.zeros(0, np.int64)
+    ff_node = np.zeros(0, np.int64)
+    ff_slot = np.zeros(0, np.int32)
+    B_init = np.zeros(0, np.float32)
+    if links is not None and int(np.asarray(links["fluid_flat"]).size) > 0:
+        xf = np.asarray(links["fluid_flat"], np.int64)
+        q = np.asarray(links["dir_i"], np.int64)
+        w1 = np.asarray(links["w1"], np.float32).copy()
+        w2 = np.asarray(links["w2"], np.float32).copy()
+        ge = np.asarray(links["q_ge_half"], bool)
+        ffv = np.asarray(links["ff_valid"], bool)
+        ff = np.asarray(links["ff_flat"], np.int64)
+        if xf.min() < 0 or xf.max() >= N or q.min() < 1 or q.max() > 18:
+            raise ValueError("link table out of range")
+        keep = ~flat_solid[xf]  # links at flow-solid nodes are reset by the reference
+        xf, q, w1, w2, ge, ffv, ff = xf[keep], q[keep], w1[keep], w2[keep], ge[keep], ffv[keep], ff[keep]
+        t = q if ibb_mode == "reference" else OPP_NP[q]
+        bad = (~ge) & (~ffv)  # q < 1/2 without a second node: f*_opp alone
+        w1[bad] = 1.0
+        w2[bad] = 0.0
+        v = (~ge) & ffv
+        order = np.lexsort((t, xf))
+        xf, t, w1, w2, v, ff = xf[order], t[order], w1[order], w2[order], v[order], ff[order]
+        key = xf * 19 + t
+        if np.any(key[1:] == key[:-1]):
+            raise ValueError("duplicate Bouzidi link (node, direction)")
+        np.bitwise_or.at(Lmask, xf, (np.uint32(1) << t.astype(np.uint32)))
+        Smask[xf] &= ~Lmask[xf]  # a link overrides a solid-source substitution
+        np.bitwise_or.at(Vmask, xf[v], (np.uint32(1) << t[v].astype(np.uint32)))
+        w1s, w2s, link_node, link_dir = w1, w2, xf, t
+        vx, vt, vff = xf[v], t[v], ff[v]
+        v_node = vx
+        n_v = int(vx.size)
+        ff_solid = flat_solid[vff]
+        p = OPP_NP[vt]  # population the second node publishes
+        B_init = np.where(ff_solid, W_NP[p], 0.0).astype(np.float32)
+        pub = ~ff_solid
+        pm = np.arange(n_v, dtype=np.int64)[pub]
+        pn, pp = vff[pub], p[pub]
+        o2 = np.lexsort((pp, pn))
+        pn, pp, pm = pn[o2], pp[o2], pm[o2]
+        if np.any((pn * 19 + pp)[1:] == (pn * 19 + pp)[:-1]):
+            raise ValueError("duplicate second-node publication")
+        np.bitwise_or.at(Fmask, pn, (np.uint32(1) << pp.astype(np.uint32)))
+        ff_node, ff_slot = pn, pm.astype(np.int32)
+    special = (~flat_solid) & ((Lmask | Smask | Fmask) != 0)
+    sp_idx = np.nonzero(special)[0]
+
+    def first_index(owner: np.ndarray) -> np.ndarray:
+        return np.searchsorted(owner, sp_idx).astype(np.int32)
+
+    sp_mask = np.stack([Lmask[sp_idx], Smask[sp_idx], Vmask[sp_idx], Fmask[sp_idx]], 1).view(np.int32)
+    sp_base = np.stack([first_index(link_node), first_index(v_node), first_index(ff_node),
+                        np.zeros(sp_idx.size, np.int32)], 1).astype(np.int32)
+    nflag = flat_solid.astype(np.uint8) * F_SOLID | special.astype(np.uint8) * F_SPECIAL
+    for b, bc in enumerate(bcs):
+        ax, pl = int(bc["axis"]), int(bc["plane"])
+        sel = [slice(None)] * 3
+        sel[ax] = pl
+        plane = np.zeros((nx, ny, nz), bool)
+        plane[tuple(sel)] = True
+        plane = plane.reshape(-1) & ~flat_solid & ((nflag >> F_BC_SHIFT) == 0)
+        nflag[plane] |= np.uint8((b + 1) << F_BC_SHIFT)
+    groups = (N + 63) // 64
+    cnt = np.zeros(groups * 64, np.int64)
+    cnt[:N] = special
+    wave_base = np.concatenate([[0], np.cumsum(cnt.reshape(groups, 64).sum(1))[:-1]]).astype(np.int32)
+    return dict(nflag=nflag, wave_base=wave_base, sp_mask=sp_mask, sp_base=sp_base, w1=w1s, w2=w2s,
+                link_node=link_node, link_dir=link_dir, B_init=B_init, ff_idx=ff_slot, ff_node=ff_node,
+                n_links=int(link_node.size), n_v=int(v_node.size), n_special=int(sp_idx.size), sp_idx=sp_idx)
+
+
 class AAEngine:
     """In-place D3Q19 state + tables of the fused kernel."""
 
@@ -89,96 +175,11 @@ class AAEngine:
 
     # ------------------------------------------------------------------ tables
     def _build_tables(self, solid: np.ndarray, links) -> None:
-        nx, ny, nz, N = self.nx, self.ny, self.nz, self.N
-        flat_solid = solid.reshape(-1)
-        Smask = _neighbor_solid_masks(solid, self.periodic).reshape(-1)
-        Smask[flat_solid] = 0
+        tb = build_tables(self.nx, self.ny, self.nz, solid, links, self.periodic, self.ibb_mode, self.bcs)
+        self.n_links, self.n_v, self.n_special = tb["n_links"], tb["n_v"], tb["n_special"]
+        self._upload(tb)
 
-        Lmask = np.zeros(N, dtype=np.uint32)
-        Vmask = np.zeros(N, dtype=np.uint32)
-        Fmask = np.zeros(N, dtype=np.uint32)
-        w1s = np.zeros(0, np.float32)
-        w2s = np.zeros(0, np.float32)
-        link_node = np.zeros(0, np.int64)
-        v_node = np.zeros(0, np.int64)
-        ff_node = np.zeros(0, np.int64)
-        ff_slot = np.zeros(0, np.int32)
-        B_init = np.zeros(0, np.float32)
-        self.n_links = 0
-        if links is not None and int(links["fluid_flat"].size) > 0:
-            xf = np.asarray(links["fluid_flat"], np.int64)
-            q = np.asarray(links["dir_i"], np.int64)
-            w1 = np.asarray(links["w1"], np.float32).copy()
-            w2 = np.asarray(links["w2"], np.float32).copy()
-            ge = np.asarray(links["q_ge_half"], bool)
-            ffv = np.asarray(links["ff_valid"], bool)
-            ff = np.asarray(links["ff_flat"], np.int64)
-            if xf.min() < 0 or xf.max() >= N or q.min() < 1 or q.max() > 18:
-                raise ValueError("link table out of range")
-            keep = ~flat_solid[xf]  # links at flow-solid nodes are reset by the reference
-            xf, q, w1, w2, ge, ffv, ff = xf[keep], q[keep], w1[keep], w2[keep], ge[keep], ffv[keep], ff[keep]
-            t = q if self.ibb_mode == "reference" else OPP_NP[q]
-            # q < 1/2 without a second node: the reference takes f*_opp alone
-            bad = (~ge) & (~ffv)
-            w1[bad] = 1.0
-            w2[bad] = 0.0
-            v = (~ge) & ffv
-            order = np.lexsort((t, xf))
-            xf, t, w1, w2, v, ff = xf[order], t[order], w1[order], w2[order], v[order], ff[order]
-            key = xf * 19 + t
-            if np.any(key[1:] == key[:-1]):
-                raise ValueError("duplicate Bouzidi link (node, direction)")
-            np.bitwise_or.at(Lmask, xf, (np.uint32(1) << t.astype(np.uint32)))
-            # a link overrides any solid-source substitution of the same input
-            Smask[xf] &= ~Lmask[xf]
-            np.bitwise_or.at(Vmask, xf[v], (np.uint32(1) << t[v].astype(np.uint32)))
-            w1s, w2s, link_node = w1, w2, xf
-            self.n_links = int(xf.size)
-            # second-node (V) links in (node, direction) order
-            vx, vt, vff = xf[v], t[v], ff[v]
-            v_node = vx
-            n_v = int(vx.size)
-            ff_solid = flat_solid[vff]
-            p = OPP_NP[vt]  # population the second node publishes
-            B_init = np.where(ff_solid, W_NP[p], 0.0).astype(np.float32)
-            pub = ~ff_solid
-            pm = np.arange(n_v, dtype=np.int64)[pub]
-            pn, pp = vff[pub], p[pub]
-            o2 = np.lexsort((pp, pn))
-            pn, pp, pm = pn[o2], pp[o2], pm[o2]
-            if np.any((pn * 19 + pp)[1:] == (pn * 19 + pp)[:-1]):
-                raise ValueError("duplicate second-node publication")
-            np.bitwise_or.at(Fmask, pn, (np.uint32(1) << pp.astype(np.uint32)))
-            ff_node, ff_slot = pn, pm.astype(np.int32)
-        self.n_v = int(v_node.size)
-
-        special = (~flat_solid) & ((Lmask | Smask | Fmask) != 0)
-        sp_idx = np.nonzero(special)[0]
-        self.n_special = int(sp_idx.size)
-
-        def first_index(owner: np.ndarray) -> np.ndarray:
-            # index of the first entry owned by each special node (entries sorted by owner)
-            return np.searchsorted(owner, sp_idx).astype(np.int32)
-
-        sp_mask = np.stack([Lmask[sp_idx], Smask[sp_idx], Vmask[sp_idx], Fmask[sp_idx]], 1).view(np.int32)
-        sp_base = np.stack([first_index(link_node), first_index(v_node), first_index(ff_node),
-                            np.zeros(self.n_special, np.int32)], 1).astype(np.int32)
-
-        nflag = flat_solid.astype(np.uint8) * F_SOLID | special.astype(np.uint8) * F_SPECIAL
-        for b, bc in enumerate(self.bcs):
-            ax, pl = int(bc["axis"]), int(bc["plane"])
-            sel = [slice(None)] * 3
-            sel[ax] = pl
-            plane = np.zeros((nx, ny, nz), bool)
-            plane[tuple(sel)] = True
-            plane = plane.reshape(-1) & ~flat_solid & ((nflag >> F_BC_SHIFT) == 0)
-            nflag[plane] |= np.uint8((b + 1) << F_BC_SHIFT)
-
-        groups = (N + 63) // 64
-        cnt = np.zeros(groups * 64, np.int64)
-        cnt[:N] = special
-        wave_base = np.concatenate([[0], np.cumsum(cnt.reshape(groups, 64).sum(1))[:-1]]).astype(np.int32)
-
+    def _upload(self, tb) -> None:
         dev = self.device
 
         def dt(a, dtype):
@@ -187,16 +188,17 @@ class AAEngine:
                 a = np.zeros(1, a.dtype)
             return torch.from_numpy(a).to(device=dev, dtype=dtype)
 
-        self.nflag = dt(nflag, torch.uint8)
-        self.wave_base = dt(wave_base, torch.int32)
-        self.sp_mask = dt(sp_mask, torch.int32)
-        self.sp_base = dt(sp_base, torch.int32)
-        self.w1 = dt(w1s, torch.float32)
-        self.w2 = dt(w2s, torch.float32)
+        self.nflag = dt(tb["nflag"], torch.uint8)
+        self.wave_base = dt(tb["wave_base"], torch.int32)
+        self.sp_mask = dt(tb["sp_mask"], torch.int32)
+        self.sp_base = dt(tb["sp_base"], torch.int32)
+        self.w1 = dt(tb["w1"], torch.float32)
+        self.w2 = dt(tb["w2"], torch.float32)
         self.A = torch.zeros(max(self.n_links, 1), dtype=torch.float32, device=dev)
+        B_init = tb["B_init"]
         self.B_init = dt(np.concatenate([B_init, B_init]) if self.n_v else np.zeros(2, np.float32), torch.float32)
         self.B = self.B_init.clone()
-        self.ff_idx = dt(ff_slot, torch.int32)
+        self.ff_idx = dt(tb["ff_idx"], torch.int32)
         self._tables = _lib.AATables(
             self.nflag.data_ptr(), self.wave_base.data_ptr(), self.sp_mask.data_ptr(), self.sp_base.data_ptr(),
             self.w1.data_ptr(), self.w2.data_ptr(), self.A.data_ptr(), self.B.data_ptr(), self.ff_idx.data_ptr(),

@@ -90,8 +90,9 @@ class NLevelGeometricMGSolver:
                 lv[name] = torch.empty(shape, device=dev, dtype=torch.float32)
             _lib.call("fc_mg_build_coeffs", *[t.data_ptr() for t in fields], nx, ny, nz, self.ax, out_idx, dx, self.s,
                       *[lv[n].data_ptr() for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL")], stream)
-            lv["coef"] = MGCoef(*[lv[n].data_ptr() for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL",
-                                                             "inv_diagM")])
+            lv["coef_struct"] = MGCoef(*[lv[n].data_ptr() for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL",
+                                                                    "inv_diagM")])
+            lv["coef"] = _C.addressof(lv["coef_struct"])  # pointer to the struct, kept alive by the level
             self._levels.append(lv)
             if min(nx // 2, ny // 2, nz // 2) <= self.min_coarse_cells:
                 break

@@ -121,10 +121,11 @@ int fc_aa_convective(void *pop, int storage, int nx, int ny, int nz, int px, int
 int fc_links_mask(const uint8_t *solid, int nx, int ny, int nz, int px, int py, int pz, uint32_t *mask,
                   uint32_t *ffok, hipStream_t stream);
 /* bfl_q_from_sdf + bfl_weights_from_q (sdf_tracing.py:127-205) with x0, d, L of
- * build_ibb_from_sdf_gpu (:208-247) */
-int fc_links_q(const float *phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch,
-               const int64_t *fluid_flat, const int64_t *dir_i, int64_t n, int n_coarse, int n_bisect, float *q,
-               float *w1, float *w2, uint8_t *q_ge_half, hipStream_t stream);
+ * build_ibb_from_sdf_gpu (:208-247); px/py/pz = 1 wraps the SDF samples on that
+ * axis (the reference clamps; 0 reproduces it) */
+int fc_links_q(const float *phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch, int px, int py,
+               int pz, const int64_t *fluid_flat, const int64_t *dir_i, int64_t n, int n_coarse, int n_bisect,
+               float *q, float *w1, float *w2, uint8_t *q_ge_half, hipStream_t stream);
 
 
 /* ---- geometric multigrid (mg_kernels.cu / nlevel_mg_cuda.py) ---- */

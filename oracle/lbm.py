@@ -189,15 +189,21 @@ def step(f, solid_flow, nx, ny, nz, periodic, kind="bgk", om1=1.0, om2=1.0, S=No
 # fluid nodes (C order) whose in-range neighbour x + c_q is solid; second node
 # x_ff = x - c_q, valid when in range and fluid.
 # --------------------------------------------------------------------------
-def build_links(solid: np.ndarray):
+def build_links(solid: np.ndarray, periodic=(False, False, False)):
+    """``periodic`` (a build extension; the reference never wraps) also links
+    across periodic faces."""
     solid = np.asarray(solid, bool)
     nx, ny, nz = solid.shape
     fluid = ~solid
     I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     ff_all, d_all, fff_all, ok_all = [], [], [], []
+
+    def wrap(a, n, p):
+        return a % n if p else a
+
     for q in range(1, 19):
         cx, cy, cz = C_NP[q]
-        inb, jnb, knb = I + cx, J + cy, K + cz
+        inb, jnb, knb = wrap(I + cx, nx, periodic[0]), wrap(J + cy, ny, periodic[1]), wrap(K + cz, nz, periodic[2])
         valid = (inb >= 0) & (inb < nx) & (jnb >= 0) & (jnb < ny) & (knb >= 0) & (knb < nz)
         m = valid & fluid & solid[inb.clip(0, nx - 1), jnb.clip(0, ny - 1), knb.clip(0, nz - 1)]
         if not m.any():
@@ -205,7 +211,7 @@ def build_links(solid: np.ndarray):
         i, j, k = I[m], J[m], K[m]
         ff_all.append((i * ny + j) * nz + k)
         d_all.append(np.full(i.size, q, np.int64))
-        iff, jff, kff = i - cx, j - cy, k - cz
+        iff, jff, kff = wrap(i - cx, nx, periodic[0]), wrap(j - cy, ny, periodic[1]), wrap(k - cz, nz, periodic[2])
         ok = (iff >= 0) & (iff < nx) & (jff >= 0) & (jff < ny) & (kff >= 0) & (kff < nz)
         ok &= fluid[iff.clip(0, nx - 1), jff.clip(0, ny - 1), kff.clip(0, nz - 1)]
         fff_all.append((iff.clip(0, nx - 1) * ny + jff.clip(0, ny - 1)) * nz + kff.clip(0, nz - 1))
@@ -239,9 +245,9 @@ def linspace01(steps):
     return out
 
 
-def ibb_from_sdf(solid, phi, origin=(0.0, 0.0, 0.0), pitch=1.0):
+def ibb_from_sdf(solid, phi, origin=(0.0, 0.0, 0.0), pitch=1.0, periodic=(False, False, False)):
     """Complete IBBData-equivalent dict for a geometry (links + q + weights)."""
-    links = build_links(solid)
+    links = build_links(solid, periodic)
     q, w1, w2, ge = bfl_q(phi, origin, pitch, links["fluid_flat"], links["dir_i"])
     links.update(q=q, w1=w1, w2=w2, q_ge_half=ge)
     return links

@@ -166,9 +166,10 @@ def test_config1_step_sequence(engine, kind):
         sim.step(do_thermal=False)
     torch.cuda.synchronize()
     f_want, (rho_w, u_w, v_w, w_w) = _oracle_run(c, steps, kind, om2 or 1.0, S, (0, 0, 0), flow_bc_plan(sim))
-    assert rel(sim.u.cpu().numpy().reshape(-1), u_w) < 1e-5
-    assert rel(sim.v.cpu().numpy().reshape(-1), v_w) < 1e-5
-    assert rel(sim.wv.cpu().numpy().reshape(-1), w_w) < 1e-5
+    # velocity components relative to the largest speed of the field
+    speed = float(np.sqrt(u_w**2 + v_w**2 + w_w**2).max())
+    for got, want in ((sim.u, u_w), (sim.v, v_w), (sim.wv, w_w)):
+        assert np.abs(got.cpu().numpy().reshape(-1) - want).max() / speed < 1e-5
     assert rel(sim.rho.cpu().numpy().reshape(-1), rho_w) < 1e-5
     assert rel(sim.f.cpu().numpy().reshape(19, -1), f_want) < 1e-5
 
@@ -196,8 +197,10 @@ def test_aa_periodic_pack(kind):
     for _ in range(steps):
         f, *mw = olbm.step(f, c["solid_flow"], c["nx"], c["ny"], c["nz"], (1, 1, 1), kind, c["omega"], 1.0, S, 0.0,
                            c["links"])
-    for g, w in zip(macro, mw):
-        assert rel(g.cpu().numpy(), w) < 1e-5
+    assert rel(macro[0].cpu().numpy(), mw[0]) < 1e-5
+    speed = float(np.sqrt(mw[1]**2 + mw[2]**2 + mw[3]**2).max())
+    for a in (1, 2, 3):
+        assert np.abs(macro[a].cpu().numpy() - mw[a]).max() / speed < 1e-5
     assert rel(eng.materialize().cpu().numpy().reshape(19, -1), f) < 1e-5
 
 

@@ -33,7 +33,9 @@ def _plates(ny=24, y1=0.7, y2=None):
 def test_poiseuille_textbook_bouzidi_guo_force(kind):
     from fluxcore3d_amd.solver.aa_engine import AAEngine
     phi, solid, (nx, ny, nz), y1, y2 = _plates()
-    links = olbm.ibb_from_sdf(solid, phi)  # walls inside the domain: no wrap needed
+    # the channel is periodic in x and z: diagonal wall links wrap across those faces
+    # (phi does not vary along x or z, so the q search is unaffected by the clamp)
+    links = olbm.ibb_from_sdf(solid, phi, periodic=(True, True, True))
     tau = 0.8
     om = 1.0 / tau
     nu = (tau - 0.5) / 3.0
@@ -53,7 +55,8 @@ def test_poiseuille_textbook_bouzidi_guo_force(kind):
     fluid = ~solid[1, :, 1]
     exact = Fx / (2 * nu) * (y - y1) * (y2 - y)
     err = np.linalg.norm(u[fluid] - exact[fluid]) / np.linalg.norm(exact[fluid])
-    assert err < 2e-2, err
+    # a float64 model of the same data flow gives 1.4e-3 (BGK) and 1.6e-3 (TRT) at this resolution
+    assert err < 5e-3, err
     assert np.abs(macro[2].cpu().numpy()).max() < 1e-3 * exact.max()
 
 
@@ -62,7 +65,7 @@ def test_reference_and_textbook_bouzidi_differ_as_documented():
     direction choice (lbm_kernels.cu:567-587) is not, so its wall slips."""
     from fluxcore3d_amd.solver.aa_engine import AAEngine
     phi, solid, (nx, ny, nz), _, _ = _plates(y1=0.5, y2=22.5)
-    links = olbm.ibb_from_sdf(solid, phi)
+    links = olbm.ibb_from_sdf(solid, phi, periodic=(True, True, True))
     out = {}
     for mode in ("reference", "textbook"):
         eng = AAEngine(nx, ny, nz, solid, periodic=(1, 1, 1), collision="bgk", omega=1.0, force=(1e-6, 0, 0),
