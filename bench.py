@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--collision", default="mrt", choices=["bgk", "trt", "mrt", "mrt_smag"])
     p.add_argument("--storage", default="fp32", choices=["fp32", "fp16", "bf16"])
     p.add_argument("--seed", type=int, default=0)
-    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--cpu-steps", type=int, default=16)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--macro-every-step", type=int, default=1)
     p.add_argument("--thermal-n", type=int, default=256)

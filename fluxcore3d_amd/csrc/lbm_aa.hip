@@ -356,124 +356,208 @@ __device__ __forceinline__ void collide_fast(const Collision& c, float f[19], fl
   }
 }
 
+// Link records staged per wavefront in LDS. The records of one wave's special
+// nodes are contiguous (links, second-node reads and publications are stored
+// node-major), so the wave moves them with coalesced loads and stores instead
+// of 18 dependent per-lane gathers; a wave above the cap uses per-lane global
+// accesses. 256 covers every wave of a porosity-0.6 sphere pack (max 202).
+constexpr int kStage = 256;
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename ST, int KIND>
 __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
                                                          int parity, int first, float* __restrict__ rho_o,
                                                          float* __restrict__ u_o, float* __restrict__ v_o,
                                                          float* __restrict__ w_o) {
+  __shared__ float sA[kWaves][kStage], sW1[kWaves][kStage], sW2[kWaves][kStage], sBv[kWaves][kStage];
+  __shared__ int32_t sFF[kWaves][kStage];
   const unsigned lb = logical_block(blockIdx.x, gridDim.x);
   const int64_t x = (int64_t)lb * kBlock + threadIdx.x;
   const bool live = x < g.N;
   const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
   const bool special = (fl & F_SPECIAL) != 0;
   const uint64_t bal = __ballot(special);
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int4 m = make_int4(0, 0, 0, 0), b = make_int4(0, 0, 0, 0);
   if (special) {
     const int64_t sidx = t.wave_base[x >> 6] + __popcll(bal & ((1ull << lane) - 1ull));
     m = t.sp_mask[sidx];
     b = t.sp_base[sidx];
   }
-  if (!live) return;
-  if (fl & F_SOLID) {
+  // wave-uniform ranges of this wave's records; every lane is still active here
+  int link_lo = 0, v_lo = 0, f_lo = 0, nL = 0;
+  bool staged = false;
+  if (bal) {
+    const int lf = __builtin_ctzll(bal), ll = 63 - __builtin_clzll(bal);
+    link_lo = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(b.x, lf));
+    v_lo = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(b.y, lf));
+    f_lo = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(b.z, lf));
+    nL = __builtin_amdgcn_readlane(b.x + __popc((uint32_t)m.x), ll) - link_lo;
+    const int nV = __builtin_amdgcn_readlane(b.y + __popc((uint32_t)m.z), ll) - v_lo;
+    const int nF = __builtin_amdgcn_readlane(b.z + __popc((uint32_t)m.w), ll) - f_lo;
+    staged = nL <= kStage && nV <= kStage && nF <= kStage;
+    if (staged) {
+      for (int i = lane; i < nL; i += 64) {
+        sA[wv][i] = t.A[link_lo + i];
+        sW1[wv][i] = t.w1[link_lo + i];
+        sW2[wv][i] = t.w2[link_lo + i];
+      }
+      if (!first) {
+        const float* Bp = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
+        for (int i = lane; i < nV; i += 64) sBv[wv][i] = Bp[i];
+      }
+      for (int i = lane; i < nF; i += 64) sFF[wv][i] = t.ff_idx[f_lo + i];
+      wave_lds_sync();
+    }
+  }
+
+  if (live && (fl & F_SOLID)) {
     if (rho_o) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
-    return;
-  }
-  const uint32_t xu = (uint32_t)x;
-  const uint32_t nynz = (uint32_t)g.ny * (uint32_t)g.nz;
-  const int i = (int)(xu / nynz);
-  const uint32_t rem = xu - (uint32_t)i * nynz;
-  const int j = (int)(rem / (uint32_t)g.nz);
-  const int k = (int)(rem - (uint32_t)j * (uint32_t)g.nz);
-  const bool plain = (fl >> F_BC_SHIFT) != 0;
-  const uint32_t PZ = (uint32_t)g.PZ, PYZ = (uint32_t)g.PY * (uint32_t)g.PZ;
+  } else if (live) {
+    const uint32_t xu = (uint32_t)x;
+    const uint32_t nynz = (uint32_t)g.ny * (uint32_t)g.nz;
+    const int i = (int)(xu / nynz);
+    const uint32_t rem = xu - (uint32_t)i * nynz;
+    const int j = (int)(rem / (uint32_t)g.nz);
+    const int k = (int)(rem - (uint32_t)j * (uint32_t)g.nz);
+    const bool plain = (fl >> F_BC_SHIFT) != 0;
+    const uint32_t PZ = (uint32_t)g.PZ, PYZ = (uint32_t)g.PY * (uint32_t)g.PZ;
 
-  // padded coordinates of x - 1, x, x + 1 along each axis
-  const int Im = wrap_or_pad(i, -1, g.nx, g.gx), I0 = i + g.gx, Ip = wrap_or_pad(i, 1, g.nx, g.gx);
-  const int Jm = wrap_or_pad(j, -1, g.ny, g.gy), J0 = j + g.gy, Jp = wrap_or_pad(j, 1, g.ny, g.gy);
-  const int Km = wrap_or_pad(k, -1, g.nz, g.gz), K0 = k + g.gz, Kp = wrap_or_pad(k, 1, g.nz, g.gz);
-  auto rowb = [&](int dx, int dy) -> uint32_t {
-    const int I = dx < 0 ? Im : (dx > 0 ? Ip : I0);
-    const int J = dy < 0 ? Jm : (dy > 0 ? Jp : J0);
-    return (uint32_t)I * PYZ + (uint32_t)J * PZ;
-  };
-  auto slot = [&](int dx, int dy, int dz) -> uint32_t {
-    return rowb(dx, dy) + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
-  };
-  const uint32_t self = (uint32_t)I0 * PYZ + (uint32_t)J0 * PZ + (uint32_t)K0;
+    // padded coordinates of x - 1, x, x + 1 along each axis
+    const int Im = wrap_or_pad(i, -1, g.nx, g.gx), I0 = i + g.gx, Ip = wrap_or_pad(i, 1, g.nx, g.gx);
+    const int Jm = wrap_or_pad(j, -1, g.ny, g.gy), J0 = j + g.gy, Jp = wrap_or_pad(j, 1, g.ny, g.gy);
+    const int Km = wrap_or_pad(k, -1, g.nz, g.gz), K0 = k + g.gz, Kp = wrap_or_pad(k, 1, g.nz, g.gz);
+    auto rowb = [&](int dx, int dy) -> uint32_t {
+      const int I = dx < 0 ? Im : (dx > 0 ? Ip : I0);
+      const int J = dy < 0 ? Jm : (dy > 0 ? Jp : J0);
+      return (uint32_t)I * PYZ + (uint32_t)J * PZ;
+    };
+    auto slot = [&](int dx, int dy, int dz) -> uint32_t {
+      return rowb(dx, dy) + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
+    };
+    const uint32_t self = (uint32_t)I0 * PYZ + (uint32_t)J0 * PZ + (uint32_t)K0;
 
-  // all 19 plain loads first (branch-free), then special overrides
-  float f[19];
-  if (parity == 0) {
+    // all 19 plain loads first (branch-free), then special overrides
+    float f[19];
+    if (parity == 0) {
 #pragma unroll
-    for (int q = 0; q < 19; ++q) f[q] = Store<ST>::ld(pop + (size_t)q * (size_t)g.Ns + self, q);
-  } else {
+      for (int q = 0; q < 19; ++q) f[q] = Store<ST>::ld(pop + (size_t)q * (size_t)g.Ns + self, q);
+    } else {
 #pragma unroll
-    for (int q = 0; q < 19; ++q)
-      f[q] = Store<ST>::ld(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), OPP(q));
-  }
-  if (special && !plain) {
-    const uint32_t L = (uint32_t)m.x, Sm = (uint32_t)m.y, V = (uint32_t)m.z;
+      for (int q = 0; q < 19; ++q)
+        f[q] = Store<ST>::ld(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), OPP(q));
+    }
+    const uint32_t L = (uint32_t)m.x, Sm = (uint32_t)m.y, V = (uint32_t)m.z, F = (uint32_t)m.w;
+    if (special && !plain) {
+      if (staged) {
+        const int l0 = b.x - link_lo, v0 = b.y - v_lo;
 #pragma unroll
-    for (int q = 1; q < 19; ++q) {
-      if ((L >> q) & 1u) {
-        const int64_t l = b.x + __popc(L & ((1u << q) - 1u));
-        float v = t.A[l];
-        if (!first && ((V >> q) & 1u))
-          v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + b.y + __popc(V & ((1u << q) - 1u))];
-        f[q] = v;
-      } else if (!first && ((Sm >> q) & 1u)) {
-        f[q] = WQ(q);
+        for (int q = 1; q < 19; ++q) {
+          const uint32_t below = (1u << q) - 1u;
+          if ((L >> q) & 1u) {
+            const int li = l0 + __popc(L & below);
+            float v = sA[wv][li];
+            if (!first && ((V >> q) & 1u)) v = v + sW2[wv][li] * sBv[wv][v0 + __popc(V & below)];
+            f[q] = v;
+          } else if (!first && ((Sm >> q) & 1u)) {
+            f[q] = WQ(q);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+          if ((L >> q) & 1u) {
+            const int64_t l = b.x + __popc(L & ((1u << q) - 1u));
+            float v = t.A[l];
+            if (!first && ((V >> q) & 1u))
+              v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + b.y + __popc(V & ((1u << q) - 1u))];
+            f[q] = v;
+          } else if (!first && ((Sm >> q) & 1u)) {
+            f[q] = WQ(q);
+          }
+        }
+      }
+    }
+
+    float sigma = 0.f;
+    if (c.sponge) sigma = c.sponge[c.sponge_axis == 0 ? i : (c.sponge_axis == 1 ? j : k)];
+    float rho, ux, uy, uz;
+    collide_fast<KIND>(c, f, rho, ux, uy, uz, sigma);
+    if (rho_o) { rho_o[x] = rho; u_o[x] = ux; v_o[x] = uy; w_o[x] = uz; }
+
+    if (parity == 0) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + self, f[q], OPP(q));
+    } else {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + slot(CX(q), CY(q), CZ(q)), f[q], q);
+    }
+    // copy-self on non-periodic faces through the ghost layer
+    if ((g.gx && (i == 0 || i == g.nx - 1)) || (g.gy && (j == 0 || j == g.ny - 1)) ||
+        (g.gz && (k == 0 || k == g.nz - 1))) {
+#pragma unroll
+      for (int q = 1; q < 19; ++q) {
+        if (!oob_source(g, i, j, k, q)) continue;
+        if (parity == 0)
+          Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), f[q], OPP(q));
+        else
+          Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + self, f[q], q);
+      }
+    }
+
+    if (special) {
+      float* Bw = t.B + (int64_t)parity * t.n_v;
+      if (staged) {
+        int li = b.x - link_lo;
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+          if ((L >> q) & 1u) {
+            float a = sW1[wv][li] * f[OPP(q)];
+            if (!((V >> q) & 1u)) a = a + sW2[wv][li] * f[q];
+            sA[wv][li] = a;
+            ++li;
+          }
+        }
+        int e = b.z - f_lo;
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+          if ((F >> q) & 1u) {
+            Bw[sFF[wv][e]] = f[q];
+            ++e;
+          }
+        }
+      } else {
+        int64_t l = b.x;
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+          if ((L >> q) & 1u) {
+            float a = t.w1[l] * f[OPP(q)];
+            if (!((V >> q) & 1u)) a = a + t.w2[l] * f[q];
+            t.A[l] = a;
+            ++l;
+          }
+        }
+        int64_t e = b.z;
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+          if ((F >> q) & 1u) {
+            Bw[t.ff_idx[e]] = f[q];
+            ++e;
+          }
+        }
       }
     }
   }
-
-  float sigma = 0.f;
-  if (c.sponge) sigma = c.sponge[c.sponge_axis == 0 ? i : (c.sponge_axis == 1 ? j : k)];
-  float rho, ux, uy, uz;
-  collide_fast<KIND>(c, f, rho, ux, uy, uz, sigma);
-  if (rho_o) { rho_o[x] = rho; u_o[x] = ux; v_o[x] = uy; w_o[x] = uz; }
-
-  if (parity == 0) {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + self, f[q], OPP(q));
-  } else {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + slot(CX(q), CY(q), CZ(q)), f[q], q);
-  }
-  // copy-self on non-periodic faces through the ghost layer
-  if ((g.gx && (i == 0 || i == g.nx - 1)) || (g.gy && (j == 0 || j == g.ny - 1)) ||
-      (g.gz && (k == 0 || k == g.nz - 1))) {
-#pragma unroll
-    for (int q = 1; q < 19; ++q) {
-      if (!oob_source(g, i, j, k, q)) continue;
-      if (parity == 0)
-        Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), f[q], OPP(q));
-      else
-        Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + self, f[q], q);
-    }
-  }
-
-  if (special) {
-    const uint32_t L = (uint32_t)m.x, V = (uint32_t)m.z, F = (uint32_t)m.w;
-    int64_t l = b.x;
-#pragma unroll
-    for (int q = 1; q < 19; ++q) {
-      if ((L >> q) & 1u) {
-        float a = t.w1[l] * f[OPP(q)];
-        if (!((V >> q) & 1u)) a = a + t.w2[l] * f[q];
-        t.A[l] = a;
-        ++l;
-      }
-    }
-    int64_t e = b.z;
-#pragma unroll
-    for (int q = 1; q < 19; ++q) {
-      if ((F >> q) & 1u) {
-        t.B[(int64_t)parity * t.n_v + t.ff_idx[e]] = f[q];
-        ++e;
-      }
-    }
+  // staged own-node Bouzidi parts back to HBM, coalesced (wave-uniform branch)
+  if (staged) {
+    wave_lds_sync();
+    for (int i = lane; i < nL; i += 64) t.A[link_lo + i] = sA[wv][i];
   }
 }
 

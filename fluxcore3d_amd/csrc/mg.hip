@@ -25,6 +25,9 @@ using namespace fc;
 namespace {
 
 constexpr int kBlock = 256;
+// FP64 reductions run a capped grid (grid-stride loops): one partial per block,
+// folded by a single workgroup; the fixed grid keeps the summation order fixed.
+constexpr unsigned kMaxPartials = 2048;
 
 struct Level {
   int nx, ny, nz;
@@ -343,9 +346,8 @@ __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restr
                                                            const double* __restrict__ r, const float* __restrict__ mask,
                                                            Coef c, Level g, double dt, double* __restrict__ partial) {
   const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (idx < n) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
     int i, j, k;
     unflat(g, idx, i, j, k);
     const int64_t nynz = (int64_t)g.ny * g.nz;
@@ -361,8 +363,8 @@ __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restr
                      (double)c.aD[idx] * p[i * nynz + (int64_t)j * g.nz + kD];
     const double a = (pc + dt * L) * (double)mask[idx];
     Ap[idx] = a;
-    v[0] = pc * a;
-    v[1] = r ? pc * r[idx] : 0.0;
+    v[0] += pc * a;
+    v[1] += r ? pc * r[idx] : 0.0;
   }
   block_partials<2>(v, partial);
 }
@@ -373,15 +375,14 @@ __global__ __launch_bounds__(kBlock) void update_xr_kernel(double* __restrict__ 
                                                            const double* __restrict__ Ap,
                                                            const double* __restrict__ scal, int64_t n,
                                                            double* __restrict__ partial) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double v[1] = {0.0};
-  if (idx < n) {
-    const double alpha = scal[S_ALPHA];
+  const double alpha = scal[S_ALPHA];
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
     x[idx] += alpha * p[idx];
     const double rn = r[idx] - alpha * Ap[idx];
     r[idx] = rn;
     r32[idx] = (float)rn;
-    v[0] = rn * rn;
+    v[0] += rn * rn;
   }
   block_partials<1>(v, partial);
 }
@@ -389,9 +390,9 @@ __global__ __launch_bounds__(kBlock) void update_xr_kernel(double* __restrict__ 
 // partial of z.Ap_old (z is the FP32 V-cycle output)
 __global__ __launch_bounds__(kBlock) void dot_z_kernel(const float* __restrict__ z, const double* __restrict__ Ap_old,
                                                        int64_t n, double* __restrict__ partial) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double v[1] = {0.0};
-  if (idx < n) v[0] = (double)z[idx] * Ap_old[idx];
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock)
+    v[0] += (double)z[idx] * Ap_old[idx];
   block_partials<1>(v, partial);
 }
 
@@ -410,9 +411,8 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
                                                                const float* __restrict__ mask, Coef c, Level g,
                                                                double dt, double* __restrict__ partial) {
   const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (idx < n) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
     int i, j, k;
     unflat(g, idx, i, j, k);
     const int64_t nynz = (int64_t)g.ny * g.nz;
@@ -431,8 +431,8 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
     const double rr = (bb - (xc + dt * L)) * m;
     r[idx] = rr;
     r32[idx] = (float)rr;
-    v[0] = rr * rr;
-    v[1] = bb * bb;
+    v[0] += rr * rr;
+    v[1] += bb * bb;
   }
   block_partials<2>(v, partial);
 }
@@ -440,11 +440,10 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
 // partial of sum(x^2) for an FP32 vector, FP64 accumulation
 __global__ __launch_bounds__(kBlock) void norm2_f32_kernel(const float* __restrict__ x, int64_t n,
                                                            double* __restrict__ partial) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double v[1] = {0.0};
-  if (idx < n) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
     const double a = (double)x[idx];
-    v[0] = a * a;
+    v[0] += a * a;
   }
   block_partials<1>(v, partial);
 }
@@ -453,12 +452,11 @@ __global__ __launch_bounds__(kBlock) void norm2_f32_kernel(const float* __restri
 __global__ __launch_bounds__(kBlock) void residual_f32_kernel(const float* __restrict__ b, const float* __restrict__ Lo,
                                                               const float* __restrict__ mask, float* __restrict__ r,
                                                               int64_t n, double* __restrict__ partial) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double v[1] = {0.0};
-  if (idx < n) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
     const float a = mask ? (b[idx] - Lo[idx]) * mask[idx] : b[idx] - Lo[idx];
     r[idx] = a;
-    v[0] = (double)a * (double)a;
+    v[0] += (double)a * (double)a;
   }
   block_partials<1>(v, partial);
 }
@@ -581,7 +579,12 @@ int fc_mg_coarse_solve(float* x, float* tmp, const float* b, const fc_mg_coef* c
 }
 
 // ---- FP64 flexible-CG kernels -------------------------------------------------
-int fc_mg_partials_count(int64_t n) { return (int)grid_for(n, kBlock); }
+static unsigned reduce_grid(int64_t n) {
+  const unsigned g = grid_for(n, kBlock);
+  return g < kMaxPartials ? g : kMaxPartials;
+}
+
+int fc_mg_partials_count(int64_t n) { return (int)reduce_grid(n); }
 
 int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* mask, const fc_mg_coef* coef,
                     double dt, int nx, int ny, int nz, int ax, int out_idx, double* partial, double* scal,
@@ -590,7 +593,7 @@ int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* m
   if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
   FC_REQUIRE(p && Ap && mask && coef && partial && scal, "apply_A64: null pointer argument");
   const int64_t n = (int64_t)nx * ny * nz;
-  const unsigned gsz = grid_for(n, kBlock);
+  const unsigned gsz = reduce_grid(n);
   hipLaunchKernelGGL(apply_A64_kernel, dim3(gsz), dim3(kBlock), 0, stream, p, Ap, r, mask, to_coef(coef), g, dt,
                      partial);
   if (int rc = check_launch("apply_A64")) return rc;
@@ -602,7 +605,7 @@ int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* m
 int fc_mg_update_xr(double* x, double* r, float* r32, const double* p, const double* Ap, int64_t n, double* partial,
                     double* scal, hipStream_t stream) {
   FC_REQUIRE(x && r && r32 && p && Ap && partial && scal, "update_xr: null pointer argument");
-  const unsigned gsz = grid_for(n, kBlock);
+  const unsigned gsz = reduce_grid(n);
   hipLaunchKernelGGL(update_xr_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, r, r32, p, Ap, scal, n, partial);
   if (int rc = check_launch("update_xr")) return rc;
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, (int)S_RR, 0);
@@ -612,7 +615,7 @@ int fc_mg_update_xr(double* x, double* r, float* r32, const double* p, const dou
 int fc_mg_new_direction(double* p, const float* z, const double* Ap_old, int64_t n, int first, double* partial,
                         double* scal, hipStream_t stream) {
   FC_REQUIRE(p && z && partial && scal, "new_direction: null pointer argument");
-  const unsigned gsz = grid_for(n, kBlock);
+  const unsigned gsz = reduce_grid(n);
   if (!first) {
     FC_REQUIRE(Ap_old, "new_direction: Ap_old required after the first iteration");
     hipLaunchKernelGGL(dot_z_kernel, dim3(gsz), dim3(kBlock), 0, stream, z, Ap_old, n, partial);
@@ -620,7 +623,7 @@ int fc_mg_new_direction(double* p, const float* z, const double* Ap_old, int64_t
     hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, (int)S_ZAP, 2);
     if (int rc = check_launch("new_direction/fold")) return rc;
   }
-  hipLaunchKernelGGL(new_dir_kernel, dim3(gsz), dim3(kBlock), 0, stream, p, z, scal, n, first);
+  hipLaunchKernelGGL(new_dir_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, p, z, scal, n, first);
   return check_launch("new_direction");
 }
 
@@ -631,7 +634,7 @@ int fc_mg_true_residual(const double* x, const double* b, double* r, float* r32,
   if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
   FC_REQUIRE(x && b && r && r32 && mask && coef && partial && scal, "true_residual: null pointer argument");
   const int64_t n = (int64_t)nx * ny * nz;
-  const unsigned gsz = grid_for(n, kBlock);
+  const unsigned gsz = reduce_grid(n);
   hipLaunchKernelGGL(true_residual_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, b, r, r32, mask, to_coef(coef), g,
                      dt, partial);
   if (int rc = check_launch("true_residual")) return rc;
@@ -642,7 +645,7 @@ int fc_mg_true_residual(const double* x, const double* b, double* r, float* r32,
 int fc_mg_norm2_f32(const float* x, int64_t n, double* partial, double* scal, int slot, hipStream_t stream) {
   FC_REQUIRE(x && partial && scal && slot >= 0 && slot < S_N, "norm2_f32: bad arguments");
   if (n == 0) return 0;
-  const unsigned gsz = grid_for(n, kBlock);
+  const unsigned gsz = reduce_grid(n);
   hipLaunchKernelGGL(norm2_f32_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, n, partial);
   if (int rc = check_launch("norm2_f32")) return rc;
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, slot, 0);
@@ -653,7 +656,7 @@ int fc_mg_residual_f32(const float* b, const float* Lo, const float* mask, float
                        double* scal, int slot, hipStream_t stream) {
   FC_REQUIRE(b && Lo && r && partial && scal && slot >= 0 && slot < S_N, "residual_f32: bad arguments");
   if (n == 0) return 0;
-  const unsigned gsz = grid_for(n, kBlock);
+  const unsigned gsz = reduce_grid(n);
   hipLaunchKernelGGL(residual_f32_kernel, dim3(gsz), dim3(kBlock), 0, stream, b, Lo, mask, r, n, partial);
   if (int rc = check_launch("residual_f32")) return rc;
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, slot, 0);
