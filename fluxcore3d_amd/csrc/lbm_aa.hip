@@ -381,6 +381,7 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
   const int64_t x = (int64_t)lb * kBlock + threadIdx.x;
   const bool live = x < g.N;
   const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
+  const bool fluid = live && !(fl & F_SOLID);
   const bool special = (fl & F_SPECIAL) != 0;
   const uint64_t bal = __ballot(special);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -390,7 +391,41 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     m = t.sp_mask[sidx];
     b = t.sp_base[sidx];
   }
-  // wave-uniform ranges of this wave's records; every lane is still active here
+
+  // node coordinates and the 19 plain loads, issued before the link staging so
+  // that their latency overlaps it
+  int i = 0, j = 0, k = 0;
+  uint32_t self = 0;
+  float f[19];
+  const uint32_t PZ = (uint32_t)g.PZ, PYZ = (uint32_t)g.PY * (uint32_t)g.PZ;
+  int Im = 0, I0 = 0, Ip = 0, Jm = 0, J0 = 0, Jp = 0, Km = 0, K0 = 0, Kp = 0;
+  auto slot = [&](int dx, int dy, int dz) -> uint32_t {
+    const int I = dx < 0 ? Im : (dx > 0 ? Ip : I0);
+    const int J = dy < 0 ? Jm : (dy > 0 ? Jp : J0);
+    return (uint32_t)I * PYZ + (uint32_t)J * PZ + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
+  };
+  if (fluid) {
+    const uint32_t xu = (uint32_t)x;
+    const uint32_t nynz = (uint32_t)g.ny * (uint32_t)g.nz;
+    i = (int)(xu / nynz);
+    const uint32_t rem = xu - (uint32_t)i * nynz;
+    j = (int)(rem / (uint32_t)g.nz);
+    k = (int)(rem - (uint32_t)j * (uint32_t)g.nz);
+    Im = wrap_or_pad(i, -1, g.nx, g.gx); I0 = i + g.gx; Ip = wrap_or_pad(i, 1, g.nx, g.gx);
+    Jm = wrap_or_pad(j, -1, g.ny, g.gy); J0 = j + g.gy; Jp = wrap_or_pad(j, 1, g.ny, g.gy);
+    Km = wrap_or_pad(k, -1, g.nz, g.gz); K0 = k + g.gz; Kp = wrap_or_pad(k, 1, g.nz, g.gz);
+    self = (uint32_t)I0 * PYZ + (uint32_t)J0 * PZ + (uint32_t)K0;
+    if (parity == 0) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) f[q] = Store<ST>::ld(pop + (size_t)q * (size_t)g.Ns + self, q);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 19; ++q)
+        f[q] = Store<ST>::ld(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), OPP(q));
+    }
+  }
+
+  // wave-uniform ranges of this wave's records; every lane takes part
   int link_lo = 0, v_lo = 0, f_lo = 0, nL = 0;
   bool staged = false;
   if (bal) {
@@ -403,71 +438,40 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     const int nF = __builtin_amdgcn_readlane(b.z + __popc((uint32_t)m.w), ll) - f_lo;
     staged = nL <= kStage && nV <= kStage && nF <= kStage;
     if (staged) {
-      for (int i = lane; i < nL; i += 64) {
-        sA[wv][i] = t.A[link_lo + i];
-        sW1[wv][i] = t.w1[link_lo + i];
-        sW2[wv][i] = t.w2[link_lo + i];
+      for (int e = lane; e < nL; e += 64) {
+        sA[wv][e] = t.A[link_lo + e];
+        sW1[wv][e] = t.w1[link_lo + e];
+        sW2[wv][e] = t.w2[link_lo + e];
       }
       if (!first) {
         const float* Bp = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
-        for (int i = lane; i < nV; i += 64) sBv[wv][i] = Bp[i];
+        for (int e = lane; e < nV; e += 64) sBv[wv][e] = Bp[e];
       }
-      for (int i = lane; i < nF; i += 64) sFF[wv][i] = t.ff_idx[f_lo + i];
+      for (int e = lane; e < nF; e += 64) sFF[wv][e] = t.ff_idx[f_lo + e];
       wave_lds_sync();
     }
   }
 
-  if (live && (fl & F_SOLID)) {
+  if (live && !fluid) {
     if (rho_o) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
-  } else if (live) {
-    const uint32_t xu = (uint32_t)x;
-    const uint32_t nynz = (uint32_t)g.ny * (uint32_t)g.nz;
-    const int i = (int)(xu / nynz);
-    const uint32_t rem = xu - (uint32_t)i * nynz;
-    const int j = (int)(rem / (uint32_t)g.nz);
-    const int k = (int)(rem - (uint32_t)j * (uint32_t)g.nz);
+  } else if (fluid) {
     const bool plain = (fl >> F_BC_SHIFT) != 0;
-    const uint32_t PZ = (uint32_t)g.PZ, PYZ = (uint32_t)g.PY * (uint32_t)g.PZ;
-
-    // padded coordinates of x - 1, x, x + 1 along each axis
-    const int Im = wrap_or_pad(i, -1, g.nx, g.gx), I0 = i + g.gx, Ip = wrap_or_pad(i, 1, g.nx, g.gx);
-    const int Jm = wrap_or_pad(j, -1, g.ny, g.gy), J0 = j + g.gy, Jp = wrap_or_pad(j, 1, g.ny, g.gy);
-    const int Km = wrap_or_pad(k, -1, g.nz, g.gz), K0 = k + g.gz, Kp = wrap_or_pad(k, 1, g.nz, g.gz);
-    auto rowb = [&](int dx, int dy) -> uint32_t {
-      const int I = dx < 0 ? Im : (dx > 0 ? Ip : I0);
-      const int J = dy < 0 ? Jm : (dy > 0 ? Jp : J0);
-      return (uint32_t)I * PYZ + (uint32_t)J * PZ;
-    };
-    auto slot = [&](int dx, int dy, int dz) -> uint32_t {
-      return rowb(dx, dy) + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
-    };
-    const uint32_t self = (uint32_t)I0 * PYZ + (uint32_t)J0 * PZ + (uint32_t)K0;
-
-    // all 19 plain loads first (branch-free), then special overrides
-    float f[19];
-    if (parity == 0) {
-#pragma unroll
-      for (int q = 0; q < 19; ++q) f[q] = Store<ST>::ld(pop + (size_t)q * (size_t)g.Ns + self, q);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 19; ++q)
-        f[q] = Store<ST>::ld(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), OPP(q));
-    }
     const uint32_t L = (uint32_t)m.x, Sm = (uint32_t)m.y, V = (uint32_t)m.z, F = (uint32_t)m.w;
     if (special && !plain) {
       if (staged) {
+        // every lane reads (clamped indices), then selects: no waits inside branches
         const int l0 = b.x - link_lo, v0 = b.y - v_lo;
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
           const uint32_t below = (1u << q) - 1u;
-          if ((L >> q) & 1u) {
-            const int li = l0 + __popc(L & below);
-            float v = sA[wv][li];
-            if (!first && ((V >> q) & 1u)) v = v + sW2[wv][li] * sBv[wv][v0 + __popc(V & below)];
-            f[q] = v;
-          } else if (!first && ((Sm >> q) & 1u)) {
-            f[q] = WQ(q);
+          const int li = min(l0 + __popc(L & below), kStage - 1);
+          const int vi = min(v0 + __popc(V & below), kStage - 1);
+          float v = sA[wv][li];
+          if (!first) {
+            const float add = sW2[wv][li] * sBv[wv][vi];
+            v = ((V >> q) & 1u) ? v + add : v;
           }
+          f[q] = ((L >> q) & 1u) ? v : ((!first && ((Sm >> q) & 1u)) ? WQ(q) : f[q]);
         }
       } else {
 #pragma unroll
@@ -514,23 +518,21 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     if (special) {
       float* Bw = t.B + (int64_t)parity * t.n_v;
       if (staged) {
-        int li = b.x - link_lo;
+        const int l0 = b.x - link_lo, e0 = b.z - f_lo;
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
-          if ((L >> q) & 1u) {
-            float a = sW1[wv][li] * f[OPP(q)];
-            if (!((V >> q) & 1u)) a = a + sW2[wv][li] * f[q];
-            sA[wv][li] = a;
-            ++li;
-          }
+          const uint32_t below = (1u << q) - 1u;
+          const int li = min(l0 + __popc(L & below), kStage - 1);
+          const float w1v = sW1[wv][li], w2v = sW2[wv][li];
+          float a = w1v * f[OPP(q)];
+          if (!((V >> q) & 1u)) a = a + w2v * f[q];
+          if ((L >> q) & 1u) sA[wv][li] = a;
         }
-        int e = b.z - f_lo;
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
-          if ((F >> q) & 1u) {
-            Bw[sFF[wv][e]] = f[q];
-            ++e;
-          }
+          const int ei = min(e0 + __popc(F & ((1u << q) - 1u)), kStage - 1);
+          const int dst = sFF[wv][ei];
+          if ((F >> q) & 1u) Bw[dst] = f[q];
         }
       } else {
         int64_t l = b.x;
@@ -557,7 +559,7 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
   // staged own-node Bouzidi parts back to HBM, coalesced (wave-uniform branch)
   if (staged) {
     wave_lds_sync();
-    for (int i = lane; i < nL; i += 64) t.A[link_lo + i] = sA[wv][i];
+    for (int e = lane; e < nL; e += 64) t.A[link_lo + e] = sA[wv][e];
   }
 }
 

@@ -41,9 +41,13 @@ class FCGWorkspace:
 
 
 def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol: float = 1e-8, max_it: int = 200,
-             precond=None, ws: Optional[FCGWorkspace] = None, check_every: int = 1) -> Tuple[torch.Tensor, dict]:
+             precond=None, ws: Optional[FCGWorkspace] = None, check_every: int = 1,
+             on_iter=None) -> Tuple[torch.Tensor, dict]:
     """Flexible CG on A x = b (A = I + dt L of mg_solver's finest level, masked rows
-    excluded). ``precond(r32, z32)`` defaults to one V-cycle of mg_solver."""
+    excluded). ``precond(r32, z32)`` defaults to one V-cycle of mg_solver; it may
+    read ||r||^2 from ``ws.scal[S_RR]`` (device FP64). ``on_iter(it, ws)`` is
+    called before each preconditioner application (it = 0 for the initial
+    residual)."""
     lv0 = mg_solver._levels[0]
     shape = lv0["shape"]
     nx, ny, nz = shape
@@ -76,6 +80,8 @@ def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
     it = 0
     converged = hist[-1] < tol
     while not converged and it < max_it:
+        if on_iter is not None:
+            on_iter(it, ws)
         precond(ws.r32, ws.z32)
         _lib.call("fc_mg_new_direction", ws.p.data_ptr(), ws.z32.data_ptr(), ws.Ap.data_ptr(), ws.n, int(it == 0),
                   ws.partial.data_ptr(), ws.scal.data_ptr(), stream)

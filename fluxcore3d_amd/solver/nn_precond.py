@@ -1,0 +1,468 @@
+"""Learned preconditioner for the conduction system A x = b: a U-Net on
+channels-last BF16 activations, BF16 weights and FP32 accumulation, run by the
+MFMA implicit-GEMM convolutions of csrc/conv.hip, used inside flexible CG with
+FP64 outer vectors (mg_pcg.fcg_fp64) and a GMG-PCG fallback.
+
+There is no counterpart in the reference: its thermal solve stops at the
+multigrid-preconditioned Krylov methods (solver/mg_pcg.py:33-187). The
+network approximates the error of a residual, z = ||r|| N(r / ||r||, log k, mask),
+so one application costs a fixed number of convolutions regardless of the
+conductivity contrast.
+
+Architecture (levels l = 0..L-1, channels C_l, 2x average pooling between
+levels, nearest upsampling + skip concatenation on the way up, leaky ReLU
+with slope 0.1, no biases):
+
+  enc_l:  conv(C_{l-1} or 8 -> C_l), conv(C_l -> C_l)
+  dec_l:  conv(C_{l+1} + C_l -> C_l), conv(C_l -> C_l)      (l < L-1)
+  head:   conv(C_0 -> 8), linear, channel 0 is the output
+
+Training (``train``) runs on the GPU through the same kernels (forward, data
+gradient with flipped weights, weight gradient); element-wise work, pooling,
+upsampling and concatenation of the training graph are torch operations.
+Inference (``UNetPrecond.apply``) uses only HIP kernels: input packing,
+convolutions (decoder convolutions read the upsampled coarse tensor and the
+skip tensor directly), pooling and output scaling.
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from .. import _lib
+
+SLOPE = 0.1
+IN_CH = 8
+HEAD_CH = 8
+
+_lib.register({
+    "fc_conv_kpad": (_lib.I32, [_lib.I32]),
+    "fc_conv3d_fwd": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.P, _lib.I32, _lib.P, _lib.I32, _lib.I32,
+                                 _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.F32, _lib.P, _lib.P, _lib.P]),
+    "fc_conv3d_flip_weights": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.P, _lib.P]),
+    "fc_conv3d_dw_workspace_bytes": (_lib.SZ, [_lib.I32, _lib.I32, _lib.I32]),
+    "fc_conv3d_bwd_weight": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.P, _lib.I32, _lib.P, _lib.I32, _lib.I32,
+                                        _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.P, _lib.P, _lib.SZ, _lib.P]),
+    "fc_unet_pack_input": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P, _lib.P]),
+    "fc_unet_avgpool2": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.P, _lib.P]),
+    "fc_unet_unpack": (_lib.I32, [_lib.P, _lib.I32, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P, _lib.P]),
+}, unchecked={"fc_conv_kpad", "fc_conv3d_dw_workspace_bytes"})
+
+
+def _stream(dev) -> int:
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+def _ptr(t: Optional[torch.Tensor]) -> int:
+    return 0 if t is None else t.data_ptr()
+
+
+def kpad(cin: int) -> int:
+    return int(_lib.call("fc_conv_kpad", cin))
+
+
+def pack_weights(w: torch.Tensor) -> torch.Tensor:
+    """[cout, 27, cin] FP32 -> BF16 [cout, kpad(cin)] with zero padding."""
+    cout, _, cin = w.shape
+    out = torch.zeros(cout, kpad(cin), device=w.device, dtype=torch.bfloat16)
+    out[:, :27 * cin] = w.detach().reshape(cout, 27 * cin).to(torch.bfloat16)
+    return out
+
+
+def conv3d(x0: torch.Tensor, wpk: torch.Tensor, cout: int, act: bool, x1: Optional[torch.Tensor] = None,
+           up0: bool = False, out_f32: bool = False, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """3x3x3 convolution, zero padding. x0 [B, D', H', W', c0] BF16 (half-size
+    when up0), x1 optional skip [B, D, H, W, c1] BF16; returns [B, D, H, W, cout]."""
+    if x0.dtype != torch.bfloat16 or not x0.is_contiguous():
+        raise ValueError("conv3d: x0 must be contiguous bfloat16")
+    B = x0.shape[0]
+    D, H, W = x0.shape[1:4]
+    if up0:
+        D, H, W = 2 * D, 2 * H, 2 * W
+    c0 = x0.shape[4]
+    c1 = 0
+    if x1 is not None:
+        if x1.dtype != torch.bfloat16 or not x1.is_contiguous() or tuple(x1.shape[:4]) != (B, D, H, W):
+            raise ValueError("conv3d: x1 must be contiguous bfloat16 of the output extents")
+        c1 = x1.shape[4]
+    if wpk.shape != (cout, kpad(c0 + c1)):
+        raise ValueError(f"conv3d: packed weights {tuple(wpk.shape)} do not match ({cout}, {kpad(c0 + c1)})")
+    if out is None:
+        out = torch.empty((B, D, H, W, cout), device=x0.device, dtype=torch.float32 if out_f32 else torch.bfloat16)
+    f32 = out.dtype == torch.float32
+    _lib.call("fc_conv3d_fwd", x0.data_ptr(), c0, int(up0), _ptr(x1), c1, wpk.data_ptr(), cout, B, D, H, W,
+              int(act), SLOPE, 0 if f32 else out.data_ptr(), out.data_ptr() if f32 else 0, _stream(x0.device))
+    return out
+
+
+def conv3d_weight_grad(x0: torch.Tensor, dy16: torch.Tensor, cout: int, n_chunks: int = 0) -> torch.Tensor:
+    """dW [cout, 27, cin] (FP32) for a single-source convolution."""
+    B, D, H, W, cin = x0.shape
+    if n_chunks <= 0:
+        n_chunks = max(1, min(256, (B * D * H) // 8))
+    ws_bytes = int(_lib.call("fc_conv3d_dw_workspace_bytes", cout, cin, n_chunks))
+    ws = torch.empty(ws_bytes // 4 + 1, device=x0.device, dtype=torch.float32)
+    dw = torch.empty((cout, 27, cin), device=x0.device, dtype=torch.float32)
+    _lib.call("fc_conv3d_bwd_weight", x0.data_ptr(), cin, 0, 0, 0, dy16.data_ptr(), cout, B, D, H, W, n_chunks,
+              dw.data_ptr(), ws.data_ptr(), ws_bytes, _stream(x0.device))
+    return dw
+
+
+def conv3d_data_grad(dy16: torch.Tensor, wpk: torch.Tensor, cin: int) -> torch.Tensor:
+    """dX [B, D, H, W, cin] (FP32): the forward kernel on flipped, transposed weights."""
+    cout = dy16.shape[4]
+    wf = torch.empty((cin, kpad(cout)), device=dy16.device, dtype=torch.bfloat16)
+    _lib.call("fc_conv3d_flip_weights", wpk.data_ptr(), cout, cin, wf.data_ptr(), _stream(dy16.device))
+    return conv3d(dy16, wf, cin, act=False, out_f32=True)
+
+
+class _ConvFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, w, act, out_f32, need_dx):
+        cout, _, cin = w.shape
+        wpk = pack_weights(w)
+        y = conv3d(x, wpk, cout, act, out_f32=out_f32)
+        ctx.save_for_backward(x, wpk, y if act else None)
+        ctx.act, ctx.cout, ctx.cin, ctx.need_dx = act, cout, cin, need_dx
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        x, wpk, y = ctx.saved_tensors
+        g = gy.float()
+        if ctx.act:
+            g = torch.where(y.float() > 0, g, g * SLOPE)
+        g16 = g.to(torch.bfloat16).contiguous()
+        dw = conv3d_weight_grad(x, g16, ctx.cout)
+        dx = conv3d_data_grad(g16, wpk, ctx.cin).to(torch.bfloat16) if ctx.need_dx else None
+        return dx, dw, None, None, None
+
+
+def _pool(x: torch.Tensor) -> torch.Tensor:
+    B, D, H, W, C = x.shape
+    return x.float().view(B, D // 2, 2, H // 2, 2, W // 2, 2, C).mean((2, 4, 6)).to(torch.bfloat16)
+
+
+def _up(x: torch.Tensor) -> torch.Tensor:
+    B, D, H, W, C = x.shape
+    return x[:, :, None, :, None, :, None, :].expand(B, D, 2, H, 2, W, 2, C).reshape(B, 2 * D, 2 * H, 2 * W, C)
+
+
+class UNetPrecond:
+    """U-Net preconditioner. ``channels`` per level (multiples of 8, <= 64)."""
+
+    def __init__(self, channels: Sequence[int] = (16, 24, 32, 32, 32), device="cuda", seed: int = 0):
+        self.channels = [int(c) for c in channels]
+        for c in self.channels:
+            if c % 8 or c > 64:
+                raise ValueError("channels must be multiples of 8 and at most 64")
+        self.device = torch.device(device)
+        self.levels = len(self.channels)
+        self.layers = self._layout()
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        self.params: List[torch.Tensor] = []
+        for name, cin, cout, act in self.layers:
+            std = math.sqrt(2.0 / (27 * cin)) if act else math.sqrt(1.0 / (27 * cin))
+            w = torch.randn((cout, 27, cin), generator=g) * std
+            self.params.append(w.to(self.device).requires_grad_(True))
+        self.feat_scale = 0.25  # log10(k) scaling of the second input channel
+        self._packed: Optional[List[torch.Tensor]] = None
+        self._bufs: Dict[Tuple[int, int, int], dict] = {}
+
+    def _layout(self):
+        C = self.channels
+        layers = []
+        for l in range(self.levels):
+            cin = IN_CH if l == 0 else C[l - 1]
+            layers.append((f"enc{l}a", cin, C[l], True))
+            layers.append((f"enc{l}b", C[l], C[l], True))
+        for l in range(self.levels - 2, -1, -1):
+            layers.append((f"dec{l}a", C[l + 1] + C[l], C[l], True))
+            layers.append((f"dec{l}b", C[l], C[l], True))
+        layers.append(("head", C[0], HEAD_CH, False))
+        return layers
+
+    def n_params(self) -> int:
+        return int(sum(p.numel() for p in self.params))
+
+    # ------------------------------------------------------------ training graph
+    def forward_train(self, x: torch.Tensor) -> torch.Tensor:
+        """x [B, D, H, W, 8] BF16 -> [B, D, H, W, 8] FP32 (channel 0 is the output)."""
+        p = iter(self.params)
+        skips = []
+        h = x
+        for l in range(self.levels):
+            if l > 0:
+                h = _pool(h)
+            h = _ConvFn.apply(h, next(p), True, False, l > 0)
+            h = _ConvFn.apply(h, next(p), True, False, True)
+            skips.append(h)
+        for l in range(self.levels - 2, -1, -1):
+            h = torch.cat([_up(h), skips[l]], dim=-1).contiguous()
+            h = _ConvFn.apply(h, next(p), True, False, True)
+            h = _ConvFn.apply(h, next(p), True, False, True)
+        return _ConvFn.apply(h, next(p), False, True, True)
+
+    # ------------------------------------------------------------------ inference
+    def refresh(self) -> None:
+        """Re-pack the BF16 weights after a parameter update."""
+        self._packed = [pack_weights(w) for w in self.params]
+
+    def _buffers(self, shape) -> dict:
+        key = tuple(int(s) for s in shape)
+        if key not in self._bufs:
+            D, H, W = key
+            f = 2 ** (self.levels - 1)
+            if D % f or H % f or W % f:
+                raise ValueError(f"grid {key} must be divisible by {f} for {self.levels} levels")
+            bf = dict(device=self.device, dtype=torch.bfloat16)
+            b = {"x": torch.empty((1, D, H, W, IN_CH), **bf), "enc": [], "tmp": [], "pool": [], "dec": []}
+            for l, c in enumerate(self.channels):
+                s = (1, D >> l, H >> l, W >> l)
+                b["tmp"].append(torch.empty(s + (c,), **bf))
+                b["enc"].append(torch.empty(s + (c,), **bf))
+                b["dec"].append(torch.empty(s + (c,), **bf))
+                if l > 0:
+                    b["pool"].append(torch.empty(s + (self.channels[l - 1],), **bf))
+            b["head"] = torch.empty((1, D, H, W, HEAD_CH), device=self.device, dtype=torch.float32)
+            self._bufs[key] = b
+        return self._bufs[key]
+
+    def apply(self, r32: torch.Tensor, z32: torch.Tensor, feat: Optional[torch.Tensor], mask: Optional[torch.Tensor],
+              scal: torch.Tensor, slot: int, shape) -> None:
+        """z = ||r|| N(r / ||r||, feat, mask) * mask with ||r||^2 = scal[slot] (FP64, device)."""
+        if self._packed is None:
+            self.refresh()
+        D, H, W = shape
+        n = D * H * W
+        b = self._buffers(shape)
+        st = _stream(self.device)
+        _lib.call("fc_unet_pack_input", r32.data_ptr(), _ptr(feat), _ptr(mask), scal.data_ptr(), int(slot), n,
+                  b["x"].data_ptr(), st)
+        wp = iter(self._packed)
+        C = self.channels
+        h = b["x"]
+        for l in range(self.levels):
+            if l > 0:
+                src = b["enc"][l - 1]
+                _lib.call("fc_unet_avgpool2", src.data_ptr(), 1, D >> (l - 1), H >> (l - 1), W >> (l - 1), C[l - 1],
+                          b["pool"][l - 1].data_ptr(), st)
+                h = b["pool"][l - 1]
+            conv3d(h, next(wp), C[l], True, out=b["tmp"][l])
+            conv3d(b["tmp"][l], next(wp), C[l], True, out=b["enc"][l])
+        h = b["enc"][self.levels - 1]
+        for l in range(self.levels - 2, -1, -1):
+            conv3d(h, next(wp), C[l], True, x1=b["enc"][l], up0=True, out=b["tmp"][l])
+            conv3d(b["tmp"][l], next(wp), C[l], True, out=b["dec"][l])
+            h = b["dec"][l]
+        conv3d(h, next(wp), HEAD_CH, False, out=b["head"])
+        _lib.call("fc_unet_unpack", b["head"].data_ptr(), HEAD_CH, _ptr(mask), scal.data_ptr(), int(slot), n,
+                  z32.data_ptr(), st)
+
+    # --------------------------------------------------------------- persistence
+    def state(self) -> Dict[str, np.ndarray]:
+        out = {"channels": np.asarray(self.channels, np.int32), "feat_scale": np.asarray([self.feat_scale])}
+        for (name, *_), w in zip(self.layers, self.params):
+            out[name] = w.detach().float().cpu().numpy()
+        return out
+
+    def save(self, path: str) -> None:
+        np.savez_compressed(path, **self.state())
+
+    @classmethod
+    def load(cls, path: str, device="cuda") -> "UNetPrecond":
+        with np.load(path, allow_pickle=False) as z:
+            net = cls(tuple(int(c) for c in z["channels"]), device=device)
+            net.feat_scale = float(z["feat_scale"][0])
+            for (name, *_), w in zip(net.layers, net.params):
+                if tuple(z[name].shape) != tuple(w.shape):
+                    raise ValueError(f"{path}: layer {name} has shape {z[name].shape}, expected {tuple(w.shape)}")
+                with torch.no_grad():
+                    w.copy_(torch.from_numpy(z[name]).to(device))
+        net.refresh()
+        return net
+
+
+def log_k_feature(k: torch.Tensor, scale: float = 0.25) -> torch.Tensor:
+    """Second input channel: scale * log10(k), FP32, flattened."""
+    return (torch.log10(k.float().clamp_min(1e-30)) * scale).reshape(-1).contiguous()
+
+
+# ------------------------------------------------------------------ training data
+def _mg_for(c, device):
+    from .nlevel_mg_cuda import NLevelGeometricMGSolver
+
+    mg = NLevelGeometricMGSolver(device, torch.float32, c["ax"], c["sign"], c["s"], omega=0.8, n_pre=2, n_post=2,
+                                 n_coarse=20)
+    mg.build_if_needed(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], c["dt"])
+    return mg
+
+
+def harvest_pairs(n: int = 128, problems: int = 6, seed0: int = 1000, snapshots=(0, 1, 2, 3, 5, 8, 12, 20, 30),
+                  smooth_per_problem: int = 4, device="cuda", feat_scale: float = 0.25):
+    """(input, target, mask) samples at n^3 on smooth-noise blob problems with
+    k_s / k_f log-uniform in [1, 1e4]:
+
+    * GMG-FCG residuals r_k at the listed iterations with their errors
+      e_k = x* - x_k (x* from an FCG solve to 1e-11);
+    * smooth random errors e (Gaussian-filtered noise at random scales) with
+      r = A e (FP64 operator of the finest level).
+
+    Inputs are [n, n, n, 8] BF16 (r / ||r||, feat_scale log10 k, mask); targets
+    e / ||r|| in FP32."""
+    from .. import geometry
+    from .mg_pcg import FCGWorkspace, S_RR, fcg_fp64
+
+    samples = []
+    gen = torch.Generator(device="cpu").manual_seed(seed0)
+    for i in range(problems):
+        c = geometry.conduction_problem(n, seed=seed0 + i, device=device)
+        mg = _mg_for(c, device)
+        lv0 = mg._levels[0]
+        mask = lv0["mask_f"].reshape(-1)
+        feat = log_k_feature(c["k"], feat_scale)
+        ws = FCGWorkspace(c["k"].shape, device)
+        _, info = fcg_fp64(mg, c["b"], tol=1e-11, max_it=800, ws=ws)
+        xs = ws.x.clone()
+        snaps = set(snapshots)
+
+        def grab(it, w):
+            if it in snaps:
+                r = w.r.clone()
+                e = (xs - w.x) * mask.double()
+                samples.append(_sample(r, e, feat, mask, n))
+
+        fcg_fp64(mg, c["b"], tol=1e-11, max_it=max(snapshots) + 1, ws=FCGWorkspace(c["k"].shape, device),
+                 on_iter=grab)
+        for _ in range(smooth_per_problem):
+            e = _smooth_noise(n, gen, device).reshape(-1).double() * mask.double()
+            r = apply_A64(mg, e)
+            samples.append(_sample(r, e, feat, mask, n))
+        del mg, ws
+    return samples
+
+
+def _sample(r64, e64, feat, mask, n):
+    rn = float(r64.norm())
+    x = torch.zeros((n, n, n, IN_CH), device=r64.device, dtype=torch.float32)
+    x[..., 0] = (r64 / rn).float().view(n, n, n)
+    x[..., 1] = feat.view(n, n, n)
+    x[..., 2] = mask.view(n, n, n)
+    return x.to(torch.bfloat16), (e64 / rn).float().view(n, n, n), mask.view(n, n, n)
+
+
+def _smooth_noise(n, gen, device):
+    sigma = float(2.0 ** torch.empty(1).uniform_(0.0, 5.0, generator=gen))
+    noise = torch.randn((n, n, n), generator=gen).to(device)
+    k = torch.fft.fftfreq(n, device=device)
+    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2
+    f = torch.fft.ifftn(torch.fft.fftn(noise) * torch.exp(-2 * (math.pi * sigma) ** 2 * k2)).real
+    return f / (f.std() + 1e-30)
+
+
+def apply_A64(mg, e64: torch.Tensor) -> torch.Tensor:
+    """A e on the finest level (FP64 vectors, FP32 coefficients), masked."""
+    from .mg_pcg import FCGWorkspace
+
+    lv0 = mg._levels[0]
+    nx, ny, nz = lv0["shape"]
+    ws = FCGWorkspace(lv0["shape"], mg.device)
+    out = torch.empty_like(e64)
+    _lib.call("fc_mg_apply_A64", e64.data_ptr(), out.data_ptr(), 0, lv0["mask_f"].data_ptr(), lv0["coef"],
+              float(mg._dt_cache), nx, ny, nz, mg.ax, lv0["out_idx"], ws.partial.data_ptr(), ws.scal.data_ptr(),
+              _stream(mg.device))
+    return out
+
+
+def train(net: UNetPrecond, samples, steps: int = 600, batch: int = 2, lr: float = 2e-3, crop: int = 0,
+          seed: int = 0, log_every: int = 50, log=print):
+    """Adam on the relative error loss ||N(r) - e||^2 / ||e||^2 (masked)."""
+    opt = torch.optim.Adam(net.params, lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=steps, eta_min=lr * 0.05)
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    hist = []
+    for step in range(steps):
+        idx = torch.randint(len(samples), (batch,), generator=gen).tolist()
+        xs, es, ms = [], [], []
+        for i in idx:
+            x, e, m = samples[i]
+            if crop:
+                n = x.shape[0]
+                o = torch.randint(0, n - crop + 1, (3,), generator=gen).tolist()
+                sl = (slice(o[0], o[0] + crop), slice(o[1], o[1] + crop), slice(o[2], o[2] + crop))
+                x, e, m = x[sl], e[sl], m[sl]
+            xs.append(x)
+            es.append(e)
+            ms.append(m)
+        x = torch.stack(xs).contiguous()
+        e = torch.stack(es)
+        m = torch.stack(ms)
+        y = net.forward_train(x)[..., 0]
+        num = (((y - e) * m) ** 2).flatten(1).sum(1)
+        den = ((e * m) ** 2).flatten(1).sum(1) + 1e-30
+        loss = (num / den).mean()
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        sched.step()
+        hist.append(float(loss))
+        if log and (step % log_every == 0 or step == steps - 1):
+            log(f"[train] step {step} loss {hist[-1]:.4f}")
+    net.refresh()
+    return hist
+
+
+# ---------------------------------------------------------------- solver driver
+def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 1e-8, max_it: int = 300,
+           x0: Optional[torch.Tensor] = None, stall_window: int = 20, stall_factor: float = 0.5, ws=None,
+           check_every: int = 1):
+    """Flexible CG with the U-Net preconditioner; when the relative residual has
+    not dropped by ``stall_factor`` over ``stall_window`` iterations (or the
+    iteration budget ends unconverged) the solve continues from the current
+    iterate with GMG-PCG. Returns (x, info) with info["fallback"]."""
+    from .mg_pcg import FCGWorkspace, S_RR, fcg_fp64
+
+    lv0 = mg._levels[0]
+    shape = lv0["shape"]
+    if ws is None:
+        ws = FCGWorkspace(shape, mg.device)
+    feat = log_k_feature(k, net.feat_scale)
+    mask = lv0["mask_f"].reshape(-1)
+    hist_it = []
+
+    class _Stall(Exception):
+        pass
+
+    def precond(r32, z32):
+        net.apply(r32, z32, feat, mask, ws.scal, S_RR, shape)
+
+    def watch(it, w):
+        if check_every > 1 and it % check_every:
+            return
+        hist_it.append(float(w.scal[S_RR].item()) ** 0.5)
+        if len(hist_it) > stall_window and hist_it[-1] > stall_factor * hist_it[-1 - stall_window]:
+            raise _Stall()
+
+    stalled = False
+    try:
+        x, info = fcg_fp64(mg, b, x0=x0, tol=tol, max_it=max_it, precond=precond, ws=ws, on_iter=watch,
+                           check_every=check_every)
+        it_nn = info["it"]
+    except _Stall:
+        stalled = True
+        it_nn = len(hist_it) * check_every
+        x = ws.x.view(shape).to(torch.float32)
+        info = {"converged": False}
+    if info["converged"]:
+        info["fallback"] = False
+        info["it_nn"] = it_nn
+        return x, info
+    x0_fb = ws.x.view(shape).clone()
+    x2, info2 = fcg_fp64(mg, b, x0=x0_fb, tol=tol, max_it=max_it, ws=FCGWorkspace(shape, mg.device))
+    info2["fallback"] = True
+    info2["it_nn"] = it_nn
+    info2["stalled"] = stalled
+    return x2, info2

@@ -1,0 +1,87 @@
+"""Train the U-Net preconditioner on the GPU and compare NN-FCG with GMG-PCG.
+
+    python -m fluxcore3d_amd.solver.train_unet --out fluxcore3d_amd/data/unet_precond.npz
+
+Training pairs come from smooth-noise blob conduction problems at 128^3 with
+k_s / k_f log-uniform in [1, 1e4] (GMG-FCG residual / error snapshots plus
+smooth random errors); evaluation uses problems with seeds outside the
+training range at 128^3 and 256^3, FP32 V-cycle / FP64 outer, tolerance 1e-8.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+import torch
+
+
+def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None) -> dict:
+    from .. import geometry
+    from .mg_pcg import FCGWorkspace, fcg_fp64
+    from .nn_precond import _mg_for, nn_fcg
+
+    c = geometry.conduction_problem(n, seed=seed, ratio=ratio, device=device)
+    mg = _mg_for(c, device)
+    ws = FCGWorkspace(c["k"].shape, device)
+    fcg_fp64(mg, c["b"], tol=tol, max_it=2, ws=ws)
+    nn_fcg(mg, net, c["b"], c["k"], tol=tol, max_it=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, gi = fcg_fp64(mg, c["b"], tol=tol, max_it=1000, ws=ws)
+    torch.cuda.synchronize()
+    t_gmg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, ni = nn_fcg(mg, net, c["b"], c["k"], tol=tol, max_it=1000)
+    torch.cuda.synchronize()
+    t_nn = time.perf_counter() - t0
+    return {"n": n, "seed": seed, "ratio": round(float(c["ratio"]), 2), "gmg_s": round(t_gmg, 4),
+            "gmg_it": gi["it"], "gmg_res": gi["res_rel"], "nn_s": round(t_nn, 4), "nn_it": ni["it"],
+            "nn_it_network": ni.get("it_nn"), "nn_fallback": ni["fallback"], "nn_res": ni["res_rel"],
+            "nn_converged": bool(ni["converged"])}
+
+
+def main(argv=None):
+    from .nn_precond import UNetPrecond, harvest_pairs, train
+
+    p = argparse.ArgumentParser()
+    p.add_argument("--out", default=os.path.join(os.path.dirname(os.path.dirname(__file__)), "data",
+                                                 "unet_precond.npz"))
+    p.add_argument("--n", type=int, default=128)
+    p.add_argument("--problems", type=int, default=6)
+    p.add_argument("--steps", type=int, default=600)
+    p.add_argument("--batch", type=int, default=2)
+    p.add_argument("--crop", type=int, default=64)
+    p.add_argument("--lr", type=float, default=2e-3)
+    p.add_argument("--channels", default="8,16,24,32,32")
+    p.add_argument("--eval-n", default="128,256")
+    p.add_argument("--seed", type=int, default=0)
+    a = p.parse_args(argv)
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    samples = harvest_pairs(n=a.n, problems=a.problems, device=dev)
+    t_h = time.perf_counter() - t0
+    print(f"[train_unet] {len(samples)} samples at {a.n}^3 in {t_h:.1f} s", flush=True)
+    net = UNetPrecond(tuple(int(c) for c in a.channels.split(",")), device=dev, seed=a.seed)
+    print(f"[train_unet] {net.n_params()} parameters", flush=True)
+    t0 = time.perf_counter()
+    hist = train(net, samples, steps=a.steps, batch=a.batch, lr=a.lr, crop=a.crop,
+                 log=lambda s: print(s, flush=True))
+    torch.cuda.synchronize()
+    t_t = time.perf_counter() - t0
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    net.save(a.out)
+    del samples
+    torch.cuda.empty_cache()
+    res = []
+    for i, n in enumerate(int(v) for v in a.eval_n.split(",")):
+        r = evaluate(net, n, seed=5000 + i, device=dev)
+        print(json.dumps(r), flush=True)
+        res.append(r)
+    print(json.dumps({"train_s": round(t_t, 1), "harvest_s": round(t_h, 1), "final_loss": hist[-1],
+                      "params": net.n_params(), "eval": res}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

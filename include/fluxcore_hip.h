@@ -195,6 +195,33 @@ size_t fc_thermal_check_workspace_bytes(int64_t n);
 int fc_thermal_physics_check(const float *T, double T_min, double T_max, int64_t n, int *flag, void *workspace,
                              size_t workspace_bytes, hipStream_t stream);
 
+/* ---- learned preconditioner (csrc/conv.hip, csrc/unet.hip) ----
+ * No reference counterpart: the reference's thermal solve ends at the
+ * multigrid-preconditioned Krylov methods (solver/mg_pcg.py:33-187). These
+ * kernels run the U-Net preconditioner used inside flexible CG.
+ * Activations are channels-last BF16 [B, D, H, W, C] (C a multiple of 8);
+ * weights BF16 [cout][fc_conv_kpad(cin)], k = tap * cin + ci with
+ * tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero padded; FP32 accumulation. */
+int fc_conv_kpad(int cin);
+/* y = act(conv3x3x3(concat(up0 ? nearest_up2(x0) : x0, x1))), zero padding;
+ * act 1 = leaky ReLU(slope); output BF16 (y_bf16) and/or FP32 (y_f32). */
+int fc_conv3d_fwd(const void *x0, int c0, int up0, const void *x1, int c1, const void *w, int cout, int B, int D,
+                  int H, int W, int act, float slope, void *y_bf16, float *y_f32, hipStream_t stream);
+/* weights of the data gradient: w_flip[ci][t*cout + co] = w[co][(26 - t)*cin + ci] */
+int fc_conv3d_flip_weights(const void *w, int cout, int cin, void *w_flip, hipStream_t stream);
+size_t fc_conv3d_dw_workspace_bytes(int cout, int cin, int n_chunks);
+/* dw[co][t][ci] (FP32) = sum over positions of dy[p, co] * x[p + o_t, ci] */
+int fc_conv3d_bwd_weight(const void *x0, int c0, int up0, const void *x1, int c1, const void *dy, int cout, int B,
+                         int D, int H, int W, int n_chunks, float *dw, void *workspace, size_t ws_bytes,
+                         hipStream_t stream);
+/* x[p] = {r[p] / sqrt(scal[slot]), feat1[p], mask[p], 0 x 5} (BF16, 8 channels) */
+int fc_unet_pack_input(const float *r, const float *feat1, const float *mask, const double *scal, int slot,
+                       int64_t n, void *out, hipStream_t stream);
+int fc_unet_avgpool2(const void *in, int B, int D, int H, int W, int C, void *out, hipStream_t stream);
+/* z[p] = sqrt(scal[slot]) * y[p * cy] * mask[p] */
+int fc_unet_unpack(const float *y, int cy, const float *mask, const double *scal, int slot, int64_t n, float *z,
+                   hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
