@@ -41,12 +41,15 @@ def parse():
     p.add_argument("--macro-every-step", type=int, default=1)
     p.add_argument("--thermal-n", type=int, default=256)
     p.add_argument("--no-thermal", action="store_true")
+    p.add_argument("--unet-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "fluxcore3d_amd",
+                                                          "data", "unet_precond.npz"))
     return p.parse_args()
 
 
-def thermal_leg(n, device, seed=0):
-    """GMG-preconditioned flexible CG (FP32 V-cycle, FP64 outer loop) to an FP64
-    relative residual of 1e-8 on the config-5 blob conduction problem."""
+def thermal_leg(n, device, seed=0, weights=None):
+    """GMG-preconditioned flexible CG (FP32 V-cycle, FP64 outer loop) and flexible
+    CG with the U-Net preconditioner (GMG-PCG fallback), both to an FP64 relative
+    residual of 1e-8, on the same 256^3 blob conduction problem."""
     import torch
 
     from fluxcore3d_amd import geometry
@@ -64,11 +67,28 @@ def thermal_leg(n, device, seed=0):
     x, info = fcg_fp64(mg, c["b"], tol=1e-8, max_it=500, ws=ws)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
-    return {"problem": f"{n}^3 smooth-noise blobs (seed {seed}), k_s/k_f = {c['ratio']:.3g}, dt = {c['dt']:g}",
-            "levels": [list(lv["shape"]) for lv in mg._levels], "gmg_pcg_s": round(secs, 4),
-            "gmg_pcg_iterations": info["it"], "final_rel_residual_fp64": info["res_rel"],
-            "converged": bool(info["converged"]), "nn_pcg_s": None,
-            "note": "NN-preconditioned FCG not built yet"}
+    out = {"problem": f"{n}^3 smooth-noise blobs (seed {seed}), k_s/k_f = {c['ratio']:.3g}, dt = {c['dt']:g}",
+           "levels": [list(lv["shape"]) for lv in mg._levels], "gmg_pcg_s": round(secs, 4),
+           "gmg_pcg_iterations": info["it"], "final_rel_residual_fp64": info["res_rel"],
+           "converged": bool(info["converged"]), "nn_pcg_s": None}
+    if weights is None or not os.path.exists(weights):
+        out["note"] = "no trained U-Net weights found; NN-PCG not run"
+        return out
+    from fluxcore3d_amd.solver.nn_precond import UNetPrecond, nn_fcg
+
+    net = UNetPrecond.load(weights, device=device)
+    nn_fcg(mg, net, c["b"], c["k"], tol=1e-8, max_it=2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xn, ni = nn_fcg(mg, net, c["b"], c["k"], tol=1e-8, max_it=500)
+    torch.cuda.synchronize()
+    nn_secs = time.perf_counter() - t0
+    out.update({"nn_pcg_s": round(nn_secs, 4), "nn_pcg_iterations": ni["it"],
+                "nn_iterations_with_network": ni.get("it_nn"), "nn_fallback_to_gmg": bool(ni["fallback"]),
+                "nn_final_rel_residual_fp64": ni["res_rel"], "nn_converged": bool(ni["converged"]),
+                "nn_weights": os.path.relpath(weights, os.path.dirname(os.path.abspath(__file__))),
+                "nn_vs_gmg_speedup": round(secs / max(nn_secs, 1e-9), 3)})
+    return out
 
 
 def build_workload(n, seed, collision, storage, device):
@@ -158,7 +178,7 @@ def main():
     ux_mean = float(macro[1][torch.as_tensor(~solid.reshape(-1), device=dev)].mean())
     thermal = None
     if not args.no_thermal:
-        thermal = thermal_leg(args.thermal_n, dev)
+        thermal = thermal_leg(args.thermal_n, dev, weights=args.unet_weights)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(solid, links, S, omega, args.collision, args.cpu_steps)

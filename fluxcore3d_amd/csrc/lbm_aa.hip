@@ -55,6 +55,7 @@ struct Grid {
 struct Tables {
   const uint8_t* nflag;      // [N]
   const int32_t* wave_base;  // [ceil(N/64)] special nodes before each 64-node group
+  const int4* wave_meta;     // [ceil(N/64) + 1] {first special, first link, first second-node link, first publish}
   const int4* sp_mask;       // [n_special] {L, S, V, F}
   const int4* sp_base;       // [n_special] {link_base, v_base, ff_base, 0}
   const float* w1;           // [n_links]
@@ -379,18 +380,24 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
   __shared__ int32_t sFF[kWaves][kStage];
   const unsigned lb = logical_block(blockIdx.x, gridDim.x);
   const int64_t x = (int64_t)lb * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // per-wave table offsets (scalar loads, independent of the node flags)
+  const int gw = __builtin_amdgcn_readfirstlane((int)(x >> 6));
+  int4 wm0 = make_int4(0, 0, 0, 0), wm1 = make_int4(0, 0, 0, 0);
+  if ((int64_t)gw * 64 < g.N) {
+    wm0 = t.wave_meta[gw];
+    wm1 = t.wave_meta[gw + 1];
+  }
+  const int nS = wm1.x - wm0.x;
+  const int link_lo = wm0.y, v_lo = wm0.z, f_lo = wm0.w;
+  const int nL = wm1.y - wm0.y, nV = wm1.z - wm0.z, nF = wm1.w - wm0.w;
   const bool live = x < g.N;
   const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
   const bool fluid = live && !(fl & F_SOLID);
   const bool special = (fl & F_SPECIAL) != 0;
-  const uint64_t bal = __ballot(special);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int4 m = make_int4(0, 0, 0, 0), b = make_int4(0, 0, 0, 0);
-  if (special) {
-    const int64_t sidx = t.wave_base[x >> 6] + __popcll(bal & ((1ull << lane) - 1ull));
-    m = t.sp_mask[sidx];
-    b = t.sp_base[sidx];
-  }
+  // the wave's special-node masks, one per lane in node order
+  int4 msk = make_int4(0, 0, 0, 0);
+  if (lane < nS) msk = t.sp_mask[wm0.x + lane];
 
   // node coordinates and the 19 plain loads, issued before the link staging so
   // that their latency overlaps it
@@ -425,32 +432,45 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     }
   }
 
-  // wave-uniform ranges of this wave's records; every lane takes part
-  int link_lo = 0, v_lo = 0, f_lo = 0, nL = 0;
-  bool staged = false;
-  if (bal) {
-    const int lf = __builtin_ctzll(bal), ll = 63 - __builtin_clzll(bal);
-    link_lo = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(b.x, lf));
-    v_lo = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(b.y, lf));
-    f_lo = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(b.z, lf));
-    nL = __builtin_amdgcn_readlane(b.x + __popc((uint32_t)m.x), ll) - link_lo;
-    const int nV = __builtin_amdgcn_readlane(b.y + __popc((uint32_t)m.z), ll) - v_lo;
-    const int nF = __builtin_amdgcn_readlane(b.z + __popc((uint32_t)m.w), ll) - f_lo;
-    staged = nL <= kStage && nV <= kStage && nF <= kStage;
-    if (staged) {
-      for (int e = lane; e < nL; e += 64) {
-        sA[wv][e] = t.A[link_lo + e];
-        sW1[wv][e] = t.w1[link_lo + e];
-        sW2[wv][e] = t.w2[link_lo + e];
-      }
-      if (!first) {
-        const float* Bp = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
-        for (int e = lane; e < nV; e += 64) sBv[wv][e] = Bp[e];
-      }
-      for (int e = lane; e < nF; e += 64) sFF[wv][e] = t.ff_idx[f_lo + e];
-      wave_lds_sync();
+  // Link records of the wave (contiguous, node-major) into LDS, and each
+  // special lane's masks and record offsets: exclusive scans of the per-node
+  // record counts over the wave's special nodes, gathered to the owner lanes.
+  // Every lane takes part (wave-uniform branches only).
+  const bool staged = nS > 0 && nL <= kStage && nV <= kStage && nF <= kStage;
+  if (staged) {
+    for (int e = lane; e < nL; e += 64) {
+      sA[wv][e] = t.A[link_lo + e];
+      sW1[wv][e] = t.w1[link_lo + e];
+      sW2[wv][e] = t.w2[link_lo + e];
+    }
+    if (!first) {
+      const float* Bp = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
+      for (int e = lane; e < nV; e += 64) sBv[wv][e] = Bp[e];
+    }
+    for (int e = lane; e < nF; e += 64) sFF[wv][e] = t.ff_idx[f_lo + e];
+  }
+  int4 m = make_int4(0, 0, 0, 0), b = make_int4(0, 0, 0, 0);
+  if (nS > 0) {
+    const int cLV = __popc((uint32_t)msk.x) | (__popc((uint32_t)msk.z) << 16);
+    const int cF = __popc((uint32_t)msk.w);
+    int sLV = cLV, sF = cF;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int a1 = __shfl_up(sLV, d), a2 = __shfl_up(sF, d);
+      if (lane >= d) { sLV += a1; sF += a2; }
+    }
+    sLV -= cLV;
+    sF -= cF;
+    const uint64_t bal = __ballot(special);
+    const int src = __popcll(bal & ((1ull << lane) - 1ull));
+    const int4 mm = make_int4(__shfl(msk.x, src), __shfl(msk.y, src), __shfl(msk.z, src), __shfl(msk.w, src));
+    const int gLV = __shfl(sLV, src), gF = __shfl(sF, src);
+    if (special) {
+      m = mm;
+      b = make_int4(link_lo + (gLV & 0xffff), v_lo + (gLV >> 16), f_lo + gF, 0);
     }
   }
+  if (staged) wave_lds_sync();
 
   if (live && !fluid) {
     if (rho_o) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
@@ -819,6 +839,7 @@ extern "C" {
 struct fc_aa_tables {
   const uint8_t* nflag;
   const int32_t* wave_base;
+  const int32_t* wave_meta;  // [ceil(N/64) + 1][4]
   const int32_t* sp_mask;  // [n_special][4]
   const int32_t* sp_base;  // [n_special][4]
   const float* w1;
@@ -848,8 +869,8 @@ int64_t fc_aa_slot_count(int nx, int ny, int nz, int px, int py, int pz) {
 
 static int pack(Grid& g, Tables& t, const fc_aa_tables* tb, int nx, int ny, int nz, int px, int py, int pz) {
   if (int rc = make_grid(g, nx, ny, nz, px, py, pz)) return rc;
-  FC_REQUIRE(tb && tb->nflag && tb->wave_base, "aa: node flag tables are required");
-  t.nflag = tb->nflag; t.wave_base = tb->wave_base;
+  FC_REQUIRE(tb && tb->nflag && tb->wave_base && tb->wave_meta, "aa: node flag tables are required");
+  t.nflag = tb->nflag; t.wave_base = tb->wave_base; t.wave_meta = (const int4*)tb->wave_meta;
   t.sp_mask = (const int4*)tb->sp_mask; t.sp_base = (const int4*)tb->sp_base;
   t.w1 = tb->w1; t.w2 = tb->w2; t.A = tb->A; t.B = tb->B; t.ff_idx = tb->ff_idx; t.n_v = tb->n_v;
   return 0;

@@ -4,7 +4,8 @@
 //  pack_input:  x[p] = {r[p] / ||r||, feat1[p], mask[p], 0, ...} (8 channels),
 //               ||r||^2 read from a device FP64 scalar (no host round trip).
 //  avgpool2:    2x2x2 mean (FP32 sum of 8 BF16 values, one rounding).
-//  unpack:      z[p] = ||r|| * y[p, 0] * mask[p] (FP32 output of the last conv).
+//  scale:       out[p] = r[p] * d[p] (symmetric diagonal scaling D^-1/2 r).
+//  unpack:      z[p] = s * y[p, 0] * mask[p] * d[p], s = sqrt(scal[slot]).
 #include "fc_common.hpp"
 
 using namespace fc;
@@ -67,13 +68,22 @@ __global__ __launch_bounds__(kBlock) void avgpool2_kernel(const __bf16* __restri
 
 __global__ __launch_bounds__(kBlock) void unpack_kernel(const float* __restrict__ y, int cy,
                                                         const float* __restrict__ mask,
+                                                        const float* __restrict__ d,
                                                         const double* __restrict__ scal, int slot, int64_t n,
                                                         float* __restrict__ z) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= n) return;
   const float nr = norm_from(scal, slot);
   const float m = mask ? mask[p] : 1.f;
-  z[p] = nr * y[p * cy] * m;
+  const float dd = d ? d[p] : 1.f;
+  z[p] = nr * y[p * cy] * m * dd;
+}
+
+__global__ __launch_bounds__(kBlock) void scale_kernel(const float* __restrict__ r, const float* __restrict__ d,
+                                                       int64_t n, float* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  out[p] = r[p] * d[p];
 }
 
 }  // namespace
@@ -100,13 +110,20 @@ int fc_unet_avgpool2(const void* in, int B, int D, int H, int W, int C, void* ou
   return check_launch("unet_avgpool2");
 }
 
-int fc_unet_unpack(const float* y, int cy, const float* mask, const double* scal, int slot, int64_t n, float* z,
-                   hipStream_t stream) {
+int fc_unet_unpack(const float* y, int cy, const float* mask, const float* d, const double* scal, int slot,
+                   int64_t n, float* z, hipStream_t stream) {
   FC_REQUIRE(y && scal && z && cy > 0 && n >= 0 && slot >= 0, "unet_unpack: bad arguments");
   if (n == 0) return 0;
-  hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, y, cy, mask, scal, slot, n,
-                     z);
+  hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, y, cy, mask, d, scal, slot,
+                     n, z);
   return check_launch("unet_unpack");
+}
+
+int fc_unet_scale(const float* r, const float* d, int64_t n, float* out, hipStream_t stream) {
+  FC_REQUIRE(r && d && out && n >= 0, "unet_scale: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, r, d, n, out);
+  return check_launch("unet_scale");
 }
 
 }  // extern "C"

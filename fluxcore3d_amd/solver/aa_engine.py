@@ -138,7 +138,12 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
     cnt = np.zeros(groups * 64, np.int64)
     cnt[:N] = special
     wave_base = np.concatenate([[0], np.cumsum(cnt.reshape(groups, 64).sum(1))[:-1]]).astype(np.int32)
-    return dict(nflag=nflag, wave_base=wave_base, sp_mask=sp_mask, sp_base=sp_base, w1=w1s, w2=w2s,
+    # per 64-node group (+ totals): first special node, link, second-node link and publish entry
+    starts = np.minimum(np.arange(groups + 1, dtype=np.int64) * 64, N)
+    wave_meta = np.stack([np.searchsorted(sp_idx, starts), np.searchsorted(link_node, starts),
+                          np.searchsorted(v_node, starts), np.searchsorted(ff_node, starts)], 1).astype(np.int32)
+    return dict(nflag=nflag, wave_base=wave_base, wave_meta=wave_meta, sp_mask=sp_mask, sp_base=sp_base, w1=w1s,
+                w2=w2s,
                 link_node=link_node, link_dir=link_dir, B_init=B_init, ff_idx=ff_slot, ff_node=ff_node,
                 n_links=int(link_node.size), n_v=int(v_node.size), n_special=int(sp_idx.size), sp_idx=sp_idx)
 
@@ -190,6 +195,7 @@ class AAEngine:
 
         self.nflag = dt(tb["nflag"], torch.uint8)
         self.wave_base = dt(tb["wave_base"], torch.int32)
+        self.wave_meta = dt(tb["wave_meta"], torch.int32)
         self.sp_mask = dt(tb["sp_mask"], torch.int32)
         self.sp_base = dt(tb["sp_base"], torch.int32)
         self.w1 = dt(tb["w1"], torch.float32)
@@ -200,7 +206,8 @@ class AAEngine:
         self.B = self.B_init.clone()
         self.ff_idx = dt(tb["ff_idx"], torch.int32)
         self._tables = _lib.AATables(
-            self.nflag.data_ptr(), self.wave_base.data_ptr(), self.sp_mask.data_ptr(), self.sp_base.data_ptr(),
+            self.nflag.data_ptr(), self.wave_base.data_ptr(), self.wave_meta.data_ptr(), self.sp_mask.data_ptr(),
+            self.sp_base.data_ptr(),
             self.w1.data_ptr(), self.w2.data_ptr(), self.A.data_ptr(), self.B.data_ptr(), self.ff_idx.data_ptr(),
             max(self.n_v, 1))
 

@@ -5,9 +5,10 @@ FP64 outer vectors (mg_pcg.fcg_fp64) and a GMG-PCG fallback.
 
 There is no counterpart in the reference: its thermal solve stops at the
 multigrid-preconditioned Krylov methods (solver/mg_pcg.py:33-187). The
-network approximates the error of a residual, z = ||r|| N(r / ||r||, log k, mask),
-so one application costs a fixed number of convolutions regardless of the
-conductivity contrast.
+network works on the symmetrically Jacobi-scaled system (D = diag(A)):
+with r~ = D^-1/2 r it returns z = D^-1/2 ||r~|| N(r~ / ||r~||, log k, mask),
+an approximation of A^-1 r, so one application costs a fixed number of
+convolutions regardless of the conductivity contrast.
 
 Architecture (levels l = 0..L-1, channels C_l, 2x average pooling between
 levels, nearest upsampling + skip concatenation on the way up, leaky ReLU
@@ -33,10 +34,12 @@ import numpy as np
 import torch
 
 from .. import _lib
+from . import mg_cuda_ext  # noqa: F401  (registers fc_mg_norm2_f32)
 
 SLOPE = 0.1
 IN_CH = 8
 HEAD_CH = 8
+NN_SLOT = 7  # FP64 scalar slot of the flexible-CG workspace holding ||D^-1/2 r||^2
 
 _lib.register({
     "fc_conv_kpad": (_lib.I32, [_lib.I32]),
@@ -48,7 +51,8 @@ _lib.register({
                                         _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.P, _lib.P, _lib.SZ, _lib.P]),
     "fc_unet_pack_input": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P, _lib.P]),
     "fc_unet_avgpool2": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.P, _lib.P]),
-    "fc_unet_unpack": (_lib.I32, [_lib.P, _lib.I32, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P, _lib.P]),
+    "fc_unet_unpack": (_lib.I32, [_lib.P, _lib.I32, _lib.P, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P, _lib.P]),
+    "fc_unet_scale": (_lib.I32, [_lib.P, _lib.P, _lib.I64, _lib.P, _lib.P]),
 }, unchecked={"fc_conv_kpad", "fc_conv3d_dw_workspace_bytes"})
 
 
@@ -228,19 +232,24 @@ class UNetPrecond:
                 if l > 0:
                     b["pool"].append(torch.empty(s + (self.channels[l - 1],), **bf))
             b["head"] = torch.empty((1, D, H, W, HEAD_CH), device=self.device, dtype=torch.float32)
+            b["rs"] = torch.empty(D * H * W, device=self.device, dtype=torch.float32)
             self._bufs[key] = b
         return self._bufs[key]
 
     def apply(self, r32: torch.Tensor, z32: torch.Tensor, feat: Optional[torch.Tensor], mask: Optional[torch.Tensor],
-              scal: torch.Tensor, slot: int, shape) -> None:
-        """z = ||r|| N(r / ||r||, feat, mask) * mask with ||r||^2 = scal[slot] (FP64, device)."""
+              dsq: torch.Tensor, scal: torch.Tensor, partial: torch.Tensor, shape) -> None:
+        """z = D^-1/2 s N(r~ / s, feat, mask) * mask with r~ = D^-1/2 r (dsq = diag(A)^-1/2)
+        and s = ||r~|| kept in scal[NN_SLOT] (FP64, device); ``partial`` is the
+        block-partial buffer of the FP64 reductions."""
         if self._packed is None:
             self.refresh()
         D, H, W = shape
         n = D * H * W
         b = self._buffers(shape)
         st = _stream(self.device)
-        _lib.call("fc_unet_pack_input", r32.data_ptr(), _ptr(feat), _ptr(mask), scal.data_ptr(), int(slot), n,
+        _lib.call("fc_unet_scale", r32.data_ptr(), dsq.data_ptr(), n, b["rs"].data_ptr(), st)
+        _lib.call("fc_mg_norm2_f32", b["rs"].data_ptr(), n, partial.data_ptr(), scal.data_ptr(), NN_SLOT, st)
+        _lib.call("fc_unet_pack_input", b["rs"].data_ptr(), _ptr(feat), _ptr(mask), scal.data_ptr(), NN_SLOT, n,
                   b["x"].data_ptr(), st)
         wp = iter(self._packed)
         C = self.channels
@@ -259,8 +268,8 @@ class UNetPrecond:
             conv3d(b["tmp"][l], next(wp), C[l], True, out=b["dec"][l])
             h = b["dec"][l]
         conv3d(h, next(wp), HEAD_CH, False, out=b["head"])
-        _lib.call("fc_unet_unpack", b["head"].data_ptr(), HEAD_CH, _ptr(mask), scal.data_ptr(), int(slot), n,
-                  z32.data_ptr(), st)
+        _lib.call("fc_unet_unpack", b["head"].data_ptr(), HEAD_CH, _ptr(mask), dsq.data_ptr(), scal.data_ptr(),
+                  NN_SLOT, n, z32.data_ptr(), st)
 
     # --------------------------------------------------------------- persistence
     def state(self) -> Dict[str, np.ndarray]:
@@ -286,6 +295,14 @@ class UNetPrecond:
         return net
 
 
+def jacobi_scale(mg) -> torch.Tensor:
+    """diag(A)^-1/2 of the finest level, A = I + dt L (masked rows: 1)."""
+    lv0 = mg._levels[0]
+    d = 1.0 + float(mg._dt_cache) * lv0["diagL"].reshape(-1).double()
+    m = lv0["mask_f"].reshape(-1) > 0
+    return torch.where(m, d.clamp_min(1e-30).rsqrt(), torch.ones_like(d)).float().contiguous()
+
+
 def log_k_feature(k: torch.Tensor, scale: float = 0.25) -> torch.Tensor:
     """Second input channel: scale * log10(k), FP32, flattened."""
     return (torch.log10(k.float().clamp_min(1e-30)) * scale).reshape(-1).contiguous()
@@ -301,8 +318,41 @@ def _mg_for(c, device):
     return mg
 
 
+class ScaledOperator:
+    """A~ u = D^-1/2 A D^-1/2 u on the finest level, written with torch tensor
+    operations for the training loss only (same stencil, ghosts and masking as
+    the HIP operator; the solver never calls this)."""
+
+    def __init__(self, mg):
+        lv0 = mg._levels[0]
+        self.shape = tuple(int(v) for v in lv0["shape"])
+        self.c = {k: lv0[k].reshape(self.shape).float().clone() for k in ("aE", "aW", "aN", "aS", "aU", "aD",
+                                                                           "diagL")}
+        self.mask = lv0["mask_f"].reshape(self.shape).float().clone()
+        self.dt = float(mg._dt_cache)
+        self.dsq = jacobi_scale(mg).reshape(self.shape).clone()
+        dev = self.mask.device
+        self.idx = []
+        for ax, n in enumerate(self.shape):
+            lo = 1 if (mg.ax != ax or int(lv0["out_idx"]) == 0) else 0
+            up = torch.cat([torch.arange(1, n), torch.tensor([n - 2])]).to(dev)
+            dn = torch.cat([torch.tensor([lo]), torch.arange(0, n - 1)]).to(dev)
+            self.idx.append((up, dn))
+
+    def __call__(self, u: torch.Tensor) -> torch.Tensor:
+        v = u * self.dsq
+        c = self.c
+        nb = []
+        for ax in range(3):
+            up, dn = self.idx[ax]
+            nb.append((v.index_select(v.dim() - 3 + ax, up), v.index_select(v.dim() - 3 + ax, dn)))
+        L = (c["diagL"] * v - c["aE"] * nb[0][0] - c["aW"] * nb[0][1] - c["aN"] * nb[1][0] - c["aS"] * nb[1][1]
+             - c["aU"] * nb[2][0] - c["aD"] * nb[2][1])
+        return (v + self.dt * L) * self.mask * self.dsq
+
+
 def harvest_pairs(n: int = 128, problems: int = 6, seed0: int = 1000, snapshots=(0, 1, 2, 3, 5, 8, 12, 20, 30),
-                  smooth_per_problem: int = 4, device="cuda", feat_scale: float = 0.25):
+                  smooth_per_problem: int = 4, device="cuda", feat_scale: float = 0.25, log=print):
     """(input, target, mask) samples at n^3 on smooth-noise blob problems with
     k_s / k_f log-uniform in [1, 1e4]:
 
@@ -311,12 +361,14 @@ def harvest_pairs(n: int = 128, problems: int = 6, seed0: int = 1000, snapshots=
     * smooth random errors e (Gaussian-filtered noise at random scales) with
       r = A e (FP64 operator of the finest level).
 
-    Inputs are [n, n, n, 8] BF16 (r / ||r||, feat_scale log10 k, mask); targets
-    e / ||r|| in FP32."""
+    In the Jacobi-scaled variables r~ = D^-1/2 r, e~ = D^1/2 e, inputs are
+    [n, n, n, 8] BF16 (r~ / ||r~||, feat_scale log10 k, mask); targets
+    e~ / ||r~|| in FP32."""
     from .. import geometry
     from .mg_pcg import FCGWorkspace, S_RR, fcg_fp64
 
     samples = []
+    ops = []
     gen = torch.Generator(device="cpu").manual_seed(seed0)
     for i in range(problems):
         c = geometry.conduction_problem(n, seed=seed0 + i, device=device)
@@ -324,6 +376,9 @@ def harvest_pairs(n: int = 128, problems: int = 6, seed0: int = 1000, snapshots=
         lv0 = mg._levels[0]
         mask = lv0["mask_f"].reshape(-1)
         feat = log_k_feature(c["k"], feat_scale)
+        dsq = jacobi_scale(mg).double()
+        ops.append(ScaledOperator(mg))
+        pid = len(ops) - 1
         ws = FCGWorkspace(c["k"].shape, device)
         _, info = fcg_fp64(mg, c["b"], tol=1e-11, max_it=800, ws=ws)
         xs = ws.x.clone()
@@ -333,25 +388,30 @@ def harvest_pairs(n: int = 128, problems: int = 6, seed0: int = 1000, snapshots=
             if it in snaps:
                 r = w.r.clone()
                 e = (xs - w.x) * mask.double()
-                samples.append(_sample(r, e, feat, mask, n))
+                samples.append(_sample(r * dsq, e / dsq, feat, mask, n, pid))
 
         fcg_fp64(mg, c["b"], tol=1e-11, max_it=max(snapshots) + 1, ws=FCGWorkspace(c["k"].shape, device),
                  on_iter=grab)
         for _ in range(smooth_per_problem):
             e = _smooth_noise(n, gen, device).reshape(-1).double() * mask.double()
             r = apply_A64(mg, e)
-            samples.append(_sample(r, e, feat, mask, n))
+            samples.append(_sample(r * dsq, e / dsq, feat, mask, n, pid))
+        if log:
+            log(f"[harvest] problem {i}: k ratio {c['ratio']:.3g}, GMG-FCG {info['it']} it to "
+                f"{info['res_rel']:.2e}, {len(samples)} samples")
         del mg, ws
-    return samples
+    return samples, ops
 
 
-def _sample(r64, e64, feat, mask, n):
+def _sample(r64, e64, feat, mask, n, pid):
+    """(input BF16 [n,n,n,8], target e~/||r~|| FP32, mask, problem id, target energy e~.r~/||r~||^2)."""
     rn = float(r64.norm())
     x = torch.zeros((n, n, n, IN_CH), device=r64.device, dtype=torch.float32)
     x[..., 0] = (r64 / rn).float().view(n, n, n)
     x[..., 1] = feat.view(n, n, n)
     x[..., 2] = mask.view(n, n, n)
-    return x.to(torch.bfloat16), (e64 / rn).float().view(n, n, n), mask.view(n, n, n)
+    energy = float((e64 * r64).sum()) / rn ** 2
+    return x.to(torch.bfloat16), (e64 / rn).float().view(n, n, n), mask.view(n, n, n), pid, energy
 
 
 def _smooth_noise(n, gen, device):
@@ -377,38 +437,35 @@ def apply_A64(mg, e64: torch.Tensor) -> torch.Tensor:
     return out
 
 
-def train(net: UNetPrecond, samples, steps: int = 600, batch: int = 2, lr: float = 2e-3, crop: int = 0,
-          seed: int = 0, log_every: int = 50, log=print):
-    """Adam on the relative error loss ||N(r) - e||^2 / ||e||^2 (masked)."""
+def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: float = 2e-3, seed: int = 0,
+          log_every: int = 50, log=print):
+    """Adam on the relative energy-norm error of the scaled system,
+    (y - e~)^T A~ (y - e~) / e~^T A~ e~, which weights every error component by
+    its eigenvalue: the quantity that sets flexible-CG convergence."""
+    with torch.no_grad():  # output gain: the energy-optimal scalar on a few samples
+        gains = []
+        for x, e, m, pid, _ in samples[:: max(1, len(samples) // 8)]:
+            y = net.forward_train(x[None].contiguous())[0, ..., 0] * m
+            gains.append(float((y * x[..., 0].float()).sum() / ((y * ops[pid](y)).sum() + 1e-30)))
+        net.params[-1].mul_(float(np.median(gains)))
     opt = torch.optim.Adam(net.params, lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=steps, eta_min=lr * 0.05)
     gen = torch.Generator(device="cpu").manual_seed(seed)
     hist = []
     for step in range(steps):
         idx = torch.randint(len(samples), (batch,), generator=gen).tolist()
-        xs, es, ms = [], [], []
-        for i in idx:
-            x, e, m = samples[i]
-            if crop:
-                n = x.shape[0]
-                o = torch.randint(0, n - crop + 1, (3,), generator=gen).tolist()
-                sl = (slice(o[0], o[0] + crop), slice(o[1], o[1] + crop), slice(o[2], o[2] + crop))
-                x, e, m = x[sl], e[sl], m[sl]
-            xs.append(x)
-            es.append(e)
-            ms.append(m)
-        x = torch.stack(xs).contiguous()
-        e = torch.stack(es)
-        m = torch.stack(ms)
+        x = torch.stack([samples[i][0] for i in idx]).contiguous()
         y = net.forward_train(x)[..., 0]
-        num = (((y - e) * m) ** 2).flatten(1).sum(1)
-        den = ((e * m) ** 2).flatten(1).sum(1) + 1e-30
-        loss = (num / den).mean()
+        loss = 0.0
+        for b, i in enumerate(idx):
+            _, e, m, pid, energy = samples[i]
+            d = (y[b] - e) * m
+            loss = loss + (d * ops[pid](d)).sum() / (energy * batch)
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()
         sched.step()
-        hist.append(float(loss))
+        hist.append(float(loss.detach()))
         if log and (step % log_every == 0 or step == steps - 1):
             log(f"[train] step {step} loss {hist[-1]:.4f}")
     net.refresh()
@@ -431,13 +488,14 @@ def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 
         ws = FCGWorkspace(shape, mg.device)
     feat = log_k_feature(k, net.feat_scale)
     mask = lv0["mask_f"].reshape(-1)
+    dsq = jacobi_scale(mg)
     hist_it = []
 
     class _Stall(Exception):
         pass
 
     def precond(r32, z32):
-        net.apply(r32, z32, feat, mask, ws.scal, S_RR, shape)
+        net.apply(r32, z32, feat, mask, dsq, ws.scal, ws.partial, shape)
 
     def watch(it, w):
         if check_every > 1 and it % check_every:
@@ -460,7 +518,9 @@ def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 
         info["fallback"] = False
         info["it_nn"] = it_nn
         return x, info
-    x0_fb = ws.x.view(shape).clone()
+    # continue from the network's iterate only if it reduced the residual
+    better = len(hist_it) > 1 and hist_it[-1] < hist_it[0]
+    x0_fb = ws.x.view(shape).clone() if better else (x0 if x0 is not None else None)
     x2, info2 = fcg_fp64(mg, b, x0=x0_fb, tol=tol, max_it=max_it, ws=FCGWorkspace(shape, mg.device))
     info2["fallback"] = True
     info2["it_nn"] = it_nn

@@ -4,7 +4,8 @@
 
 Training pairs come from smooth-noise blob conduction problems at 128^3 with
 k_s / k_f log-uniform in [1, 1e4] (GMG-FCG residual / error snapshots plus
-smooth random errors); evaluation uses problems with seeds outside the
+smooth random errors); the loss is the relative energy-norm error of the
+Jacobi-scaled system. Evaluation uses problems with seeds outside the
 training range at 128^3 and 256^3, FP32 V-cycle / FP64 outer, tolerance 1e-8.
 """
 from __future__ import annotations
@@ -52,7 +53,6 @@ def main(argv=None):
     p.add_argument("--problems", type=int, default=6)
     p.add_argument("--steps", type=int, default=600)
     p.add_argument("--batch", type=int, default=2)
-    p.add_argument("--crop", type=int, default=64)
     p.add_argument("--lr", type=float, default=2e-3)
     p.add_argument("--channels", default="8,16,24,32,32")
     p.add_argument("--eval-n", default="128,256")
@@ -60,19 +60,18 @@ def main(argv=None):
     a = p.parse_args(argv)
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    samples = harvest_pairs(n=a.n, problems=a.problems, device=dev)
+    samples, ops = harvest_pairs(n=a.n, problems=a.problems, device=dev)
     t_h = time.perf_counter() - t0
     print(f"[train_unet] {len(samples)} samples at {a.n}^3 in {t_h:.1f} s", flush=True)
     net = UNetPrecond(tuple(int(c) for c in a.channels.split(",")), device=dev, seed=a.seed)
     print(f"[train_unet] {net.n_params()} parameters", flush=True)
     t0 = time.perf_counter()
-    hist = train(net, samples, steps=a.steps, batch=a.batch, lr=a.lr, crop=a.crop,
-                 log=lambda s: print(s, flush=True))
+    hist = train(net, samples, ops, steps=a.steps, batch=a.batch, lr=a.lr, log=lambda s: print(s, flush=True))
     torch.cuda.synchronize()
     t_t = time.perf_counter() - t0
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     net.save(a.out)
-    del samples
+    del samples, ops
     torch.cuda.empty_cache()
     res = []
     for i, n in enumerate(int(v) for v in a.eval_n.split(",")):

@@ -74,6 +74,8 @@ int fc_lbm_convective_bc(float *f, const uint8_t *solid, int nx, int ny, int nz,
 typedef struct fc_aa_tables {
   const uint8_t *nflag;     /* [N] 1 solid, 2 special, bits 3..5 first BC plane + 1 */
   const int32_t *wave_base; /* [ceil(N/64)] special nodes before each 64-node group */
+  const int32_t *wave_meta; /* [ceil(N/64) + 1][4] per group: first special node, first link, first
+                               second-node link, first publish entry (last row: totals) */
   const int32_t *sp_mask;   /* [n_special][4] link, solid-source, second-node, publish masks */
   const int32_t *sp_base;   /* [n_special][4] first link, first second-node link, first publish entry */
   const float *w1;          /* [n_links] Bouzidi weights */
@@ -218,9 +220,11 @@ int fc_conv3d_bwd_weight(const void *x0, int c0, int up0, const void *x1, int c1
 int fc_unet_pack_input(const float *r, const float *feat1, const float *mask, const double *scal, int slot,
                        int64_t n, void *out, hipStream_t stream);
 int fc_unet_avgpool2(const void *in, int B, int D, int H, int W, int C, void *out, hipStream_t stream);
-/* z[p] = sqrt(scal[slot]) * y[p * cy] * mask[p] */
-int fc_unet_unpack(const float *y, int cy, const float *mask, const double *scal, int slot, int64_t n, float *z,
-                   hipStream_t stream);
+/* z[p] = sqrt(scal[slot]) * y[p * cy] * mask[p] * d[p] (mask, d optional) */
+int fc_unet_unpack(const float *y, int cy, const float *mask, const float *d, const double *scal, int slot,
+                   int64_t n, float *z, hipStream_t stream);
+/* out[p] = r[p] * d[p] */
+int fc_unet_scale(const float *r, const float *d, int64_t n, float *out, hipStream_t stream);
 
 #ifdef __cplusplus
 }

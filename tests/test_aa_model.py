@@ -124,3 +124,25 @@ def test_aa_model_matches_oracle(case):
         f, *mo = olbm.step(f, solid, nx, ny, nz, per, "bgk", om, 1.0, None, 0.0, c["links"])
     for a, b in zip(mm, mo):
         assert np.abs(a - b).max() < 1e-12
+
+
+def test_wave_offsets_match_per_node_bases():
+    """The fused step derives each special node's record offsets from the per-group
+    table (wave_meta) and an exclusive scan of record counts over the group's special
+    nodes; on a porous pack this must equal the per-node bases (sp_base)."""
+    c = cases.periodic_pack(n=24, seed=3)
+    n = c["nx"]
+    tb = build_tables(n, n, n, c["solid"], c["links"], (1, 1, 1))
+    wm, sp, mask, base = tb["wave_meta"], tb["sp_idx"], tb["sp_mask"].view(np.uint32), tb["sp_base"]
+    groups = (n ** 3 + 63) // 64
+    assert wm.shape == (groups + 1, 4)
+    assert tuple(wm[-1]) == (tb["n_special"], tb["n_links"], tb["n_v"], tb["ff_node"].size)
+    pop = np.vectorize(lambda v: bin(int(v)).count("1"))
+    for g in range(groups):
+        sel = slice(wm[g, 0], wm[g + 1, 0])
+        assert np.all(sp[sel] // 64 == g)
+        m = mask[sel]
+        for col, (a, k) in enumerate(((1, 0), (2, 2), (3, 3))):
+            cnt = pop(m[:, k]) if m.size else np.zeros(0, int)
+            offs = wm[g, a] + np.concatenate([[0], np.cumsum(cnt)[:-1]]) if cnt.size else cnt
+            assert np.array_equal(offs, base[sel, col]), (g, col)

@@ -138,8 +138,10 @@ def test_training_graph_matches_torch_fp32_reference():
 
 
 def test_inference_matches_training_graph():
-    """Fused inference (packing, two-source decoder convolutions, HIP pooling,
-    output scaling by the FP64 norm scalar) equals the training graph."""
+    """Fused inference (diagonal scaling, FP64 norm on the device, packing,
+    two-source decoder convolutions, HIP pooling, output scaling) equals the
+    training graph applied to the same scaled input."""
+    from fluxcore3d_amd import _lib
     from fluxcore3d_amd.solver import nn_precond as nn
     net = nn.UNetPrecond(channels=(8, 16, 16), device=DEV, seed=2)
     D, H, W = 8, 12, 16
@@ -148,16 +150,19 @@ def test_inference_matches_training_graph():
     r = torch.randn(n, generator=g).to(DEV)
     feat = (torch.rand(n, generator=g) - 0.5).to(DEV)
     mask = (torch.rand(n, generator=g) > 0.1).float().to(DEV)
+    dsq = (0.1 + torch.rand(n, generator=g)).to(DEV)
     scal = torch.zeros(8, dtype=torch.float64, device=DEV)
-    scal[2] = float((r.double() ** 2).sum())
+    partial = torch.empty(3 * int(_lib.call("fc_mg_partials_count", n)), dtype=torch.float64, device=DEV)
     z = torch.empty(n, device=DEV)
-    net.apply(r, z, feat, mask, scal, 2, (D, H, W))
-    rn = float(scal[2]) ** 0.5
+    net.apply(r, z, feat, mask, dsq, scal, partial, (D, H, W))
+    rs = r.double() * dsq.double()
+    s = float(rs.norm())
+    assert abs(float(scal[nn.NN_SLOT]) ** 0.5 - s) <= 1e-6 * s
     x = torch.zeros((1, D, H, W, 8), device=DEV)
-    x[..., 0] = (r / rn).view(D, H, W)
+    x[..., 0] = (rs / s).float().view(D, H, W)
     x[..., 1] = feat.view(D, H, W)
     x[..., 2] = mask.view(D, H, W)
     with torch.no_grad():
         y = net.forward_train(x.to(torch.bfloat16))[0, ..., 0].reshape(-1)
-    zr = rn * y * mask
+    zr = s * y * mask * dsq
     assert (z - zr).norm() <= 1e-2 * zr.norm()
