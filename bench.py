@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=16)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--macro-every-step", type=int, default=1)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="run a 256 MiB device-to-device copy first (reference kernel for PMC byte counters)")
     p.add_argument("--thermal-n", type=int, default=256)
     p.add_argument("--no-thermal", action="store_true")
     p.add_argument("--unet-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "fluxcore3d_amd",
@@ -139,6 +141,20 @@ def cpu_baseline(solid, links, S, omega, collision, steps):
                       f"(C/OpenMP restatement of the reference step, no body force), {dt:.1f} s"}
 
 
+def traffic_per_launch(args):
+    """HBM bytes per fused-step launch (FETCH_SIZE + WRITE_SIZE, each scaled by its
+    reading for a copy of known size) from the committed rocprofv3 PMC summary of
+    this workload, or None when no summary matches."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "aa_step_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        table = json.load(fh)
+    key = f"{args.n}^3 {args.collision} {args.storage}"
+    rec = table.get(key)
+    return None if rec is None else int(rec["fetch_bytes"] + rec["write_bytes"])
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -150,6 +166,13 @@ def main():
     if not torch.cuda.is_available():
         raise RuntimeError("bench.py needs a HIP device")
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(64 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     eng, solid, links, S, omega, info, setup_s = build_workload(args.n, args.seed, args.collision, args.storage, dev)
     N = args.n ** 3
     n_fluid = int((~solid).sum())
@@ -201,8 +224,9 @@ def main():
                    "storage": args.storage, "collision": args.collision, "ibb": "reference Bouzidi (periodic links)",
                    "macro_every_step": args.macro_every_step},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_update": bpu},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(args),
+                     "algorithmic_bytes_per_update": bpu,
+                     "algorithmic_bytes_per_launch": bpu * n_fluid},
         "cpu_baseline": cpu,
         "thermal": thermal,
         "mlups_all_nodes": round(N * args.steps / (ms * 1e-3) / 1e6, 2),

@@ -55,7 +55,10 @@ struct Grid {
 struct Tables {
   const uint8_t* nflag;      // [N]
   const int32_t* wave_base;  // [ceil(N/64)] special nodes before each 64-node group
-  const int4* wave_meta;     // [ceil(N/64) + 1] {first special, first link, first second-node link, first publish}
+  const int4* wave_meta;     // [ceil(N/64) + 1] {first link, first second-node link, first publish, first source}
+  const int32_t* lk_code;    // [n_links] node in group | t << 6 | opp t << 11 | V << 16 | override << 17 | v << 18
+  const int32_t* s_code;     // [n_src] node in group | q << 6 | override << 17
+  const int32_t* f_code;     // [n_ff] node in group | p << 6
   const int4* sp_mask;       // [n_special] {L, S, V, F}
   const int4* sp_base;       // [n_special] {link_base, v_base, ff_base, 0}
   const float* w1;           // [n_links]
@@ -357,13 +360,14 @@ __device__ __forceinline__ void collide_fast(const Collision& c, float f[19], fl
   }
 }
 
-// Link records staged per wavefront in LDS. The records of one wave's special
-// nodes are contiguous (links, second-node reads and publications are stored
-// node-major), so the wave moves them with coalesced loads and stores instead
-// of 18 dependent per-lane gathers; a wave above the cap uses per-lane global
-// accesses. 256 covers every wave of a porosity-0.6 sphere pack (max 202).
-constexpr int kStage = 256;
+// Bouzidi inputs and outputs of a wavefront go through LDS: the 64 lanes
+// deposit their 19 inputs (then their 19 post-collision values) in a 19 x 64
+// tile, and the wave walks its node-major entry lists with coalesced loads,
+// one entry per lane: link values A (+ w2 B of the second node) and solid-source
+// constants overwrite tile inputs; new A values and second-node publications
+// are read from the post-collision tile. No per-lane loop over directions.
 constexpr int kWaves = kBlock / 64;
+constexpr int kCodeV = 1 << 16, kCodeOvr = 1 << 17;
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -376,31 +380,24 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
                                                          int parity, int first, float* __restrict__ rho_o,
                                                          float* __restrict__ u_o, float* __restrict__ v_o,
                                                          float* __restrict__ w_o) {
-  __shared__ float sA[kWaves][kStage], sW1[kWaves][kStage], sW2[kWaves][kStage], sBv[kWaves][kStage];
-  __shared__ int32_t sFF[kWaves][kStage];
+  __shared__ float tile[kWaves][19][64];
   const unsigned lb = logical_block(blockIdx.x, gridDim.x);
   const int64_t x = (int64_t)lb * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // per-wave table offsets (scalar loads, independent of the node flags)
+  // per-wave entry ranges (scalar loads)
   const int gw = __builtin_amdgcn_readfirstlane((int)(x >> 6));
   int4 wm0 = make_int4(0, 0, 0, 0), wm1 = make_int4(0, 0, 0, 0);
   if ((int64_t)gw * 64 < g.N) {
     wm0 = t.wave_meta[gw];
     wm1 = t.wave_meta[gw + 1];
   }
-  const int nS = wm1.x - wm0.x;
-  const int link_lo = wm0.y, v_lo = wm0.z, f_lo = wm0.w;
-  const int nL = wm1.y - wm0.y, nV = wm1.z - wm0.z, nF = wm1.w - wm0.w;
+  const int l_lo = wm0.x, nL = wm1.x - wm0.x, v_lo = wm0.y, f_lo = wm0.z, nF = wm1.z - wm0.z;
+  const int s_lo = wm0.w, nSrc = wm1.w - wm0.w;
+  const bool special_wave = (nL | nF | nSrc) != 0;
   const bool live = x < g.N;
   const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
   const bool fluid = live && !(fl & F_SOLID);
-  const bool special = (fl & F_SPECIAL) != 0;
-  // the wave's special-node masks, one per lane in node order
-  int4 msk = make_int4(0, 0, 0, 0);
-  if (lane < nS) msk = t.sp_mask[wm0.x + lane];
 
-  // node coordinates and the 19 plain loads, issued before the link staging so
-  // that their latency overlaps it
   int i = 0, j = 0, k = 0;
   uint32_t self = 0;
   float f[19];
@@ -411,6 +408,8 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     const int J = dy < 0 ? Jm : (dy > 0 ? Jp : J0);
     return (uint32_t)I * PYZ + (uint32_t)J * PZ + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
   };
+#pragma unroll
+  for (int q = 0; q < 19; ++q) f[q] = 0.f;
   if (fluid) {
     const uint32_t xu = (uint32_t)x;
     const uint32_t nynz = (uint32_t)g.ny * (uint32_t)g.nz;
@@ -432,83 +431,33 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
     }
   }
 
-  // Link records of the wave (contiguous, node-major) into LDS, and each
-  // special lane's masks and record offsets: exclusive scans of the per-node
-  // record counts over the wave's special nodes, gathered to the owner lanes.
-  // Every lane takes part (wave-uniform branches only).
-  const bool staged = nS > 0 && nL <= kStage && nV <= kStage && nF <= kStage;
-  if (staged) {
+  // Bouzidi / solid-source inputs (wave-uniform branch; every lane takes part)
+  if (special_wave) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) tile[wv][q][lane] = f[q];
+    wave_lds_sync();
+    const float* Bprev = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
     for (int e = lane; e < nL; e += 64) {
-      sA[wv][e] = t.A[link_lo + e];
-      sW1[wv][e] = t.w1[link_lo + e];
-      sW2[wv][e] = t.w2[link_lo + e];
+      const int code = t.lk_code[l_lo + e];
+      float v = t.A[l_lo + e];
+      if (!first && (code & kCodeV)) v = v + t.w2[l_lo + e] * Bprev[code >> 18];
+      if (code & kCodeOvr) tile[wv][(code >> 6) & 31][code & 63] = v;
     }
     if (!first) {
-      const float* Bp = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
-      for (int e = lane; e < nV; e += 64) sBv[wv][e] = Bp[e];
+      for (int e = lane; e < nSrc; e += 64) {
+        const int code = t.s_code[s_lo + e];
+        const int q = (code >> 6) & 31;
+        if (code & kCodeOvr) tile[wv][q][code & 63] = WQ(q);
+      }
     }
-    for (int e = lane; e < nF; e += 64) sFF[wv][e] = t.ff_idx[f_lo + e];
-  }
-  int4 m = make_int4(0, 0, 0, 0), b = make_int4(0, 0, 0, 0);
-  if (nS > 0) {
-    const int cLV = __popc((uint32_t)msk.x) | (__popc((uint32_t)msk.z) << 16);
-    const int cF = __popc((uint32_t)msk.w);
-    int sLV = cLV, sF = cF;
+    wave_lds_sync();
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int a1 = __shfl_up(sLV, d), a2 = __shfl_up(sF, d);
-      if (lane >= d) { sLV += a1; sF += a2; }
-    }
-    sLV -= cLV;
-    sF -= cF;
-    const uint64_t bal = __ballot(special);
-    const int src = __popcll(bal & ((1ull << lane) - 1ull));
-    const int4 mm = make_int4(__shfl(msk.x, src), __shfl(msk.y, src), __shfl(msk.z, src), __shfl(msk.w, src));
-    const int gLV = __shfl(sLV, src), gF = __shfl(sF, src);
-    if (special) {
-      m = mm;
-      b = make_int4(link_lo + (gLV & 0xffff), v_lo + (gLV >> 16), f_lo + gF, 0);
-    }
+    for (int q = 0; q < 19; ++q) f[q] = tile[wv][q][lane];
   }
-  if (staged) wave_lds_sync();
 
   if (live && !fluid) {
     if (rho_o) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
   } else if (fluid) {
-    const bool plain = (fl >> F_BC_SHIFT) != 0;
-    const uint32_t L = (uint32_t)m.x, Sm = (uint32_t)m.y, V = (uint32_t)m.z, F = (uint32_t)m.w;
-    if (special && !plain) {
-      if (staged) {
-        // every lane reads (clamped indices), then selects: no waits inside branches
-        const int l0 = b.x - link_lo, v0 = b.y - v_lo;
-#pragma unroll
-        for (int q = 1; q < 19; ++q) {
-          const uint32_t below = (1u << q) - 1u;
-          const int li = min(l0 + __popc(L & below), kStage - 1);
-          const int vi = min(v0 + __popc(V & below), kStage - 1);
-          float v = sA[wv][li];
-          if (!first) {
-            const float add = sW2[wv][li] * sBv[wv][vi];
-            v = ((V >> q) & 1u) ? v + add : v;
-          }
-          f[q] = ((L >> q) & 1u) ? v : ((!first && ((Sm >> q) & 1u)) ? WQ(q) : f[q]);
-        }
-      } else {
-#pragma unroll
-        for (int q = 1; q < 19; ++q) {
-          if ((L >> q) & 1u) {
-            const int64_t l = b.x + __popc(L & ((1u << q) - 1u));
-            float v = t.A[l];
-            if (!first && ((V >> q) & 1u))
-              v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + b.y + __popc(V & ((1u << q) - 1u))];
-            f[q] = v;
-          } else if (!first && ((Sm >> q) & 1u)) {
-            f[q] = WQ(q);
-          }
-        }
-      }
-    }
-
     float sigma = 0.f;
     if (c.sponge) sigma = c.sponge[c.sponge_axis == 0 ? i : (c.sponge_axis == 1 ? j : k)];
     float rho, ux, uy, uz;
@@ -534,52 +483,26 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
           Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + self, f[q], q);
       }
     }
-
-    if (special) {
-      float* Bw = t.B + (int64_t)parity * t.n_v;
-      if (staged) {
-        const int l0 = b.x - link_lo, e0 = b.z - f_lo;
-#pragma unroll
-        for (int q = 1; q < 19; ++q) {
-          const uint32_t below = (1u << q) - 1u;
-          const int li = min(l0 + __popc(L & below), kStage - 1);
-          const float w1v = sW1[wv][li], w2v = sW2[wv][li];
-          float a = w1v * f[OPP(q)];
-          if (!((V >> q) & 1u)) a = a + w2v * f[q];
-          if ((L >> q) & 1u) sA[wv][li] = a;
-        }
-#pragma unroll
-        for (int q = 1; q < 19; ++q) {
-          const int ei = min(e0 + __popc(F & ((1u << q) - 1u)), kStage - 1);
-          const int dst = sFF[wv][ei];
-          if ((F >> q) & 1u) Bw[dst] = f[q];
-        }
-      } else {
-        int64_t l = b.x;
-#pragma unroll
-        for (int q = 1; q < 19; ++q) {
-          if ((L >> q) & 1u) {
-            float a = t.w1[l] * f[OPP(q)];
-            if (!((V >> q) & 1u)) a = a + t.w2[l] * f[q];
-            t.A[l] = a;
-            ++l;
-          }
-        }
-        int64_t e = b.z;
-#pragma unroll
-        for (int q = 1; q < 19; ++q) {
-          if ((F >> q) & 1u) {
-            Bw[t.ff_idx[e]] = f[q];
-            ++e;
-          }
-        }
-      }
-    }
   }
-  // staged own-node Bouzidi parts back to HBM, coalesced (wave-uniform branch)
-  if (staged) {
+
+  // own-node parts of the next link values and second-node publications
+  if (special_wave) {
     wave_lds_sync();
-    for (int e = lane; e < nL; e += 64) t.A[link_lo + e] = sA[wv][e];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) tile[wv][q][lane] = f[q];
+    wave_lds_sync();
+    for (int e = lane; e < nL; e += 64) {
+      const int code = t.lk_code[l_lo + e];
+      const int n = code & 63;
+      float a = t.w1[l_lo + e] * tile[wv][(code >> 11) & 31][n];
+      if (!(code & kCodeV)) a = a + t.w2[l_lo + e] * tile[wv][(code >> 6) & 31][n];
+      t.A[l_lo + e] = a;
+    }
+    float* Bw = t.B + (int64_t)parity * t.n_v;
+    for (int e = lane; e < nF; e += 64) {
+      const int code = t.f_code[f_lo + e];
+      Bw[t.ff_idx[f_lo + e]] = tile[wv][(code >> 6) & 31][code & 63];
+    }
   }
 }
 
@@ -840,6 +763,9 @@ struct fc_aa_tables {
   const uint8_t* nflag;
   const int32_t* wave_base;
   const int32_t* wave_meta;  // [ceil(N/64) + 1][4]
+  const int32_t* lk_code;
+  const int32_t* s_code;
+  const int32_t* f_code;
   const int32_t* sp_mask;  // [n_special][4]
   const int32_t* sp_base;  // [n_special][4]
   const float* w1;
@@ -871,6 +797,7 @@ static int pack(Grid& g, Tables& t, const fc_aa_tables* tb, int nx, int ny, int 
   if (int rc = make_grid(g, nx, ny, nz, px, py, pz)) return rc;
   FC_REQUIRE(tb && tb->nflag && tb->wave_base && tb->wave_meta, "aa: node flag tables are required");
   t.nflag = tb->nflag; t.wave_base = tb->wave_base; t.wave_meta = (const int4*)tb->wave_meta;
+  t.lk_code = tb->lk_code; t.s_code = tb->s_code; t.f_code = tb->f_code;
   t.sp_mask = (const int4*)tb->sp_mask; t.sp_base = (const int4*)tb->sp_base;
   t.w1 = tb->w1; t.w2 = tb->w2; t.A = tb->A; t.B = tb->B; t.ff_idx = tb->ff_idx; t.n_v = tb->n_v;
   return 0;

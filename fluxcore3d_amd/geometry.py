@@ -6,6 +6,10 @@
   spheres with radii uniform in [rmin, rmax] until the target solid fraction
   (attempts capped; the achieved porosity is returned), union SDF with the
   minimum-image distance.
+* ``gyroid``: periodic gyroid sheet, solid where
+  sin x cos y + sin y cos z + sin z cos x < level with x = 2 pi i / period.
+* ``pin_fin_plate``: base plate with a square array of cylindrical pins in a
+  walled channel (flow along the last axis).
 
 Each returns float32 ``phi`` (negative in solid) and the boolean ``solid``.
 """
@@ -137,3 +141,46 @@ def conduction_problem(n, seed=0, ratio=None, dt=1.0e4, device="cuda"):
     z = torch.zeros_like(k)
     return dict(k=k, rcp=torch.ones_like(k), u=z, v=z, w=z, mask=mask, b=b, dt=dt, ratio=ratio, ax=0, sign=1,
                 s=1.0, solid=solid)
+
+
+def gyroid(n=384, period=48.0, level=0.3, shape=None):
+    """Gyroid solid g(x) < level, g = sin x cos y + sin y cos z + sin z cos x,
+    x = 2 pi i / period. phi = (g - level) / |grad g| clipped to +-4 (a first-order
+    distance in voxels, negative in solid), so Bouzidi q is the trilinear zero
+    crossing."""
+    nx, ny, nz = shape if shape is not None else (n, n, n)
+    w = 2.0 * math.pi / period
+    i, j, k = np.meshgrid(np.arange(nx) * w, np.arange(ny) * w, np.arange(nz) * w, indexing="ij")
+    g = np.sin(i) * np.cos(j) + np.sin(j) * np.cos(k) + np.sin(k) * np.cos(i)
+    gx = np.cos(i) * np.cos(j) - np.sin(k) * np.sin(i)
+    gy = -np.sin(i) * np.sin(j) + np.cos(j) * np.cos(k)
+    gz = -np.sin(j) * np.sin(k) + np.cos(k) * np.cos(i)
+    grad = w * np.sqrt(gx ** 2 + gy ** 2 + gz ** 2)
+    phi = np.clip((g - level) / np.maximum(grad, 1e-3 * w), -4.0, 4.0).astype(np.float32)
+    return phi, phi < 0
+
+
+def pin_fin_plate(nx=192, ny=192, nz=384, plate=16, pins=8, radius=4.0, height=120, margin=None):
+    """Heat-sink channel: flow along z, a base plate of ``plate`` voxels at low y,
+    a pins x pins array of cylinders (axis along y, radius ``radius``, height
+    ``height`` above the plate) centred in x and in the middle third of z.
+    Returns phi (SDF of plate union pins, negative in solid), solid and
+    solid_flow (solid plus channel walls on +-X and +Y)."""
+    i, j, k = np.meshgrid(np.arange(nx, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                          np.arange(nz, dtype=np.float64), indexing="ij")
+    d_plate = j - (plate - 0.5)
+    margin = nx / (2 * pins) if margin is None else margin
+    xs = np.linspace(margin, nx - 1 - margin, pins)
+    zs = np.linspace(nz / 3.0, 2.0 * nz / 3.0, pins)
+    d_pin = np.full(i.shape, np.inf)
+    for cx in xs:
+        for cz in zs:
+            dr = np.sqrt((i - cx) ** 2 + (k - cz) ** 2) - radius
+            dh = j - (plate - 0.5 + height)
+            d_pin = np.minimum(d_pin, np.maximum(dr, dh))
+    phi = np.minimum(d_plate, d_pin).astype(np.float32)
+    solid = phi < 0
+    solid_flow = solid.copy()
+    solid_flow[0, :, :] = solid_flow[-1, :, :] = True
+    solid_flow[:, -1, :] = True
+    return phi, solid, solid_flow

@@ -73,7 +73,9 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
     link_dir = np.zeros(0, np.int64)
     v_node = np.zeros(0, np.int64)
     ff_node = np.zeros(0, np.int64)
+    ff_dir = np.zeros(0, np.int64)
     ff_slot = np.zeros(0, np.int32)
+    link_v = np.zeros(0, bool)
     B_init = np.zeros(0, np.float32)
     if links is not None and int(np.asarray(links["fluid_flat"]).size) > 0:
         xf = np.asarray(links["fluid_flat"], np.int64)
@@ -115,7 +117,8 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
         if np.any((pn * 19 + pp)[1:] == (pn * 19 + pp)[:-1]):
             raise ValueError("duplicate second-node publication")
         np.bitwise_or.at(Fmask, pn, (np.uint32(1) << pp.astype(np.uint32)))
-        ff_node, ff_slot = pn, pm.astype(np.int32)
+        ff_node, ff_dir, ff_slot = pn, pp, pm.astype(np.int32)
+        link_v = v
     special = (~flat_solid) & ((Lmask | Smask | Fmask) != 0)
     sp_idx = np.nonzero(special)[0]
 
@@ -138,12 +141,28 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
     cnt = np.zeros(groups * 64, np.int64)
     cnt[:N] = special
     wave_base = np.concatenate([[0], np.cumsum(cnt.reshape(groups, 64).sum(1))[:-1]]).astype(np.int32)
-    # per 64-node group (+ totals): first special node, link, second-node link and publish entry
+    # Per-entry codes of the fused step, node-major like the lists they describe:
+    #   link:    node within its 64-node group | t << 6 | opp(t) << 11 | second node << 16
+    #            | override << 17 | second-node index relative to the group's first << 18
+    #   source:  node | q << 6 | override << 17   (solid upstream, input w_q after the first step)
+    #   publish: node | p << 6                     (post-collision f_p goes to B[ff_idx])
+    # override = the node is not on an inlet/outlet plane (plane kernels rebuild those inputs).
     starts = np.minimum(np.arange(groups + 1, dtype=np.int64) * 64, N)
-    wave_meta = np.stack([np.searchsorted(sp_idx, starts), np.searchsorted(link_node, starts),
-                          np.searchsorted(v_node, starts), np.searchsorted(ff_node, starts)], 1).astype(np.int32)
-    return dict(nflag=nflag, wave_base=wave_base, wave_meta=wave_meta, sp_mask=sp_mask, sp_base=sp_base, w1=w1s,
-                w2=w2s,
+    ovr_node = (nflag >> F_BC_SHIFT) == 0
+    v_base = np.searchsorted(v_node, starts)
+    v_rank = np.cumsum(link_v) - 1
+    lk_code = ((link_node % 64) | (link_dir << 6) | (OPP_NP[link_dir] << 11) | (link_v.astype(np.int64) << 16)
+               | (ovr_node[link_node].astype(np.int64) << 17)
+               | (np.where(link_v, v_rank - v_base[link_node // 64], 0) << 18)).astype(np.int32)
+    s_row, s_q = np.nonzero(((Smask[sp_idx][:, None] >> np.arange(19, dtype=np.uint32)) & 1).astype(bool))
+    s_node = sp_idx[s_row]
+    s_code = ((s_node % 64) | (s_q << 6) | (ovr_node[s_node].astype(np.int64) << 17)).astype(np.int32)
+    f_code = ((ff_node % 64) | (ff_dir << 6)).astype(np.int32)
+    # per 64-node group (+ totals): first link, second-node link, publish entry and source entry
+    wave_meta = np.stack([np.searchsorted(link_node, starts), v_base, np.searchsorted(ff_node, starts),
+                          np.searchsorted(s_node, starts)], 1).astype(np.int32)
+    return dict(nflag=nflag, wave_base=wave_base, wave_meta=wave_meta, lk_code=lk_code, s_code=s_code,
+                f_code=f_code, n_src=int(s_node.size), sp_mask=sp_mask, sp_base=sp_base, w1=w1s, w2=w2s,
                 link_node=link_node, link_dir=link_dir, B_init=B_init, ff_idx=ff_slot, ff_node=ff_node,
                 n_links=int(link_node.size), n_v=int(v_node.size), n_special=int(sp_idx.size), sp_idx=sp_idx)
 
@@ -196,6 +215,9 @@ class AAEngine:
         self.nflag = dt(tb["nflag"], torch.uint8)
         self.wave_base = dt(tb["wave_base"], torch.int32)
         self.wave_meta = dt(tb["wave_meta"], torch.int32)
+        self.lk_code = dt(tb["lk_code"], torch.int32)
+        self.s_code = dt(tb["s_code"], torch.int32)
+        self.f_code = dt(tb["f_code"], torch.int32)
         self.sp_mask = dt(tb["sp_mask"], torch.int32)
         self.sp_base = dt(tb["sp_base"], torch.int32)
         self.w1 = dt(tb["w1"], torch.float32)
@@ -206,8 +228,8 @@ class AAEngine:
         self.B = self.B_init.clone()
         self.ff_idx = dt(tb["ff_idx"], torch.int32)
         self._tables = _lib.AATables(
-            self.nflag.data_ptr(), self.wave_base.data_ptr(), self.wave_meta.data_ptr(), self.sp_mask.data_ptr(),
-            self.sp_base.data_ptr(),
+            self.nflag.data_ptr(), self.wave_base.data_ptr(), self.wave_meta.data_ptr(), self.lk_code.data_ptr(),
+            self.s_code.data_ptr(), self.f_code.data_ptr(), self.sp_mask.data_ptr(), self.sp_base.data_ptr(),
             self.w1.data_ptr(), self.w2.data_ptr(), self.A.data_ptr(), self.B.data_ptr(), self.ff_idx.data_ptr(),
             max(self.n_v, 1))
 

@@ -173,6 +173,7 @@ class UNetPrecond:
             w = torch.randn((cout, 27, cin), generator=g) * std
             self.params.append(w.to(self.device).requires_grad_(True))
         self.feat_scale = 0.25  # log10(k) scaling of the second input channel
+        self.out_gain = 1.0  # output scale, folded into the packed head weights
         self._packed: Optional[List[torch.Tensor]] = None
         self._bufs: Dict[Tuple[int, int, int], dict] = {}
 
@@ -208,12 +209,12 @@ class UNetPrecond:
             h = torch.cat([_up(h), skips[l]], dim=-1).contiguous()
             h = _ConvFn.apply(h, next(p), True, False, True)
             h = _ConvFn.apply(h, next(p), True, False, True)
-        return _ConvFn.apply(h, next(p), False, True, True)
+        return _ConvFn.apply(h, next(p), False, True, True) * self.out_gain
 
     # ------------------------------------------------------------------ inference
     def refresh(self) -> None:
         """Re-pack the BF16 weights after a parameter update."""
-        self._packed = [pack_weights(w) for w in self.params]
+        self._packed = [pack_weights(w) for w in self.params[:-1]] + [pack_weights(self.params[-1] * self.out_gain)]
 
     def _buffers(self, shape) -> dict:
         key = tuple(int(s) for s in shape)
@@ -273,7 +274,8 @@ class UNetPrecond:
 
     # --------------------------------------------------------------- persistence
     def state(self) -> Dict[str, np.ndarray]:
-        out = {"channels": np.asarray(self.channels, np.int32), "feat_scale": np.asarray([self.feat_scale])}
+        out = {"channels": np.asarray(self.channels, np.int32), "feat_scale": np.asarray([self.feat_scale]),
+               "out_gain": np.asarray([self.out_gain])}
         for (name, *_), w in zip(self.layers, self.params):
             out[name] = w.detach().float().cpu().numpy()
         return out
@@ -286,6 +288,7 @@ class UNetPrecond:
         with np.load(path, allow_pickle=False) as z:
             net = cls(tuple(int(c) for c in z["channels"]), device=device)
             net.feat_scale = float(z["feat_scale"][0])
+            net.out_gain = float(z["out_gain"][0]) if "out_gain" in z.files else 1.0
             for (name, *_), w in zip(net.layers, net.params):
                 if tuple(z[name].shape) != tuple(w.shape):
                     raise ValueError(f"{path}: layer {name} has shape {z[name].shape}, expected {tuple(w.shape)}")
@@ -437,17 +440,21 @@ def apply_A64(mg, e64: torch.Tensor) -> torch.Tensor:
     return out
 
 
-def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: float = 2e-3, seed: int = 0,
-          log_every: int = 50, log=print):
-    """Adam on the relative energy-norm error of the scaled system,
-    (y - e~)^T A~ (y - e~) / e~^T A~ e~, which weights every error component by
-    its eigenvalue: the quantity that sets flexible-CG convergence."""
-    with torch.no_grad():  # output gain: the energy-optimal scalar on a few samples
+def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: float = 3e-4, seed: int = 0,
+          log_every: int = 50, log=print, clip: float = 1.0):
+    """Adam on the scaled residual of the network's answer, ||A~ (y - e~)||^2 with
+    ||A~ e~|| = ||r~ / ||r~|||| = 1. (The operator is not symmetric, and its
+    symmetric part is indefinite near Dirichlet and reflected Neumann rows, so
+    an energy norm is not available.) The output gain starts at the
+    least-squares optimum over a few samples."""
+    with torch.no_grad():
+        net.out_gain = 1.0
         gains = []
         for x, e, m, pid, _ in samples[:: max(1, len(samples) // 8)]:
-            y = net.forward_train(x[None].contiguous())[0, ..., 0] * m
-            gains.append(float((y * x[..., 0].float()).sum() / ((y * ops[pid](y)).sum() + 1e-30)))
-        net.params[-1].mul_(float(np.median(gains)))
+            Ay = ops[pid](net.forward_train(x[None].contiguous())[0, ..., 0] * m)
+            Ae = ops[pid](e * m)
+            gains.append(float((Ay * Ae).sum() / ((Ay * Ay).sum() + 1e-30)))
+        net.out_gain = float(np.median(gains))
     opt = torch.optim.Adam(net.params, lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=steps, eta_min=lr * 0.05)
     gen = torch.Generator(device="cpu").manual_seed(seed)
@@ -458,11 +465,11 @@ def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: 
         y = net.forward_train(x)[..., 0]
         loss = 0.0
         for b, i in enumerate(idx):
-            _, e, m, pid, energy = samples[i]
-            d = (y[b] - e) * m
-            loss = loss + (d * ops[pid](d)).sum() / (energy * batch)
+            _, e, m, pid, _ = samples[i]
+            loss = loss + (ops[pid]((y[b] - e) * m) ** 2).sum() / batch
         opt.zero_grad(set_to_none=True)
         loss.backward()
+        torch.nn.utils.clip_grad_norm_(net.params, clip)
         opt.step()
         sched.step()
         hist.append(float(loss.detach()))

@@ -4,8 +4,8 @@
 
 Training pairs come from smooth-noise blob conduction problems at 128^3 with
 k_s / k_f log-uniform in [1, 1e4] (GMG-FCG residual / error snapshots plus
-smooth random errors); the loss is the relative energy-norm error of the
-Jacobi-scaled system. Evaluation uses problems with seeds outside the
+smooth random errors); the loss is the scaled residual of the network answer on the
+Jacobi-scaled system (see nn_precond.train). Evaluation uses problems with seeds outside the
 training range at 128^3 and 256^3, FP32 V-cycle / FP64 outer, tolerance 1e-8.
 """
 from __future__ import annotations
@@ -53,7 +53,7 @@ def main(argv=None):
     p.add_argument("--problems", type=int, default=6)
     p.add_argument("--steps", type=int, default=600)
     p.add_argument("--batch", type=int, default=2)
-    p.add_argument("--lr", type=float, default=2e-3)
+    p.add_argument("--lr", type=float, default=3e-4)
     p.add_argument("--channels", default="8,16,24,32,32")
     p.add_argument("--eval-n", default="128,256")
     p.add_argument("--seed", type=int, default=0)

@@ -60,6 +60,30 @@ def config1():
                 omega=1.0 / 0.6)
 
 
+def pin_fin_channel(nx=48, ny=48, nz=96):
+    """Reduced heat-sink channel (the 192x192x384 plate + 8x8 pins workload at
+    1/4 scale): plate 4 voxels, 4x4 pins r=2 h=30, flow +Z, inlet u=0.04,
+    convective outlet, walls +-X +-Y, TRT with magic parameter 1/4."""
+    phi, solid, _ = geometry.pin_fin_plate(nx, ny, nz, plate=4, pins=4, radius=2.0, height=30)
+    cfg = Cfg(flow_dir="+Z", domain_walls=["+X", "-X", "+Y", "-Y"], domain_outlets=[], transverse_walls=False,
+              outlet_bc_mode="convective", rho_out_lat=1.0, u_lat_max=0.12, collision="trt", sponge_length=0,
+              u_in_lat=0.04)
+    solid_flow = solid.copy()
+    solid_flow[0], solid_flow[-1] = True, True
+    solid_flow[:, 0], solid_flow[:, -1] = True, True
+    links = olbm.ibb_from_sdf(solid, phi)
+    tau = 0.6
+    return dict(nx=nx, ny=ny, nz=nz, phi=phi, solid=solid, solid_flow=solid_flow, cfg=cfg, links=links,
+                omega=1.0 / tau, omega_minus=1.0 / (0.5 + 0.25 / (tau - 0.5)))
+
+
+def gyroid_box(n=48, period=48.0):
+    """One period of the gyroid (the 384^3, period-48 precision workload's unit cell)."""
+    phi, solid = geometry.gyroid(n, period=period)
+    links = olbm.ibb_from_sdf(solid, phi, periodic=(True, True, True))
+    return dict(nx=n, ny=n, nz=n, phi=phi, solid=solid, solid_flow=solid, links=links)
+
+
 def periodic_pack(n=48, seed=3):
     phi, solid, info = geometry.rsa_spheres(n, porosity=0.6, rmin=4, rmax=10, seed=seed)
     links = olbm.ibb_from_sdf(solid, phi)

@@ -29,6 +29,11 @@ def _popc_below(mask, q):
     return cnt
 
 
+def _group_of(entry_base, n_entries):
+    """Group (64-node wavefront) of each entry from the per-group first-entry table."""
+    return np.searchsorted(entry_base, np.arange(n_entries), side="right") - 1
+
+
 def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="reference"):
     tb = build_tables(nx, ny, nz, solid, links, periodic, ibb_mode)
     N = nx * ny * nz
@@ -49,15 +54,24 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
                 o |= (src[a] < 0) | (src[a] >= n)
         return o
 
+    wm = tb["wave_meta"].astype(np.int64)
+    lk, sc, fc = (tb[k].astype(np.int64) for k in ("lk_code", "s_code", "f_code"))
+    # decode the entry codes exactly as the fused kernel does
+    l_node = _group_of(wm[:, 0], lk.size) * 64 + (lk & 63)
+    l_t, l_opp = (lk >> 6) & 31, (lk >> 11) & 31
+    l_hasv, l_ovr = ((lk >> 16) & 1).astype(bool), ((lk >> 17) & 1).astype(bool)
+    l_v = wm[_group_of(wm[:, 0], lk.size), 1] + (lk >> 18)
+    s_node = _group_of(wm[:, 3], sc.size) * 64 + (sc & 63)
+    s_q, s_ovr = (sc >> 6) & 31, ((sc >> 17) & 1).astype(bool)
+    f_node = _group_of(wm[:, 2], fc.size) * 64 + (fc & 63)
+    f_p = (fc >> 6) & 31
+
     self_slot = slot((0, 0, 0))
     pop = np.zeros((19, Ns))
     pop[:, self_slot] = f0
     A = f0[tb["link_dir"], tb["link_node"]].astype(np.float64)
     B = [tb["B_init"].astype(np.float64), tb["B_init"].astype(np.float64)]
     fluid = ~solid.reshape(-1)
-    sp = tb["sp_idx"]
-    L, S, V, F = (tb["sp_mask"][:, a].view(np.uint32) for a in range(4))
-    lb, vb, fb = tb["sp_base"][:, 0], tb["sp_base"][:, 1], tb["sp_base"][:, 2]
     w1, w2 = tb["w1"].astype(np.float64), tb["w2"].astype(np.float64)
     macros = None
     for t in range(steps):
@@ -65,18 +79,12 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
         f = np.empty((19, N))
         for q in range(19):
             f[q] = pop[q, self_slot] if P == 0 else pop[OPP[q], slot(-C[q])]
-        for q in range(1, 19):
-            hl = ((L >> np.uint32(q)) & np.uint32(1)).astype(bool)
-            l = lb[hl] + _popc_below(L[hl], q)
-            val = A[l].copy()
-            if not first:
-                hv = ((V[hl] >> np.uint32(q)) & np.uint32(1)).astype(bool)
-                mv = vb[hl][hv] + _popc_below(V[hl][hv], q)
-                val[hv] = val[hv] + w2[l[hv]] * B[P ^ 1][mv]
-            f[q, sp[hl]] = val
-            if not first:
-                hs = ((S >> np.uint32(q)) & np.uint32(1)).astype(bool) & ~hl
-                f[q, sp[hs]] = olbm.W32[q]
+        val = A.copy()
+        if not first:
+            val[l_hasv] += w2[l_hasv] * B[P ^ 1][l_v[l_hasv]]
+        f[l_t[l_ovr], l_node[l_ovr]] = val[l_ovr]
+        if not first:
+            f[s_q[s_ovr], s_node[s_ovr]] = olbm.W32[s_q[s_ovr]]
         fp, rho, u, v, w = olbm.macro_collide("bgk", f, (~fluid).astype(np.uint8), om1=om)
         macros = (rho, u, v, w)
         for q in range(19):
@@ -88,15 +96,8 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
                 pop[q, slot(C[q])[fluid]] = fp[q, fluid]
                 o = oob(q) & fluid
                 pop[q, self_slot[o]] = fp[q, o]
-        for q in range(1, 19):
-            hl = ((L >> np.uint32(q)) & np.uint32(1)).astype(bool)
-            l = lb[hl] + _popc_below(L[hl], q)
-            x = sp[hl]
-            hv = ((V[hl] >> np.uint32(q)) & np.uint32(1)).astype(bool)
-            A[l] = w1[l] * fp[OPP[q], x] + np.where(hv, 0.0, w2[l] * fp[q, x])
-            hf = ((F >> np.uint32(q)) & np.uint32(1)).astype(bool)
-            e = fb[hf] + _popc_below(F[hf], q)
-            B[P][tb["ff_idx"][e]] = fp[q, sp[hf]]
+        A = w1 * fp[l_opp, l_node] + np.where(l_hasv, 0.0, w2 * fp[l_t, l_node])
+        B[P][tb["ff_idx"]] = fp[f_p, f_node]
     return macros
 
 
@@ -126,23 +127,36 @@ def test_aa_model_matches_oracle(case):
         assert np.abs(a - b).max() < 1e-12
 
 
-def test_wave_offsets_match_per_node_bases():
-    """The fused step derives each special node's record offsets from the per-group
-    table (wave_meta) and an exclusive scan of record counts over the group's special
-    nodes; on a porous pack this must equal the per-node bases (sp_base)."""
+def test_entry_codes_match_node_masks():
+    """The fused step's per-entry codes (links, solid sources, publications, with
+    per-group first-entry offsets) describe exactly the per-node masks and bases
+    used by the plane kernels (sp_mask / sp_base)."""
     c = cases.periodic_pack(n=24, seed=3)
     n = c["nx"]
     tb = build_tables(n, n, n, c["solid"], c["links"], (1, 1, 1))
-    wm, sp, mask, base = tb["wave_meta"], tb["sp_idx"], tb["sp_mask"].view(np.uint32), tb["sp_base"]
+    wm = tb["wave_meta"].astype(np.int64)
     groups = (n ** 3 + 63) // 64
     assert wm.shape == (groups + 1, 4)
-    assert tuple(wm[-1]) == (tb["n_special"], tb["n_links"], tb["n_v"], tb["ff_node"].size)
-    pop = np.vectorize(lambda v: bin(int(v)).count("1"))
-    for g in range(groups):
-        sel = slice(wm[g, 0], wm[g + 1, 0])
-        assert np.all(sp[sel] // 64 == g)
-        m = mask[sel]
-        for col, (a, k) in enumerate(((1, 0), (2, 2), (3, 3))):
-            cnt = pop(m[:, k]) if m.size else np.zeros(0, int)
-            offs = wm[g, a] + np.concatenate([[0], np.cumsum(cnt)[:-1]]) if cnt.size else cnt
-            assert np.array_equal(offs, base[sel, col]), (g, col)
+    assert tuple(wm[-1]) == (tb["n_links"], tb["n_v"], tb["ff_node"].size, tb["n_src"])
+    N = n ** 3
+    L, S, V, F = (np.zeros(N, np.int64) for _ in range(4))
+    lk = tb["lk_code"].astype(np.int64)
+    node = _group_of(wm[:, 0], lk.size) * 64 + (lk & 63)
+    assert np.array_equal(node, tb["link_node"])
+    assert np.array_equal((lk >> 6) & 31, tb["link_dir"]) and np.array_equal((lk >> 11) & 31, OPP[tb["link_dir"]])
+    np.bitwise_or.at(L, node, 1 << ((lk >> 6) & 31))
+    hv = ((lk >> 16) & 1).astype(bool)
+    np.bitwise_or.at(V, node[hv], 1 << ((lk[hv] >> 6) & 31))
+    v_idx = wm[_group_of(wm[:, 0], lk.size), 1] + (lk >> 18)
+    assert np.array_equal(v_idx[hv], np.arange(tb["n_v"]))
+    sc = tb["s_code"].astype(np.int64)
+    np.bitwise_or.at(S, _group_of(wm[:, 3], sc.size) * 64 + (sc & 63), 1 << ((sc >> 6) & 31))
+    fc = tb["f_code"].astype(np.int64)
+    f_node = _group_of(wm[:, 2], fc.size) * 64 + (fc & 63)
+    assert np.array_equal(f_node, tb["ff_node"])
+    np.bitwise_or.at(F, f_node, 1 << ((fc >> 6) & 31))
+    sp = tb["sp_idx"]
+    masks = tb["sp_mask"].view(np.uint32).astype(np.int64)
+    for col, arr in enumerate((L, S, V, F)):
+        assert np.array_equal(arr[sp], masks[:, col]), col
+        assert not np.any(np.delete(arr, sp))

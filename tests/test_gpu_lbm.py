@@ -174,6 +174,28 @@ def test_config1_step_sequence(engine, kind):
     assert rel(sim.f.cpu().numpy().reshape(19, -1), f_want) < 1e-5
 
 
+@pytest.mark.parametrize("engine", ["aa", "reference"])
+def test_pin_fin_channel_trt(engine):
+    """Heat-sink channel at 1/4 scale (48x48x96: plate + 4x4 pins, flow +Z, inlet
+    u = 0.04, convective outlet, walls, TRT with magic parameter 1/4), 120 steps
+    of the drop-in step against the oracle's reference step."""
+    from fluxcore3d_amd.solver.lbm_cuda_ext import LBMCUDAUpgrade, flow_bc_plan
+    c = cases.pin_fin_channel()
+    sim = _sim_for(c, "trt", engine, om2=c["omega_minus"])
+    LBMCUDAUpgrade.patch(sim)
+    steps = 120
+    for _ in range(steps):
+        sim.step(do_thermal=False)
+    torch.cuda.synchronize()
+    f_want, (rho_w, u_w, v_w, w_w) = _oracle_run(c, steps, "trt", c["omega_minus"], None, (0, 0, 0),
+                                                 flow_bc_plan(sim))
+    speed = float(np.sqrt(u_w**2 + v_w**2 + w_w**2).max())
+    for got, want in ((sim.u, u_w), (sim.v, v_w), (sim.wv, w_w)):
+        assert np.abs(got.cpu().numpy().reshape(-1) - want).max() / speed < 1e-5
+    assert rel(sim.rho.cpu().numpy().reshape(-1), rho_w) < 1e-5
+    assert rel(sim.f.cpu().numpy().reshape(19, -1), f_want) < 1e-5
+
+
 @pytest.mark.parametrize("kind", ["bgk", "mrt"])
 def test_aa_periodic_pack(kind):
     """Fully periodic sphere pack (48^3, porosity 0.6) with reference Bouzidi links."""

@@ -83,10 +83,13 @@ def test_reference_and_textbook_bouzidi_differ_as_documented():
     assert np.abs(out["reference"] - out["textbook"]).max() > 0.05 * out["textbook"].max()
 
 
+@pytest.mark.parametrize("geom", ["pack", "gyroid"])
 @pytest.mark.parametrize("storage,tol", [("fp32", 1e-5), ("fp16", 1e-3), ("bf16", 1e-2)])
-def test_storage_precision_vs_fp64_oracle(storage, tol):
+def test_storage_precision_vs_fp64_oracle(storage, tol, geom):
+    """FP32 / FP16 / BF16 population storage against the FP64 oracle on a sphere
+    pack (40^3) and on one period of the gyroid precision workload (48^3)."""
     from fluxcore3d_amd.solver.aa_engine import AAEngine
-    c = cases.periodic_pack(n=40, seed=5)
+    c = cases.periodic_pack(n=40, seed=5) if geom == "pack" else cases.gyroid_box()
     n = c["nx"]
     om = 1.0 / 0.55
     S = mrt_rates(om, s_e=1.2, s_bulk=1.2, s_ghost=1.2)
