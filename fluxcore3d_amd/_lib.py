@@ -58,7 +58,7 @@ _SIGS = {
     # LBM fused AA step (lbm_aa.hip)
     "fc_aa_slot_count": (I64, [I32, I32, I32, I32, I32, I32]),
     "fc_aa_step": (I32, [P, I32, I32, I32, I32, I32, I32, I32, C.POINTER(AATables), C.POINTER(AACollision),
-                         I32, I32, P, P, P, P, P]),
+                         I32, I32, P, P, P, P, I32, P]),
     "fc_aa_materialize": (I32, [P, I32, I32, I32, I32, I32, I32, I32, C.POINTER(AATables), I32, I32, P, P]),
     "fc_aa_load": (I32, [P, I32, I32, I32, I32, I32, I32, I32, C.POINTER(AATables), P, P]),
     "fc_aa_zou_he": (I32, [P, I32, I32, I32, I32, I32, I32, I32, C.POINTER(AATables), I32, I32, I32, I32, I32,

@@ -376,10 +376,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <typename ST, int KIND>
-__global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
+__global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 5)) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
                                                          int parity, int first, float* __restrict__ rho_o,
                                                          float* __restrict__ u_o, float* __restrict__ v_o,
-                                                         float* __restrict__ w_o) {
+                                                         float* __restrict__ w_o, int solid_macros) {
   __shared__ float tile[kWaves][19][64];
   const unsigned lb = logical_block(blockIdx.x, gridDim.x);
   const int64_t x = (int64_t)lb * kBlock + threadIdx.x;
@@ -397,6 +397,23 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
   const bool live = x < g.N;
   const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
   const bool fluid = live && !(fl & F_SOLID);
+
+  // Each lane's first entry of every list, requested before the populations so
+  // that their latency overlaps (entries e >= 64 are loaded in the loops below).
+  const bool hasL0 = lane < nL, hasS0 = !first && lane < nSrc, hasF0 = lane < nF;
+  int c0 = 0, sc0 = 0, fc0 = 0, ff0 = 0;
+  float a0 = 0.f, w10 = 0.f, w20 = 0.f;
+  if (hasL0) {
+    c0 = t.lk_code[l_lo + lane];
+    a0 = t.A[l_lo + lane];
+    w10 = t.w1[l_lo + lane];
+    w20 = t.w2[l_lo + lane];
+  }
+  if (hasS0) sc0 = t.s_code[s_lo + lane];
+  if (hasF0) {
+    fc0 = t.f_code[f_lo + lane];
+    ff0 = t.ff_idx[f_lo + lane];
+  }
 
   int i = 0, j = 0, k = 0;
   uint32_t self = 0;
@@ -432,19 +449,23 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
   }
 
   // Bouzidi / solid-source inputs (wave-uniform branch; every lane takes part)
+  const float* Bprev = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
   if (special_wave) {
+    float b0 = 0.f;
+    if (hasL0 && !first && (c0 & kCodeV)) b0 = Bprev[c0 >> 18];
 #pragma unroll
     for (int q = 0; q < 19; ++q) tile[wv][q][lane] = f[q];
     wave_lds_sync();
-    const float* Bprev = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
-    for (int e = lane; e < nL; e += 64) {
+    if (hasL0 && (c0 & kCodeOvr)) tile[wv][(c0 >> 6) & 31][c0 & 63] = (!first && (c0 & kCodeV)) ? a0 + w20 * b0 : a0;
+    for (int e = lane + 64; e < nL; e += 64) {
       const int code = t.lk_code[l_lo + e];
       float v = t.A[l_lo + e];
       if (!first && (code & kCodeV)) v = v + t.w2[l_lo + e] * Bprev[code >> 18];
       if (code & kCodeOvr) tile[wv][(code >> 6) & 31][code & 63] = v;
     }
     if (!first) {
-      for (int e = lane; e < nSrc; e += 64) {
+      if (hasS0 && (sc0 & kCodeOvr)) tile[wv][(sc0 >> 6) & 31][sc0 & 63] = WQ((sc0 >> 6) & 31);
+      for (int e = lane + 64; e < nSrc; e += 64) {
         const int code = t.s_code[s_lo + e];
         const int q = (code >> 6) & 31;
         if (code & kCodeOvr) tile[wv][q][code & 63] = WQ(q);
@@ -456,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
   }
 
   if (live && !fluid) {
-    if (rho_o) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
+    if (rho_o && solid_macros) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
   } else if (fluid) {
     float sigma = 0.f;
     if (c.sponge) sigma = c.sponge[c.sponge_axis == 0 ? i : (c.sponge_axis == 1 ? j : k)];
@@ -491,7 +512,13 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
 #pragma unroll
     for (int q = 0; q < 19; ++q) tile[wv][q][lane] = f[q];
     wave_lds_sync();
-    for (int e = lane; e < nL; e += 64) {
+    if (hasL0) {
+      const int n = c0 & 63;
+      float a = w10 * tile[wv][(c0 >> 11) & 31][n];
+      if (!(c0 & kCodeV)) a = a + w20 * tile[wv][(c0 >> 6) & 31][n];
+      t.A[l_lo + lane] = a;
+    }
+    for (int e = lane + 64; e < nL; e += 64) {
       const int code = t.lk_code[l_lo + e];
       const int n = code & 63;
       float a = t.w1[l_lo + e] * tile[wv][(code >> 11) & 31][n];
@@ -499,7 +526,8 @@ __global__ __launch_bounds__(kBlock) void aa_step_kernel(ST* __restrict__ pop, G
       t.A[l_lo + e] = a;
     }
     float* Bw = t.B + (int64_t)parity * t.n_v;
-    for (int e = lane; e < nF; e += 64) {
+    if (hasF0) Bw[ff0] = tile[wv][(fc0 >> 6) & 31][fc0 & 63];
+    for (int e = lane + 64; e < nF; e += 64) {
       const int code = t.f_code[f_lo + e];
       Bw[t.ff_idx[f_lo + e]] = tile[wv][(code >> 6) & 31][code & 63];
     }
@@ -805,7 +833,7 @@ static int pack(Grid& g, Tables& t, const fc_aa_tables* tb, int nx, int ny, int 
 
 int fc_aa_step(void* pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables* tb,
                const fc_aa_collision* col, int parity, int first, float* rho, float* u, float* v, float* w,
-               hipStream_t stream) {
+               int solid_macros, hipStream_t stream) {
   Grid g; Tables t;
   if (int rc = pack(g, t, tb, nx, ny, nz, px, py, pz)) return rc;
   FC_REQUIRE(pop, "aa_step: null population buffer");
@@ -822,7 +850,7 @@ int fc_aa_step(void* pop, int storage, int nx, int ny, int nz, int px, int py, i
   const unsigned grid = grid_for(g.N, kBlock);
 #define FC_STEP(T, K)                                                                                          \
   hipLaunchKernelGGL((aa_step_kernel<T, K>), dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, c, parity, first, \
-                     rho, u, v, w)
+                     rho, u, v, w, solid_macros)
 #define FC_STEP_KIND(T)                      \
   switch (c.kind) {                          \
     case 0: FC_STEP(T, 0); break;            \

@@ -27,7 +27,7 @@ namespace {
 constexpr int kBlock = 256;
 // FP64 reductions run a capped grid (grid-stride loops): one partial per block,
 // folded by a single workgroup; the fixed grid keeps the summation order fixed.
-constexpr unsigned kMaxPartials = 2048;
+constexpr unsigned kMaxPartials = 8192;
 
 struct Level {
   int nx, ny, nz;

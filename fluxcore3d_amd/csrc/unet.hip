@@ -1,11 +1,12 @@
 // unet.hip — element and resampling kernels around the convolutions of the
 // learned preconditioner (channels-last BF16 activations).
 //
-//  pack_input:  x[p] = {r[p] / ||r||, feat1[p], mask[p], 0, ...} (8 channels),
-//               ||r||^2 read from a device FP64 scalar (no host round trip).
+//  pack_input:  x[p] = {r[p] sqrt(n) / ||r||, feat1[p], mask[p], 0, ...} (8 channels):
+//               the residual at unit RMS, so that in BF16 it is not swamped by the
+//               O(1) feature channels; ||r||^2 is a device FP64 scalar (no host round trip).
 //  avgpool2:    2x2x2 mean (FP32 sum of 8 BF16 values, one rounding).
 //  scale:       out[p] = r[p] * d[p] (symmetric diagonal scaling D^-1/2 r).
-//  unpack:      z[p] = s * y[p, 0] * mask[p] * d[p], s = sqrt(scal[slot]).
+//  unpack:      z[p] = s * y[p, 0] * mask[p] * d[p], s = sqrt(scal[slot] / n).
 #include "fc_common.hpp"
 
 using namespace fc;
@@ -28,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void pack_input_kernel(const float* __restr
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= n) return;
   const float nr = norm_from(scal, slot);
-  const float inv = nr > 0.f ? 1.f / nr : 0.f;
+  const float inv = nr > 0.f ? sqrtf((float)n) / nr : 0.f;
   bf16x8 v;
   v[0] = (__bf16)(r[p] * inv);
   v[1] = (__bf16)(feat1 ? feat1[p] : 0.f);
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void unpack_kernel(const float* __restrict_
                                                         float* __restrict__ z) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= n) return;
-  const float nr = norm_from(scal, slot);
+  const float nr = norm_from(scal, slot) / sqrtf((float)n);
   const float m = mask ? mask[p] : 1.f;
   const float dd = d ? d[p] : 1.f;
   z[p] = nr * y[p * cy] * m * dd;

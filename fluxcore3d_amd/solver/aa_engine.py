@@ -196,6 +196,7 @@ class AAEngine:
         self.t = 0
         self.t_loaded = 0
         self._ws = None
+        self._macro_ptrs = None
 
     # ------------------------------------------------------------------ tables
     def _build_tables(self, solid: np.ndarray, links) -> None:
@@ -286,11 +287,15 @@ class AAEngine:
         return out
 
     def step(self, rho=None, u=None, v=None, w=None) -> None:
-        """One time step; writes pre-collision macros if the four outputs are given."""
+        """One time step; writes pre-collision macros if the four outputs are given.
+        Solid nodes hold the constants (1, 0, 0, 0); they are written when the
+        output buffers differ from the previous step's and skipped otherwise."""
         stream = _lib.stream_handle(self.device)
+        ptrs = (_lib.ptr(rho), _lib.ptr(u), _lib.ptr(v), _lib.ptr(w))
+        solid_macros = int(ptrs != self._macro_ptrs)
+        self._macro_ptrs = ptrs
         _lib.call("fc_aa_step", self.pop.data_ptr(), STORAGE_CODES[self.storage], *self._dims, self._tables,
-                  self._collision, self.t & 1, int(self.t == self.t_loaded), _lib.ptr(rho), _lib.ptr(u),
-                  _lib.ptr(v), _lib.ptr(w), stream)
+                  self._collision, self.t & 1, int(self.t == self.t_loaded), *ptrs, solid_macros, stream)
         self.t += 1
         self._apply_bcs(stream)
 

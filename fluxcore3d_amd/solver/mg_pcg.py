@@ -57,7 +57,7 @@ def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
     stream = torch.cuda.current_stream(dev).cuda_stream
     if precond is None:
         def precond(r32, z32):
-            mg_solver.precondition(r32, out=z32)
+            mg_solver.precondition(r32, out=z32, masked_input=True)
     ws.b.copy_(b.reshape(-1).to(torch.float64))
     if x0 is None:
         ws.x.zero_()

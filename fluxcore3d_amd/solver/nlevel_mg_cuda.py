@@ -222,12 +222,35 @@ class NLevelGeometricMGSolver:
         Lo.div_(self._dt_cache)
         return Lo
 
-    def precondition(self, r32: Tensor, out: Optional[Tensor] = None) -> Tensor:
-        """z = V-cycle(r) from a zero guess, masked (mg_pcg.py:178-187)."""
+    def precondition(self, r32: Tensor, out: Optional[Tensor] = None, masked_input: bool = False) -> Tensor:
+        """z = V-cycle(r) from a zero guess, masked (mg_pcg.py:178-187).
+
+        With ``masked_input`` (r already zero on masked rows, as the flexible-CG
+        residual is) the cycle reads r in place and writes z straight into
+        ``out``: the fine level's right-hand side and iterate buffers are
+        rebound for the call, and the masked rows stay zero because the
+        smoother's inverse diagonal and the prolongation carry the mask."""
         lv0 = self._levels[0]
-        lv0["b_buf"].copy_(r32.view(lv0["shape"]))
-        lv0["b_buf"].mul_(lv0["mask_f"])
-        self._vcycle(0, x_is_zero=True)
-        z = lv0["x"] if out is None else out.view(lv0["shape"]).copy_(lv0["x"])
-        z.mul_(lv0["mask_f"])
+        shape = lv0["shape"]
+        if not masked_input:
+            lv0["b_buf"].copy_(r32.view(shape))
+            lv0["b_buf"].mul_(lv0["mask_f"])
+            self._vcycle(0, x_is_zero=True)
+            z = lv0["x"] if out is None else out.view(shape).copy_(lv0["x"])
+            z.mul_(lv0["mask_f"])
+            return z
+        b_saved, x_saved, t_saved = lv0["b_buf"], lv0["x"], lv0["tmp"]
+        lv0["b_buf"] = r32.view(shape)
+        if out is not None:
+            lv0["x"] = out.view(shape)
+        try:
+            self._vcycle(0, x_is_zero=True)
+            z = lv0["x"]
+        finally:
+            lv0["b_buf"] = b_saved
+            if out is not None:
+                lv0["x"], lv0["tmp"] = x_saved, t_saved
+        if out is not None and z.data_ptr() != out.data_ptr():
+            out.view(shape).copy_(z)
+            z = out.view(shape)
         return z

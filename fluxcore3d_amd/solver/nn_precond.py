@@ -6,7 +6,8 @@ FP64 outer vectors (mg_pcg.fcg_fp64) and a GMG-PCG fallback.
 There is no counterpart in the reference: its thermal solve stops at the
 multigrid-preconditioned Krylov methods (solver/mg_pcg.py:33-187). The
 network works on the symmetrically Jacobi-scaled system (D = diag(A)):
-with r~ = D^-1/2 r it returns z = D^-1/2 ||r~|| N(r~ / ||r~||, log k, mask),
+with r~ = D^-1/2 r and s = ||r~|| / sqrt(n) it returns z = D^-1/2 s N(r~ / s, log k, mask)
+(the residual channel at unit RMS, like the O(1) feature channels),
 an approximation of A^-1 r, so one application costs a fixed number of
 convolutions regardless of the conductivity contrast.
 
@@ -277,7 +278,7 @@ class UNetPrecond:
         out = {"channels": np.asarray(self.channels, np.int32), "feat_scale": np.asarray([self.feat_scale]),
                "out_gain": np.asarray([self.out_gain])}
         for (name, *_), w in zip(self.layers, self.params):
-            out[name] = w.detach().float().cpu().numpy()
+            out[name] = w.detach().float().cpu().numpy().astype(np.float16)  # inference runs BF16 weights
         return out
 
     def save(self, path: str) -> None:
@@ -293,7 +294,7 @@ class UNetPrecond:
                 if tuple(z[name].shape) != tuple(w.shape):
                     raise ValueError(f"{path}: layer {name} has shape {z[name].shape}, expected {tuple(w.shape)}")
                 with torch.no_grad():
-                    w.copy_(torch.from_numpy(z[name]).to(device))
+                    w.copy_(torch.from_numpy(z[name].astype(np.float32)).to(device))
         net.refresh()
         return net
 
@@ -407,8 +408,9 @@ def harvest_pairs(n: int = 128, problems: int = 6, seed0: int = 1000, snapshots=
 
 
 def _sample(r64, e64, feat, mask, n, pid):
-    """(input BF16 [n,n,n,8], target e~/||r~|| FP32, mask, problem id, target energy e~.r~/||r~||^2)."""
-    rn = float(r64.norm())
+    """(input BF16 [n,n,n,8], target e~ sqrt(N)/||r~|| FP32, mask, problem id, e~.r~/||r~||^2): the
+    residual channel at unit RMS, as the inference packing kernel makes it."""
+    rn = float(r64.norm()) / float(r64.numel()) ** 0.5
     x = torch.zeros((n, n, n, IN_CH), device=r64.device, dtype=torch.float32)
     x[..., 0] = (r64 / rn).float().view(n, n, n)
     x[..., 1] = feat.view(n, n, n)
@@ -442,22 +444,21 @@ def apply_A64(mg, e64: torch.Tensor) -> torch.Tensor:
 
 def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: float = 3e-4, seed: int = 0,
           log_every: int = 50, log=print, clip: float = 1.0):
-    """Adam on the scaled residual of the network's answer, ||A~ (y - e~)||^2 with
-    ||A~ e~|| = ||r~ / ||r~|||| = 1. (The operator is not symmetric, and its
-    symmetric part is indefinite near Dirichlet and reflected Neumann rows, so
-    an energy norm is not available.) The output gain starts at the
-    least-squares optimum over a few samples."""
-    with torch.no_grad():
-        net.out_gain = 1.0
-        gains = []
-        for x, e, m, pid, _ in samples[:: max(1, len(samples) // 8)]:
-            Ay = ops[pid](net.forward_train(x[None].contiguous())[0, ..., 0] * m)
-            Ae = ops[pid](e * m)
-            gains.append(float((Ay * Ae).sum() / ((Ay * Ay).sum() + 1e-30)))
-        net.out_gain = float(np.median(gains))
+    """Adam on the one-step residual reduction of the network's direction:
+    for z = N(r~), the best multiple of A~ z removes the part of r~ along it, so
+
+        loss = 1 - (r~ . A~ z)^2 / (|r~|^2 |A~ z|^2)  in [0, 1],
+
+    the squared sine between r~ and A~ z. It is invariant to the scale of z (as
+    flexible CG is), so the large and sample-dependent dynamic range of A~^-1
+    does not enter. (An energy norm is not available: the operator is not
+    symmetric and its symmetric part is indefinite near Dirichlet and reflected
+    Neumann rows.) The output gain is set afterwards to the least-squares
+    optimum, which only matters for callers that use z without a line search."""
     opt = torch.optim.Adam(net.params, lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=steps, eta_min=lr * 0.05)
     gen = torch.Generator(device="cpu").manual_seed(seed)
+    net.out_gain = 1.0
     hist = []
     for step in range(steps):
         idx = torch.randint(len(samples), (batch,), generator=gen).tolist()
@@ -466,7 +467,11 @@ def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: 
         loss = 0.0
         for b, i in enumerate(idx):
             _, e, m, pid, _ = samples[i]
-            loss = loss + (ops[pid]((y[b] - e) * m) ** 2).sum() / batch
+            Az = ops[pid](y[b] * m)
+            with torch.no_grad():
+                r = ops[pid](e * m)
+            c = (Az * r).sum()
+            loss = loss + (1.0 - c * c / ((Az * Az).sum() * (r * r).sum() + 1e-30)) / batch
         opt.zero_grad(set_to_none=True)
         loss.backward()
         torch.nn.utils.clip_grad_norm_(net.params, clip)
@@ -475,6 +480,13 @@ def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: 
         hist.append(float(loss.detach()))
         if log and (step % log_every == 0 or step == steps - 1):
             log(f"[train] step {step} loss {hist[-1]:.4f}")
+    with torch.no_grad():
+        gains = []
+        for x, e, m, pid, _ in samples[:: max(1, len(samples) // 8)]:
+            Az = ops[pid](net.forward_train(x[None].contiguous())[0, ..., 0] * m)
+            r = ops[pid](e * m)
+            gains.append(float((Az * r).sum() / ((Az * Az).sum() + 1e-30)))
+        net.out_gain = float(np.median(gains))
     net.refresh()
     return hist
 

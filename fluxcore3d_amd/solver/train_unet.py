@@ -4,7 +4,7 @@
 
 Training pairs come from smooth-noise blob conduction problems at 128^3 with
 k_s / k_f log-uniform in [1, 1e4] (GMG-FCG residual / error snapshots plus
-smooth random errors); the loss is the scaled residual of the network answer on the
+smooth random errors); the loss is the one-step residual reduction of the network direction on the
 Jacobi-scaled system (see nn_precond.train). Evaluation uses problems with seeds outside the
 training range at 128^3 and 256^3, FP32 V-cycle / FP64 outer, tolerance 1e-8.
 """

@@ -104,9 +104,12 @@ typedef struct fc_aa_collision {
 /* storage: 0 FP32 populations, 1 FP16 deviations f - w_i, 2 BF16 deviations.
  * px/py/pz: periodic axes (non-periodic axes carry a one-cell ghost layer). */
 int64_t fc_aa_slot_count(int nx, int ny, int nz, int px, int py, int pz);
+/* One fused step (replaces the sequence of lbm_cuda_ext.py:445-517). rho/u/v/w: pre-collision
+ * macros (all four or none); solid_macros = 0 skips the constant solid-node values (1, 0, 0, 0)
+ * when the caller passes the same buffers as on the previous step. */
 int fc_aa_step(void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables *tables,
                const fc_aa_collision *collision, int parity, int first, float *rho, float *u, float *v, float *w,
-               hipStream_t stream);
+               int solid_macros, hipStream_t stream);
 int fc_aa_materialize(const void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz,
                       const fc_aa_tables *tables, int parity, int first, float *f_out, hipStream_t stream);
 int fc_aa_load(void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables *tables,
@@ -220,11 +223,11 @@ size_t fc_conv3d_dw_workspace_bytes(int cout, int cin, int n_chunks);
 int fc_conv3d_bwd_weight(const void *x0, int c0, int up0, const void *x1, int c1, const void *dy, int cout, int B,
                          int D, int H, int W, int n_chunks, float *dw, void *workspace, size_t ws_bytes,
                          hipStream_t stream);
-/* x[p] = {r[p] / sqrt(scal[slot]), feat1[p], mask[p], 0 x 5} (BF16, 8 channels) */
+/* x[p] = {r[p] sqrt(n / scal[slot]), feat1[p], mask[p], 0 x 5} (BF16, 8 channels; residual at unit RMS) */
 int fc_unet_pack_input(const float *r, const float *feat1, const float *mask, const double *scal, int slot,
                        int64_t n, void *out, hipStream_t stream);
 int fc_unet_avgpool2(const void *in, int B, int D, int H, int W, int C, void *out, hipStream_t stream);
-/* z[p] = sqrt(scal[slot]) * y[p * cy] * mask[p] * d[p] (mask, d optional) */
+/* z[p] = sqrt(scal[slot] / n) * y[p * cy] * mask[p] * d[p] (mask, d optional) */
 int fc_unet_unpack(const float *y, int cy, const float *mask, const float *d, const double *scal, int slot,
                    int64_t n, float *z, hipStream_t stream);
 /* out[p] = r[p] * d[p] */

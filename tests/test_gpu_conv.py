@@ -156,8 +156,8 @@ def test_inference_matches_training_graph():
     z = torch.empty(n, device=DEV)
     net.apply(r, z, feat, mask, dsq, scal, partial, (D, H, W))
     rs = r.double() * dsq.double()
-    s = float(rs.norm())
-    assert abs(float(scal[nn.NN_SLOT]) ** 0.5 - s) <= 1e-6 * s
+    assert abs(float(scal[nn.NN_SLOT]) ** 0.5 - float(rs.norm())) <= 1e-6 * float(rs.norm())
+    s = float(rs.norm()) / n ** 0.5  # the residual enters at unit RMS
     x = torch.zeros((1, D, H, W, 8), device=DEV)
     x[..., 0] = (rs / s).float().view(D, H, W)
     x[..., 1] = feat.view(D, H, W)
