@@ -1,0 +1,241 @@
+// krylov.hip — flexible GCR(m) in FP64 for the conduction system A x = b.
+//
+// Minimal-residual Krylov iteration for a non-symmetric operator and a
+// variable preconditioner (the learned one): with z = P(r),
+//   p = z,  w = A p,
+//   w -= sum_j c_j W_j,  p -= sum_j c_j P_j,  c_j = (w . W_j) / (W_j . W_j)   (last m pairs)
+//   alpha = (r . w) / (w . w),  x += alpha p,  r -= alpha w.
+// Each step removes the component of r along the new, orthogonalised A-image, so
+// ||r|| never grows. Vectors are FP64, the operator uses the FP32 coefficients
+// of the finest multigrid level; scalars stay in device memory. Reductions run
+// a fixed grid (one partial per block, folded by one workgroup), so the
+// summation order is fixed.
+#include "fc_common.hpp"
+
+using namespace fc;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr unsigned kMaxBlocks = 8192;
+constexpr int kMaxDirs = 8;
+
+// scalar slots (per call family)
+enum { K_RW = 0, K_WW = 1, K_RR = 2, K_ALPHA = 3, K_DOT0 = 8, K_WWHIST0 = 16, K_SLOTS = 24 };
+
+struct Stencil {
+  const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL;
+  const float* mask;
+  int nx, ny, nz, ax, out_idx;
+  double dt;
+};
+
+struct Dirs {
+  const double* W[kMaxDirs];
+  double* P[kMaxDirs];
+  int slot[kMaxDirs];  // history slot of each direction's W.W
+  int m;
+};
+
+__device__ __forceinline__ int lo_ghost(const Stencil& s, int axis) {
+  return (s.ax != axis || s.out_idx == 0) ? 1 : 0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  // wavefront sum by butterfly exchanges
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partial, int stride) {
+  __shared__ double sh[kBlock / 64][NV];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    const double s = wave_sum(v[a]);
+    if (lane == 0) sh[wid][a] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += sh[w][threadIdx.x];
+    partial[(int64_t)blockIdx.x * stride + threadIdx.x] = s;
+  }
+}
+
+// scal[dst + a] = sum over blocks of partial[b * stride + a], a < nv
+__global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__ partial, int nblocks, int stride,
+                                                      int nv, double* __restrict__ scal, int dst) {
+  __shared__ double sh[kBlock / 64];
+  for (int a = 0; a < nv; ++a) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) v += partial[(int64_t)b * stride + a];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) s += sh[w];
+      scal[dst + a] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// p = (double) z;  w = (p + dt L p) * mask
+__global__ __launch_bounds__(kBlock) void apply_kernel(const float* __restrict__ z, Stencil s, double* __restrict__ p,
+                                                       double* __restrict__ w) {
+  const int64_t n = (int64_t)s.nx * s.ny * s.nz;
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    const int64_t nynz = (int64_t)s.ny * s.nz;
+    const int i = (int)(idx / nynz), j = (int)((idx / s.nz) % s.ny), k = (int)(idx % s.nz);
+    const int iE = i < s.nx - 1 ? i + 1 : i - 1, iW = i > 0 ? i - 1 : lo_ghost(s, 0);
+    const int jN = j < s.ny - 1 ? j + 1 : j - 1, jS = j > 0 ? j - 1 : lo_ghost(s, 1);
+    const int kU = k < s.nz - 1 ? k + 1 : k - 1, kD = k > 0 ? k - 1 : lo_ghost(s, 2);
+    const double pc = (double)z[idx];
+    const double L = (double)s.diagL[idx] * pc - (double)s.aE[idx] * (double)z[iE * nynz + (int64_t)j * s.nz + k] -
+                     (double)s.aW[idx] * (double)z[iW * nynz + (int64_t)j * s.nz + k] -
+                     (double)s.aN[idx] * (double)z[i * nynz + (int64_t)jN * s.nz + k] -
+                     (double)s.aS[idx] * (double)z[i * nynz + (int64_t)jS * s.nz + k] -
+                     (double)s.aU[idx] * (double)z[i * nynz + (int64_t)j * s.nz + kU] -
+                     (double)s.aD[idx] * (double)z[i * nynz + (int64_t)j * s.nz + kD];
+    p[idx] = pc;
+    w[idx] = (pc + s.dt * L) * (double)s.mask[idx];
+  }
+}
+
+// partials of w . W_j (j < m)
+__global__ __launch_bounds__(kBlock) void dots_kernel(const double* __restrict__ w, Dirs d, int64_t n,
+                                                      double* __restrict__ partial) {
+  double v[kMaxDirs];
+#pragma unroll
+  for (int j = 0; j < kMaxDirs; ++j) v[j] = 0.0;
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    const double wi = w[idx];
+#pragma unroll
+    for (int j = 0; j < kMaxDirs; ++j)
+      if (j < d.m) v[j] += wi * d.W[j][idx];
+  }
+  block_partials<kMaxDirs>(v, partial, kMaxDirs);
+}
+
+// w -= sum c_j W_j; p -= sum c_j P_j with c_j = scal[K_DOT0 + j] / scal[slot_j]; partials of r.w, w.w
+__global__ __launch_bounds__(kBlock) void orth_kernel(double* __restrict__ w, double* __restrict__ p,
+                                                      const double* __restrict__ r, Dirs d,
+                                                      const double* __restrict__ scal, int64_t n,
+                                                      double* __restrict__ partial) {
+  double c[kMaxDirs];
+#pragma unroll
+  for (int j = 0; j < kMaxDirs; ++j) {
+    const double den = j < d.m ? scal[d.slot[j]] : 0.0;
+    c[j] = (j < d.m && den != 0.0) ? scal[K_DOT0 + j] / den : 0.0;
+  }
+  double v[2] = {0.0, 0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    double wi = w[idx], pi = p[idx];
+#pragma unroll
+    for (int j = 0; j < kMaxDirs; ++j) {
+      if (j < d.m) {
+        wi -= c[j] * d.W[j][idx];
+        pi -= c[j] * d.P[j][idx];
+      }
+    }
+    w[idx] = wi;
+    p[idx] = pi;
+    v[0] += r[idx] * wi;
+    v[1] += wi * wi;
+  }
+  block_partials<2>(v, partial, 2);
+}
+
+// alpha = r.w / w.w; x += alpha p; r -= alpha w; r32 = (float) r; partials of r.r
+__global__ __launch_bounds__(kBlock) void update_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                        float* __restrict__ r32, const double* __restrict__ p,
+                                                        const double* __restrict__ w,
+                                                        const double* __restrict__ scal, int64_t n,
+                                                        double* __restrict__ partial) {
+  const double ww = scal[K_WW];
+  const double alpha = ww > 0.0 ? scal[K_RW] / ww : 0.0;
+  double v[1] = {0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    x[idx] += alpha * p[idx];
+    const double rn = r[idx] - alpha * w[idx];
+    r[idx] = rn;
+    r32[idx] = (float)rn;
+    v[0] += rn * rn;
+  }
+  block_partials<1>(v, partial, 1);
+}
+
+__global__ void copy_scalar_kernel(double* __restrict__ scal, int src, int dst) { scal[dst] = scal[src]; }
+
+unsigned grid_of(int64_t n) {
+  const unsigned g = grid_for(n, kBlock);
+  return g < kMaxBlocks ? g : kMaxBlocks;
+}
+
+}  // namespace
+
+extern "C" {
+
+struct fc_gcr_stencil {
+  const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *mask;
+  double dt;
+  int nx, ny, nz, ax, out_idx;
+};
+
+int fc_gcr_scalar_slots(void) { return K_SLOTS; }
+int fc_gcr_max_directions(void) { return kMaxDirs; }
+int fc_gcr_partials_count(int64_t n) { return (int)grid_of(n) * kMaxDirs; }
+
+// One flexible GCR step after the preconditioner produced z32 (FP32):
+// p, w = A p, orthogonalisation against the m stored (P_j, W_j) with their W_j.W_j
+// in scal[ww_slot[j]], alpha, x/r update, ||r||^2 -> scal[K_RR], w.w -> scal[store_slot].
+int fc_gcr_step(const float* z32, const fc_gcr_stencil* st, double* p, double* w, const double* const* Wd,
+                double* const* Pd, const int* ww_slot, int m, int store_slot, double* x, double* r, float* r32,
+                double* partial, double* scal, hipStream_t stream) {
+  FC_REQUIRE(z32 && st && p && w && x && r && r32 && partial && scal, "gcr_step: null pointer argument");
+  FC_REQUIRE(m >= 0 && m <= kMaxDirs, "gcr_step: 0 <= m <= %d directions", kMaxDirs);
+  FC_REQUIRE(store_slot >= K_WWHIST0 && store_slot < K_SLOTS, "gcr_step: store slot out of range");
+  FC_REQUIRE(st->nx >= 2 && st->ny >= 2 && st->nz >= 2, "gcr_step: every extent must be >= 2");
+  Stencil s{st->aE, st->aW, st->aN, st->aS, st->aU, st->aD, st->diagL, st->mask, st->nx, st->ny, st->nz,
+            st->ax, st->out_idx, st->dt};
+  FC_REQUIRE(s.aE && s.aW && s.aN && s.aS && s.aU && s.aD && s.diagL && s.mask, "gcr_step: null coefficient");
+  const int64_t n = (int64_t)s.nx * s.ny * s.nz;
+  Dirs d{};
+  d.m = m;
+  for (int j = 0; j < m; ++j) {
+    FC_REQUIRE(Wd[j] && Pd[j], "gcr_step: null direction %d", j);
+    FC_REQUIRE(ww_slot[j] >= K_WWHIST0 && ww_slot[j] < K_SLOTS, "gcr_step: history slot out of range");
+    d.W[j] = Wd[j];
+    d.P[j] = Pd[j];
+    d.slot[j] = ww_slot[j];
+  }
+  const unsigned g = grid_of(n);
+  hipLaunchKernelGGL(apply_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, z32, s, p, w);
+  if (int rc = check_launch("gcr_step/apply")) return rc;
+  if (m > 0) {
+    hipLaunchKernelGGL(dots_kernel, dim3(g), dim3(kBlock), 0, stream, (const double*)w, d, n, partial);
+    if (int rc = check_launch("gcr_step/dots")) return rc;
+    hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, (const double*)partial, (int)g, kMaxDirs, m,
+                       scal, (int)K_DOT0);
+    if (int rc = check_launch("gcr_step/fold dots")) return rc;
+  }
+  hipLaunchKernelGGL(orth_kernel, dim3(g), dim3(kBlock), 0, stream, w, p, (const double*)r, d, (const double*)scal,
+                     n, partial);
+  if (int rc = check_launch("gcr_step/orth")) return rc;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, (const double*)partial, (int)g, 2, 2, scal,
+                     (int)K_RW);
+  if (int rc = check_launch("gcr_step/fold orth")) return rc;
+  hipLaunchKernelGGL(copy_scalar_kernel, dim3(1), dim3(1), 0, stream, scal, (int)K_WW, store_slot);
+  if (int rc = check_launch("gcr_step/store")) return rc;
+  hipLaunchKernelGGL(update_kernel, dim3(g), dim3(kBlock), 0, stream, x, r, r32, (const double*)p,
+                     (const double*)w, (const double*)scal, n, partial);
+  if (int rc = check_launch("gcr_step/update")) return rc;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, (const double*)partial, (int)g, 1, 1, scal,
+                     (int)K_RR);
+  return check_launch("gcr_step/fold update");
+}
+
+}  // extern "C"

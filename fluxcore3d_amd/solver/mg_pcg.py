@@ -14,6 +14,7 @@ reads one FP64 value per iteration for the stopping test.
 """
 from __future__ import annotations
 
+import ctypes
 from typing import Optional, Tuple
 
 import torch
@@ -122,3 +123,116 @@ def patch_mg_pcg(mg_solver, pe_threshold: float = 1.5):
 
     mg_solver.solve = types.MethodType(_solve_pcg, mg_solver)
     return mg_solver
+
+
+# ----------------------------------------------------------- flexible GCR(m)
+_lib.register({
+    "fc_gcr_scalar_slots": (_lib.I32, []),
+    "fc_gcr_max_directions": (_lib.I32, []),
+    "fc_gcr_partials_count": (_lib.I32, [_lib.I64]),
+    "fc_gcr_step": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.I32, _lib.I32, _lib.P,
+                                _lib.P, _lib.P, _lib.P, _lib.P, _lib.P]),
+}, unchecked={"fc_gcr_scalar_slots", "fc_gcr_max_directions", "fc_gcr_partials_count"})
+
+K_RR = 2
+K_WWHIST0 = 16
+
+
+class GCRStencil(ctypes.Structure):
+    _fields_ = [("aE", ctypes.c_void_p), ("aW", ctypes.c_void_p), ("aN", ctypes.c_void_p), ("aS", ctypes.c_void_p),
+                ("aU", ctypes.c_void_p), ("aD", ctypes.c_void_p), ("diagL", ctypes.c_void_p),
+                ("mask", ctypes.c_void_p), ("dt", ctypes.c_double), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("ax", ctypes.c_int), ("out_idx", ctypes.c_int)]
+
+
+class GCRWorkspace:
+    def __init__(self, shape, device, m: int = 4):
+        mmax = int(_lib.call("fc_gcr_max_directions"))
+        if not 0 <= m < mmax:
+            raise ValueError(f"GCR truncation m must be in [0, {mmax - 1}]")
+        n = int(torch.Size(shape).numel())
+        f64 = dict(device=device, dtype=torch.float64)
+        self.m, self.n = m, n
+        self.fcg = FCGWorkspace(shape, device)  # x, r, b, r32, z32 and the true-residual reductions
+        self.P = [torch.zeros(n, **f64) for _ in range(m + 1)]
+        self.W = [torch.zeros(n, **f64) for _ in range(m + 1)]
+        self.partial = torch.empty(int(_lib.call("fc_gcr_partials_count", n)), **f64)
+        self.scal = torch.zeros(int(_lib.call("fc_gcr_scalar_slots")), **f64)
+
+
+def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol: float = 1e-8, max_it: int = 300,
+             precond=None, m: int = 4, ws: Optional[GCRWorkspace] = None, on_iter=None,
+             check_every: int = 1) -> Tuple[torch.Tensor, dict]:
+    """Flexible GCR(m) on A x = b with FP64 vectors and a variable preconditioner
+    ``precond(r32, z32, ws)`` (default: one V-cycle). The residual norm never
+    increases; convergence is judged on the FP64 true residual. ``on_iter(it,
+    rel_res)`` may raise to stop the iteration early."""
+    lv0 = mg_solver._levels[0]
+    shape = lv0["shape"]
+    nx, ny, nz = shape
+    dev = mg_solver.device
+    if ws is None:
+        ws = GCRWorkspace(shape, dev, m)
+    fw = ws.fcg
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    if precond is None:
+        def precond(r32, z32, _ws):
+            mg_solver.precondition(r32, out=z32, masked_input=True)
+    fw.b.copy_(b.reshape(-1).to(torch.float64))
+    if x0 is None:
+        fw.x.zero_()
+    else:
+        fw.x.copy_(x0.reshape(-1).to(torch.float64))
+    dt = float(mg_solver._dt_cache)
+    mask = lv0["mask_f"]
+    st = GCRStencil(*[lv0[k].data_ptr() for k in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL")], mask.data_ptr(),
+                    dt, nx, ny, nz, int(mg_solver.ax), int(lv0["out_idx"]))
+
+    def true_residual():
+        _lib.call("fc_mg_true_residual", fw.x.data_ptr(), fw.b.data_ptr(), fw.r.data_ptr(), fw.r32.data_ptr(),
+                  mask.data_ptr(), lv0["coef"], dt, nx, ny, nz, mg_solver.ax, lv0["out_idx"], fw.partial.data_ptr(),
+                  fw.scal.data_ptr(), stream)
+        s = fw.scal[S_RR:S_BB + 1].cpu()
+        return float(s[0]), float(s[1])
+
+    rr, bb = true_residual()
+    bnorm = bb ** 0.5 + 1e-300
+    hist = [rr ** 0.5 / bnorm]
+    converged = hist[-1] < tol
+    it = 0
+    W_ptrs = (ctypes.c_void_p * max(m, 1))()
+    P_ptrs = (ctypes.c_void_p * max(m, 1))()
+    slots = (ctypes.c_int * max(m, 1))()
+    stopped = False
+    while not converged and it < max_it:
+        if on_iter is not None:
+            try:
+                on_iter(it, hist[-1])
+            except StopIteration:
+                stopped = True
+                break
+        precond(fw.r32, fw.z32, ws)
+        cur = it % (m + 1)
+        k = min(it, m)
+        for j in range(k):
+            src = (it - 1 - j) % (m + 1)
+            W_ptrs[j], P_ptrs[j], slots[j] = ws.W[src].data_ptr(), ws.P[src].data_ptr(), K_WWHIST0 + src
+        _lib.call("fc_gcr_step", fw.z32.data_ptr(), ctypes.byref(st), ws.P[cur].data_ptr(), ws.W[cur].data_ptr(),
+                  W_ptrs, P_ptrs, slots, k, K_WWHIST0 + cur, fw.x.data_ptr(), fw.r.data_ptr(), fw.r32.data_ptr(),
+                  ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
+        it += 1
+        if it % check_every and it < max_it:
+            continue
+        rec = float(ws.scal[K_RR].item()) ** 0.5 / bnorm
+        if not (rec < float("inf")):
+            hist.append(rec)
+            break
+        if rec < tol:
+            rr, _ = true_residual()
+            hist.append(rr ** 0.5 / bnorm)
+            converged = hist[-1] < tol
+        else:
+            hist.append(rec)
+    x = fw.x.view(shape).to(torch.float32) * mask
+    return x, {"converged": converged, "it": it, "res_rel": hist[-1], "hist": hist, "x64": fw.x.view(shape),
+               "stopped": stopped}

@@ -3,10 +3,12 @@
     python -m fluxcore3d_amd.solver.train_unet --out fluxcore3d_amd/data/unet_precond.npz
 
 Training pairs come from smooth-noise blob conduction problems at 128^3 with
-k_s / k_f log-uniform in [1, 1e4] (GMG-FCG residual / error snapshots plus
-smooth random errors); the loss is the one-step residual reduction of the network direction on the
-Jacobi-scaled system (see nn_precond.train). Evaluation uses problems with seeds outside the
-training range at 128^3 and 256^3, FP32 V-cycle / FP64 outer, tolerance 1e-8.
+k_s / k_f log-uniform in [1, 1e4] (random errors of random spectra and their
+residuals, nn_precond.harvest_pairs); the loss is the one-step residual
+reduction of the network direction on the Jacobi-scaled system
+(nn_precond.train). Evaluation uses problems with seeds outside the training
+range at 128^3 and 256^3, FP32 V-cycle / FP64 outer vectors, tolerance 1e-8:
+flexible GCR with the network against GMG-PCG (flexible CG and GCR).
 """
 from __future__ import annotations
 
@@ -20,7 +22,7 @@ import torch
 
 def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None) -> dict:
     from .. import geometry
-    from .mg_pcg import FCGWorkspace, fcg_fp64
+    from .mg_pcg import FCGWorkspace, fcg_fp64, gcr_fp64
     from .nn_precond import _mg_for, nn_fcg
 
     c = geometry.conduction_problem(n, seed=seed, ratio=ratio, device=device)
@@ -33,12 +35,19 @@ def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None) -> d
     _, gi = fcg_fp64(mg, c["b"], tol=tol, max_it=1000, ws=ws)
     torch.cuda.synchronize()
     t_gmg = time.perf_counter() - t0
+    gcr_fp64(mg, c["b"], tol=tol, max_it=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, gg = gcr_fp64(mg, c["b"], tol=tol, max_it=1000)
+    torch.cuda.synchronize()
+    t_gcr = time.perf_counter() - t0
     t0 = time.perf_counter()
     _, ni = nn_fcg(mg, net, c["b"], c["k"], tol=tol, max_it=1000)
     torch.cuda.synchronize()
     t_nn = time.perf_counter() - t0
     return {"n": n, "seed": seed, "ratio": round(float(c["ratio"]), 2), "gmg_s": round(t_gmg, 4),
-            "gmg_it": gi["it"], "gmg_res": gi["res_rel"], "nn_s": round(t_nn, 4), "nn_it": ni["it"],
+            "gmg_it": gi["it"], "gmg_res": gi["res_rel"], "gmg_gcr_s": round(t_gcr, 4), "gmg_gcr_it": gg["it"],
+            "nn_s": round(t_nn, 4), "nn_it": ni["it"],
             "nn_it_network": ni.get("it_nn"), "nn_fallback": ni["fallback"], "nn_res": ni["res_rel"],
             "nn_converged": bool(ni["converged"])}
 
@@ -51,6 +60,7 @@ def main(argv=None):
                                                  "unet_precond.npz"))
     p.add_argument("--n", type=int, default=128)
     p.add_argument("--problems", type=int, default=6)
+    p.add_argument("--per-problem", type=int, default=24)
     p.add_argument("--steps", type=int, default=600)
     p.add_argument("--batch", type=int, default=2)
     p.add_argument("--lr", type=float, default=3e-4)
@@ -60,7 +70,7 @@ def main(argv=None):
     a = p.parse_args(argv)
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    samples, ops = harvest_pairs(n=a.n, problems=a.problems, device=dev)
+    samples, ops = harvest_pairs(n=a.n, problems=a.problems, per_problem=a.per_problem, device=dev)
     t_h = time.perf_counter() - t0
     print(f"[train_unet] {len(samples)} samples at {a.n}^3 in {t_h:.1f} s", flush=True)
     net = UNetPrecond(tuple(int(c) for c in a.channels.split(",")), device=dev, seed=a.seed)

@@ -233,6 +233,24 @@ int fc_unet_unpack(const float *y, int cy, const float *mask, const float *d, co
 /* out[p] = r[p] * d[p] */
 int fc_unet_scale(const float *r, const float *d, int64_t n, float *out, hipStream_t stream);
 
+/* ---- flexible GCR(m) in FP64 (csrc/krylov.hip) ----
+ * No reference counterpart: the minimal-residual Krylov method used with the
+ * learned preconditioner (the reference's Krylov solvers are mg_pcg.py:33-187). */
+typedef struct fc_gcr_stencil {
+  const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *mask; /* finest-level operator, as fc_mg_coef */
+  double dt;
+  int nx, ny, nz, ax, out_idx;
+} fc_gcr_stencil;
+int fc_gcr_scalar_slots(void);
+int fc_gcr_max_directions(void);
+int fc_gcr_partials_count(int64_t n);
+/* p = z, w = A p, orthogonalise (w, p) against the m stored pairs (W_j, P_j) whose W_j.W_j sit in
+ * scal[ww_slot[j]], alpha = r.w / w.w, x += alpha p, r -= alpha w, r32 = (float) r;
+ * scal[2] = r.r, scal[store_slot] = w.w (slots 16..23 hold the W.W history). */
+int fc_gcr_step(const float *z32, const fc_gcr_stencil *stencil, double *p, double *w, const double *const *W,
+                double *const *P, const int *ww_slot, int m, int store_slot, double *x, double *r, float *r32,
+                double *partial, double *scal, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,7 @@ def test_python_binding_knows_every_declared_symbol():
     from fluxcore3d_amd import _lib
     import fluxcore3d_amd.pre_post.sdf_tracing  # noqa: F401  (registers its entry points)
     import fluxcore3d_amd.solver.lbm_cuda_ext  # noqa: F401
-    for mod in ("fluxcore3d_amd.solver.mg_cuda_ext", "fluxcore3d_amd.solver.thermal_cuda_ext",
+    for mod in ("fluxcore3d_amd.solver.mg_cuda_ext", "fluxcore3d_amd.solver.thermal_cuda_ext", "fluxcore3d_amd.solver.mg_pcg",
                 "fluxcore3d_amd.solver.nn_precond"):
         try:
             __import__(mod)

@@ -100,3 +100,24 @@ def test_fcg_fp64_true_residual_and_solution():
     xo, oinfo = mg.fcg_mg_fp64(h, c["b"], tol=1e-8, max_it=200)
     assert rel(x64 * m, xo) < 1e-6
     assert abs(info["it"] - oinfo["it"]) <= 2
+
+
+@pytest.mark.parametrize("m", [2, 4])
+def test_gcr_fp64_monotone_and_solution(m):
+    """Flexible GCR(m) with the V-cycle: the residual never grows, the FP64 true
+    residual of the returned x is below the tolerance (checked independently with
+    the oracle operator), and x equals the oracle FCG solution."""
+    from fluxcore3d_amd.solver.mg_pcg import gcr_fp64
+    c = thermal_cases.sphere_conduction(64, 32, 32, dt=1.0e6)
+    s, h = _solver_pair(c)
+    x, info = gcr_fp64(s, T(c["b"]), tol=1e-8, max_it=200, m=m)
+    assert info["converged"], info["hist"][-5:]
+    hist = np.asarray(info["hist"][:-1])  # the last entry is the true residual
+    assert np.all(np.diff(hist) <= 1e-12 * hist[:-1] + 1e-300)
+    co64 = tuple(a.astype(np.float64) for a in h.levels[0]["co"])
+    x64 = info["x64"].cpu().numpy()
+    msk = c["mask"].astype(np.float64)
+    r = (c["b"].astype(np.float64) * msk - mg.mop(x64, co64, c["dt"], 0, h.levels[0]["out_idx"])) * msk
+    assert np.linalg.norm(r) / np.linalg.norm(c["b"] * msk) < 1.01e-8
+    xo, _ = mg.fcg_mg_fp64(h, c["b"], tol=1e-10, max_it=300)
+    assert rel(x64 * msk, xo) < 1e-6
