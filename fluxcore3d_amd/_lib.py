@@ -27,7 +27,7 @@ SZ = C.c_size_t
 class AATables(C.Structure):
     _fields_ = [
         ("nflag", P), ("wave_base", P), ("wave_meta", P), ("lk_code", P), ("s_code", P), ("f_code", P),
-        ("sp_mask", P), ("sp_base", P),
+        ("lane_nodes", P), ("n_lanes", I64), ("sp_mask", P), ("sp_base", P),
         ("w1", P), ("w2", P), ("A", P), ("B", P), ("ff_idx", P), ("n_v", I64),
     ]
 

@@ -59,6 +59,8 @@ struct Tables {
   const int32_t* lk_code;    // [n_links] node in group | t << 6 | opp t << 11 | V << 16 | override << 17 | v << 18
   const int32_t* s_code;     // [n_src] node in group | q << 6 | override << 17
   const int32_t* f_code;     // [n_ff] node in group | p << 6
+  const int32_t* lane_nodes;  // [n_lanes] node of each lane (null: lane r is node r)
+  int64_t n_lanes;
   const int4* sp_mask;       // [n_special] {L, S, V, F}
   const int4* sp_base;       // [n_special] {link_base, v_base, ff_base, 0}
   const float* w1;           // [n_links]
@@ -379,24 +381,25 @@ template <typename ST, int KIND>
 __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 5)) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
                                                          int parity, int first, float* __restrict__ rho_o,
                                                          float* __restrict__ u_o, float* __restrict__ v_o,
-                                                         float* __restrict__ w_o, int solid_macros) {
+                                                         float* __restrict__ w_o) {
   __shared__ float tile[kWaves][19][64];
+  // lane r works on node x (every node, or the fluid nodes only); solid lanes idle
   const unsigned lb = logical_block(blockIdx.x, gridDim.x);
-  const int64_t x = (int64_t)lb * kBlock + threadIdx.x;
+  const int64_t r = (int64_t)lb * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   // per-wave entry ranges (scalar loads)
-  const int gw = __builtin_amdgcn_readfirstlane((int)(x >> 6));
+  const int gw = __builtin_amdgcn_readfirstlane((int)(r >> 6));
   int4 wm0 = make_int4(0, 0, 0, 0), wm1 = make_int4(0, 0, 0, 0);
-  if ((int64_t)gw * 64 < g.N) {
+  if ((int64_t)gw * 64 < t.n_lanes) {
     wm0 = t.wave_meta[gw];
     wm1 = t.wave_meta[gw + 1];
   }
   const int l_lo = wm0.x, nL = wm1.x - wm0.x, v_lo = wm0.y, f_lo = wm0.z, nF = wm1.z - wm0.z;
   const int s_lo = wm0.w, nSrc = wm1.w - wm0.w;
   const bool special_wave = (nL | nF | nSrc) != 0;
-  const bool live = x < g.N;
-  const uint8_t fl = live ? t.nflag[x] : (uint8_t)F_SOLID;
-  const bool fluid = live && !(fl & F_SOLID);
+  const bool live = r < t.n_lanes;
+  const int64_t x = live ? (t.lane_nodes ? (int64_t)t.lane_nodes[r] : r) : 0;
+  const bool fluid = live && !(t.nflag[x] & F_SOLID);
 
   // Each lane's first entry of every list, requested before the populations so
   // that their latency overlaps (entries e >= 64 are loaded in the loops below).
@@ -476,9 +479,7 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 5)) void aa_step_kernel(ST
     for (int q = 0; q < 19; ++q) f[q] = tile[wv][q][lane];
   }
 
-  if (live && !fluid) {
-    if (rho_o && solid_macros) { rho_o[x] = 1.f; u_o[x] = 0.f; v_o[x] = 0.f; w_o[x] = 0.f; }
-  } else if (fluid) {
+  if (fluid) {
     float sigma = 0.f;
     if (c.sponge) sigma = c.sponge[c.sponge_axis == 0 ? i : (c.sponge_axis == 1 ? j : k)];
     float rho, ux, uy, uz;
@@ -532,6 +533,15 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 5)) void aa_step_kernel(ST
       Bw[t.ff_idx[f_lo + e]] = tile[wv][(code >> 6) & 31][code & 63];
     }
   }
+}
+
+// macros of solid nodes: rho = 1, u = 0 (Q5)
+__global__ __launch_bounds__(kBlock) void aa_solid_macros_kernel(const uint8_t* __restrict__ nflag, int64_t N,
+                                                                 float* __restrict__ rho, float* __restrict__ u,
+                                                                 float* __restrict__ v, float* __restrict__ w) {
+  const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (x >= N || !(nflag[x] & F_SOLID)) return;
+  rho[x] = 1.f; u[x] = 0.f; v[x] = 0.f; w[x] = 0.f;
 }
 
 // ---- helpers for kernels that visit arbitrary nodes -------------------------
@@ -794,6 +804,8 @@ struct fc_aa_tables {
   const int32_t* lk_code;
   const int32_t* s_code;
   const int32_t* f_code;
+  const int32_t* lane_nodes;  // [n_lanes], or null when lane r is node r
+  int64_t n_lanes;
   const int32_t* sp_mask;  // [n_special][4]
   const int32_t* sp_base;  // [n_special][4]
   const float* w1;
@@ -826,6 +838,9 @@ static int pack(Grid& g, Tables& t, const fc_aa_tables* tb, int nx, int ny, int 
   FC_REQUIRE(tb && tb->nflag && tb->wave_base && tb->wave_meta, "aa: node flag tables are required");
   t.nflag = tb->nflag; t.wave_base = tb->wave_base; t.wave_meta = (const int4*)tb->wave_meta;
   t.lk_code = tb->lk_code; t.s_code = tb->s_code; t.f_code = tb->f_code;
+  t.lane_nodes = tb->lane_nodes; t.n_lanes = tb->n_lanes;
+  FC_REQUIRE(t.n_lanes >= 0 && t.n_lanes <= g.N, "aa: lane count out of range");
+  FC_REQUIRE(t.lane_nodes || t.n_lanes == g.N, "aa: lane list required unless there is one lane per node");
   t.sp_mask = (const int4*)tb->sp_mask; t.sp_base = (const int4*)tb->sp_base;
   t.w1 = tb->w1; t.w2 = tb->w2; t.A = tb->A; t.B = tb->B; t.ff_idx = tb->ff_idx; t.n_v = tb->n_v;
   return 0;
@@ -847,10 +862,16 @@ int fc_aa_step(void* pop, int storage, int nx, int ny, int nz, int px, int py, i
   c.cs_dx2 = col->cs_dx2;
   for (int d = 0; d < 3; ++d) c.F[d] = col->force[d];
   c.use_force = col->use_force; c.sponge = col->sponge; c.sponge_axis = col->sponge_axis;
-  const unsigned grid = grid_for(g.N, kBlock);
+  if (rho && solid_macros) {
+    hipLaunchKernelGGL(aa_solid_macros_kernel, dim3(grid_for(g.N, kBlock)), dim3(kBlock), 0, stream, t.nflag, g.N,
+                       rho, u, v, w);
+    if (int rc = check_launch("aa_step/solid macros")) return rc;
+  }
+  if (t.n_lanes == 0) return 0;
+  const unsigned grid = grid_for(t.n_lanes, kBlock);
 #define FC_STEP(T, K)                                                                                          \
   hipLaunchKernelGGL((aa_step_kernel<T, K>), dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, c, parity, first, \
-                     rho, u, v, w, solid_macros)
+                     rho, u, v, w)
 #define FC_STEP_KIND(T)                      \
   switch (c.kind) {                          \
     case 0: FC_STEP(T, 0); break;            \

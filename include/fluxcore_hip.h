@@ -74,12 +74,15 @@ int fc_lbm_convective_bc(float *f, const uint8_t *solid, int nx, int ny, int nz,
 typedef struct fc_aa_tables {
   const uint8_t *nflag;     /* [N] 1 solid, 2 special, bits 3..5 first BC plane + 1 */
   const int32_t *wave_base; /* [ceil(N/64)] special nodes before each 64-node group */
-  const int32_t *wave_meta; /* [ceil(N/64) + 1][4] per 64-node group: first link, first second-node link,
-                               first publish entry, first solid-source entry (last row: totals) */
-  const int32_t *lk_code;   /* [n_links] node in group | t << 6 | opp(t) << 11 | second node << 16
-                               | override << 17 | second-node index in group << 18 */
-  const int32_t *s_code;    /* [n_src] node in group | q << 6 | override << 17 (input w_q after step 1) */
-  const int32_t *f_code;    /* [n_publish] node in group | p << 6 (post-collision f_p -> B[ff_idx]) */
+  const int32_t *wave_meta; /* [ceil(n_lanes/64) + 1][4] per wavefront of 64 lanes: first link, first
+                               second-node link, first publish entry, first solid-source entry (last: totals) */
+  const int32_t *lk_code;   /* [n_links] lane | t << 6 | opp(t) << 11 | second node << 16
+                               | override << 17 | second-node index in the wavefront << 18 */
+  const int32_t *s_code;    /* [n_src] lane | q << 6 | override << 17 (input w_q after step 1) */
+  const int32_t *f_code;    /* [n_publish] lane | p << 6 (post-collision f_p -> B[ff_idx]) */
+  const int32_t *lane_nodes; /* [n_lanes] node of each lane, 64 lanes per wavefront (the fluid nodes, or
+                               NULL for one lane per node); lanes on solid nodes idle */
+  int64_t n_lanes;
   const int32_t *sp_mask;   /* [n_special][4] link, solid-source, second-node, publish masks */
   const int32_t *sp_base;   /* [n_special][4] first link, first second-node link, first publish entry */
   const float *w1;          /* [n_links] Bouzidi weights */

@@ -34,8 +34,8 @@ def _group_of(entry_base, n_entries):
     return np.searchsorted(entry_base, np.arange(n_entries), side="right") - 1
 
 
-def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="reference"):
-    tb = build_tables(nx, ny, nz, solid, links, periodic, ibb_mode)
+def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="reference", compact=True):
+    tb = build_tables(nx, ny, nz, solid, links, periodic, ibb_mode, compact=compact)
     N = nx * ny * nz
     gx, gy, gz = (0 if p else 1 for p in periodic)
     PY, PZ = ny + 2 * gy, nz + 2 * gz
@@ -57,13 +57,14 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
     wm = tb["wave_meta"].astype(np.int64)
     lk, sc, fc = (tb[k].astype(np.int64) for k in ("lk_code", "s_code", "f_code"))
     # decode the entry codes exactly as the fused kernel does
-    l_node = _group_of(wm[:, 0], lk.size) * 64 + (lk & 63)
+    fl = tb["lane_nodes"].astype(np.int64)
+    l_node = fl[_group_of(wm[:, 0], lk.size) * 64 + (lk & 63)]
     l_t, l_opp = (lk >> 6) & 31, (lk >> 11) & 31
     l_hasv, l_ovr = ((lk >> 16) & 1).astype(bool), ((lk >> 17) & 1).astype(bool)
     l_v = wm[_group_of(wm[:, 0], lk.size), 1] + (lk >> 18)
-    s_node = _group_of(wm[:, 3], sc.size) * 64 + (sc & 63)
+    s_node = fl[_group_of(wm[:, 3], sc.size) * 64 + (sc & 63)]
     s_q, s_ovr = (sc >> 6) & 31, ((sc >> 17) & 1).astype(bool)
-    f_node = _group_of(wm[:, 2], fc.size) * 64 + (fc & 63)
+    f_node = fl[_group_of(wm[:, 2], fc.size) * 64 + (fc & 63)]
     f_p = (fc >> 6) & 31
 
     self_slot = slot((0, 0, 0))
@@ -101,8 +102,9 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
     return macros
 
 
+@pytest.mark.parametrize("compact", [True, False])
 @pytest.mark.parametrize("case", ["periodic_pack", "walled_box"])
-def test_aa_model_matches_oracle(case):
+def test_aa_model_matches_oracle(case, compact):
     if case == "periodic_pack":
         c = cases.periodic_pack(n=24, seed=3)
         per = (1, 1, 1)
@@ -119,7 +121,7 @@ def test_aa_model_matches_oracle(case):
     f0 = olbm.feq(rho, *u, np.float64)
     om = 1.0 / 0.55
     steps = 6
-    mm = model_run(nx, ny, nz, solid, c["links"], per, f0, steps, om)
+    mm = model_run(nx, ny, nz, solid, c["links"], per, f0, steps, om, compact=compact)
     f = f0.copy()
     for _ in range(steps):
         f, *mo = olbm.step(f, solid, nx, ny, nz, per, "bgk", om, 1.0, None, 0.0, c["links"])
@@ -127,21 +129,24 @@ def test_aa_model_matches_oracle(case):
         assert np.abs(a - b).max() < 1e-12
 
 
-def test_entry_codes_match_node_masks():
+@pytest.mark.parametrize("compact", [True, False])
+def test_entry_codes_match_node_masks(compact):
     """The fused step's per-entry codes (links, solid sources, publications, with
-    per-group first-entry offsets) describe exactly the per-node masks and bases
-    used by the plane kernels (sp_mask / sp_base)."""
+    per-wavefront first-entry offsets over the compacted fluid-node list) describe
+    exactly the per-node masks used by the plane kernels (sp_mask)."""
     c = cases.periodic_pack(n=24, seed=3)
     n = c["nx"]
-    tb = build_tables(n, n, n, c["solid"], c["links"], (1, 1, 1))
+    tb = build_tables(n, n, n, c["solid"], c["links"], (1, 1, 1), compact=compact)
     wm = tb["wave_meta"].astype(np.int64)
-    groups = (n ** 3 + 63) // 64
-    assert wm.shape == (groups + 1, 4)
+    waves = (tb["n_lanes"] + 63) // 64
+    assert wm.shape == (waves + 1, 4)
+    fl = tb["lane_nodes"].astype(np.int64)
+    assert np.array_equal(fl, np.nonzero(~c["solid"].reshape(-1))[0] if compact else np.arange(n ** 3))
     assert tuple(wm[-1]) == (tb["n_links"], tb["n_v"], tb["ff_node"].size, tb["n_src"])
     N = n ** 3
     L, S, V, F = (np.zeros(N, np.int64) for _ in range(4))
     lk = tb["lk_code"].astype(np.int64)
-    node = _group_of(wm[:, 0], lk.size) * 64 + (lk & 63)
+    node = fl[_group_of(wm[:, 0], lk.size) * 64 + (lk & 63)]
     assert np.array_equal(node, tb["link_node"])
     assert np.array_equal((lk >> 6) & 31, tb["link_dir"]) and np.array_equal((lk >> 11) & 31, OPP[tb["link_dir"]])
     np.bitwise_or.at(L, node, 1 << ((lk >> 6) & 31))
@@ -150,9 +155,9 @@ def test_entry_codes_match_node_masks():
     v_idx = wm[_group_of(wm[:, 0], lk.size), 1] + (lk >> 18)
     assert np.array_equal(v_idx[hv], np.arange(tb["n_v"]))
     sc = tb["s_code"].astype(np.int64)
-    np.bitwise_or.at(S, _group_of(wm[:, 3], sc.size) * 64 + (sc & 63), 1 << ((sc >> 6) & 31))
+    np.bitwise_or.at(S, fl[_group_of(wm[:, 3], sc.size) * 64 + (sc & 63)], 1 << ((sc >> 6) & 31))
     fc = tb["f_code"].astype(np.int64)
-    f_node = _group_of(wm[:, 2], fc.size) * 64 + (fc & 63)
+    f_node = fl[_group_of(wm[:, 2], fc.size) * 64 + (fc & 63)]
     assert np.array_equal(f_node, tb["ff_node"])
     np.bitwise_or.at(F, f_node, 1 << ((fc >> 6) & 31))
     sp = tb["sp_idx"]

@@ -198,15 +198,17 @@ def test_pin_fin_channel_trt(engine):
     assert rel(sim.f.cpu().numpy().reshape(19, -1), f_want) < 1e-5
 
 
+@pytest.mark.parametrize("lanes", ["all", "fluid"])
 @pytest.mark.parametrize("kind", ["bgk", "mrt"])
-def test_aa_periodic_pack(kind):
-    """Fully periodic sphere pack (48^3, porosity 0.6) with reference Bouzidi links."""
+def test_aa_periodic_pack(kind, lanes):
+    """Fully periodic sphere pack (48^3, porosity 0.6) with reference Bouzidi links,
+    one lane per node and one lane per fluid node."""
     from fluxcore3d_amd.solver.aa_engine import AAEngine
     c = cases.periodic_pack()
     c["omega"] = 1.0 / 0.55
     S = cases.mrt_S(c["omega"]) if kind == "mrt" else None
     eng = AAEngine(c["nx"], c["ny"], c["nz"], c["solid_flow"], periodic=(1, 1, 1), collision=kind, omega=c["omega"],
-                   S=S, links=c["links"], device=DEV)
+                   S=S, links=c["links"], device=DEV, lanes=lanes)
     n = c["nx"] * c["ny"] * c["nz"]
     rng = np.random.default_rng(0)
     rho = 1 + 0.01 * rng.random(n)
