@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=16)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--macro-every-step", type=int, default=1)
+    p.add_argument("--also-storage", default="fp16", choices=["fp16", "bf16", "none"],
+                   help="also time the same workload with 16-bit population storage (reported beside value)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run a 256 MiB device-to-device copy first (reference kernel for PMC byte counters)")
     p.add_argument("--thermal-n", type=int, default=256)
@@ -210,6 +212,34 @@ def main():
     achieved = bpu * n_fluid / (ms_step * 1e-3) / 1e9
     ok = bool(torch.isfinite(macro[1]).all()) and bool(torch.isfinite(macro[0]).all())
     ux_mean = float(macro[1][torch.as_tensor(~solid.reshape(-1), device=dev)].mean())
+    n_links = int(eng.n_links)
+    half = None
+    if args.also_storage != "none" and args.storage == "fp32":
+        del eng
+        torch.cuda.empty_cache()
+        from fluxcore3d_amd.solver.aa_engine import AAEngine
+        from fluxcore3d_amd.solver.d3q19 import W
+
+        eng16 = AAEngine(args.n, args.n, args.n, solid, periodic=(True, True, True), collision=args.collision,
+                         omega=omega, omega_minus=1.2, S=S, force=(1e-6, 0.0, 0.0), links=links,
+                         storage=args.also_storage, device=dev)
+        eng16.load(torch.as_tensor(np.broadcast_to(W[:, None], (19, N)).copy(), device=dev))
+        for _ in range(args.warmup):
+            eng16.step(*macro)
+        torch.cuda.synchronize()
+        k16 = max(args.steps // 5, 20)
+        ev0.record()
+        for _ in range(k16):
+            eng16.step(*macro)
+        ev1.record()
+        torch.cuda.synchronize()
+        ms16 = ev0.elapsed_time(ev1) / k16
+        half = {"storage": args.also_storage, "steps": k16, "ms_per_step": round(ms16, 5),
+                "mlups": round(n_fluid / (ms16 * 1e-3) / 1e6, 2),
+                "roofline_frac": round(eng16.bytes_per_update() * n_fluid / (ms16 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_update": eng16.bytes_per_update(), "lanes": eng16.lanes}
+        del eng16
+        torch.cuda.empty_cache()
     thermal = None
     if not args.no_thermal:
         thermal = thermal_leg(args.thermal_n, dev, weights=args.unet_weights)
@@ -231,7 +261,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"flow-only throughput: {args.n}^3 {args.collision.upper()} + Bouzidi, periodic RSA "
                                f"sphere pack porosity {info['porosity']:.4f} (seed {args.seed}), body force 1e-6 x",
-                   "grid": [args.n] * 3, "fluid_nodes": n_fluid, "n_links": int(eng.n_links),
+                   "grid": [args.n] * 3, "fluid_nodes": n_fluid, "n_links": n_links,
                    "storage": args.storage, "collision": args.collision, "ibb": "reference Bouzidi (periodic links)",
                    "macro_every_step": args.macro_every_step},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -239,6 +269,7 @@ def main():
                      "algorithmic_bytes_per_update": bpu,
                      "algorithmic_bytes_per_launch": bpu * n_fluid},
         "cpu_baseline": cpu,
+        "half_storage": half,
         "thermal": thermal,
         "mlups_all_nodes": round(N * args.steps / (ms * 1e-3) / 1e6, 2),
         "wall_s_timed": round(wall, 4),
