@@ -52,7 +52,7 @@ def parse():
 
 def thermal_leg(n, device, seed=0, weights=None):
     """GMG-preconditioned flexible CG (FP32 V-cycle, FP64 outer loop, the reference's
-    PCG), GMG-preconditioned flexible GCR(4), and flexible GCR(4) with the U-Net
+    PCG), GMG-preconditioned flexible GCR(6), and flexible GCR(6) with the U-Net
     preconditioner (V-cycle fallback), all to an FP64 relative residual of 1e-8
     on the same 256^3 blob conduction problem."""
     import torch
@@ -74,18 +74,18 @@ def thermal_leg(n, device, seed=0, weights=None):
     secs = time.perf_counter() - t0
     from fluxcore3d_amd.solver.mg_pcg import GCRWorkspace, gcr_fp64
 
-    gws = GCRWorkspace(c["k"].shape, device, 4)
-    gcr_fp64(mg, c["b"], tol=1e-8, max_it=3, m=4, ws=gws)  # warm-up
+    gws = GCRWorkspace(c["k"].shape, device, 6)
+    gcr_fp64(mg, c["b"], tol=1e-8, max_it=3, m=6, ws=gws)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    xg, gi = gcr_fp64(mg, c["b"], tol=1e-8, max_it=500, m=4, ws=gws)
+    xg, gi = gcr_fp64(mg, c["b"], tol=1e-8, max_it=500, m=6, ws=gws)
     torch.cuda.synchronize()
     gcr_secs = time.perf_counter() - t0
     out = {"problem": f"{n}^3 smooth-noise blobs (seed {seed}), k_s/k_f = {c['ratio']:.3g}, dt = {c['dt']:g}",
            "levels": [list(lv["shape"]) for lv in mg._levels], "gmg_pcg_s": round(secs, 4),
            "gmg_pcg_iterations": info["it"], "final_rel_residual_fp64": info["res_rel"],
-           "converged": bool(info["converged"]), "gmg_gcr4_s": round(gcr_secs, 4),
-           "gmg_gcr4_iterations": gi["it"], "gmg_gcr4_final_rel_residual_fp64": gi["res_rel"], "nn_pcg_s": None}
+           "converged": bool(info["converged"]), "gmg_gcr6_s": round(gcr_secs, 4),
+           "gmg_gcr6_iterations": gi["it"], "gmg_gcr6_final_rel_residual_fp64": gi["res_rel"], "nn_pcg_s": None}
     if weights is None or not os.path.exists(weights):
         out["note"] = "no trained U-Net weights found; NN-PCG not run"
         return out

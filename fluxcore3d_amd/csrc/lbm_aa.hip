@@ -378,7 +378,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <typename ST, int KIND>
-__global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 5)) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
+__global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
                                                          int parity, int first, float* __restrict__ rho_o,
                                                          float* __restrict__ u_o, float* __restrict__ v_o,
                                                          float* __restrict__ w_o) {
@@ -401,16 +401,25 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 5)) void aa_step_kernel(ST
   const int64_t x = live ? (t.lane_nodes ? (int64_t)t.lane_nodes[r] : r) : 0;
   const bool fluid = live && !(t.nflag[x] & F_SOLID);
 
-  // Each lane's first entry of every list, requested before the populations so
-  // that their latency overlaps (entries e >= 64 are loaded in the loops below).
-  const bool hasL0 = lane < nL, hasS0 = !first && lane < nSrc, hasF0 = lane < nF;
-  int c0 = 0, sc0 = 0, fc0 = 0, ff0 = 0;
-  float a0 = 0.f, w10 = 0.f, w20 = 0.f;
+  // Each lane's first entry of every list (and second link entry), requested
+  // before the populations so that their latency overlaps; further entries are
+  // loaded in the loops below. (A second wave of prefetches costs registers: the
+  // kernel runs 4 waves per SIMD, which measured equal or faster than 5 waves
+  // with one prefetched entry.)
+  const bool hasL0 = lane < nL, hasS0 = !first && lane < nSrc, hasF0 = lane < nF, hasL1 = lane + 64 < nL;
+  int c0 = 0, sc0 = 0, fc0 = 0, ff0 = 0, c1 = 0;
+  float a0 = 0.f, w10 = 0.f, w20 = 0.f, a1 = 0.f, w11 = 0.f, w21 = 0.f;
   if (hasL0) {
     c0 = t.lk_code[l_lo + lane];
     a0 = t.A[l_lo + lane];
     w10 = t.w1[l_lo + lane];
     w20 = t.w2[l_lo + lane];
+  }
+  if (hasL1) {
+    c1 = t.lk_code[l_lo + lane + 64];
+    a1 = t.A[l_lo + lane + 64];
+    w11 = t.w1[l_lo + lane + 64];
+    w21 = t.w2[l_lo + lane + 64];
   }
   if (hasS0) sc0 = t.s_code[s_lo + lane];
   if (hasF0) {
@@ -454,13 +463,15 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 5)) void aa_step_kernel(ST
   // Bouzidi / solid-source inputs (wave-uniform branch; every lane takes part)
   const float* Bprev = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
   if (special_wave) {
-    float b0 = 0.f;
+    float b0 = 0.f, b1 = 0.f;
     if (hasL0 && !first && (c0 & kCodeV)) b0 = Bprev[c0 >> 18];
+    if (hasL1 && !first && (c1 & kCodeV)) b1 = Bprev[c1 >> 18];
 #pragma unroll
     for (int q = 0; q < 19; ++q) tile[wv][q][lane] = f[q];
     wave_lds_sync();
     if (hasL0 && (c0 & kCodeOvr)) tile[wv][(c0 >> 6) & 31][c0 & 63] = (!first && (c0 & kCodeV)) ? a0 + w20 * b0 : a0;
-    for (int e = lane + 64; e < nL; e += 64) {
+    if (hasL1 && (c1 & kCodeOvr)) tile[wv][(c1 >> 6) & 31][c1 & 63] = (!first && (c1 & kCodeV)) ? a1 + w21 * b1 : a1;
+    for (int e = lane + 128; e < nL; e += 64) {
       const int code = t.lk_code[l_lo + e];
       float v = t.A[l_lo + e];
       if (!first && (code & kCodeV)) v = v + t.w2[l_lo + e] * Bprev[code >> 18];
@@ -519,7 +530,13 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 5)) void aa_step_kernel(ST
       if (!(c0 & kCodeV)) a = a + w20 * tile[wv][(c0 >> 6) & 31][n];
       t.A[l_lo + lane] = a;
     }
-    for (int e = lane + 64; e < nL; e += 64) {
+    if (hasL1) {
+      const int n = c1 & 63;
+      float a = w11 * tile[wv][(c1 >> 11) & 31][n];
+      if (!(c1 & kCodeV)) a = a + w21 * tile[wv][(c1 >> 6) & 31][n];
+      t.A[l_lo + lane + 64] = a;
+    }
+    for (int e = lane + 128; e < nL; e += 64) {
       const int code = t.lk_code[l_lo + e];
       const int n = code & 63;
       float a = t.w1[l_lo + e] * tile[wv][(code >> 11) & 31][n];

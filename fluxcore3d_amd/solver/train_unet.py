@@ -35,10 +35,10 @@ def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None) -> d
     _, gi = fcg_fp64(mg, c["b"], tol=tol, max_it=1000, ws=ws)
     torch.cuda.synchronize()
     t_gmg = time.perf_counter() - t0
-    gcr_fp64(mg, c["b"], tol=tol, max_it=2)
+    gcr_fp64(mg, c["b"], tol=tol, max_it=2, m=6)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, gg = gcr_fp64(mg, c["b"], tol=tol, max_it=1000)
+    _, gg = gcr_fp64(mg, c["b"], tol=tol, max_it=1000, m=6)
     torch.cuda.synchronize()
     t_gcr = time.perf_counter() - t0
     t0 = time.perf_counter()
