@@ -146,7 +146,7 @@ class GCRStencil(ctypes.Structure):
 
 
 class GCRWorkspace:
-    def __init__(self, shape, device, m: int = 4):
+    def __init__(self, shape, device, m: int = 6):
         mmax = int(_lib.call("fc_gcr_max_directions"))
         if not 0 <= m < mmax:
             raise ValueError(f"GCR truncation m must be in [0, {mmax - 1}]")
@@ -161,7 +161,7 @@ class GCRWorkspace:
 
 
 def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol: float = 1e-8, max_it: int = 300,
-             precond=None, m: int = 4, ws: Optional[GCRWorkspace] = None, on_iter=None,
+             precond=None, m: int = 6, ws: Optional[GCRWorkspace] = None, on_iter=None,
              check_every: int = 1) -> Tuple[torch.Tensor, dict]:
     """Flexible GCR(m) on A x = b with FP64 vectors and a variable preconditioner
     ``precond(r32, z32, ws)`` (default: one V-cycle). The residual norm never

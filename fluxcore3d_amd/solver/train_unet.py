@@ -52,6 +52,44 @@ def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None) -> d
             "nn_converged": bool(ni["converged"])}
 
 
+def unet_flops(net, n: int) -> float:
+    """Forward FLOPs of one application at n^3: sum over layers of 2 * 27 * cin * cout per voxel."""
+    total = 0.0
+    for name, cin, cout, _ in net.layers:
+        lvl = int(name[3]) if name[:3] in ("enc", "dec") else 0
+        total += 2.0 * 27 * cin * cout * (n >> lvl) ** 3
+    return total
+
+
+def time_apply(weights: str, n: int, reps: int = 10) -> dict:
+    """Time one U-Net application (inference path, all HIP kernels) at n^3."""
+    from .nn_precond import UNetPrecond
+
+    dev = torch.device("cuda:0")
+    net = UNetPrecond.load(weights, device=dev)
+    N = n ** 3
+    g = torch.Generator(device="cpu").manual_seed(0)
+    r = torch.randn(N, generator=g).to(dev)
+    z = torch.empty_like(r)
+    ones = torch.ones(N, device=dev)
+    feat = torch.zeros(N, device=dev)
+    scal = torch.zeros(8, dtype=torch.float64, device=dev)
+    partial = torch.empty(3 * 8192, dtype=torch.float64, device=dev)
+    for _ in range(3):
+        net.apply(r, z, feat, ones, ones, scal, partial, (n, n, n))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        net.apply(r, z, feat, ones, ones, scal, partial, (n, n, n))
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    fl = unet_flops(net, n)
+    return {"n": n, "apply_ms": round(ms, 4), "gflop": round(fl / 1e9, 1), "tflops": round(fl / ms / 1e9, 2),
+            "mfma_bf16_peak_frac": round(fl / ms / 1e9 / 2500.0, 4)}
+
+
 def main(argv=None):
     from .nn_precond import UNetPrecond, harvest_pairs, train
 
@@ -67,7 +105,11 @@ def main(argv=None):
     p.add_argument("--channels", default="8,16,24,32,32")
     p.add_argument("--eval-n", default="128,256")
     p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--time-apply", type=int, default=0, help="only time one application at this size with --out weights")
     a = p.parse_args(argv)
+    if a.time_apply:
+        print(json.dumps(time_apply(a.out, a.time_apply)), flush=True)
+        return
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
     samples, ops = harvest_pairs(n=a.n, problems=a.problems, per_problem=a.per_problem, device=dev)
