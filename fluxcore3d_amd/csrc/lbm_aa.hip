@@ -56,13 +56,14 @@ struct Tables {
   const uint8_t* nflag;      // [N]
   const int32_t* wave_base;  // [ceil(N/64)] special nodes before each 64-node group
   const int4* wave_meta;     // [ceil(N/64) + 1] {first link, first second-node link, first publish, first source}
+  // links of a group: the nV second-node links first (entry e <-> B slot v_lo + e), then the others
   const int32_t* lk_code;    // [n_links] node in group | t << 6 | opp t << 11 | V << 16 | override << 17 | v << 18
   const int32_t* s_code;     // [n_src] node in group | q << 6 | override << 17
   const int32_t* f_code;     // [n_ff] node in group | p << 6
   const int32_t* lane_nodes;  // [n_lanes] node of each lane (null: lane r is node r)
   int64_t n_lanes;
   const int4* sp_mask;       // [n_special] {L, S, V, F}
-  const int4* sp_base;       // [n_special] {link_base, v_base, ff_base, 0}
+  const int4* sp_base;       // [n_special] {first plain link, v_base, ff_base, first second-node link}
   const float* w1;           // [n_links]
   const float* w2;           // [n_links]
   float* A;                  // [n_links]
@@ -83,14 +84,25 @@ struct Collision {
 };
 
 // ---- storage types: FP32 values, or 16-bit deviations f - w_i (rho0 = 1) --
+// ld/st: plain accesses; ld_nt/st_nt: non-temporal (streaming) accesses for the
+// step's population sweep, which touches every slot once per step.
 template <typename ST> struct Store;
 template <> struct Store<float> {
   static __device__ __forceinline__ float ld(const float* p, int) { return *p; }
   static __device__ __forceinline__ void st(float* p, float v, int) { *p = v; }
+  static __device__ __forceinline__ float ld_nt(const float* p, int) { return __builtin_nontemporal_load(p); }
+  static __device__ __forceinline__ void st_nt(float* p, float v, int) { __builtin_nontemporal_store(v, p); }
 };
 template <> struct Store<__half> {
   static __device__ __forceinline__ float ld(const __half* p, int q) { return __half2float(*p) + WQ(q); }
   static __device__ __forceinline__ void st(__half* p, float v, int q) { *p = __float2half(v - WQ(q)); }
+  static __device__ __forceinline__ float ld_nt(const __half* p, int q) {
+    const unsigned short b = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p));
+    return __half2float(__ushort_as_half(b)) + WQ(q);
+  }
+  static __device__ __forceinline__ void st_nt(__half* p, float v, int q) {
+    __builtin_nontemporal_store(__half_as_ushort(__float2half(v - WQ(q))), reinterpret_cast<unsigned short*>(p));
+  }
 };
 template <> struct Store<__hip_bfloat16> {
   static __device__ __forceinline__ float ld(const __hip_bfloat16* p, int q) {
@@ -98,6 +110,14 @@ template <> struct Store<__hip_bfloat16> {
   }
   static __device__ __forceinline__ void st(__hip_bfloat16* p, float v, int q) {
     *p = __float2bfloat16(v - WQ(q));
+  }
+  static __device__ __forceinline__ float ld_nt(const __hip_bfloat16* p, int q) {
+    const unsigned short b = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p));
+    return __bfloat162float(__ushort_as_bfloat16(b)) + WQ(q);
+  }
+  static __device__ __forceinline__ void st_nt(__hip_bfloat16* p, float v, int q) {
+    __builtin_nontemporal_store(__bfloat16_as_ushort(__float2bfloat16(v - WQ(q))),
+                                reinterpret_cast<unsigned short*>(p));
   }
 };
 
@@ -147,9 +167,10 @@ __device__ __forceinline__ void compose_inputs(const ST* __restrict__ pop, const
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     if (special && ((L >> q) & 1u)) {
-      const int64_t l = b.x + __popc(L & ((1u << q) - 1u));
+      const bool isv = (V >> q) & 1u;
+      const int64_t l = isv ? b.w + __popc(V & ((1u << q) - 1u)) : b.x + __popc(L & ~V & ((1u << q) - 1u));
       float v = t.A[l];
-      if (!first && ((V >> q) & 1u)) {
+      if (!first && isv) {
         const int64_t mv = b.y + __popc(V & ((1u << q) - 1u));
         v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + mv];
       }
@@ -383,50 +404,24 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
                                                          float* __restrict__ u_o, float* __restrict__ v_o,
                                                          float* __restrict__ w_o) {
   __shared__ float tile[kWaves][19][64];
-  // lane r works on node x (every node, or the fluid nodes only); solid lanes idle
+  // lane r works on node x (every node, or the fluid nodes only)
   const unsigned lb = logical_block(blockIdx.x, gridDim.x);
   const int64_t r = (int64_t)lb * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // per-wave entry ranges (scalar loads)
-  const int gw = __builtin_amdgcn_readfirstlane((int)(r >> 6));
-  int4 wm0 = make_int4(0, 0, 0, 0), wm1 = make_int4(0, 0, 0, 0);
-  if ((int64_t)gw * 64 < t.n_lanes) {
-    wm0 = t.wave_meta[gw];
-    wm1 = t.wave_meta[gw + 1];
-  }
-  const int l_lo = wm0.x, nL = wm1.x - wm0.x, v_lo = wm0.y, f_lo = wm0.z, nF = wm1.z - wm0.z;
-  const int s_lo = wm0.w, nSrc = wm1.w - wm0.w;
-  const bool special_wave = (nL | nF | nSrc) != 0;
   const bool live = r < t.n_lanes;
+  // per-wave entry ranges (scalar loads; waves past the last lane get empty ranges)
+  const int gw = __builtin_amdgcn_readfirstlane((int)(r >> 6));
+  const int last_w = (int)((t.n_lanes - 1) >> 6);
+  const int gwc = min(gw, last_w);
+  const int4 wm0 = t.wave_meta[gwc];
+  int4 wm1 = t.wave_meta[gwc + 1];
+  if (gw > last_w) wm1 = wm0;
   const int64_t x = live ? (t.lane_nodes ? (int64_t)t.lane_nodes[r] : r) : 0;
-  const bool fluid = live && !(t.nflag[x] & F_SOLID);
 
-  // Each lane's first entry of every list (and second link entry), requested
-  // before the populations so that their latency overlaps; further entries are
-  // loaded in the loops below. (A second wave of prefetches costs registers: the
-  // kernel runs 4 waves per SIMD, which measured equal or faster than 5 waves
-  // with one prefetched entry.)
-  const bool hasL0 = lane < nL, hasS0 = !first && lane < nSrc, hasF0 = lane < nF, hasL1 = lane + 64 < nL;
-  int c0 = 0, sc0 = 0, fc0 = 0, ff0 = 0, c1 = 0;
-  float a0 = 0.f, w10 = 0.f, w20 = 0.f, a1 = 0.f, w11 = 0.f, w21 = 0.f;
-  if (hasL0) {
-    c0 = t.lk_code[l_lo + lane];
-    a0 = t.A[l_lo + lane];
-    w10 = t.w1[l_lo + lane];
-    w20 = t.w2[l_lo + lane];
-  }
-  if (hasL1) {
-    c1 = t.lk_code[l_lo + lane + 64];
-    a1 = t.A[l_lo + lane + 64];
-    w11 = t.w1[l_lo + lane + 64];
-    w21 = t.w2[l_lo + lane + 64];
-  }
-  if (hasS0) sc0 = t.s_code[s_lo + lane];
-  if (hasF0) {
-    fc0 = t.f_code[f_lo + lane];
-    ff0 = t.ff_idx[f_lo + lane];
-  }
-
+  // Populations first. Solid lanes (one lane per node) load their slots too and
+  // discard them: with every lane of a wavefront active, each load instruction
+  // covers whole 128-byte lines, which measured faster than the same lines
+  // requested by the fluid lanes alone.
   int i = 0, j = 0, k = 0;
   uint32_t self = 0;
   float f[19];
@@ -437,9 +432,18 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
     const int J = dy < 0 ? Jm : (dy > 0 ? Jp : J0);
     return (uint32_t)I * PYZ + (uint32_t)J * PZ + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
   };
+  // Slot of direction q's access in this step's parity: the even step uses the
+  // node's own slots, so its -1/+1 neighbour coordinates collapse onto the node's.
+  // One load loop and one store loop serve both parities (no branch; w_q = w_opp(q)).
+  int aIm = 0, aIp = 0, aJm = 0, aJp = 0, aKm = 0, aKp = 0;
+  auto pslot = [&](int dx, int dy, int dz) -> uint32_t {
+    const int I = dx < 0 ? aIm : (dx > 0 ? aIp : I0);
+    const int J = dy < 0 ? aJm : (dy > 0 ? aJp : J0);
+    return (uint32_t)I * PYZ + (uint32_t)J * PZ + (uint32_t)(dz < 0 ? aKm : (dz > 0 ? aKp : K0));
+  };
 #pragma unroll
   for (int q = 0; q < 19; ++q) f[q] = 0.f;
-  if (fluid) {
+  if (live) {
     const uint32_t xu = (uint32_t)x;
     const uint32_t nynz = (uint32_t)g.ny * (uint32_t)g.nz;
     i = (int)(xu / nynz);
@@ -450,31 +454,60 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
     Jm = wrap_or_pad(j, -1, g.ny, g.gy); J0 = j + g.gy; Jp = wrap_or_pad(j, 1, g.ny, g.gy);
     Km = wrap_or_pad(k, -1, g.nz, g.gz); K0 = k + g.gz; Kp = wrap_or_pad(k, 1, g.nz, g.gz);
     self = (uint32_t)I0 * PYZ + (uint32_t)J0 * PZ + (uint32_t)K0;
-    if (parity == 0) {
+    aIm = parity ? Im : I0; aIp = parity ? Ip : I0;
+    aJm = parity ? Jm : J0; aJp = parity ? Jp : J0;
+    aKm = parity ? Km : K0; aKp = parity ? Kp : K0;
 #pragma unroll
-      for (int q = 0; q < 19; ++q) f[q] = Store<ST>::ld(pop + (size_t)q * (size_t)g.Ns + self, q);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 19; ++q)
-        f[q] = Store<ST>::ld(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), OPP(q));
-    }
+    for (int q = 0; q < 19; ++q)
+      f[q] = Store<ST>::ld_nt(pop + (size_t)(q + parity * (OPP(q) - q)) * (size_t)g.Ns + pslot(-CX(q), -CY(q), -CZ(q)), q);
   }
 
-  // Bouzidi / solid-source inputs (wave-uniform branch; every lane takes part)
+  // Each lane's first entry of every list (and second link entry), requested
+  // right behind the populations: no entry load waits for another load (the B
+  // slot of second-node link e < nV is v_lo + e), so all latencies overlap.
+  // Further entries are loaded in the loops below.
+  const int l_lo = wm0.x, nL = wm1.x - wm0.x, v_lo = wm0.y, nV = wm1.y - wm0.y, f_lo = wm0.z, nF = wm1.z - wm0.z;
+  const int s_lo = wm0.w, nSrc = wm1.w - wm0.w;
+  const bool special_wave = (nL | nF | nSrc) != 0;
   const float* Bprev = t.B + (int64_t)(parity ^ 1) * t.n_v + v_lo;
+  const bool hasL0 = lane < nL, hasS0 = !first && lane < nSrc, hasF0 = lane < nF, hasL1 = lane + 64 < nL;
+  const bool hasB0 = !first && lane < nV, hasB1 = !first && lane + 64 < nV;
+  int c0 = 0, sc0 = 0, fc0 = 0, c1 = 0;
+  uint32_t ff0 = 0;  // unsigned: widening it to an address needs no wait for the load
+  float a0 = 0.f, w10 = 0.f, w20 = 0.f, a1 = 0.f, w11 = 0.f, w21 = 0.f, b0 = 0.f, b1 = 0.f;
+  if (hasL0) {
+    c0 = t.lk_code[l_lo + lane];
+    a0 = t.A[l_lo + lane];
+    w10 = t.w1[l_lo + lane];
+    w20 = t.w2[l_lo + lane];
+  }
+  if (hasB0) b0 = Bprev[lane];
+  if (hasL1) {
+    c1 = t.lk_code[l_lo + lane + 64];
+    a1 = t.A[l_lo + lane + 64];
+    w11 = t.w1[l_lo + lane + 64];
+    w21 = t.w2[l_lo + lane + 64];
+  }
+  if (hasB1) b1 = Bprev[lane + 64];
+  if (hasS0) sc0 = t.s_code[s_lo + lane];
+  if (hasF0) {
+    fc0 = t.f_code[f_lo + lane];
+    ff0 = (uint32_t)t.ff_idx[f_lo + lane];
+  }
+  const uint8_t nfl = t.nflag[x];  // x = 0 on dead lanes: no branch around the load
+  const bool fluid = live && !(nfl & F_SOLID);
+
+  // Bouzidi / solid-source inputs (wave-uniform branch; every lane takes part)
   if (special_wave) {
-    float b0 = 0.f, b1 = 0.f;
-    if (hasL0 && !first && (c0 & kCodeV)) b0 = Bprev[c0 >> 18];
-    if (hasL1 && !first && (c1 & kCodeV)) b1 = Bprev[c1 >> 18];
 #pragma unroll
     for (int q = 0; q < 19; ++q) tile[wv][q][lane] = f[q];
     wave_lds_sync();
-    if (hasL0 && (c0 & kCodeOvr)) tile[wv][(c0 >> 6) & 31][c0 & 63] = (!first && (c0 & kCodeV)) ? a0 + w20 * b0 : a0;
-    if (hasL1 && (c1 & kCodeOvr)) tile[wv][(c1 >> 6) & 31][c1 & 63] = (!first && (c1 & kCodeV)) ? a1 + w21 * b1 : a1;
+    if (hasL0 && (c0 & kCodeOvr)) tile[wv][(c0 >> 6) & 31][c0 & 63] = hasB0 ? a0 + w20 * b0 : a0;
+    if (hasL1 && (c1 & kCodeOvr)) tile[wv][(c1 >> 6) & 31][c1 & 63] = hasB1 ? a1 + w21 * b1 : a1;
     for (int e = lane + 128; e < nL; e += 64) {
       const int code = t.lk_code[l_lo + e];
       float v = t.A[l_lo + e];
-      if (!first && (code & kCodeV)) v = v + t.w2[l_lo + e] * Bprev[code >> 18];
+      if (!first && e < nV) v = v + t.w2[l_lo + e] * Bprev[e];
       if (code & kCodeOvr) tile[wv][(code >> 6) & 31][code & 63] = v;
     }
     if (!first) {
@@ -490,23 +523,31 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
     for (int q = 0; q < 19; ++q) f[q] = tile[wv][q][lane];
   }
 
+  float rho = 1.f, ux = 0.f, uy = 0.f, uz = 0.f;
   if (fluid) {
     float sigma = 0.f;
     if (c.sponge) sigma = c.sponge[c.sponge_axis == 0 ? i : (c.sponge_axis == 1 ? j : k)];
-    float rho, ux, uy, uz;
     collide_fast<KIND>(c, f, rho, ux, uy, uz, sigma);
-    if (rho_o) { rho_o[x] = rho; u_o[x] = ux; v_o[x] = uy; w_o[x] = uz; }
-
-    if (parity == 0) {
+  } else {
+    // A solid node streams the rest state w_q (the reference's f_post = W,
+    // lbm_kernels.cu:277-283) into the slots its neighbours read next step, so
+    // their inputs from solid nodes need no substitution; macros (1, 0, 0, 0).
 #pragma unroll
-      for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + self, f[q], OPP(q));
-    } else {
-#pragma unroll
-      for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + slot(CX(q), CY(q), CZ(q)), f[q], q);
+    for (int q = 0; q < 19; ++q) f[q] = WQ(q);
+  }
+  if (live) {
+    if (rho_o) {
+      __builtin_nontemporal_store(rho, rho_o + x);
+      __builtin_nontemporal_store(ux, u_o + x);
+      __builtin_nontemporal_store(uy, v_o + x);
+      __builtin_nontemporal_store(uz, w_o + x);
     }
+#pragma unroll
+    for (int q = 0; q < 19; ++q)
+      Store<ST>::st_nt(pop + (size_t)(OPP(q) + parity * (q - OPP(q))) * (size_t)g.Ns + pslot(CX(q), CY(q), CZ(q)), f[q], q);
     // copy-self on non-periodic faces through the ghost layer
-    if ((g.gx && (i == 0 || i == g.nx - 1)) || (g.gy && (j == 0 || j == g.ny - 1)) ||
-        (g.gz && (k == 0 || k == g.nz - 1))) {
+    if (fluid && ((g.gx && (i == 0 || i == g.nx - 1)) || (g.gy && (j == 0 || j == g.ny - 1)) ||
+                  (g.gz && (k == 0 || k == g.nz - 1)))) {
 #pragma unroll
       for (int q = 1; q < 19; ++q) {
         if (!oob_source(g, i, j, k, q)) continue;
@@ -544,7 +585,7 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
       t.A[l_lo + e] = a;
     }
     float* Bw = t.B + (int64_t)parity * t.n_v;
-    if (hasF0) Bw[ff0] = tile[wv][(fc0 >> 6) & 31][fc0 & 63];
+    if (hasF0) Bw[(size_t)ff0] = tile[wv][(fc0 >> 6) & 31][fc0 & 63];
     for (int e = lane + 64; e < nF; e += 64) {
       const int code = t.f_code[f_lo + e];
       Bw[t.ff_idx[f_lo + e]] = tile[wv][(code >> 6) & 31][code & 63];
@@ -647,10 +688,10 @@ __global__ __launch_bounds__(kBlock) void aa_load_kernel(ST* __restrict__ pop, G
   if (special) {
     const int64_t sidx = t.wave_base[x >> 6] + __popcll(bal & ((1ull << lane) - 1ull));
     const int4 m = t.sp_mask[sidx], b = t.sp_base[sidx];
-    const uint32_t L = (uint32_t)m.x;
-    int64_t l = b.x;
+    const uint32_t L = (uint32_t)m.x, V = (uint32_t)m.z;
+    int64_t lp = b.x, lv = b.w;
     for (int q = 0; q < 19; ++q)
-      if ((L >> q) & 1u) t.A[l++] = in[q * g.N + x];
+      if ((L >> q) & 1u) t.A[((V >> q) & 1u) ? lv++ : lp++] = in[q * g.N + x];
   }
 }
 
@@ -879,7 +920,8 @@ int fc_aa_step(void* pop, int storage, int nx, int ny, int nz, int px, int py, i
   c.cs_dx2 = col->cs_dx2;
   for (int d = 0; d < 3; ++d) c.F[d] = col->force[d];
   c.use_force = col->use_force; c.sponge = col->sponge; c.sponge_axis = col->sponge_axis;
-  if (rho && solid_macros) {
+  // with one lane per node the step itself writes the solid nodes' macros
+  if (rho && solid_macros && t.lane_nodes) {
     hipLaunchKernelGGL(aa_solid_macros_kernel, dim3(grid_for(g.N, kBlock)), dim3(kBlock), 0, stream, t.nflag, g.N,
                        rho, u, v, w);
     if (int rc = check_launch("aa_step/solid macros")) return rc;

@@ -13,8 +13,10 @@ Tables (built once, in node order):
   wave_base[G]   special nodes before each group of 64 nodes
   sp_mask[S,4]   per special node: link mask L, solid-source mask S,
                  second-node mask V, publish mask F
-  sp_base[S,4]   first link, first V link, first publish entry
-  w1, w2, A      per link (sorted by node, then direction)
+  sp_base[S,4]   first link without a second node, first V-link slot in B,
+                 first publish entry, first link with a second node
+  w1, w2, A      per link: by wavefront (64 lanes), links with a second node
+                 first, then by node and direction
   B[2, n_v]      second-node values, double-buffered by step parity
   ff_idx[n_ff]   V-link slot each publishing node writes
 """
@@ -127,7 +129,7 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
         return np.searchsorted(owner, sp_idx).astype(np.int32)
 
     sp_mask = np.stack([Lmask[sp_idx], Smask[sp_idx], Vmask[sp_idx], Fmask[sp_idx]], 1).view(np.int32)
-    sp_base = np.stack([first_index(link_node), first_index(v_node), first_index(ff_node),
+    sp_base = np.stack([np.zeros(sp_idx.size, np.int32), first_index(v_node), first_index(ff_node),
                         np.zeros(sp_idx.size, np.int32)], 1).astype(np.int32)
     nflag = flat_solid.astype(np.uint8) * F_SOLID | special.astype(np.uint8) * F_SPECIAL
     for b, bc in enumerate(bcs):
@@ -144,11 +146,13 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
     wave_base = np.concatenate([[0], np.cumsum(cnt.reshape(groups, 64).sum(1))[:-1]]).astype(np.int32)
     # Lanes of the fused step: with `compact`, one lane per fluid node (fluid nodes in
     # node order, 64 per wavefront, `lane_nodes`); otherwise one lane per node (solid
-    # lanes idle). Entry lists are node-major, so each wavefront owns a contiguous
+    # lanes load whole lines and store w_q). Entry lists are node-major, so each wavefront owns a contiguous
     # range of every list. Per-entry codes:
     #   link:    lane | t << 6 | opp(t) << 11 | second node << 16
     #            | override << 17 | second-node index relative to the wavefront's first << 18
-    #   source:  lane | q << 6 | override << 17   (solid upstream, input w_q after the first step)
+    #   source:  lane | q << 6 | override << 17   (solid upstream, input w_q after the first step;
+    #            compacted lanes only: with one lane per node, solid lanes write w_q into the
+    #            slots their neighbours read, so the list is empty)
     #   publish: lane | p << 6                     (post-collision f_p goes to B[ff_idx])
     # override = the node is not on an inlet/outlet plane (plane kernels rebuild those inputs).
     lane_nodes = np.nonzero(~flat_solid)[0].astype(np.int64) if compact else np.arange(N, dtype=np.int64)
@@ -158,19 +162,40 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
     waves = (n_lanes + 63) // 64
     starts = np.append(lane_nodes[::64], N) if n_lanes else np.zeros(1, np.int64)
     ovr_node = (nflag >> F_BC_SHIFT) == 0
+    # Within each wavefront the links with a second node come first, so entry e < nV
+    # of a wavefront is its e-th second-node link (B slot v_lo + e) and the kernel
+    # loads B without waiting for the entry code. The second-node links keep the
+    # global (node, direction) order of the B slots.
+    link_wave = rank[link_node] // 64
+    lo = np.lexsort((link_dir, link_node, ~link_v, link_wave))
+    link_node, link_dir, w1s, w2s, link_v, link_wave = (a[lo] for a in (link_node, link_dir, w1s, w2s, link_v,
+                                                                        link_wave))
+    assert np.array_equal(link_node[link_v], v_node)
+    pos = np.arange(link_node.size)
+
+    def first_pos(sel: np.ndarray) -> np.ndarray:
+        # first position of each special node's links in the class `sel` (node-sorted)
+        return np.append(pos[sel], link_node.size)[np.searchsorted(link_node[sel], sp_idx)].astype(np.int32)
+
+    sp_base[:, 0] = first_pos(~link_v)
+    sp_base[:, 3] = first_pos(link_v)
     v_base = np.searchsorted(v_node, starts)
     v_rank = np.cumsum(link_v) - 1
     lk_code = ((rank[link_node] % 64) | (link_dir << 6) | (OPP_NP[link_dir] << 11)
                | (link_v.astype(np.int64) << 16) | (ovr_node[link_node].astype(np.int64) << 17)
                | (np.where(link_v, v_rank - v_base[rank[link_node] // 64], 0) << 18)).astype(np.int32)
-    s_row, s_q = np.nonzero(((Smask[sp_idx][:, None] >> np.arange(19, dtype=np.uint32)) & 1).astype(bool))
-    s_node = sp_idx[s_row]
+    if compact:
+        s_row, s_q = np.nonzero(((Smask[sp_idx][:, None] >> np.arange(19, dtype=np.uint32)) & 1).astype(bool))
+        s_node = sp_idx[s_row]
+    else:
+        s_q = s_node = np.zeros(0, np.int64)
     s_code = ((rank[s_node] % 64) | (s_q << 6) | (ovr_node[s_node].astype(np.int64) << 17)).astype(np.int32)
     f_code = ((rank[ff_node] % 64) | (ff_dir << 6)).astype(np.int32)
     # per wavefront (+ totals): first link, second-node link, publish entry and source entry
-    wave_meta = np.stack([np.searchsorted(link_node, starts), v_base, np.searchsorted(ff_node, starts),
-                          np.searchsorted(s_node, starts)], 1).astype(np.int32)
+    wave_meta = np.stack([np.searchsorted(link_wave, np.arange(waves + 1)), v_base,
+                          np.searchsorted(ff_node, starts), np.searchsorted(s_node, starts)], 1).astype(np.int32)
     assert wave_meta.shape[0] == waves + 1 or n_lanes == 0
+    assert np.all((pos - wave_meta[link_wave, 0] < np.diff(wave_meta[:, 1])[link_wave]) == link_v)
     return dict(nflag=nflag, wave_base=wave_base, wave_meta=wave_meta, lk_code=lk_code, s_code=s_code,
                 f_code=f_code, n_src=int(s_node.size), lane_nodes=lane_nodes.astype(np.int32), n_lanes=n_lanes, sp_mask=sp_mask, sp_base=sp_base, w1=w1s, w2=w2s,
                 link_node=link_node, link_dir=link_dir, B_init=B_init, ff_idx=ff_slot, ff_node=ff_node,
@@ -194,11 +219,11 @@ class AAEngine:
             raise ValueError("ibb_mode must be 'reference' or 'textbook'")
         self.storage = storage
         # Lane mapping of the fused step: "fluid" (one lane per fluid node) or "all"
-        # (one lane per node). On a porosity-0.6 sphere pack "fluid" is 7 % faster
-        # with 16-bit populations and 3 % slower with FP32 (its wavefronts span about
-        # twice as many cache lines per load), so "auto" picks by storage.
+        # (one lane per node). With "all" every load instruction covers whole cache
+        # lines; on a porosity-0.6 256^3 sphere pack it is 1.8x (FP32) and 1.5x (FP16)
+        # faster than "fluid", whose wavefronts gather scattered nodes. "auto" = "all".
         if lanes == "auto":
-            lanes = "all" if storage == "fp32" else "fluid"
+            lanes = "all"
         if lanes not in ("fluid", "all"):
             raise ValueError("lanes must be 'auto', 'fluid' or 'all'")
         self.lanes = lanes

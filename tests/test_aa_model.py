@@ -73,6 +73,8 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
     A = f0[tb["link_dir"], tb["link_node"]].astype(np.float64)
     B = [tb["B_init"].astype(np.float64), tb["B_init"].astype(np.float64)]
     fluid = ~solid.reshape(-1)
+    # with one lane per node, solid lanes store their f_post (= w_q) like fluid lanes
+    writer = fluid if compact else np.ones(N, bool)
     w1, w2 = tb["w1"].astype(np.float64), tb["w2"].astype(np.float64)
     macros = None
     for t in range(steps):
@@ -90,11 +92,11 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
         macros = (rho, u, v, w)
         for q in range(19):
             if P == 0:
-                pop[OPP[q], self_slot[fluid]] = fp[q, fluid]
+                pop[OPP[q], self_slot[writer]] = fp[q, writer]
                 o = oob(q) & fluid
                 pop[OPP[q], slot(-C[q])[o]] = fp[q, o]
             else:
-                pop[q, slot(C[q])[fluid]] = fp[q, fluid]
+                pop[q, slot(C[q])[writer]] = fp[q, writer]
                 o = oob(q) & fluid
                 pop[q, self_slot[o]] = fp[q, o]
         A = w1 * fp[l_opp, l_node] + np.where(l_hasv, 0.0, w2 * fp[l_t, l_node])
@@ -156,6 +158,9 @@ def test_entry_codes_match_node_masks(compact):
     assert np.array_equal(v_idx[hv], np.arange(tb["n_v"]))
     sc = tb["s_code"].astype(np.int64)
     np.bitwise_or.at(S, fl[_group_of(wm[:, 3], sc.size) * 64 + (sc & 63)], 1 << ((sc >> 6) & 31))
+    if not compact:  # solid lanes write w_q themselves: no substitution entries
+        assert sc.size == 0 and tb["n_src"] == 0
+        S[tb["sp_idx"]] = tb["sp_mask"].view(np.uint32).astype(np.int64)[:, 1]
     fc = tb["f_code"].astype(np.int64)
     f_node = fl[_group_of(wm[:, 2], fc.size) * 64 + (fc & 63)]
     assert np.array_equal(f_node, tb["ff_node"])
@@ -165,3 +170,39 @@ def test_entry_codes_match_node_masks(compact):
     for col, arr in enumerate((L, S, V, F)):
         assert np.array_equal(arr[sp], masks[:, col]), col
         assert not np.any(np.delete(arr, sp))
+
+
+@pytest.mark.parametrize("compact", [True, False])
+def test_link_order_and_node_addressing(compact):
+    """Within every wavefront the links with a second node come first (entry
+    e < nV reads B slot v_lo + e), and the per-node addressing of the load,
+    materialize and plane kernels (plain links from sp_base.x, second-node
+    links from sp_base.w, by popcount of the lower mask bits) finds each link."""
+    c = cases.periodic_pack(n=24, seed=3)
+    n = c["nx"]
+    tb = build_tables(n, n, n, c["solid"], c["links"], (1, 1, 1), compact=compact)
+    wm = tb["wave_meta"].astype(np.int64)
+    lk = tb["lk_code"].astype(np.int64)
+    grp = _group_of(wm[:, 0], lk.size)
+    e = np.arange(lk.size) - wm[grp, 0]
+    hv = ((lk >> 16) & 1).astype(bool)
+    assert hv.any() and (~hv).any()
+    assert np.array_equal(e < np.diff(wm[:, 1])[grp], hv)
+    assert np.array_equal((lk >> 18)[hv], e[hv])
+    masks = tb["sp_mask"].view(np.uint32).astype(np.int64)
+    base = tb["sp_base"].astype(np.int64)
+    found = 0
+    for row, x in enumerate(tb["sp_idx"]):
+        Lm, Vm = int(masks[row, 0]), int(masks[row, 2])
+        for q in range(19):
+            if not (Lm >> q) & 1:
+                continue
+            below = (1 << q) - 1
+            if (Vm >> q) & 1:
+                li = base[row, 3] + bin(Vm & below).count("1")
+                assert base[row, 1] + bin(Vm & below).count("1") == np.cumsum(hv)[li] - 1
+            else:
+                li = base[row, 0] + bin(Lm & ~Vm & below).count("1")
+            assert tb["link_node"][li] == x and tb["link_dir"][li] == q
+            found += 1
+    assert found == tb["n_links"]
