@@ -83,26 +83,26 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__
   }
 }
 
-// p = (double) z;  w = (p + dt L p) * mask
+// p = (double) z;  w = (p + dt L p) * mask  (one cell per thread, 32-bit index math)
 __global__ __launch_bounds__(kBlock) void apply_kernel(const float* __restrict__ z, Stencil s, double* __restrict__ p,
                                                        double* __restrict__ w) {
-  const int64_t n = (int64_t)s.nx * s.ny * s.nz;
-  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
-    const int64_t nynz = (int64_t)s.ny * s.nz;
-    const int i = (int)(idx / nynz), j = (int)((idx / s.nz) % s.ny), k = (int)(idx % s.nz);
-    const int iE = i < s.nx - 1 ? i + 1 : i - 1, iW = i > 0 ? i - 1 : lo_ghost(s, 0);
-    const int jN = j < s.ny - 1 ? j + 1 : j - 1, jS = j > 0 ? j - 1 : lo_ghost(s, 1);
-    const int kU = k < s.nz - 1 ? k + 1 : k - 1, kD = k > 0 ? k - 1 : lo_ghost(s, 2);
-    const double pc = (double)z[idx];
-    const double L = (double)s.diagL[idx] * pc - (double)s.aE[idx] * (double)z[iE * nynz + (int64_t)j * s.nz + k] -
-                     (double)s.aW[idx] * (double)z[iW * nynz + (int64_t)j * s.nz + k] -
-                     (double)s.aN[idx] * (double)z[i * nynz + (int64_t)jN * s.nz + k] -
-                     (double)s.aS[idx] * (double)z[i * nynz + (int64_t)jS * s.nz + k] -
-                     (double)s.aU[idx] * (double)z[i * nynz + (int64_t)j * s.nz + kU] -
-                     (double)s.aD[idx] * (double)z[i * nynz + (int64_t)j * s.nz + kD];
-    p[idx] = pc;
-    w[idx] = (pc + s.dt * L) * (double)s.mask[idx];
-  }
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= s.nx * s.ny * s.nz) return;
+  const unsigned nynz = (unsigned)s.ny * (unsigned)s.nz, u = (unsigned)idx;
+  const int i = (int)(u / nynz);
+  const unsigned rem = u - (unsigned)i * nynz;
+  const int j = (int)(rem / (unsigned)s.nz), k = (int)(rem - (unsigned)j * (unsigned)s.nz);
+  // neighbour offsets with the reference's ghosts (mg_kernels.cu:37-44)
+  const int oE = (i < s.nx - 1 ? 1 : -1) * (int)nynz, oW = (i > 0 ? -1 : lo_ghost(s, 0)) * (int)nynz;
+  const int oN = (j < s.ny - 1 ? 1 : -1) * s.nz, oS = (j > 0 ? -1 : lo_ghost(s, 1)) * s.nz;
+  const int oU = k < s.nz - 1 ? 1 : -1, oD = k > 0 ? -1 : lo_ghost(s, 2);
+  const double pc = (double)z[idx];
+  const double L = (double)s.diagL[idx] * pc - (double)s.aE[idx] * (double)z[idx + oE] -
+                   (double)s.aW[idx] * (double)z[idx + oW] - (double)s.aN[idx] * (double)z[idx + oN] -
+                   (double)s.aS[idx] * (double)z[idx + oS] - (double)s.aU[idx] * (double)z[idx + oU] -
+                   (double)s.aD[idx] * (double)z[idx + oD];
+  p[idx] = pc;
+  w[idx] = (pc + s.dt * L) * (double)s.mask[idx];
 }
 
 // partials of w . W_j (j < m)
@@ -203,6 +203,8 @@ int fc_gcr_step(const float* z32, const fc_gcr_stencil* st, double* p, double* w
             st->ax, st->out_idx, st->dt};
   FC_REQUIRE(s.aE && s.aW && s.aN && s.aS && s.aU && s.aD && s.diagL && s.mask, "gcr_step: null coefficient");
   const int64_t n = (int64_t)s.nx * s.ny * s.nz;
+  FC_REQUIRE(n + (int64_t)kMaxBlocks * kBlock < ((int64_t)1 << 31), "gcr_step: %lld cells exceed the 32-bit index range",
+             (long long)n);
   Dirs d{};
   d.m = m;
   for (int j = 0; j < m; ++j) {

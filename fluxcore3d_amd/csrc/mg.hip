@@ -42,43 +42,56 @@ __device__ __forceinline__ int lo_ghost(const Level& g, int this_ax) {
   return (g.ax != this_ax || g.out_idx == 0) ? 1 : 0;
 }
 
-// Lo = T + dt L(T) at node (i,j,k) with the reference's ghosts and operand order.
+// Cell counts stay below 2^31 (make_level checks it), so index math is 32-bit:
+// a 64-bit division per cell costs more than the rest of a stencil update.
+__device__ __forceinline__ void unflat(const Level& g, int idx, int& i, int& j, int& k) {
+  const unsigned nynz = (unsigned)g.ny * (unsigned)g.nz, u = (unsigned)idx;
+  i = (int)(u / nynz);
+  const unsigned rem = u - (unsigned)i * nynz;
+  j = (int)(rem / (unsigned)g.nz);
+  k = (int)(rem - (unsigned)j * (unsigned)g.nz);
+}
+
+// Offsets of the six stencil neighbours of (i,j,k) with the reference's ghosts
+// (mg_kernels.cu:37-44): high faces mirror to n-2, low faces to index 1 (or to the
+// node itself on the flow axis with the inlet at 0).
+struct Nbr {
+  int e, w, n, s, u, d;
+};
+__device__ __forceinline__ Nbr neighbours(const Level& g, int i, int j, int k) {
+  const int nynz = g.ny * g.nz;
+  Nbr o;
+  o.e = (i < g.nx - 1 ? 1 : -1) * nynz;
+  o.w = (i > 0 ? -1 : lo_ghost(g, 0)) * nynz;
+  o.n = (j < g.ny - 1 ? 1 : -1) * g.nz;
+  o.s = (j > 0 ? -1 : lo_ghost(g, 1)) * g.nz;
+  o.u = k < g.nz - 1 ? 1 : -1;
+  o.d = k > 0 ? -1 : lo_ghost(g, 2);
+  return o;
+}
+
+// Lo = T + dt L(T) at cell idx with the reference's operand order.
 template <typename TT>
-__device__ __forceinline__ float stencil_lo(const TT* __restrict__ T, const Coef& c, const Level& g, float dt,
-                                            int i, int j, int k, int64_t idx) {
-  const int64_t nynz = (int64_t)g.ny * g.nz;
-  const int iE = i < g.nx - 1 ? i + 1 : i - 1;
-  const int iW = i > 0 ? i - 1 : lo_ghost(g, 0);
-  const int jN = j < g.ny - 1 ? j + 1 : j - 1;
-  const int jS = j > 0 ? j - 1 : lo_ghost(g, 1);
-  const int kU = k < g.nz - 1 ? k + 1 : k - 1;
-  const int kD = k > 0 ? k - 1 : lo_ghost(g, 2);
+__device__ __forceinline__ float stencil_lo(const TT* __restrict__ T, const Coef& c, float dt, int idx, const Nbr& o) {
   const float TC = (float)T[idx];
-  const float TE = (float)T[iE * nynz + (int64_t)j * g.nz + k];
-  const float TW = (float)T[iW * nynz + (int64_t)j * g.nz + k];
-  const float TN = (float)T[i * nynz + (int64_t)jN * g.nz + k];
-  const float TS = (float)T[i * nynz + (int64_t)jS * g.nz + k];
-  const float TU = (float)T[i * nynz + (int64_t)j * g.nz + kU];
-  const float TD = (float)T[i * nynz + (int64_t)j * g.nz + kD];
+  const float TE = (float)T[idx + o.e];
+  const float TW = (float)T[idx + o.w];
+  const float TN = (float)T[idx + o.n];
+  const float TS = (float)T[idx + o.s];
+  const float TU = (float)T[idx + o.u];
+  const float TD = (float)T[idx + o.d];
   const float LT = c.diagL[idx] * TC - c.aE[idx] * TE - c.aW[idx] * TW - c.aN[idx] * TN - c.aS[idx] * TS -
                    c.aU[idx] * TU - c.aD[idx] * TD;
   return TC + dt * LT;
 }
 
-__device__ __forceinline__ void unflat(const Level& g, int64_t idx, int& i, int& j, int& k) {
-  const int64_t nynz = (int64_t)g.ny * g.nz;
-  i = (int)(idx / nynz);
-  j = (int)((idx / g.nz) % g.ny);
-  k = (int)(idx % g.nz);
-}
-
 __global__ __launch_bounds__(kBlock) void mop_kernel(const float* __restrict__ T, Coef c, Level g, float dt,
                                                      float* __restrict__ Lo) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= (int64_t)g.nx * g.ny * g.nz) return;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= g.nx * g.ny * g.nz) return;
   int i, j, k;
   unflat(g, idx, i, j, k);
-  Lo[idx] = stencil_lo(T, c, g, dt, i, j, k, idx);
+  Lo[idx] = stencil_lo(T, c, dt, idx, neighbours(g, i, j, k));
 }
 
 __global__ __launch_bounds__(kBlock) void jacobi_kernel(float* __restrict__ x, const float* __restrict__ b,
@@ -93,15 +106,15 @@ __global__ __launch_bounds__(kBlock) void jacobi_kernel(float* __restrict__ x, c
 __global__ __launch_bounds__(kBlock) void sweep_kernel(const float* __restrict__ x, float* __restrict__ x_out,
                                                        const float* __restrict__ b, Coef c, Level g, float dt,
                                                        int zero_guess) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= (int64_t)g.nx * g.ny * g.nz) return;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= g.nx * g.ny * g.nz) return;
   if (zero_guess) {  // x = 0: A x = 0
     x_out[idx] = 0.f + c.invD[idx] * (b[idx] - 0.f);
     return;
   }
   int i, j, k;
   unflat(g, idx, i, j, k);
-  const float lo = stencil_lo(x, c, g, dt, i, j, k, idx);
+  const float lo = stencil_lo(x, c, dt, idx, neighbours(g, i, j, k));
   x_out[idx] = x[idx] + c.invD[idx] * (b[idx] - lo);
 }
 
@@ -111,23 +124,26 @@ __global__ __launch_bounds__(kBlock) void residual_restrict_kernel(const float* 
                                                                    const float* __restrict__ mask, Coef c, Level g,
                                                                    float dt, float* __restrict__ bc, int cx, int cy,
                                                                    int cz) {
-  const int64_t cidx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (cidx >= (int64_t)cx * cy * cz) return;
-  const int I = (int)(cidx / ((int64_t)cy * cz)), J = (int)((cidx / cz) % cy), K = (int)(cidx % cz);
+  const int cidx = blockIdx.x * kBlock + threadIdx.x;
+  if (cidx >= cx * cy * cz) return;
+  const unsigned cycz = (unsigned)cy * (unsigned)cz, u = (unsigned)cidx;
+  const int I = (int)(u / cycz);
+  const unsigned rem = u - (unsigned)I * cycz;
+  const int J = (int)(rem / (unsigned)cz), K = (int)(rem - (unsigned)J * (unsigned)cz);
   float s = 0.f;
 #pragma unroll
   for (int d = 0; d < 8; ++d) {
     const int i = 2 * I + (d >> 2), j = 2 * J + ((d >> 1) & 1), k = 2 * K + (d & 1);
-    const int64_t idx = ((int64_t)i * g.ny + j) * g.nz + k;
-    const float lo = stencil_lo(x, c, g, dt, i, j, k, idx);
+    const int idx = (i * g.ny + j) * g.nz + k;
+    const float lo = stencil_lo(x, c, dt, idx, neighbours(g, i, j, k));
     s += (b[idx] - lo) * mask[idx];
   }
   bc[cidx] = s / 8.f;
 }
 
-// linear interpolation source (align_corners = False) along one axis
-__device__ __forceinline__ void lin_src(int o, int n_out, int n_in, int& i0, int& i1, float& l0, float& l1) {
-  const float scale = (float)n_in / (float)n_out;
+// linear interpolation source (align_corners = False) along one axis; scale =
+// n_in / n_out, divided once on the host (the same IEEE quotient)
+__device__ __forceinline__ void lin_src(int o, float scale, int n_in, int& i0, int& i1, float& l0, float& l1) {
   float src = scale * ((float)o + 0.5f) - 0.5f;
   src = src < 0.f ? 0.f : src;
   i0 = min((int)src, n_in - 1);
@@ -138,16 +154,20 @@ __device__ __forceinline__ void lin_src(int o, int n_out, int n_in, int& i0, int
 
 __global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __restrict__ xc, float* __restrict__ x,
                                                                  const float* __restrict__ mask, int nx, int ny,
-                                                                 int nz, int cx, int cy, int cz, float alpha) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= (int64_t)nx * ny * nz) return;
-  const int i = (int)(idx / ((int64_t)ny * nz)), j = (int)((idx / nz) % ny), k = (int)(idx % nz);
+                                                                 int nz, int cx, int cy, int cz, float sx, float sy,
+                                                                 float sz, float alpha) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= nx * ny * nz) return;
+  const unsigned nynz = (unsigned)ny * (unsigned)nz, u = (unsigned)idx;
+  const int i = (int)(u / nynz);
+  const unsigned rem = u - (unsigned)i * nynz;
+  const int j = (int)(rem / (unsigned)nz), k = (int)(rem - (unsigned)j * (unsigned)nz);
   int i0, i1, j0, j1, k0, k1;
   float a0, a1, b0, b1, c0, c1;
-  lin_src(i, nx, cx, i0, i1, a0, a1);
-  lin_src(j, ny, cy, j0, j1, b0, b1);
-  lin_src(k, nz, cz, k0, k1, c0, c1);
-  auto X = [&](int a, int b2, int c2) { return xc[((int64_t)a * cy + b2) * cz + c2]; };
+  lin_src(i, sx, cx, i0, i1, a0, a1);
+  lin_src(j, sy, cy, j0, j1, b0, b1);
+  lin_src(k, sz, cz, k0, k1, c0, c1);
+  auto X = [&](int a, int b2, int c2) { return xc[(a * cy + b2) * cz + c2]; };
   const float e = a0 * (b0 * (c0 * X(i0, j0, k0) + c1 * X(i0, j0, k1)) + b1 * (c0 * X(i0, j1, k0) + c1 * X(i0, j1, k1))) +
                   a1 * (b0 * (c0 * X(i1, j0, k0) + c1 * X(i1, j0, k1)) + b1 * (c0 * X(i1, j1, k0) + c1 * X(i1, j1, k1)));
   x[idx] = (x[idx] + alpha * e) * mask[idx];
@@ -162,13 +182,13 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(float* __restrict__ 
   float* src = x;
   float* dst = tmp;
   for (int s = 0; s < sweeps; ++s) {
-    for (int64_t idx = threadIdx.x; idx < n; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < (int)n; idx += blockDim.x) {
       if (s == 0 && zero_guess) {
         dst[idx] = 0.f + c.invD[idx] * (b[idx] - 0.f);
       } else {
         int i, j, k;
         unflat(g, idx, i, j, k);
-        const float lo = stencil_lo(src, c, g, dt, i, j, k, idx);
+        const float lo = stencil_lo(src, c, dt, idx, neighbours(g, i, j, k));
         dst[idx] = src[idx] + c.invD[idx] * (b[idx] - lo);
       }
     }
@@ -198,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void build_coeffs_kernel(const float* __res
   const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= (int64_t)g.nx * g.ny * g.nz) return;
   int i, j, k;
-  unflat(g, idx, i, j, k);
+  unflat(g, (int)idx, i, j, k);
   const float kc = kf[idx];
   const float inv_rcp = 1.0f / (rcp[idx] + 1e-30f);
   float out[6];
@@ -345,22 +365,16 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__
 __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restrict__ p, double* __restrict__ Ap,
                                                            const double* __restrict__ r, const float* __restrict__ mask,
                                                            Coef c, Level g, double dt, double* __restrict__ partial) {
-  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int n = g.nx * g.ny * g.nz;
   double v[2] = {0.0, 0.0};
-  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
     int i, j, k;
     unflat(g, idx, i, j, k);
-    const int64_t nynz = (int64_t)g.ny * g.nz;
-    const int iE = i < g.nx - 1 ? i + 1 : i - 1, iW = i > 0 ? i - 1 : lo_ghost(g, 0);
-    const int jN = j < g.ny - 1 ? j + 1 : j - 1, jS = j > 0 ? j - 1 : lo_ghost(g, 1);
-    const int kU = k < g.nz - 1 ? k + 1 : k - 1, kD = k > 0 ? k - 1 : lo_ghost(g, 2);
+    const Nbr o = neighbours(g, i, j, k);
     const double pc = p[idx];
-    const double L = (double)c.diagL[idx] * pc - (double)c.aE[idx] * p[iE * nynz + (int64_t)j * g.nz + k] -
-                     (double)c.aW[idx] * p[iW * nynz + (int64_t)j * g.nz + k] -
-                     (double)c.aN[idx] * p[i * nynz + (int64_t)jN * g.nz + k] -
-                     (double)c.aS[idx] * p[i * nynz + (int64_t)jS * g.nz + k] -
-                     (double)c.aU[idx] * p[i * nynz + (int64_t)j * g.nz + kU] -
-                     (double)c.aD[idx] * p[i * nynz + (int64_t)j * g.nz + kD];
+    const double L = (double)c.diagL[idx] * pc - (double)c.aE[idx] * p[idx + o.e] - (double)c.aW[idx] * p[idx + o.w] -
+                     (double)c.aN[idx] * p[idx + o.n] - (double)c.aS[idx] * p[idx + o.s] -
+                     (double)c.aU[idx] * p[idx + o.u] - (double)c.aD[idx] * p[idx + o.d];
     const double a = (pc + dt * L) * (double)mask[idx];
     Ap[idx] = a;
     v[0] += pc * a;
@@ -410,22 +424,16 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
                                                                float* __restrict__ r32,
                                                                const float* __restrict__ mask, Coef c, Level g,
                                                                double dt, double* __restrict__ partial) {
-  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int n = g.nx * g.ny * g.nz;
   double v[2] = {0.0, 0.0};
-  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
     int i, j, k;
     unflat(g, idx, i, j, k);
-    const int64_t nynz = (int64_t)g.ny * g.nz;
-    const int iE = i < g.nx - 1 ? i + 1 : i - 1, iW = i > 0 ? i - 1 : lo_ghost(g, 0);
-    const int jN = j < g.ny - 1 ? j + 1 : j - 1, jS = j > 0 ? j - 1 : lo_ghost(g, 1);
-    const int kU = k < g.nz - 1 ? k + 1 : k - 1, kD = k > 0 ? k - 1 : lo_ghost(g, 2);
+    const Nbr o = neighbours(g, i, j, k);
     const double xc = x[idx];
-    const double L = (double)c.diagL[idx] * xc - (double)c.aE[idx] * x[iE * nynz + (int64_t)j * g.nz + k] -
-                     (double)c.aW[idx] * x[iW * nynz + (int64_t)j * g.nz + k] -
-                     (double)c.aN[idx] * x[i * nynz + (int64_t)jN * g.nz + k] -
-                     (double)c.aS[idx] * x[i * nynz + (int64_t)jS * g.nz + k] -
-                     (double)c.aU[idx] * x[i * nynz + (int64_t)j * g.nz + kU] -
-                     (double)c.aD[idx] * x[i * nynz + (int64_t)j * g.nz + kD];
+    const double L = (double)c.diagL[idx] * xc - (double)c.aE[idx] * x[idx + o.e] - (double)c.aW[idx] * x[idx + o.w] -
+                     (double)c.aN[idx] * x[idx + o.n] - (double)c.aS[idx] * x[idx + o.s] -
+                     (double)c.aU[idx] * x[idx + o.u] - (double)c.aD[idx] * x[idx + o.d];
     const double m = (double)mask[idx];
     const double bb = b[idx] * m;
     const double rr = (bb - (xc + dt * L)) * m;
@@ -463,6 +471,9 @@ __global__ __launch_bounds__(kBlock) void residual_f32_kernel(const float* __res
 
 int make_level(Level& g, int nx, int ny, int nz, int ax, int out_idx) {
   FC_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "mg: every extent must be >= 2 (got %d,%d,%d)", nx, ny, nz);
+  // 32-bit cell indices, including the stride of the capped reduction grids
+  FC_REQUIRE((int64_t)nx * ny * nz + (int64_t)kMaxPartials * kBlock < ((int64_t)1 << 31),
+             "mg: %d x %d x %d cells exceed the 32-bit index range", nx, ny, nz);
   FC_REQUIRE(ax >= 0 && ax <= 2, "mg: flow axis must be 0, 1 or 2");
   g.nx = nx; g.ny = ny; g.nz = nz; g.ax = ax; g.out_idx = out_idx;
   return 0;
@@ -562,8 +573,11 @@ int fc_mg_prolong_correct(const float* x_coarse, float* x, const float* mask, in
   FC_REQUIRE(nx > 0 && ny > 0 && nz > 0 && cx > 0 && cy > 0 && cz > 0, "prolong_correct: extents must be positive");
   FC_REQUIRE(x_coarse && x && mask, "prolong_correct: null pointer argument");
   const int64_t n = (int64_t)nx * ny * nz;
+  FC_REQUIRE(n < ((int64_t)1 << 31), "prolong_correct: %lld cells exceed the 32-bit index range", (long long)n);
+  // the interpolation scale of F.interpolate(align_corners=False): n_in / n_out in FP32
+  const float sx = (float)cx / (float)nx, sy = (float)cy / (float)ny, sz = (float)cz / (float)nz;
   hipLaunchKernelGGL(prolong_correct_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x_coarse, x, mask,
-                     nx, ny, nz, cx, cy, cz, alpha);
+                     nx, ny, nz, cx, cy, cz, sx, sy, sz, alpha);
   return check_launch("prolong_correct");
 }
 
