@@ -31,9 +31,14 @@ def _up(x):
     return x.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
 
 
+# W >= 64 runs the LDS-staged kernel (input tile of 3 x 6 x 66 voxels) when the tile
+# fits in 64 KB: partial h and w tiles, 1-3 channel chunks; cin = 40 at W = 96 falls
+# back to the cached kernel.
 @pytest.mark.parametrize("B,D,H,W,cin,cout,act", [(2, 6, 5, 19, 8, 12, False), (1, 4, 7, 33, 16, 16, True),
                                                    (1, 3, 3, 16, 24, 48, True), (1, 5, 4, 20, 40, 64, False),
-                                                   (1, 3, 2, 70, 16, 24, True), (2, 2, 3, 40, 8, 32, False)])
+                                                   (1, 3, 2, 70, 16, 24, True), (2, 2, 3, 40, 8, 32, False),
+                                                   (1, 5, 6, 64, 8, 8, True), (2, 3, 5, 130, 24, 16, False),
+                                                   (1, 2, 9, 96, 40, 64, True)])
 def test_forward_single_source(B, D, H, W, cin, cout, act):
     from fluxcore3d_amd.solver import nn_precond as nn
     x = _bf16((B, D, H, W, cin), 1)
@@ -50,10 +55,11 @@ def test_forward_single_source(B, D, H, W, cin, cout, act):
     assert np.all(np.abs(y16 - y) <= 2.0 ** -8 * np.abs(y) + 1e-30)
 
 
-def test_forward_two_sources_with_upsample():
+@pytest.mark.parametrize("W", [16, 64])
+def test_forward_two_sources_with_upsample(W):
     from fluxcore3d_amd.solver import nn_precond as nn
-    xc = _bf16((1, 4, 3, 8, 16), 3)  # half-resolution source
-    xs = _bf16((1, 8, 6, 16, 8), 4)  # skip source at full resolution
+    xc = _bf16((1, 4, 3, W // 2, 16), 3)  # half-resolution source
+    xs = _bf16((1, 8, 6, W, 8), 4)  # skip source at full resolution
     cout = 24
     w = _bf16((cout, 27, 24), 5).float() * 0.2
     wpk = nn.pack_weights(w.to(DEV))
