@@ -16,6 +16,8 @@
 // tap, Cin tile), with per-chunk partials folded in a fixed order.
 #include "fc_common.hpp"
 
+#include <algorithm>
+
 using namespace fc;
 
 namespace {
@@ -136,124 +138,147 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(const __bf16* __restrict_
   }
 }
 
-// Same convolution with the input tile staged in LDS. A block computes 1 x 4 x 64
-// output positions (one h row of four 16-position tiles per wavefront); its
-// 3 x 6 x 66 input voxels are loaded once, in 16-byte pieces of 8 channels (zero
-// outside the volume), and every tap reads its B fragments from LDS. The cached
-// kernel above re-gathers each input voxel for all 27 taps of every position.
-constexpr int kLdsTH = 4, kLdsTW = 64, kLdsPH = kLdsTH + 2, kLdsPW = kLdsTW + 2;
+// Same convolution with the input staged in LDS. A block computes up to kLdsDZ
+// output planes of 4 x 64 positions (one h row of four 16-position tiles per
+// wavefront), walking along d with a ring of three input planes of 6 x 66
+// voxels: each output plane loads one new input plane (16-byte pieces of 8
+// channels, zero outside the volume) and every tap reads its B fragments from
+// LDS. The LDS offsets of each lane group's K steps are tabulated once per
+// block. The cached kernel above re-gathers each input voxel for all 27 taps of
+// every position.
+constexpr int kLdsTH = 4, kLdsTW = 64, kLdsPH = kLdsTH + 2, kLdsPW = kLdsTW + 2, kLdsDZ = 8;  // max planes per block
+constexpr int kLdsPlane = kLdsPH * kLdsPW;  // voxels of one input plane
+constexpr int kLdsMaxK = 128;               // K-step table entries (kpad / 8 <= 27 * 3 + 3)
 
 // tap = kk / nch for kk < 256 and nch <= 8 by a multiply-shift with the host's
-// magic = ceil(2^16 / nch) (checked exhaustively on the host)
+// magic = ceil(2^16 / nch) (checked on the host for every kk used)
 __device__ __forceinline__ int div_small(int kk, int magic) { return (kk * magic) >> 16; }
 
 template <int MT>
 __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(const __bf16* __restrict__ x0,
                                                            const __bf16* __restrict__ x1,
                                                            const __bf16* __restrict__ wt, ConvShape s, int act,
-                                                           float slope, int magic, __bf16* __restrict__ y16,
-                                                           float* __restrict__ y32) {
-  extern __shared__ bf16x8 tile_s[];  // [chunk of 8 channels][dz 3][hy 6][wx 66]
+                                                           float slope, int magic, int dz_block,
+                                                           __bf16* __restrict__ y16, float* __restrict__ y32) {
+  extern __shared__ bf16x8 ring[];  // [plane slot 3][chunk of 8 channels][hy 6][wx 66]
+  __shared__ int2 ktab[kLdsMaxK];   // per 8-wide K group: {tile offset without the plane, dz}
   constexpr int NT = kLdsTW / 16;
   const int nch = (s.c0 + s.c1) >> 3, nch0 = s.c0 >> 3;
+  const int slab = nch * kLdsPlane;  // one plane slot
   const int tiles_w = (s.W + kLdsTW - 1) / kLdsTW, tiles_h = (s.H + kLdsTH - 1) / kLdsTH;
+  const int tiles_d = (s.D + dz_block - 1) / dz_block;
   unsigned bid = blockIdx.x;
   const int tw = (int)(bid % (unsigned)tiles_w); bid /= (unsigned)tiles_w;
   const int th = (int)(bid % (unsigned)tiles_h); bid /= (unsigned)tiles_h;
-  const int d = (int)(bid % (unsigned)s.D);
-  const int b = (int)(bid / (unsigned)s.D);
-  const int w0 = tw * kLdsTW, h0 = th * kLdsTH;
-  // input tile: 16-byte pieces, zero outside the volume; 32-bit element offsets
-  const int n_items = nch * 3 * kLdsPH * kLdsPW;
-  for (int it = threadIdx.x; it < n_items; it += 256) {
-    const int wx = it % kLdsPW, r1 = it / kLdsPW;
-    const int hy = r1 % kLdsPH, r2 = r1 / kLdsPH;
-    const int dz = r2 % 3, ch = r2 / 3;
-    const int dd = d + dz - 1, hh = h0 + hy - 1, ww = w0 + wx - 1;
-    bf16x8 v;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
-    if ((unsigned)dd < (unsigned)s.D && (unsigned)hh < (unsigned)s.H && (unsigned)ww < (unsigned)s.W) {
-      if (ch < nch0) {
-        const unsigned pos = s.up0 ? ((unsigned)(b * (s.D >> 1) + (dd >> 1)) * (unsigned)(s.H >> 1) + (unsigned)(hh >> 1)) *
-                                             (unsigned)(s.W >> 1) + (unsigned)(ww >> 1)
-                                   : ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
-        v = *(const bf16x8*)(x0 + (pos * (unsigned)s.c0 + (unsigned)(ch << 3)));
-      } else {
-        const unsigned pos = ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
-        v = *(const bf16x8*)(x1 + (pos * (unsigned)s.c1 + (unsigned)((ch - nch0) << 3)));
-      }
-    }
-    tile_s[it] = v;
+  const int td = (int)(bid % (unsigned)tiles_d);
+  const int b = (int)(bid / (unsigned)tiles_d);
+  const int w0 = tw * kLdsTW, h0 = th * kLdsTH, d0 = td * dz_block, d1 = min(d0 + dz_block, s.D);
+  const int kgroups = s.kpad >> 3;
+  for (int kk = threadIdx.x; kk < kgroups; kk += 256) {
+    // K past 27 taps carries zero weights: any finite tile values serve, so the
+    // padding reads tap 26 (tile values are finite activations)
+    const int tap0 = div_small(kk, magic), chunk = kk - tap0 * nch, tap = min(tap0, 26);
+    const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
+    ktab[kk] = make_int2((chunk * kLdsPH + ty) * kLdsPW + tx, tz);
   }
-  __syncthreads();
+  // input plane dd (absolute) into ring slot dd mod 3
+  auto load_plane = [&](int dd) {
+    bf16x8* dst = ring + ((dd + 3) % 3) * slab;
+    const int n_items = nch * kLdsPlane;
+    for (int it = threadIdx.x; it < n_items; it += 256) {
+      const int wx = it % kLdsPW, r1 = it / kLdsPW;
+      const int hy = r1 % kLdsPH, ch = r1 / kLdsPH;
+      const int hh = h0 + hy - 1, ww = w0 + wx - 1;
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
+      if ((unsigned)dd < (unsigned)s.D && (unsigned)hh < (unsigned)s.H && (unsigned)ww < (unsigned)s.W) {
+        if (ch < nch0) {
+          const unsigned pos = s.up0 ? ((unsigned)(b * (s.D >> 1) + (dd >> 1)) * (unsigned)(s.H >> 1) + (unsigned)(hh >> 1)) *
+                                               (unsigned)(s.W >> 1) + (unsigned)(ww >> 1)
+                                     : ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
+          v = *(const bf16x8*)(x0 + (pos * (unsigned)s.c0 + (unsigned)(ch << 3)));
+        } else {
+          const unsigned pos = ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
+          v = *(const bf16x8*)(x1 + (pos * (unsigned)s.c1 + (unsigned)((ch - nch0) << 3)));
+        }
+      }
+      dst[it] = v;
+    }
+  };
+  load_plane(d0 - 1);
+  load_plane(d0);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int h = h0 + wv;
   const int col = lane & 15, grp = lane >> 4;
-  f32x4 acc[MT][NT];
-#pragma unroll
-  for (int m = 0; m < MT; ++m)
-#pragma unroll
-    for (int n = 0; n < NT; ++n) acc[m][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const int ksteps = s.kpad >> 5;
   const __bf16* wrow[MT];
 #pragma unroll
   for (int m = 0; m < MT; ++m)  // rows past cout read the last row (valid memory); results dropped
     wrow[m] = wt + (unsigned)min(m * 16 + col, s.cout - 1) * (unsigned)s.kpad + (unsigned)(grp << 3);
-  const bf16x8* lane_tile = tile_s + __mul24(wv, kLdsPW) + col;
-  bf16x8 av_next[MT];
-#pragma unroll
-  for (int m = 0; m < MT; ++m) av_next[m] = *(const bf16x8*)wrow[m];
-  for (int ks = 0; ks < ksteps; ++ks) {
-    bf16x8 av[MT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m) av[m] = av_next[m];
-    if (ks + 1 < ksteps) {  // next step's weights in flight during this step's MFMAs
-#pragma unroll
-      for (int m = 0; m < MT; ++m) av_next[m] = *(const bf16x8*)(wrow[m] + ((ks + 1) << 5));
-    }
-    const int kk = (ks << 2) + grp;  // this lane group's 8 K values are channels [8 chunk, +8) of tap
-    const int tap0 = div_small(kk, magic), chunk = kk - __mul24(tap0, nch);
-    // K past 27 taps carries zero weights: any finite tile values serve, so the
-    // padding reads tap 26 instead of branching (tile values are finite activations)
-    const int tap = min(tap0, 26);
-    const int tz = (tap * 57) >> 9, t3 = (tap * 11) >> 5;  // tap / 9 and tap / 3 for tap < 27
-    const int ty = t3 - 3 * tz, tx = tap - 3 * t3;
-    const bf16x8* row = lane_tile + __mul24(__mul24(chunk, 3) + tz, kLdsPH * kLdsPW) + __mul24(ty, kLdsPW) + tx;
-    bf16x8 bv[NT];
-#pragma unroll
-    for (int n = 0; n < NT; ++n) bv[n] = row[16 * n];
+  const int lane_off = wv * kLdsPW + col;
+  for (int d = d0; d < d1; ++d) {
+    load_plane(d + 1);
+    __syncthreads();  // planes d-1, d, d+1 and the K table are in LDS
+    f32x4 acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[n], acc[m][n], 0, 0, 0);
-  }
-  if (h >= s.H) return;
-  const unsigned prow = ((unsigned)(b * s.D + d) * (unsigned)s.H + (unsigned)h) * (unsigned)s.W;
+      for (int n = 0; n < NT; ++n) acc[m][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int s0 = (d + 2) % 3;  // ring slot of input plane d - 1
+    bf16x8 av_next[MT];
 #pragma unroll
-  for (int n = 0; n < NT; ++n) {
-    const int w = w0 + 16 * n + col;
-    if (w >= s.W) continue;
-    const unsigned p = prow + (unsigned)w;
+    for (int m = 0; m < MT; ++m) av_next[m] = *(const bf16x8*)wrow[m];
+    for (int ks = 0; ks < ksteps; ++ks) {
+      bf16x8 av[MT];
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const int co = m * 16 + grp * 4;
-      if (co >= s.cout) continue;
-      float v[4];
+      for (int m = 0; m < MT; ++m) av[m] = av_next[m];
+      if (ks + 1 < ksteps) {  // next step's weights in flight during this step's MFMAs
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        float a = acc[m][n][rr];
-        if (act == 1 && a < 0.f) a *= slope;
-        v[rr] = a;
+        for (int m = 0; m < MT; ++m) av_next[m] = *(const bf16x8*)(wrow[m] + ((ks + 1) << 5));
       }
-      if (y32) *(f32x4*)(y32 + (p * (unsigned)s.cout + (unsigned)co)) = (f32x4){v[0], v[1], v[2], v[3]};
-      if (y16) {
-        bf16x4 o;
+      const int2 e = ktab[(ks << 2) + grp];
+      int sl = s0 + e.y;
+      sl -= sl >= 3 ? 3 : 0;
+      const bf16x8* row = ring + sl * slab + e.x + lane_off;
+      bf16x8 bv[NT];
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) o[rr] = (__bf16)v[rr];
-        *(bf16x4*)(y16 + (p * (unsigned)s.cout + (unsigned)co)) = o;
+      for (int n = 0; n < NT; ++n) bv[n] = row[16 * n];
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[n], acc[m][n], 0, 0, 0);
+    }
+    if (h < s.H) {
+      const unsigned prow = ((unsigned)(b * s.D + d) * (unsigned)s.H + (unsigned)h) * (unsigned)s.W;
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        const int w = w0 + 16 * n + col;
+        if (w >= s.W) continue;
+        const unsigned p = prow + (unsigned)w;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int co = m * 16 + grp * 4;
+          if (co >= s.cout) continue;
+          float v[4];
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            float a = acc[m][n][rr];
+            if (act == 1 && a < 0.f) a *= slope;
+            v[rr] = a;
+          }
+          if (y32) *(f32x4*)(y32 + (p * (unsigned)s.cout + (unsigned)co)) = (f32x4){v[0], v[1], v[2], v[3]};
+          if (y16) {
+            bf16x4 o;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) o[rr] = (__bf16)v[rr];
+            *(bf16x4*)(y16 + (p * (unsigned)s.cout + (unsigned)co)) = o;
+          }
+        }
       }
     }
+    __syncthreads();  // every wavefront is done with plane d - 1 before it is overwritten
   }
 }
 
@@ -363,18 +388,21 @@ int fc_conv3d_fwd(const void* x0, int c0, int up0, const void* x1, int c1, const
   FC_REQUIRE(y_bf16 || y_f32, "conv3d_fwd: no output");
   ConvShape s{B, D, H, W, c0, c1, up0, cout, fc_conv_kpad(c0 + c1)};
   const int mt = (cout + 15) / 16;
-  // LDS-staged kernel where rows are long enough and the input tile fits in 64 KB
-  const size_t lds = (size_t)((c0 + c1) / 8) * 3 * kLdsPH * kLdsPW * sizeof(bf16x8);
+  // LDS-staged kernel where rows are long enough and the three input planes fit in 60 KB
+  const size_t lds = (size_t)((c0 + c1) / 8) * 3 * kLdsPlane * sizeof(bf16x8);
   const int64_t elems = (int64_t)B * D * H * W * (c0 + c1 > cout ? c0 + c1 : cout);
-  if (W >= kLdsTW && lds <= 65536 && elems < ((int64_t)1 << 31)) {
-    const int64_t blocks = (int64_t)B * D * ((H + kLdsTH - 1) / kLdsTH) * ((W + kLdsTW - 1) / kLdsTW);
+  if (W >= kLdsTW && lds <= 61440 && elems < ((int64_t)1 << 31) && fc_conv_kpad(c0 + c1) / 8 <= kLdsMaxK) {
+    // output planes per block: up to kLdsDZ (fewer halo reloads) while keeping >= 2048 blocks
+    const int64_t per_plane = (int64_t)B * ((H + kLdsTH - 1) / kLdsTH) * ((W + kLdsTW - 1) / kLdsTW);
+    const int dz_block = (int)std::max<int64_t>(1, std::min<int64_t>(kLdsDZ, per_plane * D / 2048));
+    const int64_t blocks = per_plane * ((D + dz_block - 1) / dz_block);
     const int nch = (c0 + c1) / 8, magic = (65536 + nch - 1) / nch;
     // kk / nch by multiply-shift: exact for every kk the kernel forms (kk < kpad / 8 <= 27 nch)
     for (int kk = 0; kk < fc_conv_kpad(c0 + c1) / 8; ++kk)
       FC_REQUIRE((kk * magic) >> 16 == kk / nch, "conv3d_fwd: reciprocal of %d channel chunks inexact", nch);
 #define FC_CONV_LDS(M)                                                                                          \
   hipLaunchKernelGGL((conv_fwd_lds_kernel<M>), dim3((unsigned)blocks), dim3(256), lds, stream, (const __bf16*)x0, \
-                     (const __bf16*)x1, (const __bf16*)w, s, act, slope, magic, (__bf16*)y_bf16, y_f32)
+                     (const __bf16*)x1, (const __bf16*)w, s, act, slope, magic, dz_block, (__bf16*)y_bf16, y_f32)
     switch (mt) {
       case 1: FC_CONV_LDS(1); break;
       case 2: FC_CONV_LDS(2); break;
