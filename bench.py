@@ -102,7 +102,11 @@ def thermal_leg(n, device, seed=0, weights=None):
                 "nn_iterations_with_network": ni.get("it_nn"), "nn_fallback_to_gmg": bool(ni["fallback"]),
                 "nn_final_rel_residual_fp64": ni["res_rel"], "nn_converged": bool(ni["converged"]),
                 "nn_weights": os.path.relpath(weights, os.path.dirname(os.path.abspath(__file__))),
-                "nn_vs_gmg_speedup": round(secs / max(nn_secs, 1e-9), 3)})
+                "nn_vs_gmg_speedup": round(secs / max(nn_secs, 1e-9), 3),
+                # with a fallback the remaining iterations are GMG-GCR(6): a speedup over GMG-PCG then
+                # comes from GCR, not from the network
+                "nn_speedup_source": "network" if not ni["fallback"] else "GMG-GCR(6) fallback after the network stalled",
+                "gmg_gcr6_vs_gmg_pcg_speedup": round(secs / max(gcr_secs, 1e-9), 3)})
     return out
 
 

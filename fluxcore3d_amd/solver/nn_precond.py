@@ -483,7 +483,7 @@ def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: 
 
 # ---------------------------------------------------------------- solver driver
 def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 1e-8, max_it: int = 300,
-           x0: Optional[torch.Tensor] = None, stall_window: int = 20, stall_factor: float = 0.5, m: int = 6,
+           x0: Optional[torch.Tensor] = None, stall_window: int = 8, stall_factor: float = 0.5, m: int = 6,
            ws=None, check_every: int = 1):
     """Flexible GCR(m) with the U-Net preconditioner (FP64 vectors, FP64 true
     residual as the judge). The operator is not symmetric and a learned
