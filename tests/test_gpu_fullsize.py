@@ -1,0 +1,167 @@
+"""The configurations at their full sizes, through properties that hold at any size
+(the CPU oracle is too slow there; it pins the same code paths at small sizes in
+test_gpu_lbm.py, test_gpu_physics.py and test_gpu_mg.py):
+
+* 256^3 sphere pack (porosity 0.6, MRT, reference Bouzidi, periodic): the fused
+  in-place step equals the reference's per-kernel sequence (collide -> stream ->
+  IBB -> reset, lbm_cuda_ext.py:445-517, run with this library's reference-semantics
+  kernels) after an odd and an even number of steps, <= 1e-5 of the field maxima;
+  the rest state stays at rest.
+* 384^3 gyroid (8 periods of 48 voxels, MRT, Bouzidi): FP16 population storage stays
+  within 1e-3 of the largest speed from FP32 storage after 100 steps, the bound the
+  48^3 unit cell meets against the FP64 oracle.
+* 512^3 blob conduction problem: GMG-preconditioned flexible GCR(6) and flexible CG
+  both reach an FP64 true relative residual of 1e-8 and agree on the solution.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _feq_t(rho, u, v, w):
+    """Second-order D3Q19 equilibrium in FP32 on the GPU (lbm_kernels.cu:77-113)."""
+    from fluxcore3d_amd.solver.d3q19 import C, W
+    c = torch.as_tensor(C, dtype=torch.float32, device=DEV)
+    wq = torch.as_tensor(W, dtype=torch.float32, device=DEV)
+    cu = 3.0 * (c[:, 0:1] * u + c[:, 1:2] * v + c[:, 2:3] * w)
+    u2 = 1.5 * (u * u + v * v + w * w)
+    return wq[:, None] * rho * (1.0 + cu + 0.5 * cu * cu - u2)
+
+
+def _pack_links(n, seed=0):
+    from fluxcore3d_amd import geometry
+    phi, solid, _ = geometry.rsa_spheres(n, porosity=0.6, rmin=4.0, rmax=10.0, seed=seed)
+    return solid, _links(phi, solid)
+
+
+def _links(phi, solid):
+    from fluxcore3d_amd.pre_post.sdf_tracing import bfl_q_and_weights, build_links_from_mask_gpu
+    per = (True, True, True)
+    ff, di, _, fff, ok = build_links_from_mask_gpu(torch.as_tensor(solid, device=DEV), periodic=per)
+    q, ge, w1, w2 = bfl_q_and_weights(torch.as_tensor(phi, device=DEV), (0.0,) * 3, 1.0, ff, di, periodic=per)
+    return dict(fluid_flat=ff, dir_i=di, ff_flat=fff, ff_valid=ok, w1=w1, w2=w2, q_ge_half=ge)
+
+
+def _perturbed(N, seed, du=0.02):
+    g = torch.Generator(device=DEV).manual_seed(seed)
+    rho = 1.0 + 0.005 * torch.rand(N, device=DEV, generator=g)
+    u, v, w = (du * torch.randn(N, device=DEV, generator=g) for _ in range(3))
+    return _feq_t(rho, u, v, w).contiguous()
+
+
+def _mrt_S(om):
+    from tests.cases import mrt_S
+    return mrt_S(om)
+
+
+def _rel(a, b):
+    return float((a - b).abs().max() / b.abs().max().clamp_min(1e-30))
+
+
+@pytest.mark.parametrize("steps", [11, 12])
+def test_pack_256_fused_step_equals_reference_sequence(steps):
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    from fluxcore3d_amd.solver.lbm_cuda_ext import LBM_CUDA
+    n = 256
+    N = n ** 3
+    solid, L = _pack_links(n)
+    om = 1.0 / 0.55
+    S = _mrt_S(om).astype(np.float32)
+    f0 = _perturbed(N, 1)
+    eng = AAEngine(n, n, n, solid, periodic=(1, 1, 1), collision="mrt", omega=om, S=S,
+                   links={k: v.cpu().numpy() for k, v in L.items()}, device=DEV)
+    eng.load(f0.view(19, n, n, n))
+    macro = [torch.empty(N, device=DEV) for _ in range(4)]
+    for _ in range(steps):
+        eng.step(*macro)
+    f_aa = eng.materialize().view(19, N)
+    del eng
+    # the reference's kernel sequence on f / f_post (reference-semantics kernels of this library)
+    f, fp = f0.clone(), torch.empty_like(f0)
+    mref = [torch.empty(N, device=DEV) for _ in range(4)]
+    st = torch.as_tensor(solid.reshape(-1).astype(np.uint8), device=DEV)
+    St = torch.as_tensor(S, device=DEV)
+    i32 = lambda t: t.to(torch.int32).contiguous()  # noqa: E731
+    from fluxcore3d_amd.solver.d3q19 import OPP
+    opp = torch.as_tensor(OPP, device=DEV)
+    args = (i32(L["fluid_flat"]), i32(L["dir_i"]), i32(opp[L["dir_i"].long()]), L["w1"].float().contiguous(),
+            L["w2"].float().contiguous(), L["q_ge_half"].to(torch.uint8).contiguous(), i32(L["ff_flat"]),
+            L["ff_valid"].to(torch.uint8).contiguous())
+    for _ in range(steps):
+        LBM_CUDA.macro_collide_mrt(f, fp, *mref, st, St)
+        LBM_CUDA.stream_pull(fp, f, n, n, n, 1, 1, 1)
+        LBM_CUDA.ibb(fp, f, *args)
+        LBM_CUDA.reset_solid(f, st)
+    assert _rel(macro[0], mref[0]) < 1e-5
+    for a in (1, 2, 3):
+        assert _rel(macro[a], mref[a]) < 1e-5, a
+    fluid = st == 0
+    assert _rel(f_aa[:, fluid], f[:, fluid]) < 1e-5
+
+
+def test_pack_256_rest_state_stays_at_rest():
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    from fluxcore3d_amd.solver.d3q19 import W
+    n = 256
+    N = n ** 3
+    solid, L = _pack_links(n)
+    om = 1.0 / 0.55
+    eng = AAEngine(n, n, n, solid, periodic=(1, 1, 1), collision="mrt", omega=om, S=_mrt_S(om),
+                   links={k: v.cpu().numpy() for k, v in L.items()}, device=DEV)
+    eng.load(torch.as_tensor(np.broadcast_to(W[:, None], (19, N)).copy(), device=DEV, dtype=torch.float32))
+    macro = [torch.empty(N, device=DEV) for _ in range(4)]
+    for _ in range(40):
+        eng.step(*macro)
+    assert float((macro[0] - 1.0).abs().max()) < 1e-5
+    assert max(float(m.abs().max()) for m in macro[1:]) < 1e-6
+
+
+def test_gyroid_384_fp16_storage_tracks_fp32():
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    n = 384
+    N = n ** 3
+    phi, solid = geometry.gyroid(n)
+    L = {k: v.cpu().numpy() for k, v in _links(phi, solid).items()}
+    om = 1.0 / 0.55
+    S = _mrt_S(om)
+    f0 = _perturbed(N, 2)
+    out = {}
+    for storage in ("fp32", "fp16"):
+        eng = AAEngine(n, n, n, solid, periodic=(1, 1, 1), collision="mrt", omega=om, S=S, links=L, storage=storage,
+                       device=DEV)
+        eng.load(f0.view(19, n, n, n))
+        macro = [torch.empty(N, device=DEV) for _ in range(4)]
+        for _ in range(100):
+            eng.step(*macro)
+        out[storage] = torch.stack(macro[1:])
+        del eng, macro
+        torch.cuda.empty_cache()
+    umax = float(out["fp32"].abs().max())
+    assert umax > 1e-3
+    assert float((out["fp16"] - out["fp32"]).abs().max()) / umax < 1e-3
+
+
+def test_blobs_512_gcr_and_fcg_reach_tolerance_and_agree():
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, gcr_fp64
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+    c = geometry.conduction_problem(512, seed=1, device=DEV)
+    mg = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, c["ax"], c["sign"], c["s"], omega=0.8, n_pre=2,
+                                 n_post=2, n_coarse=20)
+    mg.build_if_needed(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], c["dt"])
+    xg, gi = gcr_fp64(mg, c["b"], tol=1e-8, max_it=400, m=6)
+    assert gi["converged"] and gi["res_rel"] < 1e-8, gi["hist"][-3:]
+    hist = np.asarray(gi["hist"][:-1])
+    assert np.all(np.diff(hist) <= 1e-12 * hist[:-1])  # minimal residual: never grows
+    xg = gi["x64"].clone()
+    del gi
+    xf, fi = fcg_fp64(mg, c["b"], tol=1e-8, max_it=600)
+    assert fi["converged"] and fi["res_rel"] < 1e-8, fi["hist"][-3:]
+    m = c["mask"].reshape(-1).double()
+    xf64 = fi["x64"].reshape(-1) * m
+    assert float((xg.reshape(-1) * m - xf64).norm() / xf64.norm()) < 1e-6
