@@ -430,16 +430,18 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
   auto slot = [&](int dx, int dy, int dz) -> uint32_t {
     const int I = dx < 0 ? Im : (dx > 0 ? Ip : I0);
     const int J = dy < 0 ? Jm : (dy > 0 ? Jp : J0);
-    return (uint32_t)I * PYZ + (uint32_t)J * PZ + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
+    return __umul24((uint32_t)I, PYZ) + __umul24((uint32_t)J, PZ) + (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
   };
   // Slot of direction q's access in this step's parity: the even step uses the
   // node's own slots, so its -1/+1 neighbour coordinates collapse onto the node's.
   // One load loop and one store loop serve both parities (no branch; w_q = w_opp(q)).
-  int aIm = 0, aIp = 0, aJm = 0, aJp = 0, aKm = 0, aKp = 0;
-  auto pslot = [&](int dx, int dy, int dz) -> uint32_t {
-    const int I = dx < 0 ? aIm : (dx > 0 ? aIp : I0);
-    const int J = dy < 0 ? aJm : (dy > 0 ? aJp : J0);
-    return (uint32_t)I * PYZ + (uint32_t)J * PZ + (uint32_t)(dz < 0 ? aKm : (dz > 0 ? aKp : K0));
+  // The x and y terms of the three coordinates are formed once per lane (24-bit
+  // multiplies: padded extents stay below 2^12, PY * PZ below 2^24).
+  uint32_t oI[3] = {0, 0, 0}, oJ[3] = {0, 0, 0}, oK[3] = {0, 0, 0};
+  auto pslot = [&](int dx, int dy, int dz) -> uint32_t { return oI[dx + 1] + oJ[dy + 1] + oK[dz + 1]; };
+  // a direction's slots: uniform base pointer plus a 32-bit byte offset per lane
+  auto at = [&](int dir, uint32_t slot_idx) -> ST* {
+    return (ST*)((char*)(pop + (size_t)dir * (size_t)g.Ns) + slot_idx * (uint32_t)sizeof(ST));
   };
 #pragma unroll
   for (int q = 0; q < 19; ++q) f[q] = 0.f;
@@ -453,13 +455,14 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
     Im = wrap_or_pad(i, -1, g.nx, g.gx); I0 = i + g.gx; Ip = wrap_or_pad(i, 1, g.nx, g.gx);
     Jm = wrap_or_pad(j, -1, g.ny, g.gy); J0 = j + g.gy; Jp = wrap_or_pad(j, 1, g.ny, g.gy);
     Km = wrap_or_pad(k, -1, g.nz, g.gz); K0 = k + g.gz; Kp = wrap_or_pad(k, 1, g.nz, g.gz);
-    self = (uint32_t)I0 * PYZ + (uint32_t)J0 * PZ + (uint32_t)K0;
-    aIm = parity ? Im : I0; aIp = parity ? Ip : I0;
-    aJm = parity ? Jm : J0; aJp = parity ? Jp : J0;
-    aKm = parity ? Km : K0; aKp = parity ? Kp : K0;
+    oI[1] = __umul24((uint32_t)I0, PYZ); oJ[1] = __umul24((uint32_t)J0, PZ); oK[1] = (uint32_t)K0;
+    self = oI[1] + oJ[1] + oK[1];
+    oI[0] = parity ? __umul24((uint32_t)Im, PYZ) : oI[1]; oI[2] = parity ? __umul24((uint32_t)Ip, PYZ) : oI[1];
+    oJ[0] = parity ? __umul24((uint32_t)Jm, PZ) : oJ[1]; oJ[2] = parity ? __umul24((uint32_t)Jp, PZ) : oJ[1];
+    oK[0] = parity ? (uint32_t)Km : oK[1]; oK[2] = parity ? (uint32_t)Kp : oK[1];
 #pragma unroll
     for (int q = 0; q < 19; ++q)
-      f[q] = Store<ST>::ld_nt(pop + (size_t)(q + parity * (OPP(q) - q)) * (size_t)g.Ns + pslot(-CX(q), -CY(q), -CZ(q)), q);
+      f[q] = Store<ST>::ld_nt(at(q + parity * (OPP(q) - q), pslot(-CX(q), -CY(q), -CZ(q))), q);
   }
 
   // Each lane's first entry of every list (and second link entry), requested
@@ -544,7 +547,7 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
     }
 #pragma unroll
     for (int q = 0; q < 19; ++q)
-      Store<ST>::st_nt(pop + (size_t)(OPP(q) + parity * (q - OPP(q))) * (size_t)g.Ns + pslot(CX(q), CY(q), CZ(q)), f[q], q);
+      Store<ST>::st_nt(at(OPP(q) + parity * (q - OPP(q)), pslot(CX(q), CY(q), CZ(q))), f[q], q);
     // copy-self on non-periodic faces through the ghost layer
     if (fluid && ((g.gx && (i == 0 || i == g.nx - 1)) || (g.gy && (j == 0 || j == g.ny - 1)) ||
                   (g.gz && (k == 0 || k == g.nz - 1)))) {
@@ -910,7 +913,9 @@ int fc_aa_step(void* pop, int storage, int nx, int ny, int nz, int px, int py, i
   Grid g; Tables t;
   if (int rc = pack(g, t, tb, nx, ny, nz, px, py, pz)) return rc;
   FC_REQUIRE(pop, "aa_step: null population buffer");
-  FC_REQUIRE(g.Ns < ((int64_t)1 << 31), "aa_step: padded grid of %lld nodes exceeds 2^31", (long long)g.Ns);
+  FC_REQUIRE(g.Ns < ((int64_t)1 << 30), "aa_step: padded grid of %lld nodes exceeds 2^30", (long long)g.Ns);
+  FC_REQUIRE((int64_t)g.PY * g.PZ < ((int64_t)1 << 24) && nx + 2 < (1 << 12),
+             "aa_step: padded extents exceed the 24-bit slot arithmetic");
   FC_REQUIRE(col && col->kind >= 0 && col->kind <= 3, "aa_step: collision kind must be 0..3");
   FC_REQUIRE(parity == 0 || parity == 1, "aa_step: parity must be 0 or 1");
   FC_REQUIRE((rho && u && v && w) || (!rho && !u && !v && !w), "aa_step: macro outputs must be all set or all null");
