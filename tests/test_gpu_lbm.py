@@ -174,6 +174,37 @@ def test_config1_step_sequence(engine, kind):
     assert rel(sim.f.cpu().numpy().reshape(19, -1), f_want) < 1e-5
 
 
+def test_config1_full_run():
+    """Config 1 at its specified length: 64x32x32 sphere channel, BGK tau = 0.6, inlet
+    u = 0.05, convective outlet, walls, Bouzidi links, 2000 steps of the fused step
+    against the FP32 oracle (same operation order) and the FP64 oracle. The FP32
+    runs differ only by rounding (contracted multiply-adds on the GPU); the flow
+    settles towards its steady state, so the difference stays bounded."""
+    from fluxcore3d_amd.solver.lbm_cuda_ext import LBMCUDAUpgrade, flow_bc_plan
+    c = cases.config1()
+    sim = _sim_for(c, "bgk", "aa")
+    LBMCUDAUpgrade.patch(sim)
+    steps = 2000
+    for _ in range(steps):
+        sim.step(do_thermal=False)
+    torch.cuda.synchronize()
+    plan = flow_bc_plan(sim)
+    f32, (rho_w, u_w, v_w, w_w) = _oracle_run(c, steps, "bgk", 1.0, None, (0, 0, 0), plan)
+    speed = float(np.sqrt(u_w**2 + v_w**2 + w_w**2).max())
+    got = [a.cpu().numpy().reshape(-1) for a in (sim.u, sim.v, sim.wv)]
+    err32 = max(np.abs(g - w).max() for g, w in zip(got, (u_w, v_w, w_w))) / speed
+    assert err32 < 2e-5, err32
+    assert rel(sim.rho.cpu().numpy().reshape(-1), rho_w) < 1e-5
+    # FP64 oracle (reported against the FP32 path: FP32 storage and arithmetic)
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    f = np.broadcast_to(olbm.W32[:, None].astype(np.float64), (19, nx * ny * nz)).copy()
+    for _ in range(steps):
+        f, rho64, u64, v64, w64 = olbm.step(f, c["solid_flow"], nx, ny, nz, (0, 0, 0), "bgk", c["omega"], 1.0, None,
+                                            0.0, c["links"], None, plan)
+    err64 = max(np.abs(g - w).max() for g, w in zip(got, (u64, v64, w64))) / speed
+    assert err64 < 1e-4, err64
+
+
 @pytest.mark.parametrize("engine", ["aa", "reference"])
 def test_pin_fin_channel_trt(engine):
     """Heat-sink channel at 1/4 scale (48x48x96: plate + 4x4 pins, flow +Z, inlet
