@@ -139,6 +139,20 @@ __device__ __forceinline__ void mrt_df(const float fneq[19], const float S[19], 
   }
 }
 
+// Face form of the 7-point conduction coefficients. One array per axis, with the
+// cell layout: slot c holds the coefficient of the face between cell c and its
+// +1 neighbour along the axis. The last plane has no such face; its slot holds
+// the low-boundary coefficient of the first plane where the low ghost is the cell
+// itself (flow axis, inlet at 0), and is unused otherwise. Mirrored ghosts
+// (high face -> n-2, low face -> 1) reuse the face towards that neighbour.
+// pos = index along the axis, stride = its element stride.
+__device__ __forceinline__ int face_hi(int idx, int pos, int n, int stride) {
+  return pos < n - 1 ? idx : idx - stride;
+}
+__device__ __forceinline__ int face_lo(int idx, int pos, int n, int stride, int ghost_mirrors) {
+  return pos > 0 ? idx - stride : (ghost_mirrors ? idx : idx + (n - 1) * stride);
+}
+
 // Round-up grid helper.
 inline unsigned grid_for(int64_t n, int block) {
   return (unsigned)((n + block - 1) / block);

@@ -28,6 +28,7 @@ struct Stencil {
   const float* mask;
   int nx, ny, nz, ax, out_idx;
   double dt;
+  const float *fx, *fy, *fz;  // face form of the coefficients (fc_common.hpp), or null
 };
 
 struct Dirs {
@@ -64,26 +65,49 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restri
   }
 }
 
-// scal[dst + a] = sum over blocks of partial[b * stride + a], a < nv
+// scal[dst + a] = sum over blocks of partial[b * stride + a], a < nv. Each thread
+// adds its blocks b = tid, tid + kBlock, ... in that order, the loads of 4 blocks
+// (all nv values) issued together.
 __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__ partial, int nblocks, int stride,
                                                       int nv, double* __restrict__ scal, int dst) {
-  __shared__ double sh[kBlock / 64];
-  for (int a = 0; a < nv; ++a) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) v += partial[(int64_t)b * stride + a];
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < kBlock / 64; ++w) s += sh[w];
-      scal[dst + a] = s;
+  double v[kMaxDirs];
+#pragma unroll
+  for (int a = 0; a < kMaxDirs; ++a) v[a] = 0.0;
+  for (int b0 = threadIdx.x; b0 < nblocks; b0 += 4 * kBlock) {
+    double t[4][kMaxDirs];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + u * kBlock;
+#pragma unroll
+      for (int a = 0; a < kMaxDirs; ++a) t[u][a] = (b < nblocks && a < nv) ? partial[(int64_t)b * stride + a] : 0.0;
     }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (b0 + u * kBlock < nblocks) {
+#pragma unroll
+        for (int a = 0; a < kMaxDirs; ++a)
+          if (a < nv) v[a] += t[u][a];
+      }
+    }
+  }
+  __shared__ double sh[kBlock / 64][kMaxDirs];
+#pragma unroll
+  for (int a = 0; a < kMaxDirs; ++a) {
+    const double w = wave_sum(v[a]);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][a] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += sh[w][threadIdx.x];
+    scal[dst + threadIdx.x] = s;
   }
 }
 
-// p = (double) z;  w = (p + dt L p) * mask  (one cell per thread, 32-bit index math)
+// p = (double) z;  w = (p + dt L p) * mask  (one cell per thread, 32-bit index math).
+// FACE: the six coefficients come from the face arrays and diagL is their sum in
+// the coefficient build's order (the same bits as the per-cell arrays).
+template <bool FACE>
 __global__ __launch_bounds__(kBlock) void apply_kernel(const float* __restrict__ z, Stencil s, double* __restrict__ p,
                                                        double* __restrict__ w) {
   const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -96,11 +120,23 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(const float* __restrict__
   const int oE = (i < s.nx - 1 ? 1 : -1) * (int)nynz, oW = (i > 0 ? -1 : lo_ghost(s, 0)) * (int)nynz;
   const int oN = (j < s.ny - 1 ? 1 : -1) * s.nz, oS = (j > 0 ? -1 : lo_ghost(s, 1)) * s.nz;
   const int oU = k < s.nz - 1 ? 1 : -1, oD = k > 0 ? -1 : lo_ghost(s, 2);
+  float cE, cW, cN, cS, cU, cD, cL;
+  if constexpr (FACE) {
+    cE = s.fx[face_hi(idx, i, s.nx, (int)nynz)];
+    cW = s.fx[face_lo(idx, i, s.nx, (int)nynz, lo_ghost(s, 0))];
+    cN = s.fy[face_hi(idx, j, s.ny, s.nz)];
+    cS = s.fy[face_lo(idx, j, s.ny, s.nz, lo_ghost(s, 1))];
+    cU = s.fz[face_hi(idx, k, s.nz, 1)];
+    cD = s.fz[face_lo(idx, k, s.nz, 1, lo_ghost(s, 2))];
+    cL = ((((cE + cW) + cN) + cS) + cU) + cD;
+  } else {
+    cE = s.aE[idx]; cW = s.aW[idx]; cN = s.aN[idx]; cS = s.aS[idx]; cU = s.aU[idx]; cD = s.aD[idx];
+    cL = s.diagL[idx];
+  }
   const double pc = (double)z[idx];
-  const double L = (double)s.diagL[idx] * pc - (double)s.aE[idx] * (double)z[idx + oE] -
-                   (double)s.aW[idx] * (double)z[idx + oW] - (double)s.aN[idx] * (double)z[idx + oN] -
-                   (double)s.aS[idx] * (double)z[idx + oS] - (double)s.aU[idx] * (double)z[idx + oU] -
-                   (double)s.aD[idx] * (double)z[idx + oD];
+  const double L = (double)cL * pc - (double)cE * (double)z[idx + oE] - (double)cW * (double)z[idx + oW] -
+                   (double)cN * (double)z[idx + oN] - (double)cS * (double)z[idx + oS] -
+                   (double)cU * (double)z[idx + oU] - (double)cD * (double)z[idx + oD];
   p[idx] = pc;
   w[idx] = (pc + s.dt * L) * (double)s.mask[idx];
 }
@@ -183,6 +219,7 @@ struct fc_gcr_stencil {
   const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *mask;
   double dt;
   int nx, ny, nz, ax, out_idx;
+  const float *fx, *fy, *fz;
 };
 
 int fc_gcr_scalar_slots(void) { return K_SLOTS; }
@@ -200,8 +237,11 @@ int fc_gcr_step(const float* z32, const fc_gcr_stencil* st, double* p, double* w
   FC_REQUIRE(store_slot >= K_WWHIST0 && store_slot < K_SLOTS, "gcr_step: store slot out of range");
   FC_REQUIRE(st->nx >= 2 && st->ny >= 2 && st->nz >= 2, "gcr_step: every extent must be >= 2");
   Stencil s{st->aE, st->aW, st->aN, st->aS, st->aU, st->aD, st->diagL, st->mask, st->nx, st->ny, st->nz,
-            st->ax, st->out_idx, st->dt};
-  FC_REQUIRE(s.aE && s.aW && s.aN && s.aS && s.aU && s.aD && s.diagL && s.mask, "gcr_step: null coefficient");
+            st->ax, st->out_idx, st->dt, st->fx, st->fy, st->fz};
+  const int faces = (s.fx != nullptr) + (s.fy != nullptr) + (s.fz != nullptr);
+  FC_REQUIRE(faces == 0 || faces == 3, "gcr_step: face coefficients must be all set or all null");
+  FC_REQUIRE(s.mask && (faces == 3 || (s.aE && s.aW && s.aN && s.aS && s.aU && s.aD && s.diagL)),
+             "gcr_step: null coefficient");
   const int64_t n = (int64_t)s.nx * s.ny * s.nz;
   FC_REQUIRE(n + (int64_t)kMaxBlocks * kBlock < ((int64_t)1 << 31), "gcr_step: %lld cells exceed the 32-bit index range",
              (long long)n);
@@ -215,7 +255,8 @@ int fc_gcr_step(const float* z32, const fc_gcr_stencil* st, double* p, double* w
     d.slot[j] = ww_slot[j];
   }
   const unsigned g = grid_of(n);
-  hipLaunchKernelGGL(apply_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, z32, s, p, w);
+  hipLaunchKernelGGL(faces ? apply_kernel<true> : apply_kernel<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0,
+                     stream, z32, s, p, w);
   if (int rc = check_launch("gcr_step/apply")) return rc;
   if (m > 0) {
     hipLaunchKernelGGL(dots_kernel, dim3(g), dim3(kBlock), 0, stream, (const double*)w, d, n, partial);

@@ -36,6 +36,9 @@ struct Level {
 
 struct Coef {
   const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *invD;
+  // face form (fc_common.hpp face_hi/face_lo); the kernels templated on FACE
+  // read these three arrays instead of the six per-cell ones and diagL
+  const float *fx, *fy, *fz;
 };
 
 __device__ __forceinline__ int lo_ghost(const Level& g, int this_ax) {
@@ -70,19 +73,52 @@ __device__ __forceinline__ Nbr neighbours(const Level& g, int i, int j, int k) {
   return o;
 }
 
-// Lo = T + dt L(T) at cell idx with the reference's operand order.
-template <typename TT>
-__device__ __forceinline__ float stencil_lo(const TT* __restrict__ T, const Coef& c, float dt, int idx, const Nbr& o) {
-  const float TC = (float)T[idx];
-  const float TE = (float)T[idx + o.e];
-  const float TW = (float)T[idx + o.w];
-  const float TN = (float)T[idx + o.n];
-  const float TS = (float)T[idx + o.s];
-  const float TU = (float)T[idx + o.u];
-  const float TD = (float)T[idx + o.d];
-  const float LT = c.diagL[idx] * TC - c.aE[idx] * TE - c.aW[idx] * TW - c.aN[idx] * TN - c.aS[idx] * TS -
-                   c.aU[idx] * TU - c.aD[idx] * TD;
+// The six coefficients and diagL of cell (i,j,k). In face form diagL is summed
+// in build_coeffs_kernel's order, so both forms give the same bits wherever
+// fc_mg_build_faces found the face form exact.
+struct C7 {
+  float e, w, n, s, u, d, diag;
+};
+template <bool FACE>
+__device__ __forceinline__ C7 coef_at(const Coef& c, const Level& g, int idx, int i, int j, int k) {
+  C7 a;
+  if constexpr (FACE) {
+    const int sx = g.ny * g.nz;
+    a.e = c.fx[face_hi(idx, i, g.nx, sx)];
+    a.w = c.fx[face_lo(idx, i, g.nx, sx, lo_ghost(g, 0))];
+    a.n = c.fy[face_hi(idx, j, g.ny, g.nz)];
+    a.s = c.fy[face_lo(idx, j, g.ny, g.nz, lo_ghost(g, 1))];
+    a.u = c.fz[face_hi(idx, k, g.nz, 1)];
+    a.d = c.fz[face_lo(idx, k, g.nz, 1, lo_ghost(g, 2))];
+    a.diag = ((((a.e + a.w) + a.n) + a.s) + a.u) + a.d;
+  } else {
+    a.e = c.aE[idx]; a.w = c.aW[idx]; a.n = c.aN[idx]; a.s = c.aS[idx]; a.u = c.aU[idx]; a.d = c.aD[idx];
+    a.diag = c.diagL[idx];
+  }
+  return a;
+}
+
+// Lo = T + dt L(T) from the seven values, with the reference's operand order. Every
+// kernel forms Lo through this one expression, so fused and separate kernels give
+// the same bits.
+__device__ __forceinline__ float lo_of(const C7& a, float dt, float TC, float TE, float TW, float TN, float TS,
+                                       float TU, float TD) {
+  const float LT = a.diag * TC - a.e * TE - a.w * TW - a.n * TN - a.s * TS - a.u * TU - a.d * TD;
   return TC + dt * LT;
+}
+
+template <bool FACE, typename TT>
+__device__ __forceinline__ float stencil_lo(const TT* __restrict__ T, const Coef& c, const Level& g, float dt, int idx,
+                                            int i, int j, int k) {
+  const Nbr o = neighbours(g, i, j, k);
+  const C7 a = coef_at<FACE>(c, g, idx, i, j, k);
+  return lo_of(a, dt, (float)T[idx], (float)T[idx + o.e], (float)T[idx + o.w], (float)T[idx + o.n],
+               (float)T[idx + o.s], (float)T[idx + o.u], (float)T[idx + o.d]);
+}
+
+// First sweep from x = 0 (x1 = invD b), written the way sweep_kernel writes it.
+__device__ __forceinline__ float zero_sweep(const Coef& c, const float* __restrict__ b, int idx) {
+  return 0.f + c.invD[idx] * (b[idx] - 0.f);
 }
 
 __global__ __launch_bounds__(kBlock) void mop_kernel(const float* __restrict__ T, Coef c, Level g, float dt,
@@ -91,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void mop_kernel(const float* __restrict__ T
   if (idx >= g.nx * g.ny * g.nz) return;
   int i, j, k;
   unflat(g, idx, i, j, k);
-  Lo[idx] = stencil_lo(T, c, dt, idx, neighbours(g, i, j, k));
+  Lo[idx] = stencil_lo<false>(T, c, g, dt, idx, i, j, k);
 }
 
 __global__ __launch_bounds__(kBlock) void jacobi_kernel(float* __restrict__ x, const float* __restrict__ b,
@@ -103,42 +139,75 @@ __global__ __launch_bounds__(kBlock) void jacobi_kernel(float* __restrict__ x, c
 }
 
 // x_out = x + invD (b - A x): the two reference kernels in one pass.
+template <bool FACE>
 __global__ __launch_bounds__(kBlock) void sweep_kernel(const float* __restrict__ x, float* __restrict__ x_out,
                                                        const float* __restrict__ b, Coef c, Level g, float dt,
                                                        int zero_guess) {
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= g.nx * g.ny * g.nz) return;
   if (zero_guess) {  // x = 0: A x = 0
-    x_out[idx] = 0.f + c.invD[idx] * (b[idx] - 0.f);
+    x_out[idx] = zero_sweep(c, b, idx);
     return;
   }
   int i, j, k;
   unflat(g, idx, i, j, k);
-  const float lo = stencil_lo(x, c, dt, idx, neighbours(g, i, j, k));
+  const float lo = stencil_lo<FACE>(x, c, g, dt, idx, i, j, k);
   x_out[idx] = x[idx] + c.invD[idx] * (b[idx] - lo);
 }
 
+// The first two sweeps from x = 0 in one pass: x1 = invD b is formed at the seven
+// stencil points from b and invD instead of being written and read back;
+// x_out = x1 + invD (b - A x1), the same bits as two sweep_kernel launches.
+template <bool FACE>
+__global__ __launch_bounds__(kBlock) void sweep2_from_zero_kernel(float* __restrict__ x_out,
+                                                                  const float* __restrict__ b, Coef c, Level g,
+                                                                  float dt) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= g.nx * g.ny * g.nz) return;
+  int i, j, k;
+  unflat(g, idx, i, j, k);
+  const Nbr o = neighbours(g, i, j, k);
+  const C7 a = coef_at<FACE>(c, g, idx, i, j, k);
+  const float xc = zero_sweep(c, b, idx);
+  const float lo = lo_of(a, dt, xc, zero_sweep(c, b, idx + o.e), zero_sweep(c, b, idx + o.w),
+                         zero_sweep(c, b, idx + o.n), zero_sweep(c, b, idx + o.s), zero_sweep(c, b, idx + o.u),
+                         zero_sweep(c, b, idx + o.d));
+  x_out[idx] = xc + c.invD[idx] * (b[idx] - lo);
+}
+
 // coarse rhs: mean of the 8 fine residuals (b - A x) * mask (odd extents truncated).
+// A block covers 32 coarse cells of one coarse row: its four wavefronts take the
+// four fine rows (2I + di, 2J + dj), one fine cell per lane along k (contiguous
+// loads); the residuals meet in LDS and lane K of the first wavefront adds the 8
+// of coarse cell K in the order d = 4 di + 2 dj + dk.
+template <bool FACE>
 __global__ __launch_bounds__(kBlock) void residual_restrict_kernel(const float* __restrict__ x,
                                                                    const float* __restrict__ b,
                                                                    const float* __restrict__ mask, Coef c, Level g,
                                                                    float dt, float* __restrict__ bc, int cx, int cy,
-                                                                   int cz) {
-  const int cidx = blockIdx.x * kBlock + threadIdx.x;
-  if (cidx >= cx * cy * cz) return;
-  const unsigned cycz = (unsigned)cy * (unsigned)cz, u = (unsigned)cidx;
-  const int I = (int)(u / cycz);
-  const unsigned rem = u - (unsigned)I * cycz;
-  const int J = (int)(rem / (unsigned)cz), K = (int)(rem - (unsigned)J * (unsigned)cz);
-  float s = 0.f;
-#pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    const int i = 2 * I + (d >> 2), j = 2 * J + ((d >> 1) & 1), k = 2 * K + (d & 1);
+                                                                   int cz, int kchunks) {
+  static_assert(kBlock == 256, "four fine rows of 64 cells per block");
+  __shared__ float res[4][64];
+  const int row = blockIdx.x / kchunks;  // coarse (I, J)
+  const int K0 = (blockIdx.x - row * kchunks) * 32;
+  const int I = row / cy, J = row - I * cy;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = 2 * I + (wave >> 1), j = 2 * J + (wave & 1), k = 2 * K0 + lane;
+  float r = 0.f;
+  if (k < 2 * cz) {
     const int idx = (i * g.ny + j) * g.nz + k;
-    const float lo = stencil_lo(x, c, dt, idx, neighbours(g, i, j, k));
-    s += (b[idx] - lo) * mask[idx];
+    const float lo = stencil_lo<FACE>(x, c, g, dt, idx, i, j, k);
+    r = (b[idx] - lo) * mask[idx];
   }
-  bc[cidx] = s / 8.f;
+  res[wave][lane] = r;
+  __syncthreads();
+  const int K = K0 + threadIdx.x;
+  if (threadIdx.x < 32 && K < cz) {
+    float s = 0.f;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) s += res[d >> 1][2 * threadIdx.x + (d & 1)];
+    bc[(I * cy + J) * cz + K] = s / 8.f;
+  }
 }
 
 // linear interpolation source (align_corners = False) along one axis; scale =
@@ -175,6 +244,7 @@ __global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __
 
 // All sweeps of the coarsest level in one workgroup (global ping-pong, the
 // level is small enough to stay in L2); x holds the result on return.
+template <bool FACE>
 __global__ __launch_bounds__(1024) void coarse_solve_kernel(float* __restrict__ x, float* __restrict__ tmp,
                                                             const float* __restrict__ b, Coef c, Level g, float dt,
                                                             int sweeps, int zero_guess) {
@@ -184,11 +254,11 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(float* __restrict__ 
   for (int s = 0; s < sweeps; ++s) {
     for (int idx = threadIdx.x; idx < (int)n; idx += blockDim.x) {
       if (s == 0 && zero_guess) {
-        dst[idx] = 0.f + c.invD[idx] * (b[idx] - 0.f);
+        dst[idx] = zero_sweep(c, b, idx);
       } else {
         int i, j, k;
         unflat(g, idx, i, j, k);
-        const float lo = stencil_lo(src, c, dt, idx, neighbours(g, i, j, k));
+        const float lo = stencil_lo<FACE>(src, c, g, dt, idx, i, j, k);
         dst[idx] = src[idx] + c.invD[idx] * (b[idx] - lo);
       }
     }
@@ -249,6 +319,35 @@ __global__ __launch_bounds__(kBlock) void inv_diag_kernel(const float* __restric
   d = d < 1e-12f ? 1e-12f : d;
   const float r = 1.0f / d;
   invD[idx] = (r * omega) * mask[idx];
+}
+
+// Face form of one level's coefficients (fc_common.hpp face_hi/face_lo).
+__global__ __launch_bounds__(kBlock) void build_faces_kernel(Coef c, Level g, float* __restrict__ fx,
+                                                             float* __restrict__ fy, float* __restrict__ fz) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= g.nx * g.ny * g.nz) return;
+  int i, j, k;
+  unflat(g, idx, i, j, k);
+  const int sx = g.ny * g.nz;
+  fx[idx] = i < g.nx - 1 ? c.aE[idx] : (lo_ghost(g, 0) ? 0.f : c.aW[idx - (g.nx - 1) * sx]);
+  fy[idx] = j < g.ny - 1 ? c.aN[idx] : (lo_ghost(g, 1) ? 0.f : c.aS[idx - (g.ny - 1) * g.nz]);
+  fz[idx] = k < g.nz - 1 ? c.aU[idx] : (lo_ghost(g, 2) ? 0.f : c.aD[idx - (g.nz - 1)]);
+}
+
+// Count the cells whose six coefficients or diagL, read back through the face
+// form, differ in any bit from the per-cell arrays.
+__global__ __launch_bounds__(kBlock) void check_faces_kernel(Coef c, Level g, unsigned* __restrict__ mismatches) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= g.nx * g.ny * g.nz) return;
+  int i, j, k;
+  unflat(g, idx, i, j, k);
+  const C7 f = coef_at<true>(c, g, idx, i, j, k);
+  const C7 a = coef_at<false>(c, g, idx, i, j, k);
+  const bool same = __float_as_uint(f.e) == __float_as_uint(a.e) && __float_as_uint(f.w) == __float_as_uint(a.w) &&
+                    __float_as_uint(f.n) == __float_as_uint(a.n) && __float_as_uint(f.s) == __float_as_uint(a.s) &&
+                    __float_as_uint(f.u) == __float_as_uint(a.u) && __float_as_uint(f.d) == __float_as_uint(a.d) &&
+                    __float_as_uint(f.diag) == __float_as_uint(a.diag);
+  if (!same) atomicAdd(mismatches, 1u);
 }
 
 // 2x2x2 means of up to 6 fields (hierarchy build); mask_mode 1 thresholds
@@ -337,9 +436,27 @@ enum { S_PAP = 0, S_PR = 1, S_RR = 2, S_BB = 3, S_ZAP = 4, S_ALPHA = 5, S_BETA =
 // fold the partials of `nparts` blocks into scal[slot0..slot0+nv-1] (one block)
 __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__ partial, int nparts, int nv,
                                                       double* __restrict__ scal, int slot0, int update) {
+  // Each thread adds its partials p = tid, tid + kBlock, ... in that order; the
+  // loads of 8 of them are issued together (one after another they would cost a
+  // memory latency each).
   double v[3] = {0.0, 0.0, 0.0};
-  for (int p = threadIdx.x; p < nparts; p += kBlock)
-    for (int a = 0; a < nv; ++a) v[a] += partial[(int64_t)p * 3 + a];
+  for (int p0 = threadIdx.x; p0 < nparts; p0 += 8 * kBlock) {
+    double t[8][3];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int p = p0 + u * kBlock;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) t[u][a] = (p < nparts && a < nv) ? partial[(int64_t)p * 3 + a] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (p0 + u * kBlock < nparts) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          if (a < nv) v[a] += t[u][a];
+      }
+    }
+  }
   __shared__ double sh[kBlock / 64][3];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int a = 0; a < 3; ++a) {
@@ -362,6 +479,7 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__
 }
 
 // Ap = (p + dt L p) * mask in FP64 (FP32 coefficients); partials of p.Ap, p.r
+template <bool FACE>
 __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restrict__ p, double* __restrict__ Ap,
                                                            const double* __restrict__ r, const float* __restrict__ mask,
                                                            Coef c, Level g, double dt, double* __restrict__ partial) {
@@ -371,10 +489,11 @@ __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restr
     int i, j, k;
     unflat(g, idx, i, j, k);
     const Nbr o = neighbours(g, i, j, k);
+    const C7 q = coef_at<FACE>(c, g, idx, i, j, k);
     const double pc = p[idx];
-    const double L = (double)c.diagL[idx] * pc - (double)c.aE[idx] * p[idx + o.e] - (double)c.aW[idx] * p[idx + o.w] -
-                     (double)c.aN[idx] * p[idx + o.n] - (double)c.aS[idx] * p[idx + o.s] -
-                     (double)c.aU[idx] * p[idx + o.u] - (double)c.aD[idx] * p[idx + o.d];
+    const double L = (double)q.diag * pc - (double)q.e * p[idx + o.e] - (double)q.w * p[idx + o.w] -
+                     (double)q.n * p[idx + o.n] - (double)q.s * p[idx + o.s] - (double)q.u * p[idx + o.u] -
+                     (double)q.d * p[idx + o.d];
     const double a = (pc + dt * L) * (double)mask[idx];
     Ap[idx] = a;
     v[0] += pc * a;
@@ -419,6 +538,7 @@ __global__ __launch_bounds__(kBlock) void new_dir_kernel(double* __restrict__ p,
 }
 
 // true residual r = (b - A x) * mask in FP64, r32 copy, partials of r.r
+template <bool FACE>
 __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __restrict__ x,
                                                                const double* __restrict__ b, double* __restrict__ r,
                                                                float* __restrict__ r32,
@@ -430,10 +550,11 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
     int i, j, k;
     unflat(g, idx, i, j, k);
     const Nbr o = neighbours(g, i, j, k);
+    const C7 q = coef_at<FACE>(c, g, idx, i, j, k);
     const double xc = x[idx];
-    const double L = (double)c.diagL[idx] * xc - (double)c.aE[idx] * x[idx + o.e] - (double)c.aW[idx] * x[idx + o.w] -
-                     (double)c.aN[idx] * x[idx + o.n] - (double)c.aS[idx] * x[idx + o.s] -
-                     (double)c.aU[idx] * x[idx + o.u] - (double)c.aD[idx] * x[idx + o.d];
+    const double L = (double)q.diag * xc - (double)q.e * x[idx + o.e] - (double)q.w * x[idx + o.w] -
+                     (double)q.n * x[idx + o.n] - (double)q.s * x[idx + o.s] - (double)q.u * x[idx + o.u] -
+                     (double)q.d * x[idx + o.d];
     const double m = (double)mask[idx];
     const double bb = b[idx] * m;
     const double rr = (bb - (xc + dt * L)) * m;
@@ -485,14 +606,27 @@ extern "C" {
 
 struct fc_mg_coef {
   const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *invD;
+  const float *fx, *fy, *fz;
 };
 
 static Coef to_coef(const fc_mg_coef* c) {
   Coef o;
   o.aE = c->aE; o.aW = c->aW; o.aN = c->aN; o.aS = c->aS; o.aU = c->aU; o.aD = c->aD; o.diagL = c->diagL;
   o.invD = c->invD;
+  o.fx = c->fx; o.fy = c->fy; o.fz = c->fz;
   return o;
 }
+
+// 1: face form, 0: per-cell arrays, -1: partly set (an error)
+static int face_form(const fc_mg_coef* c) {
+  const int set = (c->fx != nullptr) + (c->fy != nullptr) + (c->fz != nullptr);
+  return set == 3 ? 1 : (set == 0 ? 0 : -1);
+}
+#define FC_COEF_FORM(c, what)                                                                  \
+  const int form = face_form(c);                                                               \
+  FC_REQUIRE(form >= 0, what ": face coefficients must be all set or all null");               \
+  FC_REQUIRE(form == 1 || ((c)->aE && (c)->aW && (c)->aN && (c)->aS && (c)->aU && (c)->aD &&   \
+                           (c)->diagL), what ": null coefficient array")
 
 int fc_mg_mop_fused(const float* T, const float* aE, const float* aW, const float* aN, const float* aS,
                     const float* aU, const float* aD, const float* diagL, float* Lo, double dt, int nx, int ny, int nz,
@@ -528,6 +662,22 @@ int fc_mg_build_coeffs(const float* k, const float* rho_cp, const float* u, cons
   return check_launch("build_coeffs");
 }
 
+int fc_mg_build_faces(const fc_mg_coef* coef, int nx, int ny, int nz, int ax, int out_idx, float* fx, float* fy,
+                      float* fz, unsigned* mismatches, hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(coef && coef->aE && coef->aW && coef->aN && coef->aS && coef->aU && coef->aD && coef->diagL,
+             "build_faces: the per-cell coefficient arrays are required");
+  FC_REQUIRE(fx && fy && fz && mismatches, "build_faces: null pointer argument");
+  Coef c = to_coef(coef);
+  const int64_t n = (int64_t)nx * ny * nz;
+  hipLaunchKernelGGL(build_faces_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, c, g, fx, fy, fz);
+  if (int rc = check_launch("build_faces")) return rc;
+  c.fx = fx; c.fy = fy; c.fz = fz;
+  hipLaunchKernelGGL(check_faces_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, c, g, mismatches);
+  return check_launch("build_faces/check");
+}
+
 int fc_mg_inv_diag(const float* diagL, const float* mask, double dt, double omega, float* inv_diagM, int64_t n,
                    hipStream_t stream) {
   if (n == 0) return 0;
@@ -551,10 +701,23 @@ int fc_mg_sweep(const float* x, float* x_out, const float* b, const fc_mg_coef* 
   Level g;
   if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
   FC_REQUIRE(coef && coef->invD && x && x_out && b && x != x_out, "sweep: bad arguments (x and x_out must differ)");
+  FC_COEF_FORM(coef, "sweep");
   const int64_t n = (int64_t)nx * ny * nz;
-  hipLaunchKernelGGL(sweep_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x, x_out, b, to_coef(coef),
-                     g, (float)dt, zero_guess);
+  hipLaunchKernelGGL(form ? sweep_kernel<true> : sweep_kernel<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0,
+                     stream, x, x_out, b, to_coef(coef), g, (float)dt, zero_guess);
   return check_launch("sweep");
+}
+
+int fc_mg_sweep2_from_zero(float* x_out, const float* b, const fc_mg_coef* coef, double dt, int nx, int ny, int nz,
+                           int ax, int out_idx, hipStream_t stream) {
+  Level g;
+  if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
+  FC_REQUIRE(coef && coef->invD && x_out && b && x_out != b, "sweep2_from_zero: bad arguments");
+  FC_COEF_FORM(coef, "sweep2_from_zero");
+  const int64_t n = (int64_t)nx * ny * nz;
+  hipLaunchKernelGGL(form ? sweep2_from_zero_kernel<true> : sweep2_from_zero_kernel<false>,
+                     dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x_out, b, to_coef(coef), g, (float)dt);
+  return check_launch("sweep2_from_zero");
 }
 
 int fc_mg_residual_restrict(const float* x, const float* b, const float* mask, const fc_mg_coef* coef, double dt,
@@ -562,9 +725,13 @@ int fc_mg_residual_restrict(const float* x, const float* b, const float* mask, c
   Level g;
   if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
   FC_REQUIRE(coef && x && b && mask && b_coarse, "residual_restrict: null pointer argument");
+  FC_COEF_FORM(coef, "residual_restrict");
   const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
-  hipLaunchKernelGGL(residual_restrict_kernel, dim3(grid_for((int64_t)cx * cy * cz, kBlock)), dim3(kBlock), 0,
-                     stream, x, b, mask, to_coef(coef), g, (float)dt, b_coarse, cx, cy, cz);
+  FC_REQUIRE(cx >= 1 && cy >= 1 && cz >= 1, "residual_restrict: every extent must be >= 2");
+  const int kchunks = (cz + 31) / 32;
+  const int64_t blocks = (int64_t)cx * cy * kchunks;
+  hipLaunchKernelGGL(form ? residual_restrict_kernel<true> : residual_restrict_kernel<false>, dim3((unsigned)blocks),
+                     dim3(kBlock), 0, stream, x, b, mask, to_coef(coef), g, (float)dt, b_coarse, cx, cy, cz, kchunks);
   return check_launch("residual_restrict");
 }
 
@@ -587,7 +754,8 @@ int fc_mg_coarse_solve(float* x, float* tmp, const float* b, const fc_mg_coef* c
   if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
   FC_REQUIRE(coef && coef->invD && x && tmp && b && x != tmp, "coarse_solve: bad arguments");
   FC_REQUIRE(sweeps >= 0, "coarse_solve: sweeps must be >= 0");
-  hipLaunchKernelGGL(coarse_solve_kernel, dim3(1), dim3(1024), 0, stream, x, tmp, b, to_coef(coef), g, (float)dt,
+  FC_COEF_FORM(coef, "coarse_solve");
+  hipLaunchKernelGGL(form ? coarse_solve_kernel<true> : coarse_solve_kernel<false>, dim3(1), dim3(1024), 0, stream, x, tmp, b, to_coef(coef), g, (float)dt,
                      sweeps, zero_guess);
   return check_launch("coarse_solve");
 }
@@ -606,9 +774,10 @@ int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* m
   Level g;
   if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
   FC_REQUIRE(p && Ap && mask && coef && partial && scal, "apply_A64: null pointer argument");
+  FC_COEF_FORM(coef, "apply_A64");
   const int64_t n = (int64_t)nx * ny * nz;
   const unsigned gsz = reduce_grid(n);
-  hipLaunchKernelGGL(apply_A64_kernel, dim3(gsz), dim3(kBlock), 0, stream, p, Ap, r, mask, to_coef(coef), g, dt,
+  hipLaunchKernelGGL(form ? apply_A64_kernel<true> : apply_A64_kernel<false>, dim3(gsz), dim3(kBlock), 0, stream, p, Ap, r, mask, to_coef(coef), g, dt,
                      partial);
   if (int rc = check_launch("apply_A64")) return rc;
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 2, scal, (int)S_PAP,
@@ -647,9 +816,10 @@ int fc_mg_true_residual(const double* x, const double* b, double* r, float* r32,
   Level g;
   if (int rc = make_level(g, nx, ny, nz, ax, out_idx)) return rc;
   FC_REQUIRE(x && b && r && r32 && mask && coef && partial && scal, "true_residual: null pointer argument");
+  FC_COEF_FORM(coef, "true_residual");
   const int64_t n = (int64_t)nx * ny * nz;
   const unsigned gsz = reduce_grid(n);
-  hipLaunchKernelGGL(true_residual_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, b, r, r32, mask, to_coef(coef), g,
+  hipLaunchKernelGGL(form ? true_residual_kernel<true> : true_residual_kernel<false>, dim3(gsz), dim3(kBlock), 0, stream, x, b, r, r32, mask, to_coef(coef), g,
                      dt, partial);
   if (int rc = check_launch("true_residual")) return rc;
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 2, scal, (int)S_RR, 0);

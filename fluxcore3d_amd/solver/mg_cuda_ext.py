@@ -17,9 +17,11 @@ _lib.register({
     "fc_mg_mop_fused": (I32, [P, P, P, P, P, P, P, P, P, F64, I32, I32, I32, I32, I32, P]),
     "fc_mg_jacobi_smooth": (I32, [P, P, P, P, I64, P]),
     "fc_mg_build_coeffs": (I32, [P, P, P, P, P, I32, I32, I32, I32, I32, F64, F64, P, P, P, P, P, P, P, P]),
+    "fc_mg_build_faces": (I32, [P, I32, I32, I32, I32, I32, P, P, P, P, P]),
     "fc_mg_inv_diag": (I32, [P, P, F64, F64, P, I64, P]),
     "fc_mg_restrict_field": (I32, [P, P, I32, I32, I32, I32, P]),
     "fc_mg_sweep": (I32, [P, P, P, P, F64, I32, I32, I32, I32, I32, I32, P]),
+    "fc_mg_sweep2_from_zero": (I32, [P, P, P, F64, I32, I32, I32, I32, I32, P]),
     "fc_mg_residual_restrict": (I32, [P, P, P, P, F64, I32, I32, I32, I32, I32, P, P]),
     "fc_mg_prolong_correct": (I32, [P, P, P, I32, I32, I32, I32, I32, I32, F32, P]),
     "fc_mg_coarse_solve": (I32, [P, P, P, P, F64, I32, I32, I32, I32, I32, I32, I32, P]),

@@ -142,7 +142,8 @@ class GCRStencil(ctypes.Structure):
     _fields_ = [("aE", ctypes.c_void_p), ("aW", ctypes.c_void_p), ("aN", ctypes.c_void_p), ("aS", ctypes.c_void_p),
                 ("aU", ctypes.c_void_p), ("aD", ctypes.c_void_p), ("diagL", ctypes.c_void_p),
                 ("mask", ctypes.c_void_p), ("dt", ctypes.c_double), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
-                ("nz", ctypes.c_int), ("ax", ctypes.c_int), ("out_idx", ctypes.c_int)]
+                ("nz", ctypes.c_int), ("ax", ctypes.c_int), ("out_idx", ctypes.c_int), ("fx", ctypes.c_void_p),
+                ("fy", ctypes.c_void_p), ("fz", ctypes.c_void_p)]
 
 
 class GCRWorkspace:
@@ -185,8 +186,10 @@ def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
         fw.x.copy_(x0.reshape(-1).to(torch.float64))
     dt = float(mg_solver._dt_cache)
     mask = lv0["mask_f"]
+    faces = lv0.get("faces")
     st = GCRStencil(*[lv0[k].data_ptr() for k in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL")], mask.data_ptr(),
-                    dt, nx, ny, nz, int(mg_solver.ax), int(lv0["out_idx"]))
+                    dt, nx, ny, nz, int(mg_solver.ax), int(lv0["out_idx"]),
+                    *([f.data_ptr() for f in faces] if faces is not None else [None] * 3))
 
     def true_residual():
         _lib.call("fc_mg_true_residual", fw.x.data_ptr(), fw.b.data_ptr(), fw.r.data_ptr(), fw.r32.data_ptr(),

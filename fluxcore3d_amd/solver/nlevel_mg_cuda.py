@@ -9,6 +9,14 @@ kernels; one V-cycle runs per level: two fused sweeps, one fused
 residual+restriction, the coarser level, one fused prolongation+correction
 (x += 0.75 e, x *= mask) and two fused sweeps; the coarsest level's sweeps run
 in a single workgroup launch.
+
+Coefficient storage: where a level's face coefficients are symmetric (aW of a
+cell equals aE of its -x neighbour, bit for bit, and so on: conduction with
+uniform rho*c_p and no advection), the hierarchy also keeps one face array per
+axis and the kernels read those three arrays instead of the six per-cell ones
+and diagL. fc_mg_build_faces checks every cell's seven values read back through
+the face form against the per-cell arrays; any difference keeps that level on
+the per-cell arrays. Both forms give the same bits.
 """
 from __future__ import annotations
 
@@ -25,12 +33,13 @@ Tensor = torch.Tensor
 
 
 class MGCoef(_C.Structure):
-    _fields_ = [(n, _C.c_void_p) for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL", "invD")]
+    _fields_ = [(n, _C.c_void_p) for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL", "invD", "fx", "fy", "fz")]
 
 
 class NLevelGeometricMGSolver:
     def __init__(self, device, dtype, flow_axis: int, flow_sign: int, thermal_dt_scale: float, omega: float = 0.8,
-                 n_pre: int = 2, n_post: int = 2, n_coarse: int = 20, min_coarse_cells: int = 4):
+                 n_pre: int = 2, n_post: int = 2, n_coarse: int = 20, min_coarse_cells: int = 4,
+                 face_coefficients: bool = True, fused_first_sweeps: bool = True):
         if torch.device(device).type != "cuda":
             raise RuntimeError("NLevelGeometricMGSolver runs on the HIP device only")
         if dtype != torch.float32:
@@ -45,6 +54,8 @@ class NLevelGeometricMGSolver:
         self.n_post = int(n_post)
         self.n_coarse = int(n_coarse)
         self.min_coarse_cells = int(min_coarse_cells)
+        self.face_coefficients = bool(face_coefficients)
+        self.fused_first_sweeps = bool(fused_first_sweeps)
         self._ready = False
         self._dt_cache: Optional[float] = None
         self._shape_cache: Optional[tuple] = None
@@ -93,6 +104,7 @@ class NLevelGeometricMGSolver:
             lv["coef_struct"] = MGCoef(*[lv[n].data_ptr() for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL",
                                                                     "inv_diagM")])
             lv["coef"] = _C.addressof(lv["coef_struct"])  # pointer to the struct, kept alive by the level
+            lv["faces"] = self._build_faces(lv) if self.face_coefficients else None
             self._levels.append(lv)
             if min(nx // 2, ny // 2, nz // 2) <= self.min_coarse_cells:
                 break
@@ -111,6 +123,19 @@ class NLevelGeometricMGSolver:
         self._partial = torch.empty(3 * int(_lib.call("fc_mg_partials_count", n0)), device=dev, dtype=torch.float64)
         shapes = " -> ".join(str(lv["shape"]) for lv in self._levels)
         print(f"[MG] {self._nlevels}-level hierarchy: {shapes}")
+
+    def _build_faces(self, lv) -> Optional[Tuple[Tensor, Tensor, Tensor]]:
+        """Face form of the level's coefficients, or None where it is not exact."""
+        nx, ny, nz = lv["shape"]
+        faces = tuple(torch.empty(lv["shape"], device=self.device, dtype=torch.float32) for _ in range(3))
+        bad = torch.zeros(1, device=self.device, dtype=torch.int32)
+        _lib.call("fc_mg_build_faces", lv["coef"], nx, ny, nz, self.ax, lv["out_idx"], *[f.data_ptr() for f in faces],
+                  bad.data_ptr(), self._stream())
+        if int(bad.item()) != 0:
+            return None
+        st = lv["coef_struct"]
+        st.fx, st.fy, st.fz = (f.data_ptr() for f in faces)
+        return faces
 
     def _update_inv_diag(self, dt: float) -> None:
         for lv in self._levels:
@@ -132,14 +157,21 @@ class NLevelGeometricMGSolver:
                   Lo.data_ptr(), float(self._dt_cache), nx, ny, nz, self.ax, lv["out_idx"], self._stream())
 
     def _sweeps(self, level: int, count: int, zero_guess: bool) -> None:
-        """`count` fused Jacobi sweeps on levels[level]['x'] (ping-pong through 'tmp')."""
+        """`count` fused Jacobi sweeps on levels[level]['x'] (ping-pong through 'tmp').
+        From a zero guess the first two run as one pass that forms x1 = invD b at
+        the stencil points (same bits, no x1 written or read)."""
         lv = self._levels[level]
         nx, ny, nz = lv["shape"]
+        dt = float(self._dt_cache)
         src, dst = lv["x"], lv["tmp"]
-        for s in range(count):
-            _lib.call("fc_mg_sweep", src.data_ptr(), dst.data_ptr(), lv["b_buf"].data_ptr(), lv["coef"],
-                      float(self._dt_cache), nx, ny, nz, self.ax, lv["out_idx"], int(zero_guess and s == 0),
-                      self._stream())
+        first = 0
+        if zero_guess and count >= 2 and self.fused_first_sweeps:
+            _lib.call("fc_mg_sweep2_from_zero", src.data_ptr(), lv["b_buf"].data_ptr(), lv["coef"], dt, nx, ny, nz,
+                      self.ax, lv["out_idx"], self._stream())
+            first = 2
+        for s in range(first, count):
+            _lib.call("fc_mg_sweep", src.data_ptr(), dst.data_ptr(), lv["b_buf"].data_ptr(), lv["coef"], dt, nx, ny,
+                      nz, self.ax, lv["out_idx"], int(zero_guess and s == 0), self._stream())
             src, dst = dst, src
         if src is not lv["x"]:
             lv["x"], lv["tmp"] = lv["tmp"], lv["x"]

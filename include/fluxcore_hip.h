@@ -158,13 +158,32 @@ int fc_mg_inv_diag(const float *diagL, const float *mask, double dt, double omeg
 /* _restrict_2x (nlevel_mg_cuda.py:512-519); threshold=1 gives the coarse mask (>= 0.999) */
 int fc_mg_restrict_field(const float *src, float *dst, int nx, int ny, int nz, int threshold, hipStream_t stream);
 
+/* One level's coefficients. fx, fy, fz (all set or all null) hold the face form
+ * written by fc_mg_build_faces: per axis, the coefficient of the face between a
+ * cell and its +1 neighbour, the last plane holding the low-boundary coefficient
+ * where the low ghost is the cell itself. When set, the kernels below read them
+ * instead of aE..aD and diagL (3 arrays instead of 7 per cell). */
 typedef struct fc_mg_coef {
   const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *invD;
+  const float *fx, *fy, *fz;
 } fc_mg_coef;
+
+/* No reference counterpart (a storage form of _build_coeffs_full's output,
+ * nlevel_mg_cuda.py:418-507): writes the face form of coef's per-cell arrays and
+ * adds to *mismatches the number of cells whose six coefficients or diagL, read
+ * back through it, differ in any bit (zero for conduction with uniform rho*c_p and
+ * no advection). Use the face form only when that count is zero. */
+int fc_mg_build_faces(const fc_mg_coef *coef, int nx, int ny, int nz, int ax, int out_idx, float *fx, float *fy,
+                      float *fz, unsigned *mismatches, hipStream_t stream);
 
 /* fused _Mop + _smooth_step (nlevel_mg_cuda.py:311-372): x_out = x + invD (b - A x) */
 int fc_mg_sweep(const float *x, float *x_out, const float *b, const fc_mg_coef *coef, double dt, int nx, int ny,
                 int nz, int ax, int out_idx, int zero_guess, hipStream_t stream);
+/* the first two _smooth_step sweeps of _vcycle from x = 0 (nlevel_mg_cuda.py:285-287) in one
+ * pass: x_out = x1 + invD (b - A x1), x1 = invD b formed at the stencil points; the same
+ * bits as fc_mg_sweep with zero_guess=1 followed by fc_mg_sweep */
+int fc_mg_sweep2_from_zero(float *x_out, const float *b, const fc_mg_coef *coef, double dt, int nx, int ny, int nz,
+                           int ax, int out_idx, hipStream_t stream);
 /* residual + _restrict_into of _vcycle (nlevel_mg_cuda.py:289-295) */
 int fc_mg_residual_restrict(const float *x, const float *b, const float *mask, const fc_mg_coef *coef, double dt,
                             int nx, int ny, int nz, int ax, int out_idx, float *b_coarse, hipStream_t stream);
@@ -243,6 +262,7 @@ typedef struct fc_gcr_stencil {
   const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *mask; /* finest-level operator, as fc_mg_coef */
   double dt;
   int nx, ny, nz, ax, out_idx;
+  const float *fx, *fy, *fz; /* face form (fc_mg_build_faces) or all null */
 } fc_gcr_stencil;
 int fc_gcr_scalar_slots(void);
 int fc_gcr_max_directions(void);

@@ -121,3 +121,95 @@ def test_gcr_fp64_monotone_and_solution(m):
     assert np.linalg.norm(r) / np.linalg.norm(c["b"] * msk) < 1.01e-8
     xo, _ = mg.fcg_mg_fp64(h, c["b"], tol=1e-10, max_it=300)
     assert rel(x64 * msk, xo) < 1e-6
+
+
+@pytest.mark.parametrize("ax,sign", [(0, 1), (1, -1), (2, 1)])
+def test_face_form_same_bits_as_per_cell_arrays(ax, sign):
+    """Conduction with uniform rho*c_p and no advection: every level takes the face
+    form (3 arrays instead of 7), and the V-cycle, the V-cycle iteration, flexible
+    CG and flexible GCR give the same bits and iteration counts as with the
+    per-cell arrays. sign=+1 puts the outlet at the high end of the flow axis, so
+    the low face there is the self ghost, held in the face array's last plane."""
+    from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, gcr_fp64
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+    c = dict(thermal_cases.sphere_conduction(64, 32, 32, dt=1.0e6), ax=ax, sign=sign)
+
+    def make(faces):
+        s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, ax, sign, c["s"], omega=0.6,
+                                    face_coefficients=faces)
+        s.build_if_needed(T(c["k"]), T(c["rcp"]), T(c["u"]), T(c["v"]), T(c["w"]), T(c["mask"]), c["dt"])
+        return s
+
+    sf, sc = make(True), make(False)
+    assert all(lv["faces"] is not None for lv in sf._levels)
+    assert all(lv["faces"] is None for lv in sc._levels)
+    r = T(c["b"].astype(np.float32)).view(-1)
+    assert torch.equal(sf.precondition(r).clone(), sc.precondition(r).clone())
+    xf, fi = sf.solve(T(c["b"]), max_cycles=4, tol=1e-30)
+    xc, ci = sc.solve(T(c["b"]), max_cycles=4, tol=1e-30)
+    assert torch.equal(xf, xc) and fi["res_rel"] == ci["res_rel"]
+    for solver in (fcg_fp64, gcr_fp64):
+        _, fi = solver(sf, T(c["b"]), tol=1e-8, max_it=200)
+        _, ci = solver(sc, T(c["b"]), tol=1e-8, max_it=200)
+        assert fi["converged"] and fi["it"] == ci["it"] and fi["hist"] == ci["hist"], solver.__name__
+        assert torch.equal(fi["x64"], ci["x64"]), solver.__name__
+
+
+def test_face_form_declined_with_advection_or_varying_heat_capacity():
+    """Advection or a varying rho*c_p makes the per-cell coefficients of neighbouring
+    cells differ: the build keeps those levels on the seven per-cell arrays, and the
+    V-cycle still matches the oracle."""
+    base = thermal_cases.sphere_conduction(64, 32, 32, dt=10.0)
+    rng = np.random.default_rng(5)
+    for change in (dict(u=(0.05 * rng.standard_normal(base["k"].shape)).astype(np.float32)),
+                   dict(rcp=(0.5 + rng.random(base["k"].shape)).astype(np.float32))):
+        c = dict(base, **change)
+        s, h = _solver_pair(c)
+        assert s._levels[0]["faces"] is None
+        r = c["b"].astype(np.float32)
+        assert rel(s.precondition(T(r).view(-1)).cpu().numpy(), h.precond(r)) < 1e-5
+
+
+def test_fused_first_sweeps_and_restriction_bits():
+    """The V-cycle's first two sweeps from x = 0 run as one pass
+    (fc_mg_sweep2_from_zero): same bits as two separate sweeps. The residual +
+    restriction kernel equals, bit for bit, the reference's sequence: Lo from
+    mop_fused, r = (b - Lo) * mask, the 2x2x2 sum in the order
+    (di, dj, dk) = (0,0,0), (0,0,1), ... (nlevel_mg_cuda.py:289-295,521-528), / 8."""
+    from fluxcore3d_amd import _lib
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+    c = thermal_cases.sphere_conduction(64, 32, 32, dt=10.0)
+
+    def make(fused, faces=True):
+        s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, c["ax"], c["sign"], c["s"], omega=0.6,
+                                    face_coefficients=faces, fused_first_sweeps=fused)
+        s.build_if_needed(T(c["k"]), T(c["rcp"]), T(c["u"]), T(c["v"]), T(c["w"]), T(c["mask"]), c["dt"])
+        return s
+
+    r = T(np.random.default_rng(3).standard_normal(c["k"].shape).astype(np.float32) * c["mask"]).view(-1)
+    za = make(True).precondition(r, masked_input=True).clone()
+    zb = make(False).precondition(r, masked_input=True).clone()
+    assert torch.equal(za, zb)
+    for faces in (True, False):
+        s = make(True, faces)
+        for level in range(s._nlevels - 1):
+            lv, lc = s._levels[level], s._levels[level + 1]
+            nx, ny, nz = lv["shape"]
+            rng = np.random.default_rng(level)
+            x = T(rng.standard_normal(lv["shape"]).astype(np.float32))
+            b = T(rng.standard_normal(lv["shape"]).astype(np.float32))
+            bc = torch.empty(lc["shape"], device=DEV)
+            _lib.call("fc_mg_residual_restrict", x.data_ptr(), b.data_ptr(), lv["mask_f"].data_ptr(), lv["coef"],
+                      float(s._dt_cache), nx, ny, nz, s.ax, lv["out_idx"], bc.data_ptr(),
+                      torch.cuda.current_stream().cuda_stream)
+            Lo = torch.empty(lv["shape"], device=DEV)
+            s._Mop_cuda(x, level, Lo)
+            res = ((b - Lo) * lv["mask_f"]).cpu().numpy()
+            cx, cy, cz = lc["shape"]
+            want = np.zeros((cx, cy, cz), np.float32)
+            for di in (0, 1):
+                for dj in (0, 1):
+                    for dk in (0, 1):
+                        want += res[di:2 * cx:2, dj:2 * cy:2, dk:2 * cz:2]
+            want /= np.float32(8.0)
+            np.testing.assert_array_equal(bc.cpu().numpy(), want, err_msg=f"level {level} faces {faces}")
