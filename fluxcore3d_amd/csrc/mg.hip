@@ -99,12 +99,21 @@ __device__ __forceinline__ C7 coef_at(const Coef& c, const Level& g, int idx, in
 }
 
 // Lo = T + dt L(T) from the seven values, with the reference's operand order. Every
-// kernel forms Lo through this one expression, so fused and separate kernels give
-// the same bits.
+// kernel forms Lo, the sweep update and the prolongation through these helpers,
+// compiled without multiply-add contraction: each product and sum is rounded on
+// its own, so fused and separate kernels give the same bits whatever the compiler
+// would fuse in each context.
 __device__ __forceinline__ float lo_of(const C7& a, float dt, float TC, float TE, float TW, float TN, float TS,
                                        float TU, float TD) {
+#pragma clang fp contract(off)
   const float LT = a.diag * TC - a.e * TE - a.w * TW - a.n * TN - a.s * TS - a.u * TU - a.d * TD;
   return TC + dt * LT;
+}
+
+// x + invD (b - Lo)
+__device__ __forceinline__ float sweep_update(float x, float invD, float b, float lo) {
+#pragma clang fp contract(off)
+  return x + invD * (b - lo);
 }
 
 template <bool FACE, typename TT>
@@ -118,7 +127,7 @@ __device__ __forceinline__ float stencil_lo(const TT* __restrict__ T, const Coef
 
 // First sweep from x = 0 (x1 = invD b), written the way sweep_kernel writes it.
 __device__ __forceinline__ float zero_sweep(const Coef& c, const float* __restrict__ b, int idx) {
-  return 0.f + c.invD[idx] * (b[idx] - 0.f);
+  return sweep_update(0.f, c.invD[idx], b[idx], 0.f);
 }
 
 __global__ __launch_bounds__(kBlock) void mop_kernel(const float* __restrict__ T, Coef c, Level g, float dt,
@@ -135,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void jacobi_kernel(float* __restrict__ x, c
                                                         const float* __restrict__ invD, int64_t n) {
   const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= n) return;
-  x[idx] += invD[idx] * (b[idx] - Lo[idx]);
+  x[idx] = sweep_update(x[idx], invD[idx], b[idx], Lo[idx]);
 }
 
 // x_out = x + invD (b - A x): the two reference kernels in one pass.
@@ -152,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float* __restrict__
   int i, j, k;
   unflat(g, idx, i, j, k);
   const float lo = stencil_lo<FACE>(x, c, g, dt, idx, i, j, k);
-  x_out[idx] = x[idx] + c.invD[idx] * (b[idx] - lo);
+  x_out[idx] = sweep_update(x[idx], c.invD[idx], b[idx], lo);
 }
 
 // The first two sweeps from x = 0 in one pass: x1 = invD b is formed at the seven
@@ -172,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void sweep2_from_zero_kernel(float* __restr
   const float lo = lo_of(a, dt, xc, zero_sweep(c, b, idx + o.e), zero_sweep(c, b, idx + o.w),
                          zero_sweep(c, b, idx + o.n), zero_sweep(c, b, idx + o.s), zero_sweep(c, b, idx + o.u),
                          zero_sweep(c, b, idx + o.d));
-  x_out[idx] = xc + c.invD[idx] * (b[idx] - lo);
+  x_out[idx] = sweep_update(xc, c.invD[idx], b[idx], lo);
 }
 
 // coarse rhs: mean of the 8 fine residuals (b - A x) * mask (odd extents truncated).
@@ -213,12 +222,34 @@ __global__ __launch_bounds__(kBlock) void residual_restrict_kernel(const float* 
 // linear interpolation source (align_corners = False) along one axis; scale =
 // n_in / n_out, divided once on the host (the same IEEE quotient)
 __device__ __forceinline__ void lin_src(int o, float scale, int n_in, int& i0, int& i1, float& l0, float& l1) {
+#pragma clang fp contract(off)
   float src = scale * ((float)o + 0.5f) - 0.5f;
   src = src < 0.f ? 0.f : src;
   i0 = min((int)src, n_in - 1);
   i1 = min(i0 + 1, n_in - 1);
   l1 = src - (float)i0;
   l0 = 1.f - l1;
+}
+
+// Coarse correction at fine (i,j,k): trilinear interpolation of the coarse
+// iterate (F.interpolate, align_corners=False) through X(I, J, K).
+template <typename F>
+__device__ __forceinline__ float prolong_at(const F& X, int i, int j, int k, int cx, int cy, int cz, float sx,
+                                            float sy, float sz) {
+#pragma clang fp contract(off)
+  int i0, i1, j0, j1, k0, k1;
+  float a0, a1, b0, b1, c0, c1;
+  lin_src(i, sx, cx, i0, i1, a0, a1);
+  lin_src(j, sy, cy, j0, j1, b0, b1);
+  lin_src(k, sz, cz, k0, k1, c0, c1);
+  return a0 * (b0 * (c0 * X(i0, j0, k0) + c1 * X(i0, j0, k1)) + b1 * (c0 * X(i0, j1, k0) + c1 * X(i0, j1, k1))) +
+         a1 * (b0 * (c0 * X(i1, j0, k0) + c1 * X(i1, j0, k1)) + b1 * (c0 * X(i1, j1, k0) + c1 * X(i1, j1, k1)));
+}
+
+// x = (x + alpha e) * mask (nlevel_mg_cuda.py:300-303)
+__device__ __forceinline__ float corrected(float x, float e, float alpha, float m) {
+#pragma clang fp contract(off)
+  return (x + alpha * e) * m;
 }
 
 __global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __restrict__ xc, float* __restrict__ x,
@@ -231,15 +262,8 @@ __global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __
   const int i = (int)(u / nynz);
   const unsigned rem = u - (unsigned)i * nynz;
   const int j = (int)(rem / (unsigned)nz), k = (int)(rem - (unsigned)j * (unsigned)nz);
-  int i0, i1, j0, j1, k0, k1;
-  float a0, a1, b0, b1, c0, c1;
-  lin_src(i, sx, cx, i0, i1, a0, a1);
-  lin_src(j, sy, cy, j0, j1, b0, b1);
-  lin_src(k, sz, cz, k0, k1, c0, c1);
   auto X = [&](int a, int b2, int c2) { return xc[(a * cy + b2) * cz + c2]; };
-  const float e = a0 * (b0 * (c0 * X(i0, j0, k0) + c1 * X(i0, j0, k1)) + b1 * (c0 * X(i0, j1, k0) + c1 * X(i0, j1, k1))) +
-                  a1 * (b0 * (c0 * X(i1, j0, k0) + c1 * X(i1, j0, k1)) + b1 * (c0 * X(i1, j1, k0) + c1 * X(i1, j1, k1)));
-  x[idx] = (x[idx] + alpha * e) * mask[idx];
+  x[idx] = corrected(x[idx], prolong_at(X, i, j, k, cx, cy, cz, sx, sy, sz), alpha, mask[idx]);
 }
 
 // All sweeps of the coarsest level in one workgroup (global ping-pong, the
@@ -259,7 +283,7 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(float* __restrict__ 
         int i, j, k;
         unflat(g, idx, i, j, k);
         const float lo = stencil_lo<FACE>(src, c, g, dt, idx, i, j, k);
-        dst[idx] = src[idx] + c.invD[idx] * (b[idx] - lo);
+        dst[idx] = sweep_update(src[idx], c.invD[idx], b[idx], lo);
       }
     }
     __syncthreads();

@@ -170,15 +170,18 @@ def test_face_form_declined_with_advection_or_varying_heat_capacity():
         assert rel(s.precondition(T(r).view(-1)).cpu().numpy(), h.precond(r)) < 1e-5
 
 
-def test_fused_first_sweeps_and_restriction_bits():
+@pytest.mark.parametrize("shape", [(64, 32, 32), (44, 36, 30)])
+def test_fused_first_sweeps_and_restriction_bits(shape):
     """The V-cycle's first two sweeps from x = 0 run as one pass
-    (fc_mg_sweep2_from_zero): same bits as two separate sweeps. The residual +
-    restriction kernel equals, bit for bit, the reference's sequence: Lo from
-    mop_fused, r = (b - Lo) * mask, the 2x2x2 sum in the order
-    (di, dj, dk) = (0,0,0), (0,0,1), ... (nlevel_mg_cuda.py:289-295,521-528), / 8."""
+    (fc_mg_sweep2_from_zero): the V-cycle gives the same bits as with two
+    separate sweeps, also with odd extents (44x36x30 -> 22x18x15 -> 11x9x7) and
+    through the caller's output buffer. The residual + restriction kernel equals, bit for bit,
+    the reference's sequence: Lo from mop_fused, r = (b - Lo) * mask, the 2x2x2 sum
+    in the order (di, dj, dk) = (0,0,0), (0,0,1), ... (nlevel_mg_cuda.py:289-295,
+    521-528), / 8."""
     from fluxcore3d_amd import _lib
     from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
-    c = thermal_cases.sphere_conduction(64, 32, 32, dt=10.0)
+    c = thermal_cases.sphere_conduction(*shape, dt=10.0)
 
     def make(fused, faces=True):
         s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, c["ax"], c["sign"], c["s"], omega=0.6,
@@ -187,9 +190,16 @@ def test_fused_first_sweeps_and_restriction_bits():
         return s
 
     r = T(np.random.default_rng(3).standard_normal(c["k"].shape).astype(np.float32) * c["mask"]).view(-1)
-    za = make(True).precondition(r, masked_input=True).clone()
-    zb = make(False).precondition(r, masked_input=True).clone()
+    sa, sb = make(True), make(False)
+    za = sa.precondition(r, masked_input=True).clone()
+    zb = sb.precondition(r, masked_input=True).clone()
     assert torch.equal(za, zb)
+    out = torch.full_like(r, float("nan"))
+    assert torch.equal(sa.precondition(r, out=out, masked_input=True).reshape(-1), zb.reshape(-1))
+    assert torch.equal(out, zb.reshape(-1))
+    xa, ia = sa.solve(T(c["b"]), max_cycles=3, tol=1e-30)
+    xb, ib = sb.solve(T(c["b"]), max_cycles=3, tol=1e-30)
+    assert torch.equal(xa, xb) and ia["res_rel"] == ib["res_rel"]
     for faces in (True, False):
         s = make(True, faces)
         for level in range(s._nlevels - 1):
