@@ -175,8 +175,8 @@ def traffic_per_launch(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "bench.py measures one GPU: run it with --gpus 1 (the path is not sharded "
+                                   "across GPUs)", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
 

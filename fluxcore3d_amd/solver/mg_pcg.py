@@ -98,7 +98,7 @@ def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
             hist.append(rec)
             break
         if rec < tol:
-            rr, _ = true_residual()  # the FP64 true residual is the judge
+            rr, _ = true_residual()  # the FP64 true residual decides convergence
             hist.append(rr ** 0.5 / bnorm)
             converged = hist[-1] < tol
         else:
@@ -114,8 +114,8 @@ def pcg_mg(mg_solver, b, x0=None, tol: float = 1e-3, max_it: int = 15):
 
 
 def patch_mg_pcg(mg_solver, pe_threshold: float = 1.5):
-    """Route mg_solver.solve through pcg_mg (mg_pcg.py:250-275; the BiCGSTAB
-    branch of the reference is not part of this build)."""
+    """Route mg_solver.solve through pcg_mg (mg_pcg.py:250-275; the reference's
+    BiCGSTAB branch is not provided here)."""
     import types
 
     def _solve_pcg(self, b, x0=None, max_cycles=15, tol=1e-3):

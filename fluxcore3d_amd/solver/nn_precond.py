@@ -485,8 +485,8 @@ def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: 
 def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 1e-8, max_it: int = 300,
            x0: Optional[torch.Tensor] = None, stall_window: int = 8, stall_factor: float = 0.5, m: int = 6,
            ws=None, check_every: int = 1):
-    """Flexible GCR(m) with the U-Net preconditioner (FP64 vectors, FP64 true
-    residual as the judge). The operator is not symmetric and a learned
+    """Flexible GCR(m) with the U-Net preconditioner (FP64 vectors; the FP64 true
+    residual decides convergence). The operator is not symmetric and a learned
     preconditioner is not exact, so a minimal-residual flexible method is used
     (flexible CG does not converge with weaker preconditioners on this
     operator): every step removes the part of r along the orthogonalised

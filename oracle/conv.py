@@ -1,7 +1,7 @@
 """FP64 NumPy reference of the 3x3x3 convolutions of the learned preconditioner.
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py). No reference counterpart
-exists (the U-Net is this build's addition); these restate the definitions:
+exists (the reference has no learned preconditioner); these restate the definitions:
 
   y[b, x, co]  = bias[co] + sum_{t, ci} W[co, t, ci] x[b, x + o_t, ci]   (zero padding)
   dx[b, x, ci] = sum_{t, co} W[co, t, ci] dy[b, x - o_t, co]
