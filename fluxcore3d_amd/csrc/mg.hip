@@ -266,6 +266,29 @@ __global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __
   x[idx] = corrected(x[idx], prolong_at(X, i, j, k, cx, cy, cz, sx, sy, sz), alpha, mask[idx]);
 }
 
+// Two fine cells (i, j, k), (i, j, k + 1) per thread, k even: 8-byte loads and
+// stores, and the coarse values the two interpolations share are loaded once.
+__global__ __launch_bounds__(kBlock) void prolong_correct2_kernel(const float* __restrict__ xc, float* __restrict__ x,
+                                                                  const float* __restrict__ mask, int nx, int ny,
+                                                                  int nz, int cx, int cy, int cz, float sx, float sy,
+                                                                  float sz, float alpha) {
+  const int hz = nz >> 1;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nx * ny * hz) return;
+  const unsigned nyhz = (unsigned)ny * (unsigned)hz, u = (unsigned)t;
+  const int i = (int)(u / nyhz);
+  const unsigned rem = u - (unsigned)i * nyhz;
+  const int j = (int)(rem / (unsigned)hz), k = 2 * (int)(rem - (unsigned)j * (unsigned)hz);
+  const int idx = (i * ny + j) * nz + k;
+  auto X = [&](int a, int b2, int c2) { return xc[(a * cy + b2) * cz + c2]; };
+  const float2 xv = *reinterpret_cast<const float2*>(x + idx);
+  const float2 mv = *reinterpret_cast<const float2*>(mask + idx);
+  float2 out;
+  out.x = corrected(xv.x, prolong_at(X, i, j, k, cx, cy, cz, sx, sy, sz), alpha, mv.x);
+  out.y = corrected(xv.y, prolong_at(X, i, j, k + 1, cx, cy, cz, sx, sy, sz), alpha, mv.y);
+  *reinterpret_cast<float2*>(x + idx) = out;
+}
+
 // All sweeps of the coarsest level in one workgroup (global ping-pong, the
 // level is small enough to stay in L2); x holds the result on return.
 template <bool FACE>
@@ -291,6 +314,64 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(float* __restrict__ 
   }
   if (src != x) {
     for (int64_t idx = threadIdx.x; idx < n; idx += blockDim.x) x[idx] = src[idx];
+  }
+}
+
+// The same sweeps for a coarsest level of at most kCoarseLdsCells cells: the
+// iterate ping-pongs in LDS and each thread keeps the coefficients, invD, b and
+// neighbour offsets of its cells in registers, so a sweep costs LDS reads and a
+// barrier instead of L2 round trips. Same arithmetic, same bits.
+constexpr int kCoarseThreads = 1024, kCoarseCpt = 4, kCoarseLdsCells = kCoarseThreads * kCoarseCpt;
+
+template <bool FACE>
+__global__ __launch_bounds__(kCoarseThreads) void coarse_solve_lds_kernel(float* __restrict__ x,
+                                                                          const float* __restrict__ b, Coef c,
+                                                                          Level g, float dt, int sweeps,
+                                                                          int zero_guess) {
+  __shared__ float S[2][kCoarseLdsCells];
+  const int n = g.nx * g.ny * g.nz;
+  C7 co[kCoarseCpt];
+  Nbr nb[kCoarseCpt];
+  float inv[kCoarseCpt], bb[kCoarseCpt];
+#pragma unroll
+  for (int u = 0; u < kCoarseCpt; ++u) {
+    const int idx = (int)threadIdx.x + u * kCoarseThreads;
+    if (idx < n) {
+      int i, j, k;
+      unflat(g, idx, i, j, k);
+      nb[u] = neighbours(g, i, j, k);
+      co[u] = coef_at<FACE>(c, g, idx, i, j, k);
+      inv[u] = c.invD[idx];
+      bb[u] = b[idx];
+      S[0][idx] = zero_guess ? 0.f : x[idx];
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int s = 0; s < sweeps; ++s) {
+    const float* src = S[cur];
+    float* dst = S[cur ^ 1];
+#pragma unroll
+    for (int u = 0; u < kCoarseCpt; ++u) {
+      const int idx = (int)threadIdx.x + u * kCoarseThreads;
+      if (idx < n) {
+        if (s == 0 && zero_guess) {
+          dst[idx] = sweep_update(0.f, inv[u], bb[u], 0.f);
+        } else {
+          const Nbr& o = nb[u];
+          const float lo = lo_of(co[u], dt, src[idx], src[idx + o.e], src[idx + o.w], src[idx + o.n],
+                                 src[idx + o.s], src[idx + o.u], src[idx + o.d]);
+          dst[idx] = sweep_update(src[idx], inv[u], bb[u], lo);
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int u = 0; u < kCoarseCpt; ++u) {
+    const int idx = (int)threadIdx.x + u * kCoarseThreads;
+    if (idx < n) x[idx] = S[cur][idx];
   }
 }
 
@@ -509,6 +590,9 @@ __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restr
                                                            Coef c, Level g, double dt, double* __restrict__ partial) {
   const int n = g.nx * g.ny * g.nz;
   double v[2] = {0.0, 0.0};
+  // unrolled so that several cells' gathers are in flight; each thread still adds its
+  // cells in index order
+#pragma unroll 4
   for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
     int i, j, k;
     unflat(g, idx, i, j, k);
@@ -570,6 +654,9 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
                                                                double dt, double* __restrict__ partial) {
   const int n = g.nx * g.ny * g.nz;
   double v[2] = {0.0, 0.0};
+  // unrolled so that several cells' gathers are in flight; each thread still adds its
+  // cells in index order
+#pragma unroll 4
   for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
     int i, j, k;
     unflat(g, idx, i, j, k);
@@ -767,8 +854,12 @@ int fc_mg_prolong_correct(const float* x_coarse, float* x, const float* mask, in
   FC_REQUIRE(n < ((int64_t)1 << 31), "prolong_correct: %lld cells exceed the 32-bit index range", (long long)n);
   // the interpolation scale of F.interpolate(align_corners=False): n_in / n_out in FP32
   const float sx = (float)cx / (float)nx, sy = (float)cy / (float)ny, sz = (float)cz / (float)nz;
-  hipLaunchKernelGGL(prolong_correct_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x_coarse, x, mask,
-                     nx, ny, nz, cx, cy, cz, sx, sy, sz, alpha);
+  if (nz % 2 == 0 && (reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(mask)) % 8 == 0)
+    hipLaunchKernelGGL(prolong_correct2_kernel, dim3(grid_for(n / 2, kBlock)), dim3(kBlock), 0, stream, x_coarse, x,
+                       mask, nx, ny, nz, cx, cy, cz, sx, sy, sz, alpha);
+  else
+    hipLaunchKernelGGL(prolong_correct_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x_coarse, x, mask,
+                       nx, ny, nz, cx, cy, cz, sx, sy, sz, alpha);
   return check_launch("prolong_correct");
 }
 
@@ -779,8 +870,12 @@ int fc_mg_coarse_solve(float* x, float* tmp, const float* b, const fc_mg_coef* c
   FC_REQUIRE(coef && coef->invD && x && tmp && b && x != tmp, "coarse_solve: bad arguments");
   FC_REQUIRE(sweeps >= 0, "coarse_solve: sweeps must be >= 0");
   FC_COEF_FORM(coef, "coarse_solve");
-  hipLaunchKernelGGL(form ? coarse_solve_kernel<true> : coarse_solve_kernel<false>, dim3(1), dim3(1024), 0, stream, x, tmp, b, to_coef(coef), g, (float)dt,
-                     sweeps, zero_guess);
+  if ((int64_t)nx * ny * nz <= kCoarseLdsCells)
+    hipLaunchKernelGGL(form ? coarse_solve_lds_kernel<true> : coarse_solve_lds_kernel<false>, dim3(1),
+                       dim3(kCoarseThreads), 0, stream, x, b, to_coef(coef), g, (float)dt, sweeps, zero_guess);
+  else
+    hipLaunchKernelGGL(form ? coarse_solve_kernel<true> : coarse_solve_kernel<false>, dim3(1), dim3(1024), 0, stream,
+                       x, tmp, b, to_coef(coef), g, (float)dt, sweeps, zero_guess);
   return check_launch("coarse_solve");
 }
 

@@ -223,3 +223,59 @@ def test_fused_first_sweeps_and_restriction_bits(shape):
                         want += res[di:2 * cx:2, dj:2 * cy:2, dk:2 * cz:2]
             want /= np.float32(8.0)
             np.testing.assert_array_equal(bc.cpu().numpy(), want, err_msg=f"level {level} faces {faces}")
+
+
+def test_prolongation_two_cells_per_thread_same_bits():
+    """fc_mg_prolong_correct takes two fine cells per thread (8-byte accesses) when nz
+    is even and x, mask are 8-byte aligned, else one. Both give the same bits: the
+    same buffers at a 4-byte offset force the one-cell kernel."""
+    from fluxcore3d_amd import _lib
+    for shape in ((64, 32, 32), (44, 36, 30)):
+        nx, ny, nz = shape
+        cx, cy, cz = nx // 2, ny // 2, nz // 2
+        rng = np.random.default_rng(7)
+        xc = T(rng.standard_normal((cx, cy, cz)).astype(np.float32))
+        x0 = rng.standard_normal(shape).astype(np.float32)
+        m0 = (rng.random(shape) > 0.2).astype(np.float32)
+        outs = []
+        for off in (0, 1):
+            xb = torch.zeros(nx * ny * nz + 2, device=DEV)
+            mb = torch.zeros(nx * ny * nz + 2, device=DEV)
+            xv, mv = xb[off:off + x0.size], mb[off:off + x0.size]
+            xv.copy_(T(x0).view(-1))
+            mv.copy_(T(m0).view(-1))
+            _lib.call("fc_mg_prolong_correct", xc.data_ptr(), xv.data_ptr(), mv.data_ptr(), nx, ny, nz, cx, cy, cz,
+                      0.75, torch.cuda.current_stream().cuda_stream)
+            outs.append(xv.clone())
+        assert torch.equal(outs[0], outs[1]), shape
+        want = torch.nn.functional.interpolate(xc[None, None], size=shape, mode="trilinear", align_corners=False)[0, 0]
+        ref = (T(x0) + 0.75 * want) * T(m0)
+        assert rel(outs[0].view(shape).cpu().numpy(), ref.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("zero_guess", [0, 1])
+def test_coarse_solve_lds_same_bits_as_sweeps(zero_guess):
+    """The coarsest level's sweeps in one workgroup with the iterate in LDS equal,
+    bit for bit, the same number of separate fc_mg_sweep launches."""
+    from fluxcore3d_amd import _lib
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+    c = thermal_cases.sphere_conduction(64, 32, 32, dt=10.0)
+    s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, c["ax"], c["sign"], c["s"], omega=0.6)
+    s.build_if_needed(T(c["k"]), T(c["rcp"]), T(c["u"]), T(c["v"]), T(c["w"]), T(c["mask"]), c["dt"])
+    lv = s._levels[-1]
+    nx, ny, nz = lv["shape"]
+    assert nx * ny * nz <= 4096
+    rng = np.random.default_rng(11)
+    x0 = T(rng.standard_normal(lv["shape"]).astype(np.float32))
+    b = T(rng.standard_normal(lv["shape"]).astype(np.float32))
+    st = torch.cuda.current_stream().cuda_stream
+    dt = float(s._dt_cache)
+    x, tmp = x0.clone(), torch.empty_like(x0)
+    _lib.call("fc_mg_coarse_solve", x.data_ptr(), tmp.data_ptr(), b.data_ptr(), lv["coef"], dt, nx, ny, nz, s.ax,
+              lv["out_idx"], 20, zero_guess, st)
+    src, dst = x0.clone(), torch.empty_like(x0)
+    for sweep in range(20):
+        _lib.call("fc_mg_sweep", src.data_ptr(), dst.data_ptr(), b.data_ptr(), lv["coef"], dt, nx, ny, nz, s.ax,
+                  lv["out_idx"], int(zero_guess and sweep == 0), st)
+        src, dst = dst, src
+    assert torch.equal(x, src)
