@@ -125,6 +125,12 @@ __device__ __forceinline__ float stencil_lo(const TT* __restrict__ T, const Coef
                (float)T[idx + o.s], (float)T[idx + o.u], (float)T[idx + o.d]);
 }
 
+// Streaming accesses: values read or written once per kernel bypass the cache's
+// retention, leaving L2 to the iterate and coefficient lines that neighbouring
+// cells read again.
+__device__ __forceinline__ float ld_once(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_once(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 // First sweep from x = 0 (x1 = invD b), written the way sweep_kernel writes it.
 __device__ __forceinline__ float zero_sweep(const Coef& c, const float* __restrict__ b, int idx) {
   return sweep_update(0.f, c.invD[idx], b[idx], 0.f);
@@ -161,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float* __restrict__
   int i, j, k;
   unflat(g, idx, i, j, k);
   const float lo = stencil_lo<FACE>(x, c, g, dt, idx, i, j, k);
-  x_out[idx] = sweep_update(x[idx], c.invD[idx], b[idx], lo);
+  st_once(x_out + idx, sweep_update(x[idx], ld_once(c.invD + idx), ld_once(b + idx), lo));
 }
 
 // The first two sweeps from x = 0 in one pass: x1 = invD b is formed at the seven
@@ -181,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void sweep2_from_zero_kernel(float* __restr
   const float lo = lo_of(a, dt, xc, zero_sweep(c, b, idx + o.e), zero_sweep(c, b, idx + o.w),
                          zero_sweep(c, b, idx + o.n), zero_sweep(c, b, idx + o.s), zero_sweep(c, b, idx + o.u),
                          zero_sweep(c, b, idx + o.d));
-  x_out[idx] = sweep_update(xc, c.invD[idx], b[idx], lo);
+  st_once(x_out + idx, sweep_update(xc, c.invD[idx], b[idx], lo));
 }
 
 // coarse rhs: mean of the 8 fine residuals (b - A x) * mask (odd extents truncated).
@@ -206,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void residual_restrict_kernel(const float* 
   if (k < 2 * cz) {
     const int idx = (i * g.ny + j) * g.nz + k;
     const float lo = stencil_lo<FACE>(x, c, g, dt, idx, i, j, k);
-    r = (b[idx] - lo) * mask[idx];
+    r = (ld_once(b + idx) - lo) * ld_once(mask + idx);
   }
   res[wave][lane] = r;
   __syncthreads();
@@ -281,12 +287,13 @@ __global__ __launch_bounds__(kBlock) void prolong_correct2_kernel(const float* _
   const int j = (int)(rem / (unsigned)hz), k = 2 * (int)(rem - (unsigned)j * (unsigned)hz);
   const int idx = (i * ny + j) * nz + k;
   auto X = [&](int a, int b2, int c2) { return xc[(a * cy + b2) * cz + c2]; };
-  const float2 xv = *reinterpret_cast<const float2*>(x + idx);
-  const float2 mv = *reinterpret_cast<const float2*>(mask + idx);
-  float2 out;
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 xv = __builtin_nontemporal_load(reinterpret_cast<const f2*>(x + idx));
+  const f2 mv = __builtin_nontemporal_load(reinterpret_cast<const f2*>(mask + idx));
+  f2 out;
   out.x = corrected(xv.x, prolong_at(X, i, j, k, cx, cy, cz, sx, sy, sz), alpha, mv.x);
   out.y = corrected(xv.y, prolong_at(X, i, j, k + 1, cx, cy, cz, sx, sy, sz), alpha, mv.y);
-  *reinterpret_cast<float2*>(x + idx) = out;
+  __builtin_nontemporal_store(out, reinterpret_cast<f2*>(x + idx));
 }
 
 // All sweeps of the coarsest level in one workgroup (global ping-pong, the
