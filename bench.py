@@ -52,9 +52,9 @@ def parse():
 
 def thermal_leg(n, device, seed=0, weights=None):
     """GMG-preconditioned flexible CG (FP32 V-cycle, FP64 outer loop, the reference's
-    PCG), GMG-preconditioned flexible GCR(6), and flexible GCR(6) with the U-Net
-    preconditioner (V-cycle fallback), all to an FP64 relative residual of 1e-8
-    on the same 256^3 blob conduction problem."""
+    PCG), GMG-preconditioned flexible GCR(6) and restarted flexible GMRES(16), and
+    flexible GCR(6) with the U-Net preconditioner (V-cycle fallback), all to an FP64
+    relative residual of 1e-8 on the same 256^3 blob conduction problem."""
     import torch
 
     from fluxcore3d_amd import geometry
@@ -81,11 +81,25 @@ def thermal_leg(n, device, seed=0, weights=None):
     xg, gi = gcr_fp64(mg, c["b"], tol=1e-8, max_it=500, m=6, ws=gws)
     torch.cuda.synchronize()
     gcr_secs = time.perf_counter() - t0
+    del gws
+    from fluxcore3d_amd.solver.mg_pcg import FGMRESWorkspace, fgmres_fp64
+
+    fws = FGMRESWorkspace(c["k"].shape, device, 16)
+    fgmres_fp64(mg, c["b"], tol=1e-8, max_it=3, m=16, ws=fws)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xf, fi = fgmres_fp64(mg, c["b"], tol=1e-8, max_it=500, m=16, ws=fws)
+    torch.cuda.synchronize()
+    fgmres_secs = time.perf_counter() - t0
+    del fws
     out = {"problem": f"{n}^3 smooth-noise blobs (seed {seed}), k_s/k_f = {c['ratio']:.3g}, dt = {c['dt']:g}",
            "levels": [list(lv["shape"]) for lv in mg._levels], "gmg_pcg_s": round(secs, 4),
            "gmg_pcg_iterations": info["it"], "final_rel_residual_fp64": info["res_rel"],
            "converged": bool(info["converged"]), "gmg_gcr6_s": round(gcr_secs, 4),
-           "gmg_gcr6_iterations": gi["it"], "gmg_gcr6_final_rel_residual_fp64": gi["res_rel"], "nn_pcg_s": None}
+           "gmg_gcr6_iterations": gi["it"], "gmg_gcr6_final_rel_residual_fp64": gi["res_rel"],
+           "gmg_fgmres16_s": round(fgmres_secs, 4), "gmg_fgmres16_iterations": fi["it"],
+           "gmg_fgmres16_final_rel_residual_fp64": fi["res_rel"], "gmg_fgmres16_converged": bool(fi["converged"]),
+           "gmg_fgmres16_vs_gmg_pcg_speedup": round(secs / max(fgmres_secs, 1e-9), 3), "nn_pcg_s": None}
     if weights is None or not os.path.exists(weights):
         out["note"] = "no trained U-Net weights found; NN-PCG not run"
         return out

@@ -204,6 +204,187 @@ __global__ __launch_bounds__(kBlock) void update_kernel(double* __restrict__ x, 
   block_partials<1>(v, partial, 1);
 }
 
+// ---- restarted flexible GMRES(m) --------------------------------------------------
+// The Arnoldi basis V_0..V_m is kept unnormalised in FP64 with ||V_i||^2 in device
+// scalars; the preconditioner reads an FP32 copy of the newest V_j. One Arnoldi step
+// is two passes: w = A z (z the FP32 preconditioned vector) written as V_{j+1},
+// with the partials of w . V_i, i <= j; then V_{j+1} = w - sum_i (w.V_i / V_i.V_i) V_i
+// in place (classical Gram-Schmidt), its FP32 copy and ||V_{j+1}||^2.
+constexpr int kMaxBasis = 24;
+constexpr unsigned kArnoldiBlocks = 8192;
+enum { F_DOT0 = 0, F_NV0 = kMaxBasis + 1, F_SLOTS = 2 * (kMaxBasis + 1) };
+
+struct Basis {
+  const double* V[kMaxBasis + 1];
+};
+
+// column-major partials (partial[a * nblocks + b]); block a folds column a, in
+// block order, into scal[dst + a]
+__global__ __launch_bounds__(kBlock) void fold_columns_kernel(const double* __restrict__ partial, int nblocks,
+                                                              double* __restrict__ scal, int dst) {
+  const double* col = partial + (int64_t)blockIdx.x * nblocks;
+  double v = 0.0;
+  for (int b0 = threadIdx.x; b0 < nblocks; b0 += 8 * kBlock) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = b0 + u * kBlock < nblocks ? col[b0 + u * kBlock] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (b0 + u * kBlock < nblocks) v += t[u];
+  }
+  __shared__ double sh[kBlock / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += sh[w];
+    scal[dst + blockIdx.x] = s;
+  }
+}
+
+// block sums of acc[0..NV-1] into column-major partials
+template <int NV>
+__device__ __forceinline__ void block_columns(const double* acc, double* __restrict__ partial) {
+  __shared__ double sh[kBlock / 64][NV];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    const double s = wave_sum(acc[a]);
+    if (lane == 0) sh[wid][a] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += sh[w][threadIdx.x];
+    partial[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// The basis count NV is a template parameter: the NV loads of a cell are then
+// issued together (a runtime bound would put each behind its own branch and wait
+// for it in turn).
+// w = (z + dt L z) * mask in FP64 (the same value as apply_kernel's w), written to
+// wout; partials of w . V_i for i < NV
+template <bool FACE, int NV>
+__global__ __launch_bounds__(kBlock) void arnoldi_apply_kernel(const float* __restrict__ z, Stencil s,
+                                                               double* __restrict__ wout, Basis bs,
+                                                               double* __restrict__ partial) {
+  double acc[NV];
+#pragma unroll
+  for (int a = 0; a < NV; ++a) acc[a] = 0.0;
+  const int n = s.nx * s.ny * s.nz;
+  const unsigned nynz = (unsigned)s.ny * (unsigned)s.nz;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
+    const unsigned u = (unsigned)idx;
+    const int i = (int)(u / nynz);
+    const unsigned rem = u - (unsigned)i * nynz;
+    const int j = (int)(rem / (unsigned)s.nz), k = (int)(rem - (unsigned)j * (unsigned)s.nz);
+    const int oE = (i < s.nx - 1 ? 1 : -1) * (int)nynz, oW = (i > 0 ? -1 : lo_ghost(s, 0)) * (int)nynz;
+    const int oN = (j < s.ny - 1 ? 1 : -1) * s.nz, oS = (j > 0 ? -1 : lo_ghost(s, 1)) * s.nz;
+    const int oU = k < s.nz - 1 ? 1 : -1, oD = k > 0 ? -1 : lo_ghost(s, 2);
+    float cE, cW, cN, cS, cU, cD, cL;
+    if constexpr (FACE) {
+      cE = s.fx[face_hi(idx, i, s.nx, (int)nynz)];
+      cW = s.fx[face_lo(idx, i, s.nx, (int)nynz, lo_ghost(s, 0))];
+      cN = s.fy[face_hi(idx, j, s.ny, s.nz)];
+      cS = s.fy[face_lo(idx, j, s.ny, s.nz, lo_ghost(s, 1))];
+      cU = s.fz[face_hi(idx, k, s.nz, 1)];
+      cD = s.fz[face_lo(idx, k, s.nz, 1, lo_ghost(s, 2))];
+      cL = ((((cE + cW) + cN) + cS) + cU) + cD;
+    } else {
+      cE = s.aE[idx]; cW = s.aW[idx]; cN = s.aN[idx]; cS = s.aS[idx]; cU = s.aU[idx]; cD = s.aD[idx];
+      cL = s.diagL[idx];
+    }
+    double vb[NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) vb[a] = bs.V[a][idx];
+    const double pc = (double)z[idx];
+    const double L = (double)cL * pc - (double)cE * (double)z[idx + oE] - (double)cW * (double)z[idx + oW] -
+                     (double)cN * (double)z[idx + oN] - (double)cS * (double)z[idx + oS] -
+                     (double)cU * (double)z[idx + oU] - (double)cD * (double)z[idx + oD];
+    const double wv = (pc + s.dt * L) * (double)s.mask[idx];
+    wout[idx] = wv;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) acc[a] += wv * vb[a];
+  }
+  block_columns<NV>(acc, partial);
+}
+
+// v = v - sum_{i<NV} (scal[F_DOT0+i] / scal[F_NV0+i]) V_i in place, v32 = (float) v;
+// partials of v . v
+template <int NV>
+__global__ __launch_bounds__(kBlock) void arnoldi_orth_kernel(double* __restrict__ v, float* __restrict__ v32,
+                                                              Basis bs, const double* __restrict__ scal, int64_t n,
+                                                              double* __restrict__ partial) {
+  double c[NV];
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    const double den = scal[F_NV0 + a];
+    c[a] = den > 0.0 ? scal[F_DOT0 + a] / den : 0.0;
+  }
+  double acc[1] = {0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    double vb[NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) vb[a] = bs.V[a][idx];
+    double vi = v[idx];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) vi -= c[a] * vb[a];
+    v[idx] = vi;
+    v32[idx] = (float)vi;
+    acc[0] += vi * vi;
+  }
+  block_columns<1>(acc, partial);
+}
+
+struct ZSet {
+  const float* Z[kMaxBasis];
+  double a[kMaxBasis];
+};
+
+// x += sum_{i<K} a_i Z_i (the sum formed first, in i order)
+template <int K>
+__global__ __launch_bounds__(kBlock) void arnoldi_update_x_kernel(double* __restrict__ x, ZSet zs, int64_t n) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  float zb[K];
+#pragma unroll
+  for (int a = 0; a < K; ++a) zb[a] = zs.Z[a][idx];
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < K; ++a) s += zs.a[a] * (double)zb[a];
+  x[idx] += s;
+}
+
+// launches the instantiation with NV == nv (1 <= nv <= kMaxBasis)
+template <int NV>
+struct ArnoldiLaunch {
+  static void apply(int nv, bool faces, unsigned g, hipStream_t st, const float* z, const Stencil& s, double* w,
+                    const Basis& bs, double* partial) {
+    if (nv != NV) return ArnoldiLaunch<NV - 1>::apply(nv, faces, g, st, z, s, w, bs, partial);
+    if (faces)
+      hipLaunchKernelGGL((arnoldi_apply_kernel<true, NV>), dim3(g), dim3(kBlock), 0, st, z, s, w, bs, partial);
+    else
+      hipLaunchKernelGGL((arnoldi_apply_kernel<false, NV>), dim3(g), dim3(kBlock), 0, st, z, s, w, bs, partial);
+  }
+  static void orth(int nv, unsigned g, hipStream_t st, double* v, float* v32, const Basis& bs, const double* scal,
+                   int64_t n, double* partial) {
+    if (nv != NV) return ArnoldiLaunch<NV - 1>::orth(nv, g, st, v, v32, bs, scal, n, partial);
+    hipLaunchKernelGGL(arnoldi_orth_kernel<NV>, dim3(g), dim3(kBlock), 0, st, v, v32, bs, scal, n, partial);
+  }
+  static void update(int k, hipStream_t st, double* x, const ZSet& zs, int64_t n) {
+    if (k != NV) return ArnoldiLaunch<NV - 1>::update(k, st, x, zs, n);
+    hipLaunchKernelGGL(arnoldi_update_x_kernel<NV>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, x, zs, n);
+  }
+};
+template <>
+struct ArnoldiLaunch<0> {
+  static void apply(int, bool, unsigned, hipStream_t, const float*, const Stencil&, double*, const Basis&, double*) {}
+  static void orth(int, unsigned, hipStream_t, double*, float*, const Basis&, const double*, int64_t, double*) {}
+  static void update(int, hipStream_t, double*, const ZSet&, int64_t) {}
+};
+
 __global__ void copy_scalar_kernel(double* __restrict__ scal, int src, int dst) { scal[dst] = scal[src]; }
 
 unsigned grid_of(int64_t n) {
@@ -279,6 +460,68 @@ int fc_gcr_step(const float* z32, const fc_gcr_stencil* st, double* p, double* w
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, (const double*)partial, (int)g, 1, 1, scal,
                      (int)K_RR);
   return check_launch("gcr_step/fold update");
+}
+
+
+int fc_fgmres_max_basis(void) { return kMaxBasis; }
+int fc_fgmres_scalar_slots(void) { return F_SLOTS; }
+int fc_fgmres_dot_slot(int i) { return F_DOT0 + i; }
+int fc_fgmres_norm_slot(int i) { return F_NV0 + i; }
+int fc_fgmres_partials_count(int64_t n) {
+  const unsigned g = grid_for(n, kBlock);
+  return (int)(g < kArnoldiBlocks ? g : kArnoldiBlocks) * (kMaxBasis + 1);
+}
+
+// One Arnoldi step of flexible GMRES: V[j+1] = A z32 orthogonalised against V[0..j]
+// (classical Gram-Schmidt, coefficients w.V_i / V_i.V_i), v32 = (float) V[j+1];
+// scal[dot_slot(i)] = w.V_i (w before the orthogonalisation), scal[norm_slot(j+1)] =
+// ||V[j+1]||^2. scal[norm_slot(i)], i <= j, must hold ||V_i||^2.
+int fc_fgmres_arnoldi(const float* z32, const fc_gcr_stencil* st, double* const* V, int j, float* v32,
+                      double* partial, double* scal, hipStream_t stream) {
+  FC_REQUIRE(z32 && st && V && v32 && partial && scal, "fgmres_arnoldi: null pointer argument");
+  FC_REQUIRE(j >= 0 && j < kMaxBasis, "fgmres_arnoldi: 0 <= j < %d", kMaxBasis);
+  FC_REQUIRE(st->nx >= 2 && st->ny >= 2 && st->nz >= 2, "fgmres_arnoldi: every extent must be >= 2");
+  Stencil s{st->aE, st->aW, st->aN, st->aS, st->aU, st->aD, st->diagL, st->mask, st->nx, st->ny, st->nz,
+            st->ax, st->out_idx, st->dt, st->fx, st->fy, st->fz};
+  const int faces = (s.fx != nullptr) + (s.fy != nullptr) + (s.fz != nullptr);
+  FC_REQUIRE(faces == 0 || faces == 3, "fgmres_arnoldi: face coefficients must be all set or all null");
+  FC_REQUIRE(s.mask && (faces == 3 || (s.aE && s.aW && s.aN && s.aS && s.aU && s.aD && s.diagL)),
+             "fgmres_arnoldi: null coefficient");
+  const int64_t n = (int64_t)s.nx * s.ny * s.nz;
+  FC_REQUIRE(n + (int64_t)kArnoldiBlocks * kBlock < ((int64_t)1 << 31),
+             "fgmres_arnoldi: %lld cells exceed the 32-bit index range", (long long)n);
+  Basis bs{};
+  for (int i = 0; i <= j + 1; ++i) {
+    FC_REQUIRE(V[i], "fgmres_arnoldi: null basis vector %d", i);
+    bs.V[i] = V[i];
+  }
+  const unsigned g0 = grid_for(n, kBlock), g = g0 < kArnoldiBlocks ? g0 : kArnoldiBlocks;
+  double* w = V[j + 1];
+  ArnoldiLaunch<kMaxBasis>::apply(j + 1, faces == 3, g, stream, z32, s, w, bs, partial);
+  if (int rc = check_launch("fgmres_arnoldi/apply")) return rc;
+  hipLaunchKernelGGL(fold_columns_kernel, dim3(j + 1), dim3(kBlock), 0, stream, (const double*)partial, (int)g, scal,
+                     (int)F_DOT0);
+  if (int rc = check_launch("fgmres_arnoldi/fold dots")) return rc;
+  ArnoldiLaunch<kMaxBasis>::orth(j + 1, g, stream, w, v32, bs, (const double*)scal, n, partial);
+  if (int rc = check_launch("fgmres_arnoldi/orth")) return rc;
+  hipLaunchKernelGGL(fold_columns_kernel, dim3(1), dim3(kBlock), 0, stream, (const double*)partial, (int)g, scal,
+                     (int)(F_NV0 + j + 1));
+  return check_launch("fgmres_arnoldi/fold norm");
+}
+
+// x += sum_{i<k} a[i] Z[i] (a on the host, Z FP32 device vectors)
+int fc_fgmres_update_x(double* x, const float* const* Z, const double* a, int k, int64_t n, hipStream_t stream) {
+  FC_REQUIRE(x && Z && a, "fgmres_update_x: null pointer argument");
+  FC_REQUIRE(k >= 0 && k <= kMaxBasis, "fgmres_update_x: 0 <= k <= %d", kMaxBasis);
+  if (k == 0 || n == 0) return 0;
+  ZSet zs{};
+  for (int i = 0; i < k; ++i) {
+    FC_REQUIRE(Z[i], "fgmres_update_x: null vector %d", i);
+    zs.Z[i] = Z[i];
+    zs.a[i] = a[i];
+  }
+  ArnoldiLaunch<kMaxBasis>::update(k, stream, x, zs, n);
+  return check_launch("fgmres_update_x");
 }
 
 }  // extern "C"

@@ -239,3 +239,140 @@ def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
     x = fw.x.view(shape).to(torch.float32) * mask
     return x, {"converged": converged, "it": it, "res_rel": hist[-1], "hist": hist, "x64": fw.x.view(shape),
                "stopped": stopped}
+
+
+# ---------------------------------------------------- restarted flexible GMRES(m)
+_lib.register({
+    "fc_fgmres_max_basis": (_lib.I32, []),
+    "fc_fgmres_scalar_slots": (_lib.I32, []),
+    "fc_fgmres_dot_slot": (_lib.I32, [_lib.I32]),
+    "fc_fgmres_norm_slot": (_lib.I32, [_lib.I32]),
+    "fc_fgmres_partials_count": (_lib.I32, [_lib.I64]),
+    "fc_fgmres_arnoldi": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.I32, _lib.P, _lib.P, _lib.P, _lib.P]),
+    "fc_fgmres_update_x": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P]),
+}, unchecked={"fc_fgmres_max_basis", "fc_fgmres_scalar_slots", "fc_fgmres_dot_slot", "fc_fgmres_norm_slot",
+              "fc_fgmres_partials_count"})
+
+
+class FGMRESWorkspace:
+    """Arnoldi basis V_1..V_m (FP64; V_0 is the FP64 true-residual buffer), the m
+    preconditioned vectors Z_j in FP32 (the V-cycle's own output, stored as is), the
+    FP32 copy of the newest basis vector and the device scalars."""
+
+    def __init__(self, shape, device, m: int = 16):
+        mmax = int(_lib.call("fc_fgmres_max_basis"))
+        if not 1 <= m <= mmax:
+            raise ValueError(f"FGMRES restart length m must be in [1, {mmax}]")
+        n = int(torch.Size(shape).numel())
+        self.m, self.n = m, n
+        self.fcg = FCGWorkspace(shape, device)  # x, b, r (= V_0), r32 and the true-residual reductions
+        self.V = [torch.zeros(n, device=device, dtype=torch.float64) for _ in range(m)]
+        self.Z = [torch.zeros(n, device=device, dtype=torch.float32) for _ in range(m)]
+        self.v32 = torch.zeros(n, device=device, dtype=torch.float32)
+        self.partial = torch.empty(int(_lib.call("fc_fgmres_partials_count", n)), device=device, dtype=torch.float64)
+        self.scal = torch.zeros(int(_lib.call("fc_fgmres_scalar_slots")), device=device, dtype=torch.float64)
+        self.dot_slot = [int(_lib.call("fc_fgmres_dot_slot", i)) for i in range(m + 1)]
+        self.norm_slot = [int(_lib.call("fc_fgmres_norm_slot", i)) for i in range(m + 1)]
+
+
+def fgmres_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol: float = 1e-8,
+                max_it: int = 300, m: int = 16, precond=None, ws: Optional[FGMRESWorkspace] = None
+                ) -> Tuple[torch.Tensor, dict]:
+    """Restarted flexible GMRES(m) (Saad 1993) on A x = b with FP64 vectors and a
+    variable preconditioner ``precond(v32, z32)`` (default: one V-cycle).
+
+    Each Arnoldi step is two passes over the grid (fc_fgmres_arnoldi): w = A z with
+    the dot products against the basis, then classical Gram-Schmidt in place with
+    the FP32 copy for the next preconditioner call. The basis is kept unnormalised;
+    the host forms the normalised Hessenberg column from the device scalars
+    (h_ij = w.V_i / (|V_j| |V_i|), h_j+1,j = |V_j+1| / |V_j|) and applies Givens
+    rotations in FP64. Within a cycle the residual estimate |g_j+1| never grows; at
+    each restart x += sum_j (y_j / |V_j|) Z_j and the FP64 true residual decides
+    convergence and starts the next cycle."""
+    import math
+
+    lv0 = mg_solver._levels[0]
+    shape = lv0["shape"]
+    nx, ny, nz = shape
+    dev = mg_solver.device
+    if ws is None or ws.m != m:
+        ws = FGMRESWorkspace(shape, dev, m)
+    fw = ws.fcg
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    if precond is None:
+        def precond(v32, z32):
+            mg_solver.precondition(v32, out=z32, masked_input=True)
+    fw.b.copy_(b.reshape(-1).to(torch.float64))
+    if x0 is None:
+        fw.x.zero_()
+    else:
+        fw.x.copy_(x0.reshape(-1).to(torch.float64))
+    dt = float(mg_solver._dt_cache)
+    mask = lv0["mask_f"]
+    faces = lv0.get("faces")
+    st = GCRStencil(*[lv0[k].data_ptr() for k in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL")], mask.data_ptr(),
+                    dt, nx, ny, nz, int(mg_solver.ax), int(lv0["out_idx"]),
+                    *([f.data_ptr() for f in faces] if faces is not None else [None] * 3))
+
+    def true_residual():
+        _lib.call("fc_mg_true_residual", fw.x.data_ptr(), fw.b.data_ptr(), fw.r.data_ptr(), fw.r32.data_ptr(),
+                  mask.data_ptr(), lv0["coef"], dt, nx, ny, nz, mg_solver.ax, lv0["out_idx"], fw.partial.data_ptr(),
+                  fw.scal.data_ptr(), stream)
+        s = fw.scal[S_RR:S_BB + 1].cpu()
+        return float(s[0]), float(s[1])
+
+    Vp = (ctypes.c_void_p * (m + 1))(fw.r.data_ptr(), *[v.data_ptr() for v in ws.V])
+    Zp = (ctypes.c_void_p * m)(*[z.data_ptr() for z in ws.Z])
+    rr, bb = true_residual()
+    bnorm = bb ** 0.5 + 1e-300
+    hist = [rr ** 0.5 / bnorm]
+    converged = hist[-1] < tol
+    it = 0
+    while not converged and it < max_it and rr > 0.0:
+        ws.scal[ws.norm_slot[0]:ws.norm_slot[0] + 1].copy_(fw.scal[S_RR:S_RR + 1])  # |V_0|^2 on the device
+        g = [rr ** 0.5] + [0.0] * m
+        R = [[0.0] * m for _ in range(m + 1)]  # rotated Hessenberg columns
+        cs, sn = [], []
+        nv = [rr]
+        vin, k = fw.r32, 0
+        for j in range(m):
+            precond(vin, ws.Z[j])
+            _lib.call("fc_fgmres_arnoldi", ws.Z[j].data_ptr(), ctypes.byref(st), Vp, j, ws.v32.data_ptr(),
+                      ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
+            sc = ws.scal.cpu().tolist()
+            nv.append(sc[ws.norm_slot[j + 1]])
+            hj = nv[j] ** 0.5
+            col = [sc[ws.dot_slot[i]] / (hj * nv[i] ** 0.5) for i in range(j + 1)] + [nv[j + 1] ** 0.5 / hj]
+            for i in range(j):
+                a, c2 = col[i], col[i + 1]
+                col[i], col[i + 1] = cs[i] * a + sn[i] * c2, -sn[i] * a + cs[i] * c2
+            d = math.hypot(col[j], col[j + 1])
+            if not (d > 0.0 and math.isfinite(d)):
+                break
+            cs.append(col[j] / d)
+            sn.append(col[j + 1] / d)
+            col[j], col[j + 1] = d, 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            for i in range(j + 1):
+                R[i][j] = col[i]
+            it += 1
+            k = j + 1
+            hist.append(abs(g[j + 1]) / bnorm)
+            if hist[-1] < 0.5 * tol or it >= max_it or nv[j + 1] <= 0.0:
+                break
+            vin = ws.v32
+        if k == 0:
+            break
+        y = [0.0] * k
+        for i in range(k - 1, -1, -1):
+            y[i] = (g[i] - sum(R[i][q] * y[q] for q in range(i + 1, k))) / R[i][i]
+        a = (ctypes.c_double * k)(*[y[i] / nv[i] ** 0.5 for i in range(k)])
+        _lib.call("fc_fgmres_update_x", fw.x.data_ptr(), Zp, a, k, ws.n, stream)
+        rr, _ = true_residual()
+        hist.append(rr ** 0.5 / bnorm)
+        converged = hist[-1] < tol
+        if not math.isfinite(hist[-1]):
+            break
+    x = fw.x.view(shape).to(torch.float32) * mask
+    return x, {"converged": converged, "it": it, "res_rel": hist[-1], "hist": hist, "x64": fw.x.view(shape)}

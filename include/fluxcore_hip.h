@@ -274,6 +274,25 @@ int fc_gcr_step(const float *z32, const fc_gcr_stencil *stencil, double *p, doub
                 double *const *P, const int *ww_slot, int m, int store_slot, double *x, double *r, float *r32,
                 double *partial, double *scal, hipStream_t stream);
 
+/* ---- restarted flexible GMRES(m) in FP64 (csrc/krylov.hip) ----
+ * No reference counterpart: a minimal-residual Krylov method for the conduction system with
+ * the V-cycle (or the learned preconditioner) as a variable preconditioner; the reference's
+ * Krylov solvers are mg_pcg.py:33-187. The basis V_0..V_m stays unnormalised (FP64), the
+ * preconditioned vectors Z_j are the preconditioner's FP32 output. scal holds
+ * fc_fgmres_scalar_slots() doubles: w.V_i at fc_fgmres_dot_slot(i), ||V_i||^2 at
+ * fc_fgmres_norm_slot(i); partial holds fc_fgmres_partials_count(n) doubles. */
+int fc_fgmres_max_basis(void);
+int fc_fgmres_scalar_slots(void);
+int fc_fgmres_dot_slot(int i);
+int fc_fgmres_norm_slot(int i);
+int fc_fgmres_partials_count(int64_t n);
+/* one Arnoldi step: V[j+1] = A z32 (with w.V_i, i <= j), then V[j+1] -= sum_i (w.V_i / V_i.V_i) V_i,
+ * v32 = (float) V[j+1], ||V[j+1]||^2; needs ||V_i||^2, i <= j, in scal */
+int fc_fgmres_arnoldi(const float *z32, const fc_gcr_stencil *stencil, double *const *V, int j, float *v32,
+                      double *partial, double *scal, hipStream_t stream);
+/* x += sum_{i<k} a[i] Z[i]; a is a host array of k doubles */
+int fc_fgmres_update_x(double *x, const float *const *Z, const double *a, int k, int64_t n, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,8 +10,9 @@ test_gpu_lbm.py, test_gpu_physics.py and test_gpu_mg.py):
 * 384^3 gyroid (8 periods of 48 voxels, MRT, Bouzidi): FP16 population storage stays
   within 1e-3 of the largest speed from FP32 storage after 100 steps, the bound the
   48^3 unit cell meets against the FP64 oracle.
-* 512^3 blob conduction problem: GMG-preconditioned flexible GCR(6) and flexible CG
-  both reach an FP64 true relative residual of 1e-8 and agree on the solution.
+* 512^3 blob conduction problem: GMG-preconditioned flexible GCR(6), flexible CG and
+  restarted flexible GMRES(16) reach an FP64 true relative residual of 1e-8 and agree
+  on the solution.
 """
 import numpy as np
 import pytest
@@ -146,9 +147,9 @@ def test_gyroid_384_fp16_storage_tracks_fp32():
     assert float((out["fp16"] - out["fp32"]).abs().max()) / umax < 1e-3
 
 
-def test_blobs_512_gcr_and_fcg_reach_tolerance_and_agree():
+def test_blobs_512_gcr_fcg_and_fgmres_reach_tolerance_and_agree():
     from fluxcore3d_amd import geometry
-    from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, gcr_fp64
+    from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, fgmres_fp64, gcr_fp64
     from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
     c = geometry.conduction_problem(512, seed=1, device=DEV)
     mg = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, c["ax"], c["sign"], c["s"], omega=0.8, n_pre=2,
@@ -163,5 +164,9 @@ def test_blobs_512_gcr_and_fcg_reach_tolerance_and_agree():
     xf, fi = fcg_fp64(mg, c["b"], tol=1e-8, max_it=600)
     assert fi["converged"] and fi["res_rel"] < 1e-8, fi["hist"][-3:]
     m = c["mask"].reshape(-1).double()
-    xf64 = fi["x64"].reshape(-1) * m
+    xf64 = fi["x64"].reshape(-1).clone() * m
+    del fi
     assert float((xg.reshape(-1) * m - xf64).norm() / xf64.norm()) < 1e-6
+    _, ri = fgmres_fp64(mg, c["b"], tol=1e-8, max_it=400, m=16)
+    assert ri["converged"] and ri["res_rel"] < 1e-8, ri["hist"][-3:]
+    assert float((ri["x64"].reshape(-1) * m - xf64).norm() / xf64.norm()) < 1e-6

@@ -279,3 +279,74 @@ def test_coarse_solve_lds_same_bits_as_sweeps(zero_guess):
                   lv["out_idx"], int(zero_guess and sweep == 0), st)
         src, dst = dst, src
     assert torch.equal(x, src)
+
+
+def test_fgmres_arnoldi_step_against_fp64_operator():
+    """One Arnoldi step (fc_fgmres_arnoldi) against an FP64 restatement: w = A z with
+    the oracle operator on the FP32 coefficients, dot products with the basis, classical
+    Gram-Schmidt, the FP32 copy and the squared norm."""
+    import ctypes
+    from fluxcore3d_amd import _lib
+    from fluxcore3d_amd.solver.mg_pcg import FGMRESWorkspace, GCRStencil
+    c = thermal_cases.sphere_conduction(64, 32, 32, dt=1.0e6)
+    s, h = _solver_pair(c)
+    lv0 = s._levels[0]
+    shape = lv0["shape"]
+    n = int(np.prod(shape))
+    ws = FGMRESWorkspace(shape, torch.device(DEV), 4)
+    rng = np.random.default_rng(21)
+    m = c["mask"].astype(np.float64).reshape(-1)
+    V0 = rng.standard_normal(n) * m
+    V1 = rng.standard_normal(n) * m
+    z = (rng.standard_normal(n) * m).astype(np.float32)
+    ws.fcg.r.copy_(T(V0))
+    ws.V[0].copy_(T(V1))
+    ws.scal[ws.norm_slot[0]] = float(V0 @ V0)
+    ws.scal[ws.norm_slot[1]] = float(V1 @ V1)
+    faces = lv0["faces"]
+    st = GCRStencil(*[lv0[k].data_ptr() for k in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL")],
+                    lv0["mask_f"].data_ptr(), float(s._dt_cache), *shape, int(s.ax), int(lv0["out_idx"]),
+                    *[f.data_ptr() for f in faces])
+    Vp = (ctypes.c_void_p * 5)(ws.fcg.r.data_ptr(), *[v.data_ptr() for v in ws.V])
+    zt = T(z)
+    _lib.call("fc_fgmres_arnoldi", zt.data_ptr(), ctypes.byref(st), Vp, 1, ws.v32.data_ptr(), ws.partial.data_ptr(),
+              ws.scal.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    co64 = tuple(a.astype(np.float64) for a in h.levels[0]["co"])
+    w = (mg.mop(z.astype(np.float64).reshape(shape), co64, c["dt"], 0, h.levels[0]["out_idx"]).reshape(-1)) * m
+    d0, d1 = w @ V0, w @ V1
+    v2 = w - d0 / (V0 @ V0) * V0 - d1 / (V1 @ V1) * V1
+    sc = ws.scal.cpu().numpy()
+    assert abs(sc[ws.dot_slot[0]] - d0) <= 1e-12 * np.linalg.norm(w) * np.linalg.norm(V0)
+    assert abs(sc[ws.dot_slot[1]] - d1) <= 1e-12 * np.linalg.norm(w) * np.linalg.norm(V1)
+    got = ws.V[1].cpu().numpy()
+    assert np.abs(got - v2).max() <= 1e-12 * np.abs(w).max()
+    assert abs(sc[ws.norm_slot[2]] - v2 @ v2) <= 1e-12 * (v2 @ v2)
+    np.testing.assert_array_equal(ws.v32.cpu().numpy(), got.astype(np.float32))
+
+
+@pytest.mark.parametrize("m", [4, 16])
+def test_fgmres_fp64_converges_to_the_solution(m):
+    """Restarted flexible GMRES(m) with the V-cycle: the residual estimates never grow
+    within a cycle, the stored basis is orthogonal, the FP64 true residual of the
+    returned x is below the tolerance (checked with the oracle operator) and x equals
+    the oracle FCG solution."""
+    from fluxcore3d_amd.solver.mg_pcg import FGMRESWorkspace, fgmres_fp64
+    c = thermal_cases.sphere_conduction(64, 32, 32, dt=1.0e6)
+    s, h = _solver_pair(c)
+    ws = FGMRESWorkspace(s._levels[0]["shape"], torch.device(DEV), m)
+    x, info = fgmres_fp64(s, T(c["b"]), tol=1e-8, max_it=300, m=m, ws=ws)
+    assert info["converged"], info["hist"][-5:]
+    hist = info["hist"]  # estimates, with the true residual after each cycle
+    assert all(b <= a * 1.01 for a, b in zip(hist, hist[1:])), hist
+    k = info["it"] - m * ((info["it"] - 1) // m)  # steps of the last cycle: V_1..V_k are stored
+    if k >= 2:
+        Vs = torch.stack([v / v.norm() for v in ws.V[:k]])
+        G = (Vs @ Vs.T).cpu().numpy()
+        assert np.abs(G - np.eye(k)).max() < 1e-8
+    co64 = tuple(a.astype(np.float64) for a in h.levels[0]["co"])
+    x64 = info["x64"].cpu().numpy()
+    msk = c["mask"].astype(np.float64)
+    r = (c["b"].astype(np.float64) * msk - mg.mop(x64, co64, c["dt"], 0, h.levels[0]["out_idx"])) * msk
+    assert np.linalg.norm(r) / np.linalg.norm(c["b"] * msk) < 1.01e-8
+    xo, _ = mg.fcg_mg_fp64(h, c["b"], tol=1e-10, max_it=300)
+    assert rel(x64 * msk, xo) < 1e-6
