@@ -357,6 +357,46 @@ __global__ __launch_bounds__(kBlock) void arnoldi_update_x_kernel(double* __rest
   x[idx] += s;
 }
 
+// Givens state of one GMRES cycle (FP64, device): R (upper triangle, column-major with
+// leading dimension kMaxBasis + 1), the rotations, g and the residual estimate.
+enum { G_R = 0, G_CS = (kMaxBasis + 1) * kMaxBasis, G_SN = G_CS + kMaxBasis, G_G = G_SN + kMaxBasis,
+       G_EST = G_G + kMaxBasis + 1, G_SLOTS = G_EST + 1 };
+
+// Column j of the normalised Hessenberg matrix from the Arnoldi scalars
+// (h_ij = w.V_i / (|V_j| |V_i|), h_{j+1,j} = |V_{j+1}| / |V_j|), the previous rotations
+// applied, a new rotation zeroing h_{j+1,j}, g rotated; est = |g_{j+1}| / bnorm.
+// j = 0 starts the cycle: g = (|V_0|, 0, ...).
+__global__ void arnoldi_givens_kernel(const double* __restrict__ scal, double* __restrict__ st, int j,
+                                      double bnorm) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double* R = st + G_R;
+  double* cs = st + G_CS;
+  double* sn = st + G_SN;
+  double* g = st + G_G;
+  if (j == 0) {
+    for (int i = 0; i <= kMaxBasis; ++i) g[i] = 0.0;
+    g[0] = sqrt(scal[F_NV0]);
+  }
+  double col[kMaxBasis + 1];
+  const double hj = sqrt(scal[F_NV0 + j]);
+  for (int i = 0; i <= j; ++i) col[i] = scal[F_DOT0 + i] / (hj * sqrt(scal[F_NV0 + i]));
+  col[j + 1] = sqrt(scal[F_NV0 + j + 1]) / hj;
+  for (int i = 0; i < j; ++i) {
+    const double a = col[i], b = col[i + 1];
+    col[i] = cs[i] * a + sn[i] * b;
+    col[i + 1] = -sn[i] * a + cs[i] * b;
+  }
+  const double d = hypot(col[j], col[j + 1]);
+  const double c = d > 0.0 ? col[j] / d : 1.0, sv = d > 0.0 ? col[j + 1] / d : 0.0;
+  cs[j] = c;
+  sn[j] = sv;
+  col[j] = d;
+  g[j + 1] = -sv * g[j];
+  g[j] = c * g[j];
+  for (int i = 0; i <= j; ++i) R[j * (kMaxBasis + 1) + i] = col[i];
+  st[G_EST] = fabs(g[j + 1]) / bnorm;
+}
+
 // launches the instantiation with NV == nv (1 <= nv <= kMaxBasis)
 template <int NV>
 struct ArnoldiLaunch {
@@ -522,6 +562,28 @@ int fc_fgmres_update_x(double* x, const float* const* Z, const double* a, int k,
   }
   ArnoldiLaunch<kMaxBasis>::update(k, stream, x, zs, n);
   return check_launch("fgmres_update_x");
+}
+
+
+int fc_fgmres_givens_slots(void) { return G_SLOTS; }
+int fc_fgmres_givens_layout(int* r, int* g, int* est, int* ld) {
+  FC_REQUIRE(r && g && est && ld, "fgmres_givens_layout: null pointer argument");
+  *r = G_R;
+  *g = G_G;
+  *est = G_EST;
+  *ld = kMaxBasis + 1;
+  return 0;
+}
+
+// Givens update of Arnoldi step j on the device (one thread): reads the scalars that
+// fc_fgmres_arnoldi left in scal, updates state (fc_fgmres_givens_slots() doubles)
+// and writes the residual estimate |g_{j+1}| / bnorm at the est slot.
+int fc_fgmres_givens(const double* scal, double* state, int j, double bnorm, hipStream_t stream) {
+  FC_REQUIRE(scal && state, "fgmres_givens: null pointer argument");
+  FC_REQUIRE(j >= 0 && j < kMaxBasis, "fgmres_givens: 0 <= j < %d", kMaxBasis);
+  FC_REQUIRE(bnorm > 0.0, "fgmres_givens: bnorm must be positive");
+  hipLaunchKernelGGL(arnoldi_givens_kernel, dim3(1), dim3(64), 0, stream, scal, state, j, bnorm);
+  return check_launch("fgmres_givens");
 }
 
 }  // extern "C"

@@ -250,8 +250,11 @@ _lib.register({
     "fc_fgmres_partials_count": (_lib.I32, [_lib.I64]),
     "fc_fgmres_arnoldi": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.I32, _lib.P, _lib.P, _lib.P, _lib.P]),
     "fc_fgmres_update_x": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P]),
+    "fc_fgmres_givens_slots": (_lib.I32, []),
+    "fc_fgmres_givens_layout": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P]),
+    "fc_fgmres_givens": (_lib.I32, [_lib.P, _lib.P, _lib.I32, _lib.F64, _lib.P]),
 }, unchecked={"fc_fgmres_max_basis", "fc_fgmres_scalar_slots", "fc_fgmres_dot_slot", "fc_fgmres_norm_slot",
-              "fc_fgmres_partials_count"})
+              "fc_fgmres_partials_count", "fc_fgmres_givens_slots"})
 
 
 class FGMRESWorkspace:
@@ -273,6 +276,13 @@ class FGMRESWorkspace:
         self.scal = torch.zeros(int(_lib.call("fc_fgmres_scalar_slots")), device=device, dtype=torch.float64)
         self.dot_slot = [int(_lib.call("fc_fgmres_dot_slot", i)) for i in range(m + 1)]
         self.norm_slot = [int(_lib.call("fc_fgmres_norm_slot", i)) for i in range(m + 1)]
+        lay = [ctypes.c_int() for _ in range(4)]
+        _lib.call("fc_fgmres_givens_layout", *[ctypes.byref(v) for v in lay])
+        self.g_r, self.g_g, self.g_est, self.g_ld = (v.value for v in lay)
+        self.gstate = torch.zeros(int(_lib.call("fc_fgmres_givens_slots")), device=device, dtype=torch.float64)
+        # residual estimates come back through pinned memory, one step behind
+        self.est_host = torch.zeros(m, dtype=torch.float64, pin_memory=True)
+        self.events = [torch.cuda.Event() for _ in range(m)]
 
 
 def fgmres_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol: float = 1e-8,
@@ -284,11 +294,13 @@ def fgmres_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, t
     Each Arnoldi step is two passes over the grid (fc_fgmres_arnoldi): w = A z with
     the dot products against the basis, then classical Gram-Schmidt in place with
     the FP32 copy for the next preconditioner call. The basis is kept unnormalised;
-    the host forms the normalised Hessenberg column from the device scalars
-    (h_ij = w.V_i / (|V_j| |V_i|), h_j+1,j = |V_j+1| / |V_j|) and applies Givens
-    rotations in FP64. Within a cycle the residual estimate |g_j+1| never grows; at
-    each restart x += sum_j (y_j / |V_j|) Z_j and the FP64 true residual decides
-    convergence and starts the next cycle."""
+    a one-thread kernel (fc_fgmres_givens) forms the normalised Hessenberg column
+    (h_ij = w.V_i / (|V_j| |V_i|), h_j+1,j = |V_j+1| / |V_j|) and applies the Givens
+    rotations in FP64 on the device. The host reads each residual estimate one step
+    late, from pinned memory, while the next step is already queued, so the GPU does
+    not wait for it; convergence seen at step j ends the cycle after step j + 1, whose
+    direction is used too. At each restart x += sum_j (y_j / |V_j|) Z_j and the FP64
+    true residual decides convergence and starts the next cycle."""
     import math
 
     lv0 = mg_solver._levels[0]
@@ -330,43 +342,34 @@ def fgmres_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, t
     it = 0
     while not converged and it < max_it and rr > 0.0:
         ws.scal[ws.norm_slot[0]:ws.norm_slot[0] + 1].copy_(fw.scal[S_RR:S_RR + 1])  # |V_0|^2 on the device
-        g = [rr ** 0.5] + [0.0] * m
-        R = [[0.0] * m for _ in range(m + 1)]  # rotated Hessenberg columns
-        cs, sn = [], []
-        nv = [rr]
         vin, k = fw.r32, 0
         for j in range(m):
             precond(vin, ws.Z[j])
             _lib.call("fc_fgmres_arnoldi", ws.Z[j].data_ptr(), ctypes.byref(st), Vp, j, ws.v32.data_ptr(),
                       ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
-            sc = ws.scal.cpu().tolist()
-            nv.append(sc[ws.norm_slot[j + 1]])
-            hj = nv[j] ** 0.5
-            col = [sc[ws.dot_slot[i]] / (hj * nv[i] ** 0.5) for i in range(j + 1)] + [nv[j + 1] ** 0.5 / hj]
-            for i in range(j):
-                a, c2 = col[i], col[i + 1]
-                col[i], col[i + 1] = cs[i] * a + sn[i] * c2, -sn[i] * a + cs[i] * c2
-            d = math.hypot(col[j], col[j + 1])
-            if not (d > 0.0 and math.isfinite(d)):
-                break
-            cs.append(col[j] / d)
-            sn.append(col[j + 1] / d)
-            col[j], col[j + 1] = d, 0.0
-            g[j + 1] = -sn[j] * g[j]
-            g[j] = cs[j] * g[j]
-            for i in range(j + 1):
-                R[i][j] = col[i]
+            _lib.call("fc_fgmres_givens", ws.scal.data_ptr(), ws.gstate.data_ptr(), j, bnorm, stream)
+            ws.est_host[j:j + 1].copy_(ws.gstate[ws.g_est:ws.g_est + 1], non_blocking=True)
+            ws.events[j].record()
             it += 1
             k = j + 1
-            hist.append(abs(g[j + 1]) / bnorm)
-            if hist[-1] < 0.5 * tol or it >= max_it or nv[j + 1] <= 0.0:
+            if j >= 1:  # the estimate of the previous step; this step is already queued
+                ws.events[j - 1].synchronize()
+                e = float(ws.est_host[j - 1])
+                hist.append(e)
+                if not e >= 0.5 * tol:  # converged (or not finite): stop after this step
+                    break
+            if it >= max_it:
                 break
             vin = ws.v32
-        if k == 0:
-            break
+        ws.events[k - 1].synchronize()
+        hist.append(float(ws.est_host[k - 1]))
+        gst = ws.gstate.cpu().tolist()
+        nv = [float(v) for v in ws.scal[ws.norm_slot[0]:ws.norm_slot[0] + k].cpu()]
         y = [0.0] * k
         for i in range(k - 1, -1, -1):
-            y[i] = (g[i] - sum(R[i][q] * y[q] for q in range(i + 1, k))) / R[i][i]
+            rii = gst[ws.g_r + i * ws.g_ld + i]
+            acc = gst[ws.g_g + i] - sum(gst[ws.g_r + q * ws.g_ld + i] * y[q] for q in range(i + 1, k))
+            y[i] = acc / rii if rii != 0.0 else 0.0
         a = (ctypes.c_double * k)(*[y[i] / nv[i] ** 0.5 for i in range(k)])
         _lib.call("fc_fgmres_update_x", fw.x.data_ptr(), Zp, a, k, ws.n, stream)
         rr, _ = true_residual()

@@ -292,6 +292,12 @@ int fc_fgmres_arnoldi(const float *z32, const fc_gcr_stencil *stencil, double *c
                       double *partial, double *scal, hipStream_t stream);
 /* x += sum_{i<k} a[i] Z[i]; a is a host array of k doubles */
 int fc_fgmres_update_x(double *x, const float *const *Z, const double *a, int k, int64_t n, hipStream_t stream);
+/* the Givens rotations of step j on the device (one thread), so the host need not wait for
+ * each step: state holds fc_fgmres_givens_slots() doubles (R column-major with leading
+ * dimension *ld from offset *r, g from *g, the estimate |g_j+1| / bnorm at *est) */
+int fc_fgmres_givens_slots(void);
+int fc_fgmres_givens_layout(int *r, int *g, int *est, int *ld);
+int fc_fgmres_givens(const double *scal, double *state, int j, double bnorm, hipStream_t stream);
 
 #ifdef __cplusplus
 }
