@@ -100,6 +100,7 @@ def thermal_leg(n, device, seed=0, weights=None):
            "gmg_fgmres16_s": round(fgmres_secs, 4), "gmg_fgmres16_iterations": fi["it"],
            "gmg_fgmres16_final_rel_residual_fp64": fi["res_rel"], "gmg_fgmres16_converged": bool(fi["converged"]),
            "gmg_fgmres16_vs_gmg_pcg_speedup": round(secs / max(fgmres_secs, 1e-9), 3), "nn_pcg_s": None}
+    out["cht"] = cht_leg(device)
     if weights is None or not os.path.exists(weights):
         out["note"] = "no trained U-Net weights found; NN-PCG not run"
         return out
@@ -121,6 +122,40 @@ def thermal_leg(n, device, seed=0, weights=None):
                 # comes from GCR, not from the network
                 "nn_speedup_source": "network" if not ni["fallback"] else "GMG-GCR(6) fallback after the network stalled",
                 "gmg_gcr6_vs_gmg_pcg_speedup": round(secs / max(gcr_secs, 1e-9), 3)})
+    return out
+
+
+def cht_leg(device):
+    """The coupled heat-sink conduction-advection system at full size (pin-fin plate,
+    192x192x384, upwind advection, varying rho*c_p: non-symmetric): flexible CG (the
+    reference's PCG, reported whether or not it converges), GMG-GCR(6) and
+    GMG-FGMRES(16) to an FP64 relative residual of 1e-8."""
+    import math
+
+    import torch
+
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, fgmres_fp64, gcr_fp64
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+
+    c = geometry.cht_problem(scale=1.0, device=device)
+    mg = NLevelGeometricMGSolver(device, torch.float32, c["ax"], c["sign"], c["s"], omega=0.8, n_pre=2, n_post=2,
+                                 n_coarse=20)
+    mg.build_if_needed(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], c["dt"])
+    out = {"problem": "pin-fin heat sink 192x192x384, k_s/k_f = 1000, rho*c_p ratio 2000, channel flow 0.04 along z "
+                      "(synthetic), dt = 1000", "face_form": mg._levels[0]["faces"] is not None}
+    for name, fn, kw, cap in (("gmg_pcg", fcg_fp64, {}, 100), ("gmg_gcr6", gcr_fp64, {"m": 6}, 400),
+                              ("gmg_fgmres16", fgmres_fp64, {"m": 16}, 400)):
+        fn(mg, c["b"], tol=1e-8, max_it=2, **kw)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, info = fn(mg, c["b"], tol=1e-8, max_it=cap, **kw)
+        torch.cuda.synchronize()
+        res = info["res_rel"]
+        out.update({f"{name}_s": round(time.perf_counter() - t0, 4), f"{name}_iterations": info["it"],
+                    f"{name}_converged": bool(info["converged"]),
+                    f"{name}_final_rel_residual_fp64": res if math.isfinite(res) else None})
+        del info
     return out
 
 

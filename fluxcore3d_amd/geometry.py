@@ -143,6 +143,36 @@ def conduction_problem(n, seed=0, ratio=None, dt=1.0e4, device="cuda"):
                 s=1.0, solid=solid)
 
 
+def cht_problem(scale=1.0, dt=1.0e3, u_max=0.04, device="cuda"):
+    """Conduction-advection system A x = b of the coupled heat-sink configuration
+    (pin_fin_plate at 192x192x384 times ``scale``): k_s/k_f = 1000, rho*c_p of the
+    solid 2000 times the fluid's, a channel flow along z in the fluid (parabolic in
+    x and y, peak ``u_max``, zero in solids and walls; it stands in for the flow
+    solution), Dirichlet x = 0 on the inlet plane (k = 0), Neumann elsewhere, and a
+    unit heat input spread over the bottom layer of the base plate, times dt. The
+    upwind advection makes A non-symmetric."""
+    import torch
+
+    nx, ny, nz = int(192 * scale), int(192 * scale), int(384 * scale)
+    phi, solid, solid_flow = pin_fin_plate(nx, ny, nz, plate=max(2, int(16 * scale)), radius=4.0 * scale,
+                                           height=int(120 * scale))
+    S = torch.as_tensor(solid, device=device)
+    open_ = ~torch.as_tensor(solid_flow, device=device)
+    k = torch.where(S, torch.tensor(1000.0, device=device), torch.tensor(1.0, device=device)).float()
+    rcp = torch.where(S, torch.tensor(2000.0, device=device), torch.tensor(1.0, device=device)).float()
+    x = torch.linspace(0.0, 1.0, nx, device=device)[:, None, None]
+    y = torch.linspace(0.0, 1.0, ny, device=device)[None, :, None]
+    w = (u_max * (4 * x * (1 - x)) * (4 * y * (1 - y)) * open_.float()).expand(nx, ny, nz).contiguous()
+    zero = torch.zeros_like(k)
+    mask = torch.ones_like(k)
+    mask[:, :, 0] = 0.0
+    src = torch.zeros_like(k)
+    src[:, 0, :] = S[:, 0, :].float()
+    b = dt * src / src.sum() * mask
+    return dict(k=k, rcp=rcp, u=zero, v=zero, w=w, mask=mask, b=b, dt=dt, ax=2, sign=1, s=1.0, solid=S,
+                shape=(nx, ny, nz))
+
+
 def gyroid(n=384, period=48.0, level=0.3, shape=None):
     """Gyroid solid g(x) < level, g = sin x cos y + sin y cos z + sin z cos x,
     x = 2 pi i / period. phi = (g - level) / |grad g| clipped to +-4 (a first-order
