@@ -52,9 +52,10 @@ def parse():
 
 def thermal_leg(n, device, seed=0, weights=None):
     """GMG-preconditioned flexible CG (FP32 V-cycle, FP64 outer loop, the reference's
-    PCG), GMG-preconditioned flexible GCR(6) and restarted flexible GMRES(16), and
-    flexible GCR(6) with the U-Net preconditioner (V-cycle fallback), all to an FP64
-    relative residual of 1e-8 on the same 256^3 blob conduction problem."""
+    PCG), GMG-preconditioned flexible GCR(6), restarted flexible GMRES(16) and the
+    reference's BiCGSTAB, and flexible GCR(6) with the U-Net preconditioner (V-cycle
+    fallback), all to a relative residual of 1e-8 on the same 256^3 blob conduction
+    problem."""
     import torch
 
     from fluxcore3d_amd import geometry
@@ -92,6 +93,16 @@ def thermal_leg(n, device, seed=0, weights=None):
     torch.cuda.synchronize()
     fgmres_secs = time.perf_counter() - t0
     del fws
+    from fluxcore3d_amd.solver.mg_pcg import BiCGSTABWorkspace, bicgstab_mg
+
+    bws = BiCGSTABWorkspace(c["k"].shape, device)
+    bicgstab_mg(mg, c["b"], tol=1e-8, max_it=2, ws=bws)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, bi = bicgstab_mg(mg, c["b"], tol=1e-8, max_it=500, ws=bws)
+    torch.cuda.synchronize()
+    bicg_secs = time.perf_counter() - t0
+    del bws
     out = {"problem": f"{n}^3 smooth-noise blobs (seed {seed}), k_s/k_f = {c['ratio']:.3g}, dt = {c['dt']:g}",
            "levels": [list(lv["shape"]) for lv in mg._levels], "gmg_pcg_s": round(secs, 4),
            "gmg_pcg_iterations": info["it"], "final_rel_residual_fp64": info["res_rel"],
@@ -99,7 +110,12 @@ def thermal_leg(n, device, seed=0, weights=None):
            "gmg_gcr6_iterations": gi["it"], "gmg_gcr6_final_rel_residual_fp64": gi["res_rel"],
            "gmg_fgmres16_s": round(fgmres_secs, 4), "gmg_fgmres16_iterations": fi["it"],
            "gmg_fgmres16_final_rel_residual_fp64": fi["res_rel"], "gmg_fgmres16_converged": bool(fi["converged"]),
-           "gmg_fgmres16_vs_gmg_pcg_speedup": round(secs / max(fgmres_secs, 1e-9), 3), "nn_pcg_s": None}
+           "gmg_fgmres16_vs_gmg_pcg_speedup": round(secs / max(fgmres_secs, 1e-9), 3),
+           # the reference's patch_mg_pcg picks BiCGSTAB here (0.1 max(diagL) above 1.5); its tolerance is on the
+           # recursive residual, as in the reference
+           "gmg_bicgstab_s": round(bicg_secs, 4), "gmg_bicgstab_iterations": bi["it"],
+           "gmg_bicgstab_converged": bool(bi["converged"]), "gmg_bicgstab_final_rel_residual": bi["res_rel"],
+           "nn_pcg_s": None}
     out["cht"] = cht_leg(device)
     if weights is None or not os.path.exists(weights):
         out["note"] = "no trained U-Net weights found; NN-PCG not run"
@@ -128,14 +144,14 @@ def thermal_leg(n, device, seed=0, weights=None):
 def cht_leg(device):
     """The coupled heat-sink conduction-advection system at full size (pin-fin plate,
     192x192x384, upwind advection, varying rho*c_p: non-symmetric): flexible CG (the
-    reference's PCG, reported whether or not it converges), GMG-GCR(6) and
-    GMG-FGMRES(16) to an FP64 relative residual of 1e-8."""
+    reference's PCG, reported whether or not it converges), the reference's BiCGSTAB
+    branch, GMG-GCR(6) and GMG-FGMRES(16) to an FP64 relative residual of 1e-8."""
     import math
 
     import torch
 
     from fluxcore3d_amd import geometry
-    from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, fgmres_fp64, gcr_fp64
+    from fluxcore3d_amd.solver.mg_pcg import bicgstab_mg, fcg_fp64, fgmres_fp64, gcr_fp64
     from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
 
     c = geometry.cht_problem(scale=1.0, device=device)
@@ -144,8 +160,8 @@ def cht_leg(device):
     mg.build_if_needed(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], c["dt"])
     out = {"problem": "pin-fin heat sink 192x192x384, k_s/k_f = 1000, rho*c_p ratio 2000, channel flow 0.04 along z "
                       "(synthetic), dt = 1000", "face_form": mg._levels[0]["faces"] is not None}
-    for name, fn, kw, cap in (("gmg_pcg", fcg_fp64, {}, 100), ("gmg_gcr6", gcr_fp64, {"m": 6}, 400),
-                              ("gmg_fgmres16", fgmres_fp64, {"m": 16}, 400)):
+    for name, fn, kw, cap in (("gmg_pcg", fcg_fp64, {}, 100), ("gmg_bicgstab", bicgstab_mg, {}, 400),
+                              ("gmg_gcr6", gcr_fp64, {"m": 6}, 400), ("gmg_fgmres16", fgmres_fp64, {"m": 16}, 400)):
         fn(mg, c["b"], tol=1e-8, max_it=2, **kw)  # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()

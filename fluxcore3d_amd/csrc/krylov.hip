@@ -397,6 +397,108 @@ __global__ void arnoldi_givens_kernel(const double* __restrict__ scal, double* _
   st[G_EST] = fabs(g[j + 1]) / bnorm;
 }
 
+// ---- MG-preconditioned BiCGSTAB (mg_pcg.py:103-175) ------------------------------
+// FP64 vectors, FP32 preconditioned vectors; the scalars (alpha, beta, omega) are
+// formed on the host from the dot products, in the reference's order.
+
+// out = (z + dt L z) [* mask] in FP64 from the FP32 z (the reference's Lo of the
+// preconditioned vector; it leaves the masked rows in)
+template <bool FACE>
+__global__ __launch_bounds__(kBlock) void krylov_apply_kernel(const float* __restrict__ z, Stencil s,
+                                                              double* __restrict__ out, int use_mask) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= s.nx * s.ny * s.nz) return;
+  const unsigned nynz = (unsigned)s.ny * (unsigned)s.nz, u = (unsigned)idx;
+  const int i = (int)(u / nynz);
+  const unsigned rem = u - (unsigned)i * nynz;
+  const int j = (int)(rem / (unsigned)s.nz), k = (int)(rem - (unsigned)j * (unsigned)s.nz);
+  const int oE = (i < s.nx - 1 ? 1 : -1) * (int)nynz, oW = (i > 0 ? -1 : lo_ghost(s, 0)) * (int)nynz;
+  const int oN = (j < s.ny - 1 ? 1 : -1) * s.nz, oS = (j > 0 ? -1 : lo_ghost(s, 1)) * s.nz;
+  const int oU = k < s.nz - 1 ? 1 : -1, oD = k > 0 ? -1 : lo_ghost(s, 2);
+  float cE, cW, cN, cS, cU, cD, cL;
+  if constexpr (FACE) {
+    cE = s.fx[face_hi(idx, i, s.nx, (int)nynz)];
+    cW = s.fx[face_lo(idx, i, s.nx, (int)nynz, lo_ghost(s, 0))];
+    cN = s.fy[face_hi(idx, j, s.ny, s.nz)];
+    cS = s.fy[face_lo(idx, j, s.ny, s.nz, lo_ghost(s, 1))];
+    cU = s.fz[face_hi(idx, k, s.nz, 1)];
+    cD = s.fz[face_lo(idx, k, s.nz, 1, lo_ghost(s, 2))];
+    cL = ((((cE + cW) + cN) + cS) + cU) + cD;
+  } else {
+    cE = s.aE[idx]; cW = s.aW[idx]; cN = s.aN[idx]; cS = s.aS[idx]; cU = s.aU[idx]; cD = s.aD[idx];
+    cL = s.diagL[idx];
+  }
+  const double pc = (double)z[idx];
+  const double L = (double)cL * pc - (double)cE * (double)z[idx + oE] - (double)cW * (double)z[idx + oW] -
+                   (double)cN * (double)z[idx + oN] - (double)cS * (double)z[idx + oS] -
+                   (double)cU * (double)z[idx + oU] - (double)cD * (double)z[idx + oD];
+  const double lo = pc + s.dt * L;
+  out[idx] = use_mask ? lo * (double)s.mask[idx] : lo;
+}
+
+struct DotPairs {
+  const double* a[3];
+  const double* b[3];
+};
+
+// partials of a_i . b_i, i < NP
+template <int NP>
+__global__ __launch_bounds__(kBlock) void krylov_dots_kernel(DotPairs d, int64_t n, double* __restrict__ partial) {
+  double acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+#pragma unroll 2
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[q] += d.a[q][idx] * d.b[q][idx];
+  }
+  block_columns<NP>(acc, partial);
+}
+
+// p = (beta p + r) - (beta omega) v;  p32 = (float) (p * mask)
+__global__ __launch_bounds__(kBlock) void bicgstab_dir_kernel(double* __restrict__ p, const double* __restrict__ r,
+                                                              const double* __restrict__ v,
+                                                              const float* __restrict__ mask, double beta,
+                                                              double bw, float* __restrict__ p32, int64_t n) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const double pn = (beta * p[idx] + r[idx]) - bw * v[idx];
+  p[idx] = pn;
+  p32[idx] = (float)(pn * (double)mask[idx]);
+}
+
+// s = (r - alpha v) * mask;  s32 = (float) s
+__global__ __launch_bounds__(kBlock) void bicgstab_s_kernel(double* __restrict__ sv, const double* __restrict__ r,
+                                                            const double* __restrict__ v,
+                                                            const float* __restrict__ mask, double alpha,
+                                                            float* __restrict__ s32, int64_t n) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const double sn = (r[idx] - alpha * v[idx]) * (double)mask[idx];
+  sv[idx] = sn;
+  s32[idx] = (float)sn;
+}
+
+// x = ((x + alpha p_hat) + omega s_hat) * mask;  r = (s - omega t) * mask (omega_r = 0 and
+// s_hat = nullptr: x = (x + alpha p_hat) * mask only, r untouched)
+__global__ __launch_bounds__(kBlock) void bicgstab_x_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                            const double* __restrict__ sv,
+                                                            const double* __restrict__ t,
+                                                            const float* __restrict__ p_hat,
+                                                            const float* __restrict__ s_hat,
+                                                            const float* __restrict__ mask, double alpha,
+                                                            double omega, int64_t n) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n) return;
+  const double m = (double)mask[idx];
+  double xv = x[idx] + alpha * (double)p_hat[idx];
+  if (s_hat) {
+    xv = xv + omega * (double)s_hat[idx];
+    r[idx] = (sv[idx] - omega * t[idx]) * m;
+  }
+  x[idx] = xv * m;
+}
+
 // launches the instantiation with NV == nv (1 <= nv <= kMaxBasis)
 template <int NV>
 struct ArnoldiLaunch {
@@ -584,6 +686,79 @@ int fc_fgmres_givens(const double* scal, double* state, int j, double bnorm, hip
   FC_REQUIRE(bnorm > 0.0, "fgmres_givens: bnorm must be positive");
   hipLaunchKernelGGL(arnoldi_givens_kernel, dim3(1), dim3(64), 0, stream, scal, state, j, bnorm);
   return check_launch("fgmres_givens");
+}
+
+
+// ---- BiCGSTAB building blocks ----
+int fc_krylov_apply(const float* z32, const fc_gcr_stencil* st, double* out, int use_mask, hipStream_t stream) {
+  FC_REQUIRE(z32 && st && out, "krylov_apply: null pointer argument");
+  FC_REQUIRE(st->nx >= 2 && st->ny >= 2 && st->nz >= 2, "krylov_apply: every extent must be >= 2");
+  Stencil s{st->aE, st->aW, st->aN, st->aS, st->aU, st->aD, st->diagL, st->mask, st->nx, st->ny, st->nz,
+            st->ax, st->out_idx, st->dt, st->fx, st->fy, st->fz};
+  const int faces = (s.fx != nullptr) + (s.fy != nullptr) + (s.fz != nullptr);
+  FC_REQUIRE(faces == 0 || faces == 3, "krylov_apply: face coefficients must be all set or all null");
+  FC_REQUIRE((!use_mask || s.mask) && (faces == 3 || (s.aE && s.aW && s.aN && s.aS && s.aU && s.aD && s.diagL)),
+             "krylov_apply: null coefficient");
+  const int64_t n = (int64_t)s.nx * s.ny * s.nz;
+  FC_REQUIRE(n < ((int64_t)1 << 31), "krylov_apply: %lld cells exceed the 32-bit index range", (long long)n);
+  hipLaunchKernelGGL(faces ? krylov_apply_kernel<true> : krylov_apply_kernel<false>, dim3(grid_for(n, kBlock)),
+                     dim3(kBlock), 0, stream, z32, s, out, use_mask);
+  return check_launch("krylov_apply");
+}
+
+// scal[slot0 + i] = a[i] . b[i] for i < npairs (1..3), FP64, fixed summation order;
+// partial holds fc_gcr_partials_count(n) doubles
+int fc_krylov_dots(const double* const* a, const double* const* b, int npairs, int64_t n, double* partial,
+                   double* scal, int slot0, hipStream_t stream) {
+  FC_REQUIRE(a && b && partial && scal, "krylov_dots: null pointer argument");
+  FC_REQUIRE(npairs >= 1 && npairs <= 3, "krylov_dots: 1 <= npairs <= 3");
+  FC_REQUIRE(slot0 >= 0 && slot0 + npairs <= K_SLOTS, "krylov_dots: slots out of range");
+  DotPairs d{};
+  for (int q = 0; q < npairs; ++q) {
+    FC_REQUIRE(a[q] && b[q], "krylov_dots: null vector in pair %d", q);
+    d.a[q] = a[q];
+    d.b[q] = b[q];
+  }
+  const unsigned g = grid_of(n);
+  if (npairs == 1)
+    hipLaunchKernelGGL(krylov_dots_kernel<1>, dim3(g), dim3(kBlock), 0, stream, d, n, partial);
+  else if (npairs == 2)
+    hipLaunchKernelGGL(krylov_dots_kernel<2>, dim3(g), dim3(kBlock), 0, stream, d, n, partial);
+  else
+    hipLaunchKernelGGL(krylov_dots_kernel<3>, dim3(g), dim3(kBlock), 0, stream, d, n, partial);
+  if (int rc = check_launch("krylov_dots")) return rc;
+  hipLaunchKernelGGL(fold_columns_kernel, dim3(npairs), dim3(kBlock), 0, stream, (const double*)partial, (int)g, scal,
+                     slot0);
+  return check_launch("krylov_dots/fold");
+}
+
+int fc_bicgstab_direction(double* p, const double* r, const double* v, const float* mask, double beta, double omega,
+                          float* p32, int64_t n, hipStream_t stream) {
+  FC_REQUIRE(p && r && v && mask && p32, "bicgstab_direction: null pointer argument");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(bicgstab_dir_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, p, r, v, mask, beta,
+                     beta * omega, p32, n);
+  return check_launch("bicgstab_direction");
+}
+
+int fc_bicgstab_s(double* s, const double* r, const double* v, const float* mask, double alpha, float* s32, int64_t n,
+                  hipStream_t stream) {
+  FC_REQUIRE(s && r && v && mask && s32, "bicgstab_s: null pointer argument");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(bicgstab_s_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, s, r, v, mask, alpha,
+                     s32, n);
+  return check_launch("bicgstab_s");
+}
+
+int fc_bicgstab_update(double* x, double* r, const double* s, const double* t, const float* p_hat,
+                       const float* s_hat, const float* mask, double alpha, double omega, int64_t n,
+                       hipStream_t stream) {
+  FC_REQUIRE(x && p_hat && mask, "bicgstab_update: null pointer argument");
+  FC_REQUIRE(!s_hat || (r && s && t), "bicgstab_update: r, s and t are needed with s_hat");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(bicgstab_x_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x, r, s, t, p_hat, s_hat,
+                     mask, alpha, omega, n);
+  return check_launch("bicgstab_update");
 }
 
 }  // extern "C"

@@ -11,6 +11,10 @@ preconditioner is symmetric. Convergence is judged on the true FP64 residual
 
 All scalars (p.Ap, p.r, alpha, beta, r.r) stay in device memory; the host
 reads one FP64 value per iteration for the stopping test.
+
+Also here: ``bicgstab_mg`` (the reference's BiCGSTAB branch, mg_pcg.py:103-175) and
+``patch_mg_pcg``'s switch to it; flexible GCR(m) (``gcr_fp64``) and restarted flexible
+GMRES(m) (``fgmres_fp64``), minimal-residual methods for the non-symmetric systems.
 """
 from __future__ import annotations
 
@@ -114,11 +118,15 @@ def pcg_mg(mg_solver, b, x0=None, tol: float = 1e-3, max_it: int = 15):
 
 
 def patch_mg_pcg(mg_solver, pe_threshold: float = 1.5):
-    """Route mg_solver.solve through pcg_mg (mg_pcg.py:250-275; the reference's
-    BiCGSTAB branch is not provided here)."""
+    """Route mg_solver.solve through PCG or BiCGSTAB (mg_pcg.py:250-275): with the
+    reference's heuristic pe = 0.1 * max(diagL) of the finest level, above
+    ``pe_threshold`` bicgstab_mg, else pcg_mg."""
     import types
 
     def _solve_pcg(self, b, x0=None, max_cycles=15, tol=1e-3):
+        pe = float(self._levels[0]["diagL"].max()) * 0.1
+        if pe > pe_threshold:
+            return bicgstab_mg(self, b, x0, tol=tol, max_it=max_cycles)
         return pcg_mg(self, b, x0, tol=tol, max_it=max_cycles)
 
     mg_solver.solve = types.MethodType(_solve_pcg, mg_solver)
@@ -379,3 +387,116 @@ def fgmres_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, t
             break
     x = fw.x.view(shape).to(torch.float32) * mask
     return x, {"converged": converged, "it": it, "res_rel": hist[-1], "hist": hist, "x64": fw.x.view(shape)}
+
+
+# ------------------------------------------------- MG-preconditioned BiCGSTAB
+_lib.register({
+    "fc_krylov_apply": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.I32, _lib.P]),
+    "fc_krylov_dots": (_lib.I32, [_lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P, _lib.P, _lib.I32, _lib.P]),
+    "fc_bicgstab_direction": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.F64, _lib.F64, _lib.P, _lib.I64,
+                                         _lib.P]),
+    "fc_bicgstab_s": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.F64, _lib.P, _lib.I64, _lib.P]),
+    "fc_bicgstab_update": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.F64, _lib.F64,
+                                      _lib.I64, _lib.P]),
+})
+
+
+class BiCGSTABWorkspace:
+    """FP64 vectors x, r, r_hat, p, v, s, t and b; FP32 copies of p and s for the
+    V-cycle and its outputs p_hat, s_hat; reduction scratch."""
+
+    def __init__(self, shape, device):
+        n = int(torch.Size(shape).numel())
+        f64 = dict(device=device, dtype=torch.float64)
+        f32 = dict(device=device, dtype=torch.float32)
+        self.n = n
+        for name in ("x", "r", "r_hat", "p", "v", "s", "t", "b"):
+            setattr(self, name, torch.zeros(n, **f64))
+        for name in ("p32", "s32", "p_hat", "s_hat"):
+            setattr(self, name, torch.zeros(n, **f32))
+        self.partial = torch.empty(int(_lib.call("fc_gcr_partials_count", n)), **f64)
+        self.scal = torch.zeros(int(_lib.call("fc_gcr_scalar_slots")), **f64)
+        self.fcg = FCGWorkspace(shape, device)  # the initial residual (true-residual kernel)
+
+
+def bicgstab_mg(mg_solver, b, x0=None, tol: float = 1e-3, max_it: int = 15,
+                ws: Optional[BiCGSTABWorkspace] = None) -> Tuple[torch.Tensor, dict]:
+    """bicgstab_mg (mg_pcg.py:103-175): BiCGSTAB with one V-cycle as preconditioner, for
+    operators with advection. Same steps, guards and masking as the reference (v = A
+    p_hat and t = A s_hat keep the masked rows; s, r and x are masked; the tolerance is
+    on the recursive ||r|| / ||b||), with FP64 vectors and dot products on the GPU and
+    the V-cycle in FP32. Returns (x, {"converged", "it", "res_rel"}) like the reference,
+    plus "hist" and "x64"."""
+    lv0 = mg_solver._levels[0]
+    shape = lv0["shape"]
+    nx, ny, nz = shape
+    dev = mg_solver.device
+    if ws is None:
+        ws = BiCGSTABWorkspace(shape, dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    mask = lv0["mask_f"]
+    dt = float(mg_solver._dt_cache)
+    faces = lv0.get("faces")
+    st = GCRStencil(*[lv0[k].data_ptr() for k in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL")], mask.data_ptr(),
+                    dt, nx, ny, nz, int(mg_solver.ax), int(lv0["out_idx"]),
+                    *([f.data_ptr() for f in faces] if faces is not None else [None] * 3))
+    n = ws.n
+    P2 = ctypes.c_void_p * 3
+
+    def dots(*pairs):
+        a = P2(*[p[0].data_ptr() for p in pairs])
+        c = P2(*[p[1].data_ptr() for p in pairs])
+        _lib.call("fc_krylov_dots", a, c, len(pairs), n, ws.partial.data_ptr(), ws.scal.data_ptr(), 0, stream)
+        return ws.scal[:len(pairs)].tolist()
+
+    def precond(v32, out):
+        mg_solver.precondition(v32, out=out, masked_input=True)
+
+    ws.b.copy_(b.reshape(-1).to(torch.float64))
+    if x0 is None:
+        ws.x.zero_()
+    else:
+        ws.x.copy_(x0.reshape(-1).to(torch.float64))
+        ws.x.mul_(mask.reshape(-1))
+    fw = ws.fcg
+    _lib.call("fc_mg_true_residual", ws.x.data_ptr(), ws.b.data_ptr(), ws.r.data_ptr(), fw.r32.data_ptr(),
+              mask.data_ptr(), lv0["coef"], dt, nx, ny, nz, mg_solver.ax, lv0["out_idx"], fw.partial.data_ptr(),
+              fw.scal.data_ptr(), stream)
+    ws.r_hat.copy_(ws.r)
+    bnorm = dots((ws.b, ws.b))[0] ** 0.5 + 1e-30
+    rho = alpha = omega = 1.0
+    ws.v.zero_()
+    ws.p.zero_()
+    converged, it, hist = False, 0, []
+    for it in range(max_it):
+        rr, rho_new = dots((ws.r, ws.r), (ws.r_hat, ws.r))
+        res = rr ** 0.5 / bnorm
+        hist.append(res)
+        if res < tol:
+            converged = True
+            break
+        if abs(rho_new) < 1e-30:
+            break
+        beta = (rho_new / (rho + 1e-30)) * (alpha / (omega + 1e-30))
+        _lib.call("fc_bicgstab_direction", ws.p.data_ptr(), ws.r.data_ptr(), ws.v.data_ptr(), mask.data_ptr(), beta,
+                  omega, ws.p32.data_ptr(), n, stream)
+        precond(ws.p32, ws.p_hat)
+        _lib.call("fc_krylov_apply", ws.p_hat.data_ptr(), ctypes.byref(st), ws.v.data_ptr(), 0, stream)
+        alpha = rho_new / (dots((ws.r_hat, ws.v))[0] + 1e-30)
+        _lib.call("fc_bicgstab_s", ws.s.data_ptr(), ws.r.data_ptr(), ws.v.data_ptr(), mask.data_ptr(), alpha,
+                  ws.s32.data_ptr(), n, stream)
+        if dots((ws.s, ws.s))[0] ** 0.5 / bnorm < tol:
+            _lib.call("fc_bicgstab_update", ws.x.data_ptr(), None, None, None, ws.p_hat.data_ptr(), None,
+                      mask.data_ptr(), alpha, 0.0, n, stream)
+            converged = True
+            break
+        precond(ws.s32, ws.s_hat)
+        _lib.call("fc_krylov_apply", ws.s_hat.data_ptr(), ctypes.byref(st), ws.t.data_ptr(), 0, stream)
+        ts, tt = dots((ws.t, ws.s), (ws.t, ws.t))
+        omega = ts / (tt + 1e-30)
+        _lib.call("fc_bicgstab_update", ws.x.data_ptr(), ws.r.data_ptr(), ws.s.data_ptr(), ws.t.data_ptr(),
+                  ws.p_hat.data_ptr(), ws.s_hat.data_ptr(), mask.data_ptr(), alpha, omega, n, stream)
+        rho = rho_new
+    res_final = dots((ws.r, ws.r))[0] ** 0.5 / bnorm
+    x = ws.x.view(shape).to(torch.float32) * mask
+    return x, {"converged": converged, "it": it + 1, "res_rel": res_final, "hist": hist, "x64": ws.x.view(shape)}

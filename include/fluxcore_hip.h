@@ -299,6 +299,26 @@ int fc_fgmres_givens_slots(void);
 int fc_fgmres_givens_layout(int *r, int *g, int *est, int *ld);
 int fc_fgmres_givens(const double *scal, double *state, int j, double bnorm, hipStream_t stream);
 
+/* ---- MG-preconditioned BiCGSTAB (bicgstab_mg, mg_pcg.py:103-175; its branch of patch_mg_pcg :250-275) ----
+ * FP64 vectors, FP32 preconditioned vectors; the scalars alpha, beta, omega are formed by the
+ * host from the dot products, in the reference's order. */
+/* out = (z + dt L z) [* mask if use_mask], FP64 from FP32 z (the reference's _Mop keeps masked rows) */
+int fc_krylov_apply(const float *z32, const fc_gcr_stencil *stencil, double *out, int use_mask, hipStream_t stream);
+/* scal[slot0 + i] = a[i] . b[i], i < npairs (1..3); partial: fc_gcr_partials_count(n) doubles */
+int fc_krylov_dots(const double *const *a, const double *const *b, int npairs, int64_t n, double *partial,
+                   double *scal, int slot0, hipStream_t stream);
+/* p = (beta p + r) - beta omega v; p32 = (float) (p * mask) */
+int fc_bicgstab_direction(double *p, const double *r, const double *v, const float *mask, double beta, double omega,
+                          float *p32, int64_t n, hipStream_t stream);
+/* s = (r - alpha v) * mask; s32 = (float) s */
+int fc_bicgstab_s(double *s, const double *r, const double *v, const float *mask, double alpha, float *s32, int64_t n,
+                  hipStream_t stream);
+/* x = ((x + alpha p_hat) + omega s_hat) * mask and r = (s - omega t) * mask; with s_hat = NULL
+ * only x = (x + alpha p_hat) * mask */
+int fc_bicgstab_update(double *x, double *r, const double *s, const double *t, const float *p_hat,
+                       const float *s_hat, const float *mask, double alpha, double omega, int64_t n,
+                       hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

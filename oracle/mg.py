@@ -214,6 +214,58 @@ def pcg_mg(h: Hierarchy, b, x0=None, tol=1e-3, max_it=15):
     return x, {"converged": converged, "it": it + 1, "res_rel": hist[-1] if hist else None, "hist": hist}
 
 
+def bicgstab_mg(h: Hierarchy, b, x0=None, tol=1e-3, max_it=15, outer=None):
+    """bicgstab_mg (mg_pcg.py:103-175): BiCGSTAB preconditioned by one V-cycle, with the
+    reference's masking (v = A p_hat and t = A s_hat keep the masked rows; s, r and x
+    are masked) and its 1e-30 guards. ``outer`` is the dtype of the vectors and dot
+    products (default: the hierarchy's, as in the reference); the V-cycle sees its
+    input cast to the hierarchy dtype."""
+    hd = h.levels[0]["co"][0].dtype
+    od = np.dtype(outer) if outer is not None else hd
+    t_ = od.type
+    m = h.levels[0]["mask"].astype(od)
+    co = tuple(c.astype(od) for c in h.levels[0]["co"])
+    A = lambda v: mop(v, co, h.dt, h.ax, h.levels[0]["out_idx"])  # noqa: E731  (unmasked Lo, as _Mop)
+    P = lambda v: h.precond(v.astype(hd)).astype(od)  # noqa: E731
+    dot = lambda a, c: float(np.dot(a.ravel(), c.ravel()))  # noqa: E731
+    b = b.astype(od)
+    x = np.zeros_like(b) if x0 is None else x0.astype(od) * m
+    r = (b - A(x)) * m
+    r_hat = r.copy()
+    bnorm = dot(b, b) ** 0.5 + 1e-30
+    rho = alpha = omega = 1.0
+    v = np.zeros_like(b)
+    p = np.zeros_like(b)
+    converged, it, hist = False, 0, []
+    for it in range(max_it):
+        res = dot(r, r) ** 0.5 / bnorm
+        hist.append(res)
+        if res < tol:
+            converged = True
+            break
+        rho_new = dot(r_hat, r)
+        if abs(rho_new) < 1e-30:
+            break
+        beta = (rho_new / (rho + 1e-30)) * (alpha / (omega + 1e-30))
+        p = (t_(beta) * p + r) - t_(beta * omega) * v
+        p_hat = P(p)
+        v = A(p_hat)
+        alpha = rho_new / (dot(r_hat, v) + 1e-30)
+        s = (r - t_(alpha) * v) * m
+        if dot(s, s) ** 0.5 / bnorm < tol:
+            x = x + t_(alpha) * p_hat
+            converged = True
+            break
+        s_hat = P(s)
+        t = A(s_hat)
+        omega = dot(t, s) / (dot(t, t) + 1e-30)
+        x = ((x + t_(alpha) * p_hat) + t_(omega) * s_hat) * m
+        r = (s - t_(omega) * t) * m
+        rho = rho_new
+    x = x * m
+    return x, {"converged": converged, "it": it + 1, "res_rel": dot(r, r) ** 0.5 / bnorm, "hist": hist}
+
+
 def fcg_mg_fp64(h: Hierarchy, b, tol=1e-8, max_it=100, truncation=1):
     """Flexible CG (Notay 2000, truncation 1) with FP64 vectors and residual;
     the preconditioner (one FP32 V-cycle of ``h``) sees the residual cast to

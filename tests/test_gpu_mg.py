@@ -385,3 +385,41 @@ def test_cht_advection_system_gcr_and_fgmres():
         x64 = x.cpu().numpy()
         r = (b - mg.mop(x64, co64, c["dt"], c["ax"], out_idx)) * msk
         assert np.linalg.norm(r) / np.linalg.norm(b) < 1.01e-8
+
+
+def test_bicgstab_against_oracle_and_patch_routing():
+    """MG-preconditioned BiCGSTAB (mg_pcg.py:103-175) on the non-symmetric heat-sink
+    system (1/4 scale): its first residuals equal the oracle's BiCGSTAB (FP64 vectors,
+    the FP32 hierarchy) to 1e-4, it reaches an FP64 residual of 1e-8 (checked with the
+    oracle's operator) with the GCR solution, and patch_mg_pcg routes solve() by the
+    reference's heuristic pe = 0.1 max(diagL) (mg_pcg.py:250-275)."""
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.solver.mg_pcg import bicgstab_mg, gcr_fp64, patch_mg_pcg, pcg_mg
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+    c = geometry.cht_problem(scale=0.25, device=DEV)
+    s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, c["ax"], c["sign"], c["s"], omega=0.8)
+    s.build_if_needed(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], c["dt"])
+    f32 = lambda t: t.cpu().numpy().astype(np.float32)  # noqa: E731
+    h = mg.Hierarchy(f32(c["k"]), f32(c["rcp"]), f32(c["u"]), f32(c["v"]), f32(c["w"]), f32(c["mask"]) > 0,
+                     c["ax"], c["sign"], c["s"], omega=0.8)
+    h.set_dt(c["dt"])
+    b = f32(c["b"])
+    _, oi = mg.bicgstab_mg(h, b, tol=1e-30, max_it=4, outer=np.float64)
+    _, gi4 = bicgstab_mg(s, c["b"], tol=1e-30, max_it=4)
+    assert np.allclose(gi4["hist"], oi["hist"], rtol=1e-4, atol=0.0), (gi4["hist"], oi["hist"])
+    _, bi = bicgstab_mg(s, c["b"], tol=1e-8, max_it=300)
+    assert bi["converged"], bi["hist"][-3:]
+    xb = bi["x64"].clone()
+    _, gi = gcr_fp64(s, c["b"], tol=1e-8, max_it=300, m=6)
+    assert float((xb - gi["x64"]).norm() / gi["x64"].norm()) < 1e-6
+    co64 = tuple(a.astype(np.float64) for a in h.levels[0]["co"])
+    msk = c["mask"].cpu().numpy().astype(np.float64)
+    b64 = c["b"].cpu().numpy().astype(np.float64) * msk
+    r = (b64 - mg.mop(xb.cpu().numpy(), co64, c["dt"], c["ax"], h.levels[0]["out_idx"])) * msk
+    assert np.linalg.norm(r) / np.linalg.norm(b64) < 1.01e-8
+    pe = float(s._levels[0]["diagL"].max()) * 0.1
+    for thr, direct in ((pe * 0.5, bicgstab_mg), (pe * 2.0, pcg_mg)):
+        patch_mg_pcg(s, pe_threshold=thr)
+        xp, pinfo = s.solve(c["b"], max_cycles=6, tol=1e-30)
+        xd, dinfo = direct(s, c["b"], tol=1e-30, max_it=6)
+        assert torch.equal(xp, xd) and pinfo["res_rel"] == dinfo["res_rel"], direct.__name__
