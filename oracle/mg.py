@@ -266,6 +266,48 @@ def bicgstab_mg(h: Hierarchy, b, x0=None, tol=1e-3, max_it=15, outer=None):
     return x, {"converged": converged, "it": it + 1, "res_rel": dot(r, r) ** 0.5 / bnorm, "hist": hist}
 
 
+def reconstruct(T0, theta, ax, out_idx):
+    """thermal_reconstruct (thermal_kernels.cu:61-114): T = T0 + theta, then every face
+    cell copies its inner neighbour (Neumann), faces taken in the priority x, y, z; the
+    low face of the flow axis is the inlet (Dirichlet) unless out_idx == 0."""
+    val = T0 + theta
+    out = val.copy()
+    n = val.shape
+    done = np.zeros(n, bool)
+    for a in range(3):
+        hi = [slice(None)] * 3
+        hi[a] = n[a] - 1
+        src_hi = [slice(None)] * 3
+        src_hi[a] = n[a] - 2
+        sel = ~done[tuple(hi)]
+        out[tuple(hi)][sel] = val[tuple(src_hi)][sel]
+        done[tuple(hi)] = True
+        if a != ax or out_idx == 0:
+            lo = [slice(None)] * 3
+            lo[a] = 0
+            src_lo = [slice(None)] * 3
+            src_lo[a] = 1
+            sel = ~done[tuple(lo)]
+            out[tuple(lo)][sel] = val[tuple(src_lo)][sel]
+            done[tuple(lo)] = True
+    return out
+
+
+def thermal_step(h: Hierarchy, T_old, T0, src, mask, dt, tol=1e-3, max_cycles=20):
+    """One implicit pseudo-time step of the fused thermal driver (thermal_cuda_ext.py:
+    55-201): LT0 = L(T0) (L_apply_fine), rhs = mask ((T_old + dt src) - (T0 + dt LT0)),
+    theta0 = mask (T_old - T0), theta from the V-cycle iteration (solve), theta *= mask,
+    T = reconstruct(T0 + theta). ``h`` must have set_dt(dt)."""
+    d = T_old.dtype.type
+    lv = h.levels[0]
+    LT0 = (h.mop(T0, 0) - T0) / d(h.dt)
+    rhs = mask * ((T_old + d(dt) * src) - (T0 + d(dt) * LT0))
+    theta0 = mask * (T_old - T0)
+    theta, hist = h.solve(rhs, x0=theta0, max_cycles=max_cycles, tol=tol)
+    theta = theta * mask
+    return reconstruct(T0, theta, h.ax, lv["out_idx"]), hist
+
+
 def fcg_mg_fp64(h: Hierarchy, b, tol=1e-8, max_it=100, truncation=1):
     """Flexible CG (Notay 2000, truncation 1) with FP64 vectors and residual;
     the preconditioner (one FP32 V-cycle of ``h``) sees the residual cast to

@@ -143,3 +143,25 @@ def test_bicgstab_reaches_direct_solution_on_advection_system():
     h32.set_dt(0.3)
     x32, info32 = mg.bicgstab_mg(h32, b.astype(np.float32), tol=1e-5, max_it=200)
     assert info32["converged"] and np.abs(x32 - x_direct).max() / np.abs(x_direct).max() < 1e-3
+
+
+@pytest.mark.parametrize("ax,out_idx", [(0, 8), (0, 0), (2, 4)])
+def test_reconstruct_matches_face_loop(ax, out_idx):
+    """The vectorised reconstruction against a per-cell loop over the faces in
+    priority x, y, z (thermal_kernels.cu:61-114)."""
+    rng = np.random.default_rng(1)
+    n = (9, 7, 5)
+    T0, th = (rng.standard_normal(n).astype(np.float32) for _ in range(2))
+    val = T0 + th
+    want = val.copy()
+    for c in np.ndindex(*n):
+        src = list(c)
+        for a in range(3):
+            if c[a] == n[a] - 1:
+                src[a] = c[a] - 1
+                break
+            if c[a] == 0 and (a != ax or out_idx == 0):
+                src[a] = 1
+                break
+        want[c] = val[tuple(src)]
+    np.testing.assert_array_equal(mg.reconstruct(T0, th, ax, out_idx), want)
