@@ -590,17 +590,15 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__
   }
 }
 
-// Ap = (p + dt L p) * mask in FP64 (FP32 coefficients); partials of p.Ap, p.r
+// Ap = (p + dt L p) * mask in FP64 (FP32 coefficients), one cell per thread; block
+// partials of p.Ap, p.r (folded in two stages: fold_runs_kernel, then fold_kernel)
 template <bool FACE>
 __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restrict__ p, double* __restrict__ Ap,
                                                            const double* __restrict__ r, const float* __restrict__ mask,
                                                            Coef c, Level g, double dt, double* __restrict__ partial) {
-  const int n = g.nx * g.ny * g.nz;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  // unrolled so that several cells' gathers are in flight; each thread still adds its
-  // cells in index order
-#pragma unroll 4
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
+  if (idx < g.nx * g.ny * g.nz) {
     int i, j, k;
     unflat(g, idx, i, j, k);
     const Nbr o = neighbours(g, i, j, k);
@@ -611,10 +609,34 @@ __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restr
                      (double)q.d * p[idx + o.d];
     const double a = (pc + dt * L) * (double)mask[idx];
     Ap[idx] = a;
-    v[0] += pc * a;
-    v[1] += r ? pc * r[idx] : 0.0;
+    v[0] = pc * a;
+    v[1] = r ? pc * r[idx] : 0.0;
   }
   block_partials<2>(v, partial);
+}
+
+// First stage of a two-stage fold: block b adds the partials of blocks
+// [b * run, (b + 1) * run) (stride 3, in block order) into out[b * 3 + a].
+constexpr int kFoldRuns = 256;
+__global__ __launch_bounds__(kBlock) void fold_runs_kernel(const double* __restrict__ partial, int nparts, int run,
+                                                           int nv, double* __restrict__ out) {
+  const int b0 = blockIdx.x * run, b1 = min(b0 + run, nparts);
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int q = b0 + (int)threadIdx.x; q < b1; q += kBlock)
+    for (int a = 0; a < nv; ++a) v[a] += partial[(int64_t)q * 3 + a];
+  __shared__ double sh[kBlock / 64][3];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int a = 0; a < 3; ++a) {
+    const double s = wave_sum(v[a]);
+    if (lane == 0) sh[wid][a] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int a = 0; a < nv; ++a) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) s += sh[w][a];
+      out[(int64_t)blockIdx.x * 3 + a] = s;
+    }
 }
 
 // x += alpha p; r -= alpha Ap; r32 = (float) r; partial of r.r
@@ -892,7 +914,12 @@ static unsigned reduce_grid(int64_t n) {
   return g < kMaxPartials ? g : kMaxPartials;
 }
 
-int fc_mg_partials_count(int64_t n) { return (int)reduce_grid(n); }
+// block partials (3 doubles each) for the reductions: the capped grids, or one block per
+// 256 cells (apply_A64) followed by the runs of its two-stage fold
+int fc_mg_partials_count(int64_t n) {
+  const int64_t g = grid_for(n, kBlock) + kFoldRuns;
+  return (int)(g > (int64_t)reduce_grid(n) ? g : (int64_t)reduce_grid(n));
+}
 
 int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* mask, const fc_mg_coef* coef,
                     double dt, int nx, int ny, int nz, int ax, int out_idx, double* partial, double* scal,
@@ -902,11 +929,16 @@ int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* m
   FC_REQUIRE(p && Ap && mask && coef && partial && scal, "apply_A64: null pointer argument");
   FC_COEF_FORM(coef, "apply_A64");
   const int64_t n = (int64_t)nx * ny * nz;
-  const unsigned gsz = reduce_grid(n);
-  hipLaunchKernelGGL(form ? apply_A64_kernel<true> : apply_A64_kernel<false>, dim3(gsz), dim3(kBlock), 0, stream, p, Ap, r, mask, to_coef(coef), g, dt,
-                     partial);
+  const int nb = (int)grid_for(n, kBlock);
+  hipLaunchKernelGGL(form ? apply_A64_kernel<true> : apply_A64_kernel<false>, dim3(nb), dim3(kBlock), 0, stream, p,
+                     Ap, r, mask, to_coef(coef), g, dt, partial);
   if (int rc = check_launch("apply_A64")) return rc;
-  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 2, scal, (int)S_PAP,
+  const int runs = nb < kFoldRuns ? nb : kFoldRuns, run = (nb + runs - 1) / runs;
+  double* second = partial + (int64_t)nb * 3;  // within fc_mg_partials_count(n) * 3 doubles
+  hipLaunchKernelGGL(fold_runs_kernel, dim3(runs), dim3(kBlock), 0, stream, (const double*)partial, nb, run, 2,
+                     second);
+  if (int rc = check_launch("apply_A64/fold runs")) return rc;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, (const double*)second, runs, 2, scal, (int)S_PAP,
                      r ? 1 : 0);
   return check_launch("apply_A64/fold");
 }
