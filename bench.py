@@ -1,14 +1,14 @@
 """Benchmark of the fused D3Q19 step on one MI355X.
 
-Workload (BASELINE config "flow-only throughput"): 256^3 lattice, FP32
+Workload (flow-only throughput): 256^3 lattice, FP32
 populations, MRT (d'Humieres moment basis, s_nu = 1/0.55, all other
 non-conserved rates 1.2), Bouzidi links on a random sequential-addition pack
 of non-overlapping spheres (radii uniform 4-10, porosity 0.6, seed 0), fully
 periodic, Guo body force 1e-6 along x, macros written every step.
 
 Timing: W warm-up steps, then K steps bracketed by synchronisation and HIP
-events on the stream the kernels run on. value = fluid-node MLUPS (solid
-nodes move no populations); roofline.achieved = 152 B x fluid nodes per
+events on the stream the kernels run on. value = fluid-node MLUPS (only
+fluid-node updates are counted); roofline.achieved = 152 B x fluid nodes per
 launch / average launch time. cpu_baseline: the FP32 oracle (C + OpenMP
 restatement of the reference step) on the host cores over a few steps of the
 same workload.
