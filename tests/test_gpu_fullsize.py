@@ -1,8 +1,12 @@
-"""The configurations at their full sizes, through properties that hold at any size
-(the CPU oracle is too slow there; it pins the same code paths at small sizes in
-test_gpu_lbm.py, test_gpu_physics.py and test_gpu_mg.py):
+"""The configurations at their full sizes. The CPU oracle pins the same code paths at
+small sizes in test_gpu_lbm.py, test_gpu_physics.py and test_gpu_mg.py; at full size it
+checks the link table and two flow steps, and properties that hold at any size check
+the rest:
 
-* 256^3 sphere pack (porosity 0.6, MRT, reference Bouzidi, periodic): the fused
+* 256^3 sphere pack (porosity 0.6, MRT, reference Bouzidi, periodic): links, q and the
+  Bouzidi weights bit for bit against the oracle's builder (q and the weights on the
+  links whose samples stay inside the box), and two fused steps (one of
+  each parity) within 1e-5 of the FP32 oracle's reference step; the fused
   in-place step equals the reference's per-kernel sequence (collide -> stream ->
   IBB -> reset, lbm_cuda_ext.py:445-517, run with this library's reference-semantics
   kernels) after an odd and an even number of steps, <= 1e-5 of the field maxima;
@@ -102,6 +106,56 @@ def test_pack_256_fused_step_equals_reference_sequence(steps):
         assert _rel(macro[a], mref[a]) < 1e-5, a
     fluid = st == 0
     assert _rel(f_aa[:, fluid], f[:, fluid]) < 1e-5
+
+
+def test_pack_256_links_q_and_two_steps_match_oracle():
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.pre_post.sdf_tracing import bfl_q_and_weights
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    from oracle import lbm as olbm
+    n = 256
+    N = n ** 3
+    phi, solid, _ = geometry.rsa_spheres(n, porosity=0.6, rmin=4.0, rmax=10.0, seed=0)
+    L = _links(phi, solid)
+    q, _, _, _ = bfl_q_and_weights(torch.as_tensor(phi, device=DEV), (0.0,) * 3, 1.0, L["fluid_flat"], L["dir_i"],
+                                   periodic=(True, True, True))
+    links = {k: v.cpu().numpy() for k, v in L.items()}
+    ref = olbm.ibb_from_sdf(solid, phi, periodic=(True, True, True))
+    assert ref["fluid_flat"].size > 10 ** 6
+    for k in ("fluid_flat", "dir_i", "ff_flat", "ff_valid"):
+        np.testing.assert_array_equal(links[k], ref[k], err_msg=k)
+    # q: the oracle clamps SDF samples at the faces as the reference does, the periodic
+    # build wraps them; on links whose samples stay inside the box (node 1..n-3 on
+    # every axis) both read the same eight values and must agree bit for bit
+    ff = links["fluid_flat"]
+    inner = np.ones(ff.size, bool)
+    for c in (ff // (n * n), (ff // n) % n, ff % n):
+        inner &= (c >= 1) & (c <= n - 3)
+    assert inner.mean() > 0.9
+    links["q"] = q.cpu().numpy()
+    for k in ("q", "w1", "w2", "q_ge_half"):
+        np.testing.assert_array_equal(links[k][inner], ref[k][inner], err_msg=k)
+    # the bench's MRT rates, from a perturbed equilibrium, no body force (the oracle has none)
+    om = 1.0 / 0.55
+    S = np.full(19, 1.2, np.float32)
+    S[[0, 3, 5, 7]] = 0.0
+    S[[9, 11, 13, 14, 15]] = om
+    f0 = _perturbed(N, 3)
+    eng = AAEngine(n, n, n, solid, periodic=(1, 1, 1), collision="mrt", omega=om, omega_minus=1.2, S=S,
+                   links=links, device=DEV)
+    eng.load(f0.view(19, n, n, n))
+    macro = [torch.empty(N, device=DEV) for _ in range(4)]
+    f = f0.cpu().numpy()
+    del f0
+    for _ in range(2):
+        eng.step(*macro)
+        f, *mw = olbm.step(f, solid, n, n, n, (1, 1, 1), "mrt", om, 1.2, S, 0.0, links)
+    assert _rel(macro[0].cpu(), torch.from_numpy(mw[0])) < 1e-5
+    speed = float(np.sqrt(mw[1] ** 2 + mw[2] ** 2 + mw[3] ** 2).max())
+    assert speed > 1e-3
+    for a in (1, 2, 3):
+        assert float(np.abs(macro[a].cpu().numpy() - mw[a]).max()) / speed < 1e-5, a
+    assert _rel(eng.materialize().view(19, N).cpu(), torch.from_numpy(f)) < 1e-5
 
 
 def test_pack_256_rest_state_stays_at_rest():
