@@ -14,6 +14,8 @@ the rest:
 * 384^3 gyroid (8 periods of 48 voxels, MRT, Bouzidi): FP16 population storage stays
   within 1e-3 of the largest speed from FP32 storage after 100 steps, the bound the
   48^3 unit cell meets against the FP64 oracle.
+* 256^3 blob conduction problem (the bench's): the hierarchy and one V-cycle against
+  the oracle's, and the FGMRES(16) solution's FP64 residual with the oracle's operator.
 * 512^3 blob conduction problem: GMG-preconditioned flexible GCR(6), flexible CG and
   restarted flexible GMRES(16) reach an FP64 true relative residual of 1e-8 and agree
   on the solution.
@@ -199,6 +201,42 @@ def test_gyroid_384_fp16_storage_tracks_fp32():
     umax = float(out["fp32"].abs().max())
     assert umax > 1e-3
     assert float((out["fp16"] - out["fp32"]).abs().max()) / umax < 1e-3
+
+
+def test_blobs_256_hierarchy_vcycle_and_fgmres_against_oracle():
+    """The bench's 256^3 blob problem (k_s/k_f = 353): level shapes, masks and inverse
+    diagonals, and one V-cycle (face-form coefficients on every level) against the
+    oracle's FP32 hierarchy; the GMG-FGMRES(16) solution's FP64 residual measured with
+    the oracle's coefficients and operator."""
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.solver.mg_pcg import fgmres_fp64
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+    from oracle import mg as omg
+    c = geometry.conduction_problem(256, seed=0, device=DEV)
+    s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, c["ax"], c["sign"], c["s"], omega=0.8, n_pre=2,
+                                n_post=2, n_coarse=20)
+    s.build_if_needed(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], c["dt"])
+    assert all(lv["faces"] is not None for lv in s._levels)
+    host = {k: c[k].cpu().numpy() for k in ("k", "rcp", "u", "v", "w", "mask", "b")}
+    h = omg.Hierarchy(host["k"], host["rcp"], host["u"], host["v"], host["w"], host["mask"], c["ax"], c["sign"],
+                      c["s"], omega=0.8)
+    h.set_dt(c["dt"])
+    assert [lv["shape"] for lv in s._levels] == [lv["shape"] for lv in h.levels]
+    for a, b in zip(s._levels, h.levels):
+        np.testing.assert_array_equal(a["mask_f"].cpu().numpy(), b["mask"])
+        assert _rel(a["inv_diagM"].cpu(), torch.from_numpy(b["invD"])) < 1e-6
+    r = host["b"].astype(np.float32)
+    z = s.precondition(torch.as_tensor(r, device=DEV).view(-1)).cpu()
+    assert _rel(z.reshape(-1), torch.from_numpy(h.precond(r)).reshape(-1)) < 1e-5
+    _, info = fgmres_fp64(s, c["b"], tol=1e-8, max_it=300, m=16)
+    assert info["converged"], info["hist"][-3:]
+    shape = s._levels[0]["shape"]
+    co64 = tuple(a.astype(np.float64) for a in h.levels[0]["co"])
+    msk = host["mask"].astype(np.float64)
+    bb = host["b"].astype(np.float64) * msk
+    x64 = info["x64"].reshape(shape).cpu().numpy()
+    res = (bb - omg.mop(x64, co64, c["dt"], c["ax"], h.levels[0]["out_idx"])) * msk
+    assert np.linalg.norm(res) / np.linalg.norm(bb) < 1.01e-8
 
 
 def test_blobs_512_gcr_fcg_and_fgmres_reach_tolerance_and_agree():
