@@ -86,7 +86,8 @@ def thermal_leg(n, device, seed=0, weights=None):
     from fluxcore3d_amd.solver.mg_pcg import FGMRESWorkspace, fgmres_fp64
 
     fws = FGMRESWorkspace(c["k"].shape, device, 16)
-    fgmres_fp64(mg, c["b"], tol=1e-8, max_it=3, m=16, ws=fws)  # warm-up
+    # warm-up: one full cycle launches the Arnoldi kernels of every basis count once
+    fgmres_fp64(mg, c["b"], tol=1e-8, max_it=16, m=16, ws=fws)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     xf, fi = fgmres_fp64(mg, c["b"], tol=1e-8, max_it=500, m=16, ws=fws)
@@ -162,7 +163,7 @@ def cht_leg(device):
                       "(synthetic), dt = 1000", "face_form": mg._levels[0]["faces"] is not None}
     for name, fn, kw, cap in (("gmg_pcg", fcg_fp64, {}, 100), ("gmg_bicgstab", bicgstab_mg, {}, 400),
                               ("gmg_gcr6", gcr_fp64, {"m": 6}, 400), ("gmg_fgmres16", fgmres_fp64, {"m": 16}, 400)):
-        fn(mg, c["b"], tol=1e-8, max_it=2, **kw)  # warm-up
+        fn(mg, c["b"], tol=1e-8, max_it=max(2, kw.get("m", 0)), **kw)  # warm-up: a full cycle of every kernel
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _, info = fn(mg, c["b"], tol=1e-8, max_it=cap, **kw)

@@ -153,6 +153,46 @@ __device__ __forceinline__ int face_lo(int idx, int pos, int n, int stride, int 
   return pos > 0 ? idx - stride : (ghost_mirrors ? idx : idx + (n - 1) * stride);
 }
 
+// FP64 sum over a 64-wide wavefront with DPP lane moves (quad permutes, half-row
+// and row mirrors), the four 16-lane rows combined through readlane. Every lane
+// of the wavefront must be active; all lanes return the sum.
+__device__ __forceinline__ double dpp_move(double v, int ctrl_sel) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  switch (ctrl_sel) {
+    case 0:  // quad_perm [1,0,3,2]
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+      break;
+    case 1:  // quad_perm [2,3,0,1]
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+      break;
+    case 2:  // row_half_mirror
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xF, 0xF, false);
+      break;
+    default:  // row_mirror
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xF, 0xF, false);
+      break;
+  }
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_move(v, 0);
+  v += dpp_move(v, 1);
+  v += dpp_move(v, 2);
+  v += dpp_move(v, 3);
+  // every lane of each 16-lane row now holds that row's sum
+  const int lo0 = __builtin_amdgcn_readlane(__double2loint(v), 0), hi0 = __builtin_amdgcn_readlane(__double2hiint(v), 0);
+  const int lo1 = __builtin_amdgcn_readlane(__double2loint(v), 16), hi1 = __builtin_amdgcn_readlane(__double2hiint(v), 16);
+  const int lo2 = __builtin_amdgcn_readlane(__double2loint(v), 32), hi2 = __builtin_amdgcn_readlane(__double2hiint(v), 32);
+  const int lo3 = __builtin_amdgcn_readlane(__double2loint(v), 48), hi3 = __builtin_amdgcn_readlane(__double2hiint(v), 48);
+  return (__hiloint2double(hi0, lo0) + __hiloint2double(hi1, lo1)) +
+         (__hiloint2double(hi2, lo2) + __hiloint2double(hi3, lo3));
+}
+
 // Round-up grid helper.
 inline unsigned grid_for(int64_t n, int block) {
   return (unsigned)((n + block - 1) / block);

@@ -42,19 +42,13 @@ __device__ __forceinline__ int lo_ghost(const Stencil& s, int axis) {
   return (s.ax != axis || s.out_idx == 0) ? 1 : 0;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  // wavefront sum by butterfly exchanges
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
 template <int NV>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partial, int stride) {
   __shared__ double sh[kBlock / 64][NV];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int a = 0; a < NV; ++a) {
-    const double s = wave_sum(v[a]);
+    const double s = wave_sum_dpp(v[a]);
     if (lane == 0) sh[wid][a] = s;
   }
   __syncthreads();
@@ -93,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__
   __shared__ double sh[kBlock / 64][kMaxDirs];
 #pragma unroll
   for (int a = 0; a < kMaxDirs; ++a) {
-    const double w = wave_sum(v[a]);
+    const double w = wave_sum_dpp(v[a]);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][a] = w;
   }
   __syncthreads();
@@ -233,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void fold_columns_kernel(const double* __re
       if (b0 + u * kBlock < nblocks) v += t[u];
   }
   __shared__ double sh[kBlock / 64];
-  v = wave_sum(v);
+  v = wave_sum_dpp(v);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -250,7 +244,7 @@ __device__ __forceinline__ void block_columns(const double* acc, double* __restr
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int a = 0; a < NV; ++a) {
-    const double s = wave_sum(acc[a]);
+    const double s = wave_sum_dpp(acc[a]);
     if (lane == 0) sh[wid][a] = s;
   }
   __syncthreads();

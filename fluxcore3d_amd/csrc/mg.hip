@@ -481,45 +481,6 @@ __global__ __launch_bounds__(kBlock) void restrict_field_kernel(const float* __r
 #pragma clang fp contract(on)
 
 // ---- FP64 outer loop ---------------------------------------------------------
-// Sum over a 64-wide wavefront with DPP lane moves (quad permutes, half-row and
-// row mirrors), rows combined through readlane.
-__device__ __forceinline__ double dpp_move(double v, int ctrl_sel) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  switch (ctrl_sel) {
-    case 0:  // quad_perm [1,0,3,2]
-      lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
-      hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
-      break;
-    case 1:  // quad_perm [2,3,0,1]
-      lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
-      hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
-      break;
-    case 2:  // row_half_mirror
-      lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xF, 0xF, false);
-      hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xF, 0xF, false);
-      break;
-    default:  // row_mirror
-      lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xF, 0xF, false);
-      hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xF, 0xF, false);
-      break;
-  }
-  return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-  v += dpp_move(v, 0);
-  v += dpp_move(v, 1);
-  v += dpp_move(v, 2);
-  v += dpp_move(v, 3);
-  // every lane of each 16-lane row now holds that row's sum
-  const int lo0 = __builtin_amdgcn_readlane(__double2loint(v), 0), hi0 = __builtin_amdgcn_readlane(__double2hiint(v), 0);
-  const int lo1 = __builtin_amdgcn_readlane(__double2loint(v), 16), hi1 = __builtin_amdgcn_readlane(__double2hiint(v), 16);
-  const int lo2 = __builtin_amdgcn_readlane(__double2loint(v), 32), hi2 = __builtin_amdgcn_readlane(__double2hiint(v), 32);
-  const int lo3 = __builtin_amdgcn_readlane(__double2loint(v), 48), hi3 = __builtin_amdgcn_readlane(__double2hiint(v), 48);
-  return (__hiloint2double(hi0, lo0) + __hiloint2double(hi1, lo1)) +
-         (__hiloint2double(hi2, lo2) + __hiloint2double(hi3, lo3));
-}
-
 // block sum of up to 3 values into partial[blockIdx][0..2]
 template <int NV>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partial) {
@@ -527,7 +488,7 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restri
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int a = 0; a < NV; ++a) {
-    const double s = wave_sum(v[a]);
+    const double s = wave_sum_dpp(v[a]);
     if (lane == 0) sh[wid][a] = s;
   }
   __syncthreads();
@@ -572,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__
   __shared__ double sh[kBlock / 64][3];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int a = 0; a < 3; ++a) {
-    const double s = wave_sum(v[a]);
+    const double s = wave_sum_dpp(v[a]);
     if (lane == 0) sh[wid][a] = s;
   }
   __syncthreads();
@@ -627,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void fold_runs_kernel(const double* __restr
   __shared__ double sh[kBlock / 64][3];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int a = 0; a < 3; ++a) {
-    const double s = wave_sum(v[a]);
+    const double s = wave_sum_dpp(v[a]);
     if (lane == 0) sh[wid][a] = s;
   }
   __syncthreads();
