@@ -21,7 +21,7 @@ constexpr unsigned kMaxBlocks = 8192;
 constexpr int kMaxDirs = 8;
 
 // scalar slots (per call family)
-enum { K_RW = 0, K_WW = 1, K_RR = 2, K_ALPHA = 3, K_DOT0 = 8, K_WWHIST0 = 16, K_SLOTS = 24 };
+enum { K_RW = 0, K_WW = 1, K_RR = 2, K_ALPHA = 3, K_STOP = 7, K_DOT0 = 8, K_WWHIST0 = 16, K_SLOTS = 24 };
 
 struct Stencil {
   const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL;
@@ -187,8 +187,15 @@ __global__ __launch_bounds__(kBlock) void update_kernel(double* __restrict__ x, 
                                                         double* __restrict__ partial) {
   const double ww = scal[K_WW];
   const double alpha = ww > 0.0 ? scal[K_RW] / ww : 0.0;
+  // a stopped solve (scal[K_STOP] set by fc_solver_stop_check) keeps x and r
+  const bool stop = scal[K_STOP] != 0.0;
   double v[1] = {0.0};
   for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    if (stop) {
+      const double rn = r[idx];
+      v[0] += rn * rn;
+      continue;
+    }
     x[idx] += alpha * p[idx];
     const double rn = r[idx] - alpha * w[idx];
     r[idx] = rn;
@@ -540,6 +547,7 @@ struct fc_gcr_stencil {
 };
 
 int fc_gcr_scalar_slots(void) { return K_SLOTS; }
+int fc_gcr_stop_slot(void) { return K_STOP; }
 int fc_gcr_max_directions(void) { return kMaxDirs; }
 int fc_gcr_partials_count(int64_t n) { return (int)grid_of(n) * kMaxDirs; }
 

@@ -504,7 +504,7 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restri
 
 // Device scalar slots used by the FCG kernels (FP64):
 // S_PAP/S_PR (A p, one fold), S_RR/S_BB (true residual, one fold), S_ZAP, alpha, beta.
-enum { S_PAP = 0, S_PR = 1, S_RR = 2, S_BB = 3, S_ZAP = 4, S_ALPHA = 5, S_BETA = 6, S_N = 8 };
+enum { S_PAP = 0, S_PR = 1, S_RR = 2, S_BB = 3, S_ZAP = 4, S_ALPHA = 5, S_BETA = 6, S_STOP = 7, S_N = 8 };
 
 // fold the partials of `nparts` blocks into scal[slot0..slot0+nv-1] (one block)
 __global__ __launch_bounds__(kBlock) void fold_kernel(const double* __restrict__ partial, int nparts, int nv,
@@ -608,7 +608,14 @@ __global__ __launch_bounds__(kBlock) void update_xr_kernel(double* __restrict__ 
                                                            double* __restrict__ partial) {
   double v[1] = {0.0};
   const double alpha = scal[S_ALPHA];
+  // a stopped solve (scal[S_STOP] set) keeps x and r and re-forms the same r.r
+  const bool stop = scal[S_STOP] != 0.0;
   for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    if (stop) {
+      const double rn = r[idx];
+      v[0] += rn * rn;
+      continue;
+    }
     x[idx] += alpha * p[idx];
     const double rn = r[idx] - alpha * Ap[idx];
     r[idx] = rn;
@@ -627,12 +634,28 @@ __global__ __launch_bounds__(kBlock) void dot_z_kernel(const float* __restrict__
   block_partials<1>(v, partial);
 }
 
-// p = z - beta p_old (in place on p)
+// p = z - beta p_old (in place on p); p is kept once the solve has stopped
 __global__ __launch_bounds__(kBlock) void new_dir_kernel(double* __restrict__ p, const float* __restrict__ z,
                                                          const double* __restrict__ scal, int64_t n, int first) {
   const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n) return;
+  if (idx >= n || scal[S_STOP] != 0.0) return;
   p[idx] = first ? (double)z[idx] : (double)z[idx] - scal[S_BETA] * p[idx];
+}
+
+// Stopping test on the device: scal[stop_slot] becomes 1 once sqrt(scal[rr_slot]) / bnorm
+// is below tol or not finite, and stays 1 until cleared. The host can then queue the
+// next iteration before it reads this one's residual: the kernels of that iteration
+// which would change x, r or the search directions see the flag and keep them.
+__global__ void stop_check_kernel(double* __restrict__ scal, int rr_slot, int stop_slot, double bnorm, double tol,
+                                  int clear) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (clear) {
+    scal[stop_slot] = 0.0;
+    return;
+  }
+  if (scal[stop_slot] != 0.0) return;
+  const double rec = sqrt(scal[rr_slot]) / bnorm;
+  scal[stop_slot] = (!(rec < INFINITY) || rec < tol) ? 1.0 : 0.0;
 }
 
 // true residual r = (b - A x) * mask in FP64, r32 copy, partials of r.r
@@ -913,6 +936,17 @@ int fc_mg_update_xr(double* x, double* r, float* r32, const double* p, const dou
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, (int)S_RR, 0);
   return check_launch("update_xr/fold");
 }
+
+int fc_solver_stop_check(double* scal, int rr_slot, int stop_slot, double bnorm, double tol, int clear,
+                         hipStream_t stream) {
+  FC_REQUIRE(scal, "stop_check: null scalar array");
+  FC_REQUIRE(rr_slot >= 0 && stop_slot >= 0 && rr_slot != stop_slot, "stop_check: bad slots");
+  FC_REQUIRE(bnorm > 0.0, "stop_check: ||b|| must be positive");
+  hipLaunchKernelGGL(stop_check_kernel, dim3(1), dim3(64), 0, stream, scal, rr_slot, stop_slot, bnorm, tol, clear);
+  return check_launch("stop_check");
+}
+
+int fc_mg_stop_slot(void) { return S_STOP; }
 
 int fc_mg_new_direction(double* p, const float* z, const double* Ap_old, int64_t n, int first, double* partial,
                         double* scal, hipStream_t stream) {

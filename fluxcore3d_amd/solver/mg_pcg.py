@@ -9,8 +9,11 @@ plain CG can stall before 1e-8, while flexible CG is CG whenever the
 preconditioner is symmetric. Convergence is judged on the true FP64 residual
 ||b - A x|| / ||b|| (checked whenever the recursive residual drops below tol).
 
-All scalars (p.Ap, p.r, alpha, beta, r.r) stay in device memory; the host
-reads one FP64 value per iteration for the stopping test.
+All scalars (p.Ap, p.r, alpha, beta, r.r) stay in device memory. The stopping
+test runs on the device (fc_solver_stop_check) and the host reads its result one
+iteration late from pinned memory, so it queues iteration k+1 while the GPU still
+runs iteration k; once the test has stopped the solve, the kernels of the queued
+iteration leave x, r and the directions unchanged (FCG and GCR).
 
 Also here: ``bicgstab_mg`` (the reference's BiCGSTAB branch, mg_pcg.py:103-175) and
 ``patch_mg_pcg``'s switch to it; flexible GCR(m) (``gcr_fp64``) and restarted flexible
@@ -27,6 +30,94 @@ from .. import _lib
 from . import mg_cuda_ext  # noqa: F401
 
 S_PAP, S_PR, S_RR, S_BB = 0, 1, 2, 3
+
+_lib.register({
+    "fc_solver_stop_check": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.F64, _lib.F64, _lib.I32, _lib.P]),
+    "fc_mg_stop_slot": (_lib.I32, []),
+    "fc_gcr_stop_slot": (_lib.I32, []),
+}, unchecked={"fc_mg_stop_slot", "fc_gcr_stop_slot"})
+
+
+class _Lookahead:
+    """Host side of the device stopping test (fc_solver_stop_check): after each queued
+    iteration its r.r and stop flag (scalar slots rr_slot..stop_slot) are copied to
+    pinned memory behind its kernels, and the host reads them one iteration later, so
+    it never waits for the GPU to drain before it queues the next iteration."""
+
+    def __init__(self, scal, rr_slot, stop_slot, bnorm, tol, stream):
+        self.scal, self.rr_slot, self.stop_slot = scal, rr_slot, stop_slot
+        self.bnorm, self.tol, self.stream = bnorm, tol, stream
+        self.host = torch.zeros((2, stop_slot - rr_slot + 1), dtype=torch.float64, pin_memory=True)
+        self.events = [torch.cuda.Event(), torch.cuda.Event()]
+
+    def _check(self, clear):
+        _lib.call("fc_solver_stop_check", self.scal.data_ptr(), self.rr_slot, self.stop_slot, self.bnorm, self.tol,
+                  clear, self.stream)
+
+    def clear(self):
+        self._check(1)
+
+    def push(self, j):
+        self._check(0)
+        self.host[j & 1].copy_(self.scal[self.rr_slot:self.stop_slot + 1], non_blocking=True)
+        self.events[j & 1].record()
+
+    def pull(self, j):
+        self.events[j & 1].synchronize()
+        return float(self.host[j & 1, 0]), float(self.host[j & 1, -1]) != 0.0
+
+
+# Grids from this many cells up queue each iteration before reading the previous one's
+# stopping test (at 128^3: FCG 34 ms instead of 41 ms, GCR(6) 23 ms instead of 25 ms;
+# equal at 256^3). Below it the host's launches take longer than the GPU's work, and
+# the test's extra launch and copy cost more than the wait they save.
+LOOKAHEAD_MIN_CELLS = 1 << 20
+
+
+def _use_lookahead(lookahead, n, on_iter, check_every):
+    if on_iter is not None or check_every != 1:
+        return False  # the callback and sparse checks need the loop that waits after every step
+    return n >= LOOKAHEAD_MIN_CELLS if lookahead is None else bool(lookahead)
+
+
+def _clear_stop(scal, rr_slot, stop_slot, stream):
+    _lib.call("fc_solver_stop_check", scal.data_ptr(), rr_slot, stop_slot, 1.0, 0.0, 1, stream)
+
+
+def _iterate_ahead(step, look, bnorm, tol, max_it, true_residual, hist):
+    """step(j) for j = 0, 1, ..., each queued one iteration ahead of its stopping test.
+    The history, the iteration count and the iterate equal those of a loop that tests
+    after every step: the step queued past the stopping point runs with the device
+    flag set and leaves x, r and the directions unchanged. Returns (iterations,
+    converged)."""
+    step(0)
+    look.push(0)
+    queued, j = 1, 0
+    while True:
+        if queued < max_it:
+            step(queued)
+            look.push(queued)
+            queued += 1
+        rr, stop = look.pull(j)
+        it = j + 1
+        rec = rr ** 0.5 / bnorm
+        if stop:
+            if not (rec < float("inf")):
+                hist.append(rec)
+                return it, False
+            rr_t, _ = true_residual()  # the FP64 true residual decides convergence
+            hist.append(rr_t ** 0.5 / bnorm)
+            if hist[-1] < tol:
+                return it, True
+            if it >= max_it:
+                return it, False
+            look.clear()
+            queued = it  # the step queued behind this one changed nothing: queue it again
+        else:
+            hist.append(rec)
+            if it >= max_it:
+                return it, False
+        j = it
 
 
 class FCGWorkspace:
@@ -47,12 +138,14 @@ class FCGWorkspace:
 
 def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol: float = 1e-8, max_it: int = 200,
              precond=None, ws: Optional[FCGWorkspace] = None, check_every: int = 1,
-             on_iter=None) -> Tuple[torch.Tensor, dict]:
+             on_iter=None, lookahead: Optional[bool] = None) -> Tuple[torch.Tensor, dict]:
     """Flexible CG on A x = b (A = I + dt L of mg_solver's finest level, masked rows
     excluded). ``precond(r32, z32)`` defaults to one V-cycle of mg_solver; it may
     read ||r||^2 from ``ws.scal[S_RR]`` (device FP64). ``on_iter(it, ws)`` is
     called before each preconditioner application (it = 0 for the initial
-    residual)."""
+    residual). ``lookahead`` (default: from LOOKAHEAD_MIN_CELLS cells up, never with
+    on_iter or check_every > 1) queues each iteration before the previous one's
+    stopping test is read, with the same result."""
     lv0 = mg_solver._levels[0]
     shape = lv0["shape"]
     nx, ny, nz = shape
@@ -79,21 +172,29 @@ def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
         s = ws.scal[S_RR:S_BB + 1].cpu()
         return float(s[0]), float(s[1])
 
-    rr, bb = true_residual()
-    bnorm = bb ** 0.5 + 1e-300
-    hist = [rr ** 0.5 / bnorm]
-    it = 0
-    converged = hist[-1] < tol
-    while not converged and it < max_it:
-        if on_iter is not None:
-            on_iter(it, ws)
+    def step(j):
         precond(ws.r32, ws.z32)
-        _lib.call("fc_mg_new_direction", ws.p.data_ptr(), ws.z32.data_ptr(), ws.Ap.data_ptr(), ws.n, int(it == 0),
+        _lib.call("fc_mg_new_direction", ws.p.data_ptr(), ws.z32.data_ptr(), ws.Ap.data_ptr(), ws.n, int(j == 0),
                   ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
         _lib.call("fc_mg_apply_A64", ws.p.data_ptr(), ws.Ap.data_ptr(), ws.r.data_ptr(), mask.data_ptr(),
                   lv0["coef"], dt, nx, ny, nz, ax, oi, ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
         _lib.call("fc_mg_update_xr", ws.x.data_ptr(), ws.r.data_ptr(), ws.r32.data_ptr(), ws.p.data_ptr(),
                   ws.Ap.data_ptr(), ws.n, ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
+
+    _clear_stop(ws.scal, S_RR, S_STOP, stream)  # the workspace may hold a finished solve's flag
+    rr, bb = true_residual()
+    bnorm = bb ** 0.5 + 1e-300
+    hist = [rr ** 0.5 / bnorm]
+    it = 0
+    converged = hist[-1] < tol
+    if not converged and max_it > 0 and _use_lookahead(lookahead, ws.n, on_iter, check_every):
+        look = _Lookahead(ws.scal, S_RR, S_STOP, bnorm, tol, stream)
+        it, converged = _iterate_ahead(step, look, bnorm, tol, max_it, true_residual, hist)
+        max_it = it  # done: the loop below tests after every step for on_iter / check_every
+    while not converged and it < max_it:
+        if on_iter is not None:
+            on_iter(it, ws)
+        step(it)
         it += 1
         if it % check_every and it < max_it:
             continue
@@ -144,6 +245,8 @@ _lib.register({
 
 K_RR = 2
 K_WWHIST0 = 16
+S_STOP = int(_lib.call("fc_mg_stop_slot"))
+K_STOP = int(_lib.call("fc_gcr_stop_slot"))
 
 
 class GCRStencil(ctypes.Structure):
@@ -171,11 +274,11 @@ class GCRWorkspace:
 
 def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol: float = 1e-8, max_it: int = 300,
              precond=None, m: int = 6, ws: Optional[GCRWorkspace] = None, on_iter=None,
-             check_every: int = 1) -> Tuple[torch.Tensor, dict]:
+             check_every: int = 1, lookahead: Optional[bool] = None) -> Tuple[torch.Tensor, dict]:
     """Flexible GCR(m) on A x = b with FP64 vectors and a variable preconditioner
     ``precond(r32, z32, ws)`` (default: one V-cycle). The residual norm never
     increases; convergence is judged on the FP64 true residual. ``on_iter(it,
-    rel_res)`` may raise to stop the iteration early."""
+    rel_res)`` may raise to stop the iteration early. ``lookahead`` as in fcg_fp64."""
     lv0 = mg_solver._levels[0]
     shape = lv0["shape"]
     nx, ny, nz = shape
@@ -206,15 +309,32 @@ def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
         s = fw.scal[S_RR:S_BB + 1].cpu()
         return float(s[0]), float(s[1])
 
+    W_ptrs = (ctypes.c_void_p * max(m, 1))()
+    P_ptrs = (ctypes.c_void_p * max(m, 1))()
+    slots = (ctypes.c_int * max(m, 1))()
+
+    def step(i):
+        precond(fw.r32, fw.z32, ws)
+        cur = i % (m + 1)
+        k = min(i, m)
+        for j in range(k):
+            src = (i - 1 - j) % (m + 1)
+            W_ptrs[j], P_ptrs[j], slots[j] = ws.W[src].data_ptr(), ws.P[src].data_ptr(), K_WWHIST0 + src
+        _lib.call("fc_gcr_step", fw.z32.data_ptr(), ctypes.byref(st), ws.P[cur].data_ptr(), ws.W[cur].data_ptr(),
+                  W_ptrs, P_ptrs, slots, k, K_WWHIST0 + cur, fw.x.data_ptr(), fw.r.data_ptr(), fw.r32.data_ptr(),
+                  ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
+
+    _clear_stop(ws.scal, K_RR, K_STOP, stream)  # the workspace may hold a finished solve's flag
     rr, bb = true_residual()
     bnorm = bb ** 0.5 + 1e-300
     hist = [rr ** 0.5 / bnorm]
     converged = hist[-1] < tol
     it = 0
-    W_ptrs = (ctypes.c_void_p * max(m, 1))()
-    P_ptrs = (ctypes.c_void_p * max(m, 1))()
-    slots = (ctypes.c_int * max(m, 1))()
     stopped = False
+    if not converged and max_it > 0 and _use_lookahead(lookahead, ws.n, on_iter, check_every):
+        look = _Lookahead(ws.scal, K_RR, K_STOP, bnorm, tol, stream)
+        it, converged = _iterate_ahead(step, look, bnorm, tol, max_it, true_residual, hist)
+        max_it = it  # done: the loop below tests after every step for on_iter / check_every
     while not converged and it < max_it:
         if on_iter is not None:
             try:
@@ -222,15 +342,7 @@ def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
             except StopIteration:
                 stopped = True
                 break
-        precond(fw.r32, fw.z32, ws)
-        cur = it % (m + 1)
-        k = min(it, m)
-        for j in range(k):
-            src = (it - 1 - j) % (m + 1)
-            W_ptrs[j], P_ptrs[j], slots[j] = ws.W[src].data_ptr(), ws.P[src].data_ptr(), K_WWHIST0 + src
-        _lib.call("fc_gcr_step", fw.z32.data_ptr(), ctypes.byref(st), ws.P[cur].data_ptr(), ws.W[cur].data_ptr(),
-                  W_ptrs, P_ptrs, slots, k, K_WWHIST0 + cur, fw.x.data_ptr(), fw.r.data_ptr(), fw.r32.data_ptr(),
-                  ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
+        step(it)
         it += 1
         if it % check_every and it < max_it:
             continue

@@ -212,6 +212,15 @@ int fc_mg_new_direction(double *p, const float *z, const double *Ap_old, int64_t
 int fc_mg_true_residual(const double *x, const double *b, double *r, float *r32, const float *mask,
                         const fc_mg_coef *coef, double dt, int nx, int ny, int nz, int ax, int out_idx,
                         double *partial, double *scal, hipStream_t stream);
+/* Stopping test on the device, in place of the host test after every iteration of
+ * pcg_mg (mg_pcg.py:69-72): scal[stop_slot] = 1 once sqrt(scal[rr_slot]) / bnorm < tol
+ * or is not finite, sticky until a call with clear != 0. With the flag set,
+ * fc_mg_new_direction and fc_mg_update_xr (stop slot fc_mg_stop_slot()) and
+ * fc_gcr_step (fc_gcr_stop_slot()) leave x, r and the directions unchanged, so the
+ * host may queue iteration k+1 before it reads iteration k's result. */
+int fc_solver_stop_check(double *scal, int rr_slot, int stop_slot, double bnorm, double tol, int clear,
+                         hipStream_t stream);
+int fc_mg_stop_slot(void);
 
 /* ---- thermal step kernels (thermal_kernels.cu:198-263) ---- */
 int fc_thermal_rhs(const float *T_old, const float *T0, const float *src, const float *LT0, const float *mask,
@@ -265,6 +274,7 @@ typedef struct fc_gcr_stencil {
   const float *fx, *fy, *fz; /* face form (fc_mg_build_faces) or all null */
 } fc_gcr_stencil;
 int fc_gcr_scalar_slots(void);
+int fc_gcr_stop_slot(void);
 int fc_gcr_max_directions(void);
 int fc_gcr_partials_count(int64_t n);
 /* p = z, w = A p, orthogonalise (w, p) against the m stored pairs (W_j, P_j) whose W_j.W_j sit in

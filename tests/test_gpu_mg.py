@@ -75,6 +75,36 @@ def test_vcycle_matches_oracle():
     assert rel(z, h.precond(r)) < 1e-5
 
 
+@pytest.mark.parametrize("solver", ["fcg", "gcr"])
+def test_queued_iteration_leaves_the_solve_unchanged(solver):
+    """FCG and GCR queue iteration k+1 before they read iteration k's stopping test
+    (which runs on the device). Iterate, iteration count and history equal, bit for
+    bit, those of the loop that waits for the test after every step (lookahead=False;
+    an on_iter callback also selects it): at convergence, at max_it, and at a tolerance near the
+    FP64 floor where the recursive residual can pass tol before the true one. The
+    workspace is reused between calls, so a finished solve's flag must not leak."""
+    from fluxcore3d_amd.solver.mg_pcg import FCGWorkspace, GCRWorkspace, fcg_fp64, gcr_fp64
+    c = thermal_cases.sphere_conduction(64, 32, 32, dt=10.0)
+    s, _ = _solver_pair(c)
+    shape = s._levels[0]["shape"]
+    if solver == "fcg":
+        fn, kw, ws = fcg_fp64, {}, FCGWorkspace(shape, torch.device(DEV))
+    else:
+        fn, kw, ws = gcr_fp64, {"m": 6}, GCRWorkspace(shape, torch.device(DEV), 6)
+    b = T(c["b"])
+    for tol, max_it in ((1e-8, 200), (1e-8, 5), (1e-14, 60)):
+        _, ahead = fn(s, b, tol=tol, max_it=max_it, ws=ws, lookahead=True, **kw)
+        x_ahead = ahead["x64"].clone()
+        _, wait = fn(s, b, tol=tol, max_it=max_it, ws=ws, lookahead=False, **kw)
+        assert ahead["it"] == wait["it"] and ahead["converged"] == wait["converged"], (tol, max_it)
+        assert ahead["hist"] == wait["hist"], (tol, max_it)
+        assert torch.equal(x_ahead, wait["x64"]), (tol, max_it)
+        if max_it == 200:
+            assert ahead["converged"] and ahead["res_rel"] < tol
+        if max_it == 5:
+            assert ahead["it"] == 5 and not ahead["converged"]
+
+
 def test_solve_cycles_match_oracle():
     c = thermal_cases.sphere_conduction(64, 32, 32, dt=10.0)
     s, h = _solver_pair(c)
