@@ -212,7 +212,9 @@ __global__ __launch_bounds__(kBlock) void update_kernel(double* __restrict__ x, 
 // with the partials of w . V_i, i <= j; then V_{j+1} = w - sum_i (w.V_i / V_i.V_i) V_i
 // in place (classical Gram-Schmidt), its FP32 copy and ||V_{j+1}||^2.
 constexpr int kMaxBasis = 24;
-constexpr unsigned kArnoldiBlocks = 8192;
+// grid cap of the Arnoldi kernels (at 256^3: 2 cells per thread; 8192 / 16384 / 32768 /
+// 65536 blocks gave 77.5 / 76.3 / 75.7 / 76.0 ms for the FGMRES(16) bench solve)
+constexpr unsigned kArnoldiBlocks = 32768;
 enum { F_DOT0 = 0, F_NV0 = kMaxBasis + 1, F_SLOTS = 2 * (kMaxBasis + 1) };
 
 struct Basis {
