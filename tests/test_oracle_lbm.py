@@ -312,3 +312,125 @@ def test_convective_mass_and_copy():
     ratio = g[unk][:, -1][:, fluid] / f[unk][:, -2][:, fluid]
     assert np.abs(ratio - ratio.flat[0]).max() < 1e-12  # same correction factor everywhere
     np.testing.assert_array_equal(g[:, -1, 0, 0], f[:, -1, 0, 0])  # solid node untouched
+
+
+# --- Smagorinsky MRT (lbm_kernels.cu:1018-1117) and sponge (:505-528) -------
+# Both oracle functions are checked against what the model fixes, computed
+# here from first principles: the non-equilibrium stress tensor as an einsum
+# over the lattice vectors, its Frobenius norm from numpy.linalg.norm, the
+# eddy viscosity as the positive root of its quadratic from numpy.roots, and
+# the relaxation rate each moment row actually received, read back from the
+# oracle's output through the moment matrix.
+
+S_SMAG = np.array([0, 1.19, 1.4, 0, 1.4, 0, 1.4, 0, 1.4, 1.8, 1.8, 1.8, 1.8, 1.8, 1.8, 1.8, 1.2, 1.2, 1.2])
+VISCOUS_ROWS = [9, 11, 13, 14, 15]
+
+
+def _feq_closed(rho, u):
+    """w_i rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u), FP64, with the weight literals."""
+    w = lbm.W32.astype(np.float64)[:, None]
+    cu = C.astype(np.float64) @ u
+    return w * rho * (1 + 3 * cu + 4.5 * cu**2 - 1.5 * (u * u).sum(0))
+
+
+def _rates_applied(f, f_post, feq):
+    """Per-row relaxation rate s_k = [M(f - f_post)]_k / [M(f - feq)]_k."""
+    M = lbm.moment_matrix().astype(np.float64)
+    return (M @ (f - f_post)) / (M @ (f - feq))
+
+
+def _smag_state(n=24, seed=21):
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.05 * rng.standard_normal(n)
+    u = 0.05 * rng.standard_normal((3, n))
+    f = _feq_closed(rho, u) * (1.0 + 0.02 * rng.standard_normal((19, n)))
+    return f, np.zeros(n, np.uint8)
+
+
+def test_smagorinsky_with_cs_zero_is_mrt_with_omega0_on_viscous_rows():
+    f, solid = _smag_state()
+    om0 = 1.7
+    fs, *_ = lbm.macro_collide("mrt_smag", f, solid, om1=om0, S=S_SMAG, cs_dx2=0.0)
+    S2 = S_SMAG.copy()
+    S2[VISCOUS_ROWS] = om0
+    fm, *_ = lbm.macro_collide("mrt", f, solid, S=S2)
+    assert np.abs(fs - fm).max() < 1e-14
+    # the global S of the viscous rows is replaced, not used
+    S3 = S_SMAG.copy()
+    S3[VISCOUS_ROWS] = 0.3
+    fs3, *_ = lbm.macro_collide("mrt_smag", f, solid, om1=om0, S=S3, cs_dx2=0.0)
+    assert np.abs(fs3 - fm).max() < 1e-14
+
+
+@pytest.mark.parametrize("cs_dx2", [1e-3, 1e-2, 0.1])
+def test_smagorinsky_rate_is_the_root_of_the_eddy_viscosity_quadratic(cs_dx2):
+    f, solid = _smag_state(seed=int(cs_dx2 * 1e4))
+    om0 = 1.9
+    fs, rho, u, v, w = lbm.macro_collide("mrt_smag", f, solid, om1=om0, S=S_SMAG, cs_dx2=cs_dx2)
+    uu = np.stack([u, v, w])
+    feq = _feq_closed(rho, uu)
+    # rho and u are the node's own moments (u through 1/(rho + 1e-12))
+    assert np.abs(rho - f.sum(0)).max() < 1e-14
+    assert np.abs(uu - (C.T.astype(np.float64) @ f) / f.sum(0)).max() < 1e-12
+    Pi = np.einsum("qn,qa,qb->nab", f - feq, C.astype(np.float64), C.astype(np.float64))
+    pif = np.linalg.norm(Pi, "fro", axis=(1, 2))
+    nu0 = (1.0 / om0 - 0.5) / 3.0
+    cs4 = (1.0 / 3.0) ** 2
+    rates = _rates_applied(f, fs, feq)
+    clamped = 0
+    for n in range(f.shape[1]):
+        # nu (nu - nu0) = (Cs dx)^2 |Pi| / (2 rho cs^4)
+        roots = np.roots([1.0, -nu0, -cs_dx2 * pif[n] / (2.0 * rho[n] * cs4)])
+        nu = float(roots.real.max())
+        om = 1.0 / (3.0 * nu + 0.5)
+        if om > 1.99:
+            om, clamped = float(np.float32(1.99)), clamped + 1
+        np.testing.assert_allclose(rates[VISCOUS_ROWS, n], om, rtol=1e-9)
+        others = [k for k in range(19) if k not in VISCOUS_ROWS + [0, 3, 5, 7]]
+        np.testing.assert_allclose(rates[others, n], S_SMAG[others], rtol=1e-9)
+    # the eddy viscosity lowered the rate below omega0 everywhere
+    assert rates[9].max() < om0 and clamped == 0
+
+
+def test_smagorinsky_rate_clamped_at_1_99():
+    f, solid = _smag_state(seed=5)
+    om0 = 1.995  # nu0 > 0 but 1/(3 nu0 + 1/2) = 1.995 > 1.99
+    fs, rho, u, v, w = lbm.macro_collide("mrt_smag", f, solid, om1=om0, S=S_SMAG, cs_dx2=0.0)
+    rates = _rates_applied(f, fs, _feq_closed(rho, np.stack([u, v, w])))
+    np.testing.assert_allclose(rates[VISCOUS_ROWS], float(np.float32(1.99)), rtol=1e-10)
+
+
+def _sponge_state(n=40, seed=13):
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.05 * rng.standard_normal(n)
+    u = 0.05 * rng.standard_normal((3, n))
+    f = _feq_closed(rho, u) * (1.0 + 0.05 * rng.standard_normal((19, n)))
+    # macros that are not the populations' own: the sponge takes them as given
+    rho_s = 1.0 + 0.05 * rng.standard_normal(n)
+    u_s = 0.05 * rng.standard_normal((3, n))
+    return f, rho_s, u_s
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_sponge_sigma_zero_or_negative_is_the_identity(dtype):
+    f, rho, u = _sponge_state()
+    f = f.astype(dtype)
+    sig = np.zeros(f.shape[1])
+    sig[::3] = -0.5
+    got = lbm.sponge(f, rho, u[0], u[1], u[2], sig)
+    np.testing.assert_array_equal(got, f)
+
+
+def test_sponge_sigma_one_gives_feq_and_partial_sigma_interpolates():
+    f, rho, u = _sponge_state()
+    feq = _feq_closed(rho, u)
+    got = lbm.sponge(f, rho, u[0], u[1], u[2], np.ones(f.shape[1]))
+    assert np.abs(got - feq).max() < 1e-15
+    sig = np.random.default_rng(2).random(f.shape[1])
+    got = lbm.sponge(f, rho, u[0], u[1], u[2], sig)
+    assert np.abs(got - ((1 - sig) * f + sig * feq)).max() < 1e-15
+    # the sponge pulls the momentum towards rho_s u_s by the factor sigma (to
+    # the 5e-8 by which the weight literals miss sum w c c = 1/3)
+    j = C.T.astype(np.float64) @ got
+    j_want = (1 - sig) * (C.T.astype(np.float64) @ f) + sig * rho * u
+    assert np.abs(j - j_want).max() < 1e-7 * np.abs(rho * u).max()
