@@ -1,8 +1,9 @@
 """ctypes binding of libfluxcore_hip.so (the C ABI declared in include/fluxcore_hip.h).
 
 The library is loaded from the package directory. There is no fallback: if the
-library is missing or a GPU call fails, a RuntimeError carries the message of
-``fc_last_error()``.
+library is missing, ``lib()`` raises (the extension modules then report their
+``*_AVAILABLE`` flags as False), and if a GPU call fails, a RuntimeError carries
+the message of ``fc_last_error()``.
 """
 from __future__ import annotations
 
@@ -101,6 +102,15 @@ def lib():
                 _bind(handle, _SIGS)
                 _lib = handle
     return _lib
+
+
+def load_error() -> str | None:
+    """Load the library; return None on success, else the reason it failed."""
+    try:
+        lib()
+    except (RuntimeError, OSError) as e:
+        return str(e)
+    return None
 
 
 def exported_symbols() -> list:

@@ -382,9 +382,25 @@ __global__ void arnoldi_givens_kernel(const double* __restrict__ scal, double* _
     for (int i = 0; i <= kMaxBasis; ++i) g[i] = 0.0;
     g[0] = sqrt(scal[F_NV0]);
   }
+  double* Rj = R + j * (kMaxBasis + 1);
+  const double nvj = scal[F_NV0 + j];
+  if (!(nvj > 0.0)) {
+    // V_j = 0 (exact breakdown): step j-1 already gave the solution in this Krylov
+    // space (its estimate g[j] is 0). Column j is left empty with an identity
+    // rotation; the host drops it from the update.
+    for (int i = 0; i <= j; ++i) Rj[i] = 0.0;
+    cs[j] = 1.0;
+    sn[j] = 0.0;
+    g[j + 1] = 0.0;
+    st[G_EST] = fabs(g[j]) / bnorm;
+    return;
+  }
   double col[kMaxBasis + 1];
-  const double hj = sqrt(scal[F_NV0 + j]);
-  for (int i = 0; i <= j; ++i) col[i] = scal[F_DOT0 + i] / (hj * sqrt(scal[F_NV0 + i]));
+  const double hj = sqrt(nvj);
+  for (int i = 0; i <= j; ++i) {
+    const double nvi = scal[F_NV0 + i];
+    col[i] = nvi > 0.0 ? scal[F_DOT0 + i] / (hj * sqrt(nvi)) : 0.0;
+  }
   col[j + 1] = sqrt(scal[F_NV0 + j + 1]) / hj;
   for (int i = 0; i < j; ++i) {
     const double a = col[i], b = col[i + 1];
@@ -398,7 +414,7 @@ __global__ void arnoldi_givens_kernel(const double* __restrict__ scal, double* _
   col[j] = d;
   g[j + 1] = -sv * g[j];
   g[j] = c * g[j];
-  for (int i = 0; i <= j; ++i) R[j * (kMaxBasis + 1) + i] = col[i];
+  for (int i = 0; i <= j; ++i) Rj[i] = col[i];
   st[G_EST] = fabs(g[j + 1]) / bnorm;
 }
 

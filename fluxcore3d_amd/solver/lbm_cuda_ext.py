@@ -9,11 +9,14 @@ replaces ``sim.step`` with the fused in-place AA step (one kernel launch per
 step plus the inlet/outlet plane kernels). ``cfg.fc_engine = "reference"``
 selects the reference's per-kernel sequence instead.
 
-There is no PyTorch fallback inside this module: on a machine without a GPU
-``LBM_CUDA_AVAILABLE`` is False and the calls raise.
+There is no PyTorch arithmetic inside this module. As in the reference, importing it
+never raises: without a GPU, or when libfluxcore_hip.so cannot be loaded,
+``LBM_CUDA_AVAILABLE`` is False (and ``LBM_CUDA`` None when the library is missing),
+``patch`` leaves the simulation to the caller's own step, and calls raise.
 """
 from __future__ import annotations
 
+import sys
 import time
 import types
 
@@ -131,8 +134,14 @@ class _LBMKernels:
 
 
 def _load_extension() -> bool:
+    """Load libfluxcore_hip.so; on failure LBM_CUDA is None and LBM_CUDA_AVAILABLE is
+    False, as the reference leaves them when its extension fails (lbm_cuda_ext.py:87-91)."""
     global LBM_CUDA, LBM_CUDA_AVAILABLE
-    _lib.lib()  # raises when libfluxcore_hip.so is missing
+    err = _lib.load_error()
+    if err is not None:
+        print(f"[LBM_CUDA] libfluxcore_hip.so not loaded: {err}", file=sys.stderr)
+        LBM_CUDA, LBM_CUDA_AVAILABLE = None, False
+        return False
     LBM_CUDA = _LBMKernels()
     LBM_CUDA_AVAILABLE = bool(torch.cuda.is_available())
     return LBM_CUDA_AVAILABLE

@@ -7,6 +7,8 @@ entry points used by nlevel_mg_cuda.py are registered here as well.
 """
 from __future__ import annotations
 
+import sys
+
 import torch
 
 from .. import _lib
@@ -71,6 +73,13 @@ class _MGKernels:
         _lib.call("fc_mg_jacobi_smooth", *ptrs, x.numel(), torch.cuda.current_stream(x.device).cuda_stream)
 
 
-_lib.lib()
-MG_CUDA = _MGKernels()
-MG_CUDA_AVAILABLE = bool(torch.cuda.is_available())
+# As in the reference (mg_cuda_ext.py:59-62), a library that cannot be loaded leaves
+# MG_CUDA None and MG_CUDA_AVAILABLE False instead of failing the import.
+_err = _lib.load_error()
+if _err is not None:
+    print(f"[MG_CUDA] libfluxcore_hip.so not loaded: {_err}", file=sys.stderr)
+    MG_CUDA = None
+    MG_CUDA_AVAILABLE = False
+else:
+    MG_CUDA = _MGKernels()
+    MG_CUDA_AVAILABLE = bool(torch.cuda.is_available())

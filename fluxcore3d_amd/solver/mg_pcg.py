@@ -159,8 +159,9 @@ def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
     ws.b.copy_(b.reshape(-1).to(torch.float64))
     if x0 is None:
         ws.x.zero_()
-    else:
+    else:  # Dirichlet rows start at 0 (mg_pcg.py:50-51): the stencil reads them
         ws.x.copy_(x0.reshape(-1).to(torch.float64))
+        ws.x.mul_(lv0["mask_f"].reshape(-1))
     dt = float(mg_solver._dt_cache)
     ax, oi = mg_solver.ax, lv0["out_idx"]
     mask = lv0["mask_f"]
@@ -181,6 +182,7 @@ def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
         _lib.call("fc_mg_update_xr", ws.x.data_ptr(), ws.r.data_ptr(), ws.r32.data_ptr(), ws.p.data_ptr(),
                   ws.Ap.data_ptr(), ws.n, ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
 
+    S_STOP = _stop_slot("mg")
     _clear_stop(ws.scal, S_RR, S_STOP, stream)  # the workspace may hold a finished solve's flag
     rr, bb = true_residual()
     bnorm = bb ** 0.5 + 1e-300
@@ -245,8 +247,11 @@ _lib.register({
 
 K_RR = 2
 K_WWHIST0 = 16
-S_STOP = int(_lib.call("fc_mg_stop_slot"))
-K_STOP = int(_lib.call("fc_gcr_stop_slot"))
+
+
+def _stop_slot(which: str) -> int:
+    """Scalar slot of the device stopping flag of the FCG ("mg") or GCR ("gcr") workspace."""
+    return int(_lib.call(f"fc_{which}_stop_slot"))
 
 
 class GCRStencil(ctypes.Structure):
@@ -293,8 +298,9 @@ def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
     fw.b.copy_(b.reshape(-1).to(torch.float64))
     if x0 is None:
         fw.x.zero_()
-    else:
+    else:  # Dirichlet rows start at 0 (mg_pcg.py:50-51): the stencil reads them
         fw.x.copy_(x0.reshape(-1).to(torch.float64))
+        fw.x.mul_(lv0["mask_f"].reshape(-1))
     dt = float(mg_solver._dt_cache)
     mask = lv0["mask_f"]
     faces = lv0.get("faces")
@@ -324,6 +330,7 @@ def gcr_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
                   W_ptrs, P_ptrs, slots, k, K_WWHIST0 + cur, fw.x.data_ptr(), fw.r.data_ptr(), fw.r32.data_ptr(),
                   ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
 
+    K_STOP = _stop_slot("gcr")
     _clear_stop(ws.scal, K_RR, K_STOP, stream)  # the workspace may hold a finished solve's flag
     rr, bb = true_residual()
     bnorm = bb ** 0.5 + 1e-300
@@ -437,8 +444,9 @@ def fgmres_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, t
     fw.b.copy_(b.reshape(-1).to(torch.float64))
     if x0 is None:
         fw.x.zero_()
-    else:
+    else:  # Dirichlet rows start at 0 (mg_pcg.py:50-51): the stencil reads them
         fw.x.copy_(x0.reshape(-1).to(torch.float64))
+        fw.x.mul_(lv0["mask_f"].reshape(-1))
     dt = float(mg_solver._dt_cache)
     mask = lv0["mask_f"]
     faces = lv0.get("faces")
@@ -485,6 +493,13 @@ def fgmres_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, t
         hist.append(float(ws.est_host[k - 1]))
         gst = ws.gstate.cpu().tolist()
         nv = [float(v) for v in ws.scal[ws.norm_slot[0]:ws.norm_slot[0] + k].cpu()]
+        # an exact breakdown (V_i = 0, or an empty pivot) ends the basis at column i:
+        # that column and any queued after it do not enter the update
+        k_used = next((i for i in range(k) if not (nv[i] > 0.0 and gst[ws.g_r + i * ws.g_ld + i] != 0.0)), k)
+        it -= k - k_used
+        k = k_used
+        if k == 0:
+            break
         y = [0.0] * k
         for i in range(k - 1, -1, -1):
             rii = gst[ws.g_r + i * ws.g_ld + i]

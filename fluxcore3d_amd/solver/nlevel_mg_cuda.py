@@ -40,10 +40,15 @@ class NLevelGeometricMGSolver:
     def __init__(self, device, dtype, flow_axis: int, flow_sign: int, thermal_dt_scale: float, omega: float = 0.8,
                  n_pre: int = 2, n_post: int = 2, n_coarse: int = 20, min_coarse_cells: int = 4,
                  face_coefficients: bool = True, fused_first_sweeps: bool = True):
+        # The reference falls through to its PyTorch operator for CPU or float64 input
+        # (nlevel_mg_cuda.py:71-79); this solver runs the HIP kernels only.
         if torch.device(device).type != "cuda":
-            raise RuntimeError("NLevelGeometricMGSolver runs on the HIP device only")
+            raise RuntimeError("NLevelGeometricMGSolver runs on the HIP device only (no CPU path)")
         if dtype != torch.float32:
-            raise RuntimeError("the multigrid V-cycle runs in float32")
+            raise RuntimeError("the multigrid V-cycle runs in float32 (no float64 path)")
+        err = _lib.load_error()
+        if err is not None:
+            raise RuntimeError(f"NLevelGeometricMGSolver needs libfluxcore_hip.so: {err}")
         self.device = torch.device(device)
         self.dtype = dtype
         self.ax = int(flow_axis)

@@ -8,6 +8,8 @@ by the fused kernel and every reduction on the GPU.
 """
 from __future__ import annotations
 
+import sys
+
 import types
 
 import torch
@@ -73,9 +75,16 @@ class _ThermalKernels:
         return flag
 
 
-_lib.lib()
-THERMAL_CUDA = _ThermalKernels()
-THERMAL_CUDA_AVAILABLE = bool(torch.cuda.is_available())
+# As in the reference (thermal_cuda_ext.py:14-15,44-46), a library that cannot be loaded leaves
+# THERMAL_CUDA None and THERMAL_CUDA_AVAILABLE False instead of failing the import.
+_err = _lib.load_error()
+if _err is not None:
+    print(f"[THERMAL_CUDA] libfluxcore_hip.so not loaded: {_err}", file=sys.stderr)
+    THERMAL_CUDA = None
+    THERMAL_CUDA_AVAILABLE = False
+else:
+    THERMAL_CUDA = _ThermalKernels()
+    THERMAL_CUDA_AVAILABLE = bool(torch.cuda.is_available())
 
 
 def _temperature_step_mg_fused(self, dt: float, tol: float = 1e-3, max_cycles: int = 20) -> dict:

@@ -308,6 +308,23 @@ def thermal_step(h: Hierarchy, T_old, T0, src, mask, dt, tol=1e-3, max_cycles=20
     return reconstruct(T0, theta, h.ax, lv["out_idx"]), hist
 
 
+def thermal_step_guarded(h: Hierarchy, T_old, T0, src, mask, dt, T_lo, T_hi, tol=1e-3, max_cycles=20):
+    """thermal_step with the fused driver's validity check (thermal_cuda_ext.py:146-160,
+    physics_check_kernel thermal_kernels.cu:175-200): the candidate T0 + theta (theta
+    masked) is rejected when any value is non-finite or outside [T_lo, T_hi].
+    Returns (T_new, hist), T_new None on rejection."""
+    d = T_old.dtype.type
+    LT0 = (h.mop(T0, 0) - T0) / d(h.dt)
+    rhs = mask * ((T_old + d(dt) * src) - (T0 + d(dt) * LT0))
+    theta0 = mask * (T_old - T0)
+    theta, hist = h.solve(rhs, x0=theta0, max_cycles=max_cycles, tol=tol)
+    theta = theta * mask
+    cand = T0 + theta
+    if not np.isfinite(cand).all() or cand.min() < d(T_lo) or cand.max() > d(T_hi):
+        return None, hist
+    return reconstruct(T0, theta, h.ax, h.levels[0]["out_idx"]), hist
+
+
 def fcg_mg_fp64(h: Hierarchy, b, tol=1e-8, max_it=100, truncation=1):
     """Flexible CG (Notay 2000, truncation 1) with FP64 vectors and residual;
     the preconditioner (one FP32 V-cycle of ``h``) sees the residual cast to

@@ -45,3 +45,36 @@ def test_error_channel_without_gpu():
     rc = _lib.lib().fc_lbm_stream_pull(None, None, 0, 1, 1, 0, 0, 0, None)
     assert rc != 0
     assert b"positive" in _lib.lib().fc_last_error()
+
+
+def test_missing_library_leaves_the_flags_false():
+    """As the reference's extension modules do when their build fails
+    (lbm_cuda_ext.py:87-91, mg_cuda_ext.py:59-62, thermal_cuda_ext.py:44-46): the
+    imports succeed, *_AVAILABLE is False and the kernel objects are None; the
+    multigrid solver and patch() do no arithmetic of their own."""
+    import subprocess
+    import sys
+    code = """
+import sys
+sys.path.insert(0, %r)
+from fluxcore3d_amd import _lib
+_lib.LIB_PATH = '/nonexistent/libfluxcore_hip.so'
+from fluxcore3d_amd.solver import lbm_cuda_ext, mg_cuda_ext, thermal_cuda_ext, mg_pcg, nlevel_mg_cuda
+assert lbm_cuda_ext.LBM_CUDA is None and not lbm_cuda_ext.LBM_CUDA_AVAILABLE
+assert mg_cuda_ext.MG_CUDA is None and not mg_cuda_ext.MG_CUDA_AVAILABLE
+assert thermal_cuda_ext.THERMAL_CUDA is None and not thermal_cuda_ext.THERMAL_CUDA_AVAILABLE
+import torch
+try:
+    nlevel_mg_cuda.NLevelGeometricMGSolver(torch.device('cuda'), torch.float32, 0, 1, 1.0)
+    raise SystemExit('solver built without the library')
+except RuntimeError as e:
+    assert 'libfluxcore_hip.so' in str(e), e
+class Sim: pass
+sim = Sim()
+lbm_cuda_ext.LBMCUDAUpgrade.patch(sim)
+assert not hasattr(sim, '_solid_u8')
+print('OK')
+""" % ROOT
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0 and "OK" in res.stdout, res.stdout + res.stderr
+    assert "libfluxcore_hip.so not loaded" in res.stderr

@@ -1,8 +1,8 @@
 """GPU parity of the D3Q19 kernels against the FP32 oracle, through the C ABI.
 
-Tolerances: the reference kernels are FP32 with fast-math contraction, the
-HIP kernels FP32 with contraction and the oracle FP32 without it, so values
-agree to a few ulp per operation; fields are compared as
+Tolerances: the reference kernels are FP32 with fast-math contraction; the
+HIP kernels and the oracle are FP32 without contraction, so values agree to a
+few ulp per operation (BGK and TRT bit for bit); fields are compared as
 max|a - b| / max|b| <= 1e-5 (velocity, density, populations), streaming and
 solid resets bit-exactly.
 """
@@ -177,9 +177,11 @@ def test_config1_step_sequence(engine, kind):
 def test_config1_full_run():
     """Config 1 at its specified length: 64x32x32 sphere channel, BGK tau = 0.6, inlet
     u = 0.05, convective outlet, walls, Bouzidi links, 2000 steps of the fused step
-    against the FP32 oracle (same operation order) and the FP64 oracle. The FP32
-    runs differ only by rounding (contracted multiply-adds on the GPU); the flow
-    settles towards its steady state, so the difference stays bounded."""
+    against the FP32 oracle (same operation order) and the FP64 oracle. The library
+    is built without multiply-add contraction, so the fused BGK step rounds where
+    the oracle does: the velocity fields agree to 1e-5 of the largest speed (0 was
+    measured on an MI355X). The FP32 path is 1e-5 of the largest speed from the
+    FP64 oracle, as the FP32 oracle is."""
     from fluxcore3d_amd.solver.lbm_cuda_ext import LBMCUDAUpgrade, flow_bc_plan
     c = cases.config1()
     sim = _sim_for(c, "bgk", "aa")
@@ -193,7 +195,7 @@ def test_config1_full_run():
     speed = float(np.sqrt(u_w**2 + v_w**2 + w_w**2).max())
     got = [a.cpu().numpy().reshape(-1) for a in (sim.u, sim.v, sim.wv)]
     err32 = max(np.abs(g - w).max() for g, w in zip(got, (u_w, v_w, w_w))) / speed
-    assert err32 < 2e-5, err32
+    assert err32 <= 1e-5, err32
     assert rel(sim.rho.cpu().numpy().reshape(-1), rho_w) < 1e-5
     # FP64 oracle (reported against the FP32 path: FP32 storage and arithmetic)
     nx, ny, nz = c["nx"], c["ny"], c["nz"]
@@ -220,11 +222,11 @@ def test_pin_fin_channel_trt(engine):
     torch.cuda.synchronize()
     f_want, (rho_w, u_w, v_w, w_w) = _oracle_run(c, steps, "trt", c["omega_minus"], None, (0, 0, 0),
                                                  flow_bc_plan(sim))
-    # FP32 rounding (contracted multiply-adds on the GPU, none in the oracle) over 120
-    # steps on 221k nodes: both engines sit at 1.1e-5 of the largest speed
+    # no multiply-add contraction in the library or the oracle: both engines round
+    # where the oracle does (0 difference measured on an MI355X)
     speed = float(np.sqrt(u_w**2 + v_w**2 + w_w**2).max())
     for got, want in ((sim.u, u_w), (sim.v, v_w), (sim.wv, w_w)):
-        assert np.abs(got.cpu().numpy().reshape(-1) - want).max() / speed < 3e-5
+        assert np.abs(got.cpu().numpy().reshape(-1) - want).max() / speed <= 1e-5
     assert rel(sim.rho.cpu().numpy().reshape(-1), rho_w) < 1e-5
     assert rel(sim.f.cpu().numpy().reshape(19, -1), f_want) < 1e-5
 

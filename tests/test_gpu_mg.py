@@ -111,8 +111,8 @@ def test_solve_cycles_match_oracle():
     x, info = s.solve(T(c["b"]), max_cycles=6, tol=1e-30)
     xo, hist = h.solve(c["b"].astype(np.float32), max_cycles=6, tol=1e-30)
     assert info["it"] == len(hist) == 6
-    assert rel(x.cpu().numpy(), xo) < 1e-4
-    assert abs(info["res_rel"] - hist[-1]) / hist[-1] < 1e-3
+    assert rel(x.cpu().numpy(), xo) < 1e-5  # 4e-7 measured on an MI355X
+    assert abs(info["res_rel"] - hist[-1]) / hist[-1] < 1e-5
 
 
 def test_fcg_fp64_true_residual_and_solution():
@@ -453,3 +453,47 @@ def test_bicgstab_against_oracle_and_patch_routing():
         xp, pinfo = s.solve(c["b"], max_cycles=6, tol=1e-30)
         xd, dinfo = direct(s, c["b"], tol=1e-30, max_it=6)
         assert torch.equal(xp, xd) and pinfo["res_rel"] == dinfo["res_rel"], direct.__name__
+
+
+@pytest.mark.parametrize("solver", ["pcg_mg", "gcr", "fgmres"])
+def test_initial_guess_on_dirichlet_rows_is_ignored(solver):
+    """The reference's pcg_mg masks x0 before its first residual (mg_pcg.py:50-51):
+    values of x0 on Dirichlet rows (mask 0) must not leak into the neighbouring
+    unknowns through the stencil. The solve from a random x0 that is nonzero on the
+    masked wall planes equals the solve from x0 * mask, and the oracle's FCG solution."""
+    from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, fgmres_fp64, gcr_fp64
+    c = thermal_cases.sphere_conduction(32, 16, 16, dt=1.0e3)
+    s, h = _solver_pair(c, omega=0.8)
+    rng = np.random.default_rng(3)
+    x0 = (10.0 * rng.standard_normal(c["k"].shape)).astype(np.float32)
+    assert np.abs(x0[~c["mask"]]).min() > 0
+    fn, kw = {"pcg_mg": (fcg_fp64, {}), "gcr": (gcr_fp64, {"m": 6}), "fgmres": (fgmres_fp64, {"m": 16})}[solver]
+    _, a = fn(s, T(c["b"]), x0=T(x0), tol=1e-10, max_it=400, **kw)
+    xa = a["x64"].clone()
+    _, b = fn(s, T(c["b"]), x0=T(x0 * c["mask"]), tol=1e-10, max_it=400, **kw)
+    assert a["converged"] and b["converged"]
+    assert float((xa - b["x64"]).abs().max()) <= 1e-8 * float(b["x64"].abs().max())
+    assert float(xa.view(-1)[torch.as_tensor(~c["mask"].reshape(-1), device=DEV)].abs().max()) == 0.0
+    h.set_dt(c["dt"])
+    xo, _ = mg.fcg_mg_fp64(h, c["b"].astype(np.float32), tol=1e-12, max_it=400)
+    got = xa.cpu().numpy().reshape(c["k"].shape)
+    assert np.abs(got - xo).max() <= 1e-7 * np.abs(xo).max()
+
+
+def test_fgmres_exact_breakdown_with_identity_operator():
+    """dt = 0 makes A = I. With the identity preconditioner and a right-hand side with
+    one nonzero entry, every sum is exact: A z = r, h_00 = 1 and the second basis
+    vector is exactly zero (an exact breakdown), while the step that uses it is
+    already queued. The Givens kernel leaves that column empty, the host drops it,
+    and the solve ends after one iteration with x = b exactly."""
+    from fluxcore3d_amd.solver.mg_pcg import fgmres_fp64
+    c = thermal_cases.sphere_conduction(16, 8, 8, dt=0.0)
+    s, _ = _solver_pair(c)
+    b = np.zeros(c["k"].shape, np.float32)
+    b[5, 3, 4] = 1.5
+    assert c["mask"][5, 3, 4]
+    x, info = fgmres_fp64(s, T(b), tol=1e-12, max_it=16, m=16, precond=lambda v, z: z.copy_(v))
+    x64 = info["x64"].cpu().numpy()
+    assert np.isfinite(x64).all() and np.isfinite(info["hist"]).all()
+    assert info["converged"] and info["it"] == 1, info
+    np.testing.assert_array_equal(x64, b.astype(np.float64))
