@@ -39,7 +39,10 @@ def test_poiseuille_textbook_bouzidi_guo_force(kind):
     tau = 0.8
     om = 1.0 / tau
     nu = (tau - 0.5) / 3.0
-    Fx = 1e-6
+    # u_max = 6e-3: large enough that FP32 rounding of the populations (f ~ 1/18) stays
+    # well below the discretisation error measured here; Re ~ 1, a linear regime in which
+    # the relative error does not depend on Fx
+    Fx = 1e-5
     om_minus = 1.0 / (0.5 + 0.1875 / (tau - 0.5))
     eng = AAEngine(nx, ny, nz, solid, periodic=(1, 1, 1), collision=kind, omega=om, omega_minus=om_minus,
                    S=mrt_rates(om) if kind == "mrt" else None, force=(Fx, 0, 0), links=links, ibb_mode="textbook",
