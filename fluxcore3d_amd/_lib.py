@@ -56,6 +56,7 @@ _SIGS = {
     "fc_lbm_zou_he_bc": (I32, [P, P, I32, I32, I32, I32, I32, I32, I32, F32, F32, I32, P]),
     "fc_lbm_convective_bc_workspace_bytes": (SZ, [I32, I32]),
     "fc_lbm_convective_bc": (I32, [P, P, I32, I32, I32, I32, I32, I32, I32, F32, I32, I32, P, SZ, P]),
+    "fc_lbm_convective_bc_noalloc": (I32, [P, P, I32, I32, I32, I32, I32, I32, I32, F32, I32, I32, P]),
     # LBM fused AA step (lbm_aa.hip)
     "fc_aa_slot_count": (I64, [I32, I32, I32, I32, I32, I32]),
     "fc_aa_step": (I32, [P, I32, I32, I32, I32, I32, I32, I32, C.POINTER(AATables), C.POINTER(AACollision),

@@ -9,6 +9,8 @@
 // 2^31/19 nodes are safe.
 #include "fc_common.hpp"
 
+#include <mutex>
+
 using namespace fc;
 
 namespace {
@@ -325,6 +327,64 @@ __global__ __launch_bounds__(kBlock) void convective_pass2(float* __restrict__ f
   for (int q = 0; q < 19; ++q) f[q * N + fo] *= corr;
 }
 
+// Scratch of fc_lbm_convective_bc_noalloc: the FP64 partial sums (outlet density,
+// fluid-node count) of convective_pass1, two per block. Planes of up to
+// kConvScratchBlocks * kBlock = 1,048,576 nodes; 64 KiB held by every device the
+// library runs on until the process ends. Pass 1 writes every slot that pass 2
+// reads, so nothing has to start at zero.
+constexpr int kConvScratchBlocks = 4096;
+static __device__ __align__(256) double scratch[2 * kConvScratchBlocks];
+
+// Device address of `scratch` per device (the host-side name is only a stand-in).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int convective_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(2, "convective_bc_noalloc: hipGetDevice failed: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(1, "convective_bc_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(2, "convective_bc_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// Argument checks shared by both convective entry points.
+int convective_args(int nx, int ny, int nz, int axis, int outlet_idx, int interior_idx, int unknown_cax, int n1,
+                    int n2) {
+  FC_REQUIRE(nx > 0 && ny > 0 && nz > 0, "convective_bc: grid extents must be positive");
+  FC_REQUIRE(axis >= 0 && axis <= 2, "convective_bc: axis must be 0, 1 or 2");
+  FC_REQUIRE(unknown_cax == 1 || unknown_cax == -1, "convective_bc: unknown_cax must be +1 or -1");
+  const int e1 = axis == 0 ? ny : nx, e2 = axis == 2 ? ny : nz;
+  FC_REQUIRE(n1 == e1 && n2 == e2, "convective_bc: plane dims (%d,%d) do not match axis %d (%d,%d)", n1, n2, axis, e1,
+             e2);
+  const int n_ax = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  FC_REQUIRE(outlet_idx >= 0 && outlet_idx < n_ax && interior_idx >= 0 && interior_idx < n_ax,
+             "convective_bc: plane/interior index out of range");
+  return 0;
+}
+
+int convective_launch(float* f, const uint8_t* solid, int nx, int ny, int nz, int axis, int outlet_idx,
+                      int interior_idx, int unknown_cax, float rho_out_lat, int n1, int n2, double* partial,
+                      hipStream_t stream) {
+  const unsigned g = grid_for((int64_t)n1 * n2, kBlock);
+  hipLaunchKernelGGL(convective_pass1, dim3(g), dim3(kBlock), 0, stream, f, solid, nx, ny, nz, axis, outlet_idx,
+                     interior_idx, unknown_cax, n1, n2, partial);
+  int rc = check_launch("convective_bc/pass1");
+  if (rc) return rc;
+  hipLaunchKernelGGL(convective_pass2, dim3(g), dim3(kBlock), 0, stream, f, solid, nx, ny, nz, axis, outlet_idx, n1,
+                     n2, rho_out_lat, (const double*)partial, (int)g);
+  return check_launch("convective_bc/pass2");
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -438,26 +498,29 @@ size_t fc_lbm_convective_bc_workspace_bytes(int n1, int n2) {
 int fc_lbm_convective_bc(float* f, const uint8_t* solid, int nx, int ny, int nz, int axis, int outlet_idx,
                          int interior_idx, int unknown_cax, float rho_out_lat, int n1, int n2, void* workspace,
                          size_t workspace_bytes, hipStream_t stream) {
-  FC_REQUIRE(nx > 0 && ny > 0 && nz > 0, "convective_bc: grid extents must be positive");
-  FC_REQUIRE(axis >= 0 && axis <= 2, "convective_bc: axis must be 0, 1 or 2");
-  FC_REQUIRE(unknown_cax == 1 || unknown_cax == -1, "convective_bc: unknown_cax must be +1 or -1");
-  const int e1 = axis == 0 ? ny : nx, e2 = axis == 2 ? ny : nz;
-  FC_REQUIRE(n1 == e1 && n2 == e2, "convective_bc: plane dims (%d,%d) do not match axis %d (%d,%d)", n1, n2, axis, e1,
-             e2);
-  const int n_ax = axis == 0 ? nx : (axis == 1 ? ny : nz);
-  FC_REQUIRE(outlet_idx >= 0 && outlet_idx < n_ax && interior_idx >= 0 && interior_idx < n_ax,
-             "convective_bc: plane/interior index out of range");
+  int rc = convective_args(nx, ny, nz, axis, outlet_idx, interior_idx, unknown_cax, n1, n2);
+  if (rc) return rc;
   FC_REQUIRE(f && solid, "convective_bc: null pointer argument");
   const size_t need = fc_lbm_convective_bc_workspace_bytes(n1, n2);
   FC_REQUIRE(workspace && workspace_bytes >= need, "convective_bc: workspace of %zu bytes required", need);
-  const unsigned g = grid_for((int64_t)n1 * n2, kBlock);
-  hipLaunchKernelGGL(convective_pass1, dim3(g), dim3(kBlock), 0, stream, f, solid, nx, ny, nz, axis, outlet_idx,
-                     interior_idx, unknown_cax, n1, n2, (double*)workspace);
-  int rc = check_launch("convective_bc/pass1");
+  return convective_launch(f, solid, nx, ny, nz, axis, outlet_idx, interior_idx, unknown_cax, rho_out_lat, n1, n2,
+                           (double*)workspace, stream);
+}
+
+int fc_lbm_convective_bc_noalloc(float* f, const uint8_t* solid, int nx, int ny, int nz, int axis, int outlet_idx,
+                                 int interior_idx, int unknown_cax, float rho_out_lat, int n1, int n2,
+                                 hipStream_t stream) {
+  int rc = convective_args(nx, ny, nz, axis, outlet_idx, interior_idx, unknown_cax, n1, n2);
   if (rc) return rc;
-  hipLaunchKernelGGL(convective_pass2, dim3(g), dim3(kBlock), 0, stream, f, solid, nx, ny, nz, axis, outlet_idx, n1,
-                     n2, rho_out_lat, (const double*)workspace, (int)g);
-  return check_launch("convective_bc/pass2");
+  FC_REQUIRE((int64_t)n1 * n2 <= (int64_t)kConvScratchBlocks * kBlock,
+             "convective_bc_noalloc: outlet plane of %lld nodes exceeds the %d-node scratch limit; "
+             "use fc_lbm_convective_bc with a workspace", (long long)n1 * n2, kConvScratchBlocks * kBlock);
+  FC_REQUIRE(f && solid, "convective_bc_noalloc: null pointer argument");
+  double* partial = nullptr;
+  rc = convective_scratch(&partial);
+  if (rc) return rc;
+  return convective_launch(f, solid, nx, ny, nz, axis, outlet_idx, interior_idx, unknown_cax, rho_out_lat, n1, n2,
+                           partial, stream);
 }
 
 }  // extern "C"

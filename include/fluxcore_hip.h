@@ -68,6 +68,16 @@ size_t fc_lbm_convective_bc_workspace_bytes(int n1, int n2);
 int fc_lbm_convective_bc(float *f, const uint8_t *solid, int nx, int ny, int nz, int axis, int outlet_idx,
                          int interior_idx, int unknown_cax, float rho_out_lat, int n1, int n2, void *workspace,
                          size_t workspace_bytes, hipStream_t stream);
+/* convective_bc without a workspace and without any allocation. No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * partial sums live in a 64 KiB static device array of the library (one per GPU,
+ * held until the process ends), so the outlet plane is limited to n1 * n2 <=
+ * 1,048,576 nodes (larger planes return an error). Calls that overlap on one GPU
+ * (different streams) race on that array. Same results as fc_lbm_convective_bc. */
+int fc_lbm_convective_bc_noalloc(float *f, const uint8_t *solid, int nx, int ny, int nz, int axis, int outlet_idx,
+                                 int interior_idx, int unknown_cax, float rho_out_lat, int n1, int n2,
+                                 hipStream_t stream);
 
 /* ---- fused in-place step (AA pattern): replaces the kernel sequence of
  *      LBMCUDAUpgrade._step_cuda (lbm_cuda_ext.py:445-517) ---- */
