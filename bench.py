@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--collision", default="mrt", choices=["bgk", "trt", "mrt", "mrt_smag"])
     p.add_argument("--storage", default="fp32", choices=["fp32", "fp16", "bf16"])
     p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--layout", default="dense", choices=["dense", "compact"],
+                   help="population layout of the fused step (compact: only fluid and ghost nodes own slots)")
     p.add_argument("--cpu-steps", type=int, default=16)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--macro-every-step", type=int, default=1)
@@ -176,7 +178,7 @@ def cht_leg(device):
     return out
 
 
-def build_workload(n, seed, collision, storage, device):
+def build_workload(n, seed, collision, storage, device, layout="dense"):
     import torch
 
     from fluxcore3d_amd import geometry
@@ -198,7 +200,7 @@ def build_workload(n, seed, collision, storage, device):
     S[[0, 3, 5, 7]] = 0.0
     S[[9, 11, 13, 14, 15]] = omega
     eng = AAEngine(n, n, n, solid, periodic=per, collision=collision, omega=omega, omega_minus=1.2, S=S,
-                   force=(1e-6, 0.0, 0.0), links=links, storage=storage, device=device)
+                   force=(1e-6, 0.0, 0.0), links=links, storage=storage, device=device, layout=layout)
     f0 = torch.as_tensor(np.broadcast_to(W[:, None], (19, n ** 3)).copy(), device=device)
     eng.load(f0)
     setup_s = time.perf_counter() - t0
@@ -256,7 +258,8 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    eng, solid, links, S, omega, info, setup_s = build_workload(args.n, args.seed, args.collision, args.storage, dev)
+    eng, solid, links, S, omega, info, setup_s = build_workload(args.n, args.seed, args.collision, args.storage, dev,
+                                                                args.layout)
     N = args.n ** 3
     n_fluid = int((~solid).sum())
     macro = [torch.empty(N, device=dev) for _ in range(4)]

@@ -30,6 +30,7 @@ class AATables(C.Structure):
         ("nflag", P), ("wave_base", P), ("wave_meta", P), ("lk_code", P), ("s_code", P), ("f_code", P),
         ("lane_nodes", P), ("n_lanes", I64), ("sp_mask", P), ("sp_base", P),
         ("w1", P), ("w2", P), ("A", P), ("B", P), ("ff_idx", P), ("n_v", I64),
+        ("row_meta", P), ("wpr", I32), ("n_store", I64),
     ]
 
 

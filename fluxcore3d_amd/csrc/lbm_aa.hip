@@ -70,6 +70,10 @@ struct Tables {
   float* B;                  // [2][n_v]
   const int32_t* ff_idx;     // [n_ff]
   int64_t n_v;
+  // compact storage (null: dense padded storage): per row R = I * PY + J and word
+  // w = K / 32 of the padded grid, {presence mask, present nodes before the word}
+  const uint2* row_meta;
+  int wpr;                   // words per row
 };
 
 struct Collision {
@@ -126,11 +130,23 @@ __device__ __forceinline__ int wrap_or_pad(int c, int d, int n, int g) {
   if (g) return v + 1;  // ghost layer: -1 -> 0, n -> n+1
   return v < 0 ? v + n : (v >= n ? v - n : v);
 }
-__device__ __forceinline__ int64_t slot_node(const Grid& g, int i, int j, int k, int dx, int dy, int dz) {
+// Storage index of the node at offset (dx, dy, dz) from interior node (i, j, k):
+// its padded flat index (dense storage), or its rank among the nodes that own
+// slots (compact storage). Returns false for a node without slots (compact
+// storage only): it streams w_q and receives nothing.
+__device__ __forceinline__ bool node_storage(const Grid& g, const Tables& t, int i, int j, int k, int dx, int dy,
+                                             int dz, int64_t& s) {
   const int I = wrap_or_pad(i, dx, g.nx, g.gx);
   const int J = wrap_or_pad(j, dy, g.ny, g.gy);
   const int K = wrap_or_pad(k, dz, g.nz, g.gz);
-  return ((int64_t)I * g.PY + J) * g.PZ + K;
+  if (!t.row_meta) {
+    s = ((int64_t)I * g.PY + J) * g.PZ + K;
+    return true;
+  }
+  const uint2 m = t.row_meta[((int64_t)I * g.PY + J) * t.wpr + (K >> 5)];
+  const uint32_t b = (uint32_t)K & 31u;
+  s = (int64_t)m.y + __popc(m.x & ((1u << b) - 1u));
+  return ((m.x >> b) & 1u) != 0u;
 }
 // true when x - c_q leaves the domain through a non-periodic face
 __device__ __forceinline__ bool oob_source(const Grid& g, int i, int j, int k, int q) {
@@ -143,16 +159,25 @@ __device__ __forceinline__ int rank_below(uint32_t mask) {
   return __popc(mask & ((1u << BIT) - 1u));
 }
 
-// Plain slot that holds input population q of node (i,j,k) at this parity.
+// Plain slot that holds input population q of node (i,j,k) at this parity. With
+// compact storage a slot of a node without slots reads as w_q and is not written.
 template <typename ST>
-__device__ __forceinline__ float load_plain(const ST* pop, const Grid& g, int parity, int i, int j, int k, int q) {
-  if (parity == 0) return Store<ST>::ld(pop + (int64_t)q * g.Ns + slot_node(g, i, j, k, 0, 0, 0), q);
-  return Store<ST>::ld(pop + (int64_t)OPP(q) * g.Ns + slot_node(g, i, j, k, -CX(q), -CY(q), -CZ(q)), OPP(q));
+__device__ __forceinline__ float load_plain(const ST* pop, const Grid& g, const Tables& t, int parity, int i, int j,
+                                            int k, int q) {
+  int64_t s;
+  const int d = parity == 0 ? 0 : -1;
+  if (!node_storage(g, t, i, j, k, d * CX(q), d * CY(q), d * CZ(q), s)) return WQ(q);
+  const int dir = parity == 0 ? q : OPP(q);
+  return Store<ST>::ld(pop + (int64_t)dir * g.Ns + s, dir);
 }
 template <typename ST>
-__device__ __forceinline__ void store_plain(ST* pop, const Grid& g, int parity, int i, int j, int k, int q, float v) {
-  if (parity == 0) Store<ST>::st(pop + (int64_t)q * g.Ns + slot_node(g, i, j, k, 0, 0, 0), v, q);
-  else Store<ST>::st(pop + (int64_t)OPP(q) * g.Ns + slot_node(g, i, j, k, -CX(q), -CY(q), -CZ(q)), v, OPP(q));
+__device__ __forceinline__ void store_plain(ST* pop, const Grid& g, const Tables& t, int parity, int i, int j, int k,
+                                            int q, float v) {
+  int64_t s;
+  const int d = parity == 0 ? 0 : -1;
+  if (!node_storage(g, t, i, j, k, d * CX(q), d * CY(q), d * CZ(q), s)) return;
+  const int dir = parity == 0 ? q : OPP(q);
+  Store<ST>::st(pop + (int64_t)dir * g.Ns + s, v, dir);
 }
 
 // All 19 inputs of a node: plain slots, rest weights for solid sources and
@@ -178,7 +203,7 @@ __device__ __forceinline__ void compose_inputs(const ST* __restrict__ pop, const
     } else if (special && !first && ((S >> q) & 1u)) {
       f[q] = WQ(q);  // after the first step a solid source delivers the rest weight
     } else {
-      f[q] = load_plain(pop, g, parity, i, j, k, q);
+      f[q] = load_plain(pop, g, t, parity, i, j, k, q);
     }
   }
 }
@@ -398,7 +423,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename ST, int KIND>
+template <typename ST, int KIND, bool CMP>
 __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST* __restrict__ pop, Grid g, Tables t, Collision c,
                                                          int parity, int first, float* __restrict__ rho_o,
                                                          float* __restrict__ u_o, float* __restrict__ v_o,
@@ -445,6 +470,12 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
   };
 #pragma unroll
   for (int q = 0; q < 19; ++q) f[q] = 0.f;
+  // compact storage: presence (bit q) and storage index of x - c_q, the node that
+  // wrote input q (q = 0: the node itself); x + c_q is x - c_opp(q)
+  uint32_t pres = 0;
+  uint32_t nb[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) nb[q] = 0;
   if (live) {
     const uint32_t xu = (uint32_t)x;
     const uint32_t nynz = (uint32_t)g.ny * (uint32_t)g.nz;
@@ -460,9 +491,38 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
     oI[0] = parity ? __umul24((uint32_t)Im, PYZ) : oI[1]; oI[2] = parity ? __umul24((uint32_t)Ip, PYZ) : oI[1];
     oJ[0] = parity ? __umul24((uint32_t)Jm, PZ) : oJ[1]; oJ[2] = parity ? __umul24((uint32_t)Jp, PZ) : oJ[1];
     oK[0] = parity ? (uint32_t)Km : oK[1]; oK[2] = parity ? (uint32_t)Kp : oK[1];
+    if constexpr (!CMP) {
 #pragma unroll
-    for (int q = 0; q < 19; ++q)
-      f[q] = Store<ST>::ld_nt(at(q + parity * (OPP(q) - q), pslot(-CX(q), -CY(q), -CZ(q))), q);
+      for (int q = 0; q < 19; ++q)
+        f[q] = Store<ST>::ld_nt(at(q + parity * (OPP(q) - q), pslot(-CX(q), -CY(q), -CZ(q))), q);
+    } else {
+      const uint32_t wpr = (uint32_t)t.wpr;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        const int dx = -CX(q), dy = -CY(q), dz = -CZ(q);
+        const uint32_t I = (uint32_t)(dx < 0 ? Im : (dx > 0 ? Ip : I0));
+        const uint32_t J = (uint32_t)(dy < 0 ? Jm : (dy > 0 ? Jp : J0));
+        const uint32_t K = (uint32_t)(dz < 0 ? Km : (dz > 0 ? Kp : K0));
+        const uint2 m = t.row_meta[(__umul24(I, (uint32_t)g.PY) + J) * wpr + (K >> 5)];
+        const uint32_t b = K & 31u;
+        pres |= ((m.x >> b) & 1u) << q;
+        nb[q] = m.y + __popc(m.x & ((1u << b) - 1u));
+      }
+      // Nodes with slots load their inputs (an input written by a node without
+      // slots is w_q after the first step); the others have nothing to load.
+      // Temporal accesses: a wavefront's slots start anywhere in a cache line, so
+      // lines are shared with the neighbouring wavefronts and are merged in L2.
+      if (pres & 1u) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+          const bool from = ((pres >> q) & 1u) != 0u;
+          float v = WQ(q);
+          if (from || (first && parity == 0))
+            v = Store<ST>::ld(at(q + parity * (OPP(q) - q), parity ? nb[q] : nb[0]), q);
+          f[q] = v;
+        }
+      }
+    }
   }
 
   // Each lane's first entry of every list (and second link entry), requested
@@ -545,19 +605,34 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
       __builtin_nontemporal_store(uy, v_o + x);
       __builtin_nontemporal_store(uz, w_o + x);
     }
+    if constexpr (!CMP) {
 #pragma unroll
-    for (int q = 0; q < 19; ++q)
-      Store<ST>::st_nt(at(OPP(q) + parity * (q - OPP(q)), pslot(CX(q), CY(q), CZ(q))), f[q], q);
+      for (int q = 0; q < 19; ++q)
+        Store<ST>::st_nt(at(OPP(q) + parity * (q - OPP(q)), pslot(CX(q), CY(q), CZ(q))), f[q], q);
+    } else if (pres & 1u) {
+      // even: the node's own slots; odd: slots of x + c_q, where that node has slots
+#pragma unroll
+      for (int q = 0; q < 19; ++q)
+        if (parity == 0 || ((pres >> OPP(q)) & 1u))
+          Store<ST>::st(at(OPP(q) + parity * (q - OPP(q)), parity ? nb[OPP(q)] : nb[0]), f[q], q);
+    }
     // copy-self on non-periodic faces through the ghost layer
     if (fluid && ((g.gx && (i == 0 || i == g.nx - 1)) || (g.gy && (j == 0 || j == g.ny - 1)) ||
                   (g.gz && (k == 0 || k == g.nz - 1)))) {
 #pragma unroll
       for (int q = 1; q < 19; ++q) {
         if (!oob_source(g, i, j, k, q)) continue;
-        if (parity == 0)
-          Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), f[q], OPP(q));
-        else
-          Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + self, f[q], q);
+        if constexpr (!CMP) {
+          if (parity == 0)
+            Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + slot(-CX(q), -CY(q), -CZ(q)), f[q], OPP(q));
+          else
+            Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + self, f[q], q);
+        } else {
+          if (parity == 0)
+            Store<ST>::st(pop + (size_t)OPP(q) * (size_t)g.Ns + nb[q], f[q], OPP(q));
+          else
+            Store<ST>::st(pop + (size_t)q * (size_t)g.Ns + nb[0], f[q], q);
+        }
       }
     }
   }
@@ -633,11 +708,12 @@ __device__ __forceinline__ void node_inputs(const ST* pop, const Grid& g, const 
 }
 
 template <typename ST>
-__device__ __forceinline__ void store_inputs(ST* pop, const Grid& g, int parity, int64_t x, const float f[19]) {
+__device__ __forceinline__ void store_inputs(ST* pop, const Grid& g, const Tables& t, int parity, int64_t x,
+                                             const float f[19]) {
   const int64_t nynz = (int64_t)g.ny * g.nz;
   const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
 #pragma unroll
-  for (int q = 0; q < 19; ++q) store_plain(pop, g, parity, i, j, k, q, f[q]);
+  for (int q = 0; q < 19; ++q) store_plain(pop, g, t, parity, i, j, k, q, f[q]);
 }
 
 // Reference-layout copy of the populations entering the next step
@@ -685,9 +761,11 @@ __global__ __launch_bounds__(kBlock) void aa_load_kernel(ST* __restrict__ pop, G
   if (!live) return;
   const int64_t nynz = (int64_t)g.ny * g.nz;
   const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
-  const int64_t self = slot_node(g, i, j, k, 0, 0, 0);
+  int64_t self;
+  if (node_storage(g, t, i, j, k, 0, 0, 0, self)) {
 #pragma unroll
-  for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (int64_t)q * g.Ns + self, in[q * g.N + x], q);
+    for (int q = 0; q < 19; ++q) Store<ST>::st(pop + (int64_t)q * g.Ns + self, in[q * g.N + x], q);
+  }
   if (special) {
     const int64_t sidx = t.wave_base[x >> 6] + __popcll(bal & ((1ull << lane) - 1ull));
     const int4 m = t.sp_mask[sidx], b = t.sp_base[sidx];
@@ -740,7 +818,7 @@ __global__ __launch_bounds__(kBlock) void aa_zou_he_kernel(ST* __restrict__ pop,
     float gi[19];
     node_inputs(pop, g, t, parity, first, fin, bc, gi);
     if (uax * (float)sign < 0.f) {
-      store_inputs(pop, g, parity, fb, gi);
+      store_inputs(pop, g, t, parity, fb, gi);
       return;
     }
     float r, jx, jy, jz;
@@ -768,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void aa_zou_he_kernel(ST* __restrict__ pop,
                     2.f * ((float)(CX(q) * CY(q)) * pxy + (float)(CX(q) * CZ(q)) * pxz + (float)(CY(q) * CZ(q)) * pyz);
     out[q] = feq[q] + WQ(q) * s * 4.5f;
   }
-  store_inputs(pop, g, parity, fb, out);
+  store_inputs(pop, g, t, parity, fb, out);
 }
 
 // Convective outlet on the AA state, pass 1 (copy + partial sums) and pass 2
@@ -794,7 +872,7 @@ __global__ __launch_bounds__(kBlock) void aa_convective_pass1(ST* __restrict__ p
         if (C_AX(q, axis) == unknown_cax) a[q] = b2[q];
         rho += a[q];
       }
-      store_inputs(pop, g, parity, fo, a);
+      store_inputs(pop, g, t, parity, fo, a);
       r = rho; cnt = 1.0;
     }
   }
@@ -835,8 +913,8 @@ __global__ __launch_bounds__(kBlock) void aa_convective_pass2(ST* __restrict__ p
   const int i = (int)(fo / nynz), j = (int)((fo / g.nz) % g.ny), k = (int)(fo % g.nz);
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const float v = load_plain(pop, g, parity, i, j, k, q) * s_corr;
-    store_plain(pop, g, parity, i, j, k, q, v);
+    const float v = load_plain(pop, g, t, parity, i, j, k, q) * s_corr;
+    store_plain(pop, g, t, parity, i, j, k, q, v);
   }
 }
 
@@ -875,6 +953,9 @@ struct fc_aa_tables {
   float* B;
   const int32_t* ff_idx;
   int64_t n_v;
+  const int32_t* row_meta;  // [rows * wpr][2], or null for dense storage
+  int wpr;
+  int64_t n_store;
 };
 
 struct fc_aa_collision {
@@ -902,8 +983,17 @@ static int pack(Grid& g, Tables& t, const fc_aa_tables* tb, int nx, int ny, int 
   t.lane_nodes = tb->lane_nodes; t.n_lanes = tb->n_lanes;
   FC_REQUIRE(t.n_lanes >= 0 && t.n_lanes <= g.N, "aa: lane count out of range");
   FC_REQUIRE(t.lane_nodes || t.n_lanes == g.N, "aa: lane list required unless there is one lane per node");
+  FC_REQUIRE(!tb->row_meta || !t.lane_nodes, "aa: compact storage runs with one lane per node");
   t.sp_mask = (const int4*)tb->sp_mask; t.sp_base = (const int4*)tb->sp_base;
   t.w1 = tb->w1; t.w2 = tb->w2; t.A = tb->A; t.B = tb->B; t.ff_idx = tb->ff_idx; t.n_v = tb->n_v;
+  t.row_meta = (const uint2*)tb->row_meta;
+  t.wpr = tb->wpr;
+  if (t.row_meta) {
+    FC_REQUIRE(t.wpr == (g.PZ + 31) / 32, "aa: compact storage needs %d words per row (got %d)", (g.PZ + 31) / 32,
+               t.wpr);
+    FC_REQUIRE(tb->n_store > 0 && tb->n_store <= g.Ns, "aa: compact storage count out of range");
+    g.Ns = tb->n_store;  // stride between the direction arrays
+  }
   return 0;
 }
 
@@ -933,9 +1023,13 @@ int fc_aa_step(void* pop, int storage, int nx, int ny, int nz, int px, int py, i
   }
   if (t.n_lanes == 0) return 0;
   const unsigned grid = grid_for(t.n_lanes, kBlock);
-#define FC_STEP(T, K)                                                                                          \
-  hipLaunchKernelGGL((aa_step_kernel<T, K>), dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, c, parity, first, \
-                     rho, u, v, w)
+#define FC_STEP(T, K)                                                                                            \
+  if (t.row_meta)                                                                                                  \
+    hipLaunchKernelGGL((aa_step_kernel<T, K, true>), dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, c, parity, \
+                       first, rho, u, v, w);                                                                       \
+  else                                                                                                             \
+    hipLaunchKernelGGL((aa_step_kernel<T, K, false>), dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, c,        \
+                       parity, first, rho, u, v, w)
 #define FC_STEP_KIND(T)                      \
   switch (c.kind) {                          \
     case 0: FC_STEP(T, 0); break;            \

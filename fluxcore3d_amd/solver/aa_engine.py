@@ -59,6 +59,54 @@ def _neighbor_solid_masks(solid: np.ndarray, periodic) -> np.ndarray:
     return S
 
 
+def build_storage(nx, ny, nz, solid: np.ndarray, periodic, bcs=()) -> dict:
+    """Compact population storage: which nodes of the padded grid own slots, and
+    the per-row records that turn padded coordinates into storage indices.
+
+    A node owns slots (is present) when it is a fluid node, a ghost node of a
+    non-periodic face (copy-self values) or a solid node upstream of a node on an
+    inlet/outlet plane (the plane kernels rebuild every input of their nodes,
+    including those streamed from solids). Inputs streamed from a node without slots
+    are w_q (the reference streams f_post = W out of solids, lbm_kernels.cu:277-283).
+
+    Storage index of padded node (I, J, K): with R = I * PY + J and word w = K / 32,
+    row_meta[R * wpr + w] = (mask, start): bit b of mask is the presence of K = 32 w + b
+    and start counts the present nodes before the word, so the index is
+    start + popcount(mask & ((1 << b) - 1)). Nodes are numbered in padded C order."""
+    solid = np.asarray(solid, bool).reshape(nx, ny, nz)
+    g = [0 if p else 1 for p in periodic]
+    PX, PY, PZ = nx + 2 * g[0], ny + 2 * g[1], nz + 2 * g[2]
+    present = np.ones((PX, PY, PZ), bool)
+    present[g[0]:g[0] + nx, g[1]:g[1] + ny, g[2]:g[2] + nz] = ~solid
+    plane_nodes = np.zeros((nx, ny, nz), bool)
+    for bc in bcs:
+        sel = [slice(None)] * 3
+        sel[int(bc["axis"])] = int(bc["plane"])
+        plane_nodes[tuple(sel)] = True
+    plane_nodes &= ~solid
+    if plane_nodes.any():
+        ii, jj, kk = np.nonzero(plane_nodes)
+        for q in range(1, 19):
+            c = C_NP[q]
+            src = []
+            for a, (v, n) in enumerate(((ii, nx), (jj, ny), (kk, nz))):
+                w = v - c[a]
+                src.append(w + 1 if g[a] else np.mod(w, n))
+            present[src[0], src[1], src[2]] = True
+    wpr = (PZ + 31) // 32
+    bits = np.zeros((PX * PY, wpr * 32), bool)
+    bits[:, :PZ] = present.reshape(PX * PY, PZ)
+    words = bits.reshape(-1, 32)
+    mask = (words.astype(np.uint64) << np.arange(32, dtype=np.uint64)).sum(1).astype(np.uint32)
+    cnt = words.sum(1).astype(np.int64)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    n_store = int(cnt.sum())
+    if n_store >= 2 ** 30:
+        raise ValueError("compact storage of more than 2^30 nodes")
+    row_meta = np.stack([mask.view(np.int32), start.astype(np.int32)], 1)
+    return dict(row_meta=row_meta, wpr=wpr, n_store=n_store, present=present, PY=PY, PZ=PZ, gpad=tuple(g))
+
+
 def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="reference", bcs=(),
                  compact: bool = True) -> dict:
     """Node/link tables of the fused step (pure NumPy, node order)."""
@@ -207,7 +255,8 @@ class AAEngine:
 
     def __init__(self, nx, ny, nz, solid_flow, *, periodic=(False, False, False), collision="bgk",
                  omega=1.0, omega_minus=1.0, S=None, cs_dx2=0.0, force=None, sponge=None, sponge_axis=0,
-                 links=None, ibb_mode="reference", bcs=(), storage="fp32", device="cuda", lanes="auto"):
+                 links=None, ibb_mode="reference", bcs=(), storage="fp32", device="cuda", lanes="auto",
+                 layout="auto"):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.N = self.nx * self.ny * self.nz
         self.periodic = tuple(bool(p) for p in periodic)
@@ -227,6 +276,16 @@ class AAEngine:
         if lanes not in ("fluid", "all"):
             raise ValueError("lanes must be 'auto', 'fluid' or 'all'")
         self.lanes = lanes
+        # Population layout: "dense" (every node of the padded grid owns 19 slots) or
+        # "compact" (only fluid, ghost and plane-feeding solid nodes own slots,
+        # build_storage; one lane per node). "auto" = "dense".
+        if layout == "auto":
+            layout = "dense"
+        if layout not in ("dense", "compact"):
+            raise ValueError("layout must be 'auto', 'dense' or 'compact'")
+        if layout == "compact" and lanes != "all":
+            raise ValueError("compact layout runs with one lane per node (lanes='all')")
+        self.layout = layout
         self.device = torch.device(device)
         self.ibb_mode = ibb_mode
         self.bcs = [dict(b) for b in bcs]
@@ -235,7 +294,11 @@ class AAEngine:
         solid = np.ascontiguousarray(np.asarray(solid_flow, dtype=bool).reshape(self.nx, self.ny, self.nz))
         self._build_tables(solid, links)
         self._collision = self._make_collision(collision, omega, omega_minus, S, cs_dx2, force, sponge, sponge_axis)
-        self.Ns = int(_lib.call("fc_aa_slot_count", self.nx, self.ny, self.nz, *[int(p) for p in self.periodic]))
+        if self.layout == "compact":
+            self.Ns = self._n_store
+        else:
+            self.Ns = int(_lib.call("fc_aa_slot_count", self.nx, self.ny, self.nz,
+                                    *[int(p) for p in self.periodic]))
         self.pop = torch.zeros(19 * self.Ns, dtype=STORAGE_DTYPES[storage], device=self.device)
         self.t = 0
         self.t_loaded = 0
@@ -247,6 +310,12 @@ class AAEngine:
         tb = build_tables(self.nx, self.ny, self.nz, solid, links, self.periodic, self.ibb_mode, self.bcs,
                           compact=self.lanes == "fluid")
         self.n_links, self.n_v, self.n_special = tb["n_links"], tb["n_v"], tb["n_special"]
+        if self.layout == "compact":
+            sb = build_storage(self.nx, self.ny, self.nz, solid, self.periodic, self.bcs)
+            tb.update(row_meta=sb["row_meta"], wpr=sb["wpr"], n_store=sb["n_store"])
+            self._n_store = sb["n_store"]
+        else:
+            tb.update(row_meta=None, wpr=0, n_store=0)
         self._upload(tb)
 
     def _upload(self, tb) -> None:
@@ -277,12 +346,13 @@ class AAEngine:
         self.B_init = dt(np.concatenate([B_init, B_init]) if self.n_v else np.zeros(2, np.float32), torch.float32)
         self.B = self.B_init.clone()
         self.ff_idx = dt(tb["ff_idx"], torch.int32)
+        self.row_meta = None if tb["row_meta"] is None else dt(tb["row_meta"], torch.int32)
         self._tables = _lib.AATables(
             self.nflag.data_ptr(), self.wave_base.data_ptr(), self.wave_meta.data_ptr(), self.lk_code.data_ptr(),
             self.s_code.data_ptr(), self.f_code.data_ptr(), _lib.ptr(self.lane_nodes), self.n_lanes,
             self.sp_mask.data_ptr(), self.sp_base.data_ptr(),
             self.w1.data_ptr(), self.w2.data_ptr(), self.A.data_ptr(), self.B.data_ptr(), self.ff_idx.data_ptr(),
-            max(self.n_v, 1))
+            max(self.n_v, 1), _lib.ptr(self.row_meta), int(tb["wpr"]), int(tb["n_store"]))
 
     def _make_collision(self, collision, omega, omega_minus, S, cs_dx2, force, sponge, sponge_axis):
         c = _lib.AACollision()

@@ -101,6 +101,14 @@ typedef struct fc_aa_tables {
   float *B;                 /* [2][n_v] second-node part, by step parity */
   const int32_t *ff_idx;    /* [n_publish] slot in B written by each publishing node */
   int64_t n_v;
+  /* Compact storage (NULL: dense, every padded node owns slots). Per padded row
+   * R = I * PY + J and word w = K / 32: {mask, start}, bit b of mask set when node
+   * K = 32 w + b owns slots, start = nodes owning slots before the word (padded C
+   * order); storage index = start + popcount(mask & ((1 << b) - 1)). Nodes without
+   * slots stream w_q. One lane per node only. */
+  const int32_t *row_meta;  /* [PX * PY * wpr][2] */
+  int wpr;                  /* words per row, (PZ + 31) / 32 */
+  int64_t n_store;          /* nodes owning slots: the population arrays are [19][n_store] */
 } fc_aa_tables;
 
 typedef struct fc_aa_collision {

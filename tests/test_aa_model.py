@@ -7,7 +7,7 @@ the CPU, independently of the GPU.
 import numpy as np
 import pytest
 
-from fluxcore3d_amd.solver.aa_engine import build_tables
+from fluxcore3d_amd.solver.aa_engine import build_storage, build_tables
 from oracle import lbm as olbm
 from tests import cases
 
@@ -34,8 +34,24 @@ def _group_of(entry_base, n_entries):
     return np.searchsorted(entry_base, np.arange(n_entries), side="right") - 1
 
 
-def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="reference", compact=True):
+def _locate(sb, I, J, K):
+    """Presence and storage index of padded nodes, as the fused kernel computes them
+    from the per-row records (mask, start) of build_storage."""
+    rec = sb["row_meta"][(I * sb["PY"] + J) * sb["wpr"] + (K >> 5)]
+    mask = rec[:, 0].view(np.uint32).astype(np.uint64)
+    b = (K & 31).astype(np.uint64)
+    below = mask & ((np.uint64(1) << b) - np.uint64(1))
+    cnt = np.zeros(I.shape, np.int64)
+    for bit in range(32):
+        cnt += ((below >> np.uint64(bit)) & np.uint64(1)).astype(np.int64)
+    return ((mask >> b) & np.uint64(1)).astype(bool), rec[:, 1].astype(np.int64) + cnt
+
+
+def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="reference", compact=True,
+              storage="dense"):
     tb = build_tables(nx, ny, nz, solid, links, periodic, ibb_mode, compact=compact)
+    if storage == "compact":
+        return _model_run_compact(nx, ny, nz, solid, links, periodic, f0, steps, om, tb)
     N = nx * ny * nz
     gx, gy, gz = (0 if p else 1 for p in periodic)
     PY, PZ = ny + 2 * gy, nz + 2 * gz
@@ -102,6 +118,120 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
         A = w1 * fp[l_opp, l_node] + np.where(l_hasv, 0.0, w2 * fp[l_t, l_node])
         B[P][tb["ff_idx"]] = fp[f_p, f_node]
     return macros
+
+
+def _model_run_compact(nx, ny, nz, solid, links, periodic, f0, steps, om, tb):
+    """One lane per node over compact storage: lanes whose node has no slots do
+    nothing; an input whose writer x - c_q has no slots is w_q (after the first
+    step); a store to x + c_q happens only where that node has slots."""
+    sb = build_storage(nx, ny, nz, solid, periodic)
+    N = nx * ny * nz
+    gx, gy, gz = sb["gpad"]
+    ii, jj, kk = np.unravel_index(np.arange(N), (nx, ny, nz))
+
+    def loc(d):
+        return _locate(sb, _wrap_or_pad(ii, d[0], nx, gx), _wrap_or_pad(jj, d[1], ny, gy),
+                       _wrap_or_pad(kk, d[2], nz, gz))
+
+    def oob(q):
+        src = (ii - C[q, 0], jj - C[q, 1], kk - C[q, 2])
+        o = np.zeros(N, bool)
+        for a, (g, n) in enumerate(((gx, nx), (gy, ny), (gz, nz))):
+            if g:
+                o |= (src[a] < 0) | (src[a] >= n)
+        return o
+
+    wm = tb["wave_meta"].astype(np.int64)
+    lk, fc = tb["lk_code"].astype(np.int64), tb["f_code"].astype(np.int64)
+    l_node = _group_of(wm[:, 0], lk.size) * 64 + (lk & 63)
+    l_t, l_opp = (lk >> 6) & 31, (lk >> 11) & 31
+    l_hasv, l_ovr = ((lk >> 16) & 1).astype(bool), ((lk >> 17) & 1).astype(bool)
+    l_v = wm[_group_of(wm[:, 0], lk.size), 1] + (lk >> 18)
+    f_node = _group_of(wm[:, 2], fc.size) * 64 + (fc & 63)
+    f_p = (fc >> 6) & 31
+    self_p, self_i = loc((0, 0, 0))
+    up = [loc(-C[q]) for q in range(19)]  # writer x - c_q of input q
+    assert np.array_equal(self_p, ~solid.reshape(-1))
+    pop = np.full((19, sb["n_store"]), np.nan)
+    pop[:, self_i[self_p]] = f0[:, self_p]
+    A = f0[tb["link_dir"], tb["link_node"]].astype(np.float64)
+    B = [tb["B_init"].astype(np.float64), tb["B_init"].astype(np.float64)]
+    fluid = ~solid.reshape(-1)
+    w1, w2 = tb["w1"].astype(np.float64), tb["w2"].astype(np.float64)
+    macros = None
+    for t in range(steps):
+        P, first = t & 1, t == 0
+        f = np.empty((19, N))
+        for q in range(19):
+            up_p, up_i = up[q]
+            if P == 0:
+                f[q] = np.where(fluid, pop[q, np.where(fluid, self_i, 0)], 0.0)
+            else:
+                f[q] = np.where(up_p, pop[OPP[q], np.where(up_p, up_i, 0)], 0.0)
+            if not first:
+                f[q] = np.where(up_p, f[q], olbm.W32[q])
+        val = A.copy()
+        if not first:
+            val[l_hasv] += w2[l_hasv] * B[P ^ 1][l_v[l_hasv]]
+        f[l_t[l_ovr], l_node[l_ovr]] = val[l_ovr]
+        fp, rho, u, v, w = olbm.macro_collide("bgk", f, (~fluid).astype(np.uint8), om1=om)
+        assert np.isfinite(fp[:, fluid]).all()
+        macros = (rho, u, v, w)
+        for q in range(19):
+            if P == 0:
+                pop[OPP[q], self_i[fluid]] = fp[q, fluid]
+                o = oob(q) & fluid
+                pop[OPP[q], up[q][1][o]] = fp[q, o]
+            else:
+                dn_p, dn_i = up[OPP[q]]  # x + c_q
+                wsel = fluid & dn_p
+                pop[q, dn_i[wsel]] = fp[q, wsel]
+                o = oob(q) & fluid
+                pop[q, self_i[o]] = fp[q, o]
+        A = w1 * fp[l_opp, l_node] + np.where(l_hasv, 0.0, w2 * fp[l_t, l_node])
+        B[P][tb["ff_idx"]] = fp[f_p, f_node]
+    return macros
+
+
+def test_storage_records_count_present_nodes():
+    """build_storage: present = fluid, ghost and plane-feeding solid nodes; the
+    storage index of every present node is its rank in padded C order."""
+    c = cases.config1()
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    bcs = [dict(axis=0, plane=0), dict(axis=0, plane=nx - 1)]
+    sb = build_storage(nx, ny, nz, c["solid_flow"], (0, 0, 0), bcs)
+    pres = sb["present"]
+    assert pres.shape == (nx + 2, ny + 2, nz + 2)
+    inner = pres[1:-1, 1:-1, 1:-1]
+    assert np.all(inner[~c["solid_flow"]])
+    # walls next to the inlet/outlet planes feed plane nodes: they keep slots
+    assert inner[0][c["solid_flow"][0]].any() and not inner[nx // 2][c["solid_flow"][nx // 2]].any()
+    I, J, K = (a.astype(np.int64) for a in np.nonzero(np.ones_like(pres)))
+    p, idx = _locate(sb, I, J, K)
+    assert np.array_equal(p, pres.reshape(-1))
+    assert np.array_equal(idx[p], np.arange(sb["n_store"]))
+
+
+@pytest.mark.parametrize("case", ["periodic_pack", "walled_box"])
+def test_compact_storage_model_matches_oracle(case):
+    if case == "periodic_pack":
+        c = cases.periodic_pack(n=24, seed=3)
+        per, solid = (1, 1, 1), c["solid"]
+    else:
+        c = cases.config1()
+        per, solid = (0, 0, 0), c["solid_flow"]
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    N = nx * ny * nz
+    rng = np.random.default_rng(0)
+    f0 = olbm.feq(1 + 0.01 * rng.random(N), *(0.02 * rng.standard_normal((3, N))), np.float64)
+    om = 1.0 / 0.55
+    steps = 7
+    mm = model_run(nx, ny, nz, solid, c["links"], per, f0, steps, om, compact=False, storage="compact")
+    f = f0.copy()
+    for _ in range(steps):
+        f, *mo = olbm.step(f, solid, nx, ny, nz, per, "bgk", om, 1.0, None, 0.0, c["links"])
+    for a, b in zip(mm, mo):
+        assert np.abs(a - b).max() < 1e-12
 
 
 @pytest.mark.parametrize("compact", [True, False])
