@@ -101,25 +101,15 @@ __device__ float sdf_tri(const Sdf& s, float x, float y, float z) {
   return cxy0 + fz * (cxy1 - cxy0);
 }
 
-__global__ __launch_bounds__(kBlock) void link_q_kernel(Sdf s, const int64_t* __restrict__ fluid_flat,
-                                                        const int64_t* __restrict__ dir_i, int64_t n, int n_coarse,
-                                                        int n_bisect, float* __restrict__ q_out,
-                                                        float* __restrict__ w1, float* __restrict__ w2,
-                                                        uint8_t* __restrict__ ge) {
-  const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (l >= n) return;
+// q in (0, 1] along the segment x0 + t d, t in [0, L] (bfl_q_from_sdf,
+// sdf_tracing.py:127-184): n_coarse + 1 samples at torch.linspace(0, 1) * L, the
+// first + -> (not +) crossing, n_bisect halvings keeping the solid-side end hi,
+// q = hi / (L + 1e-30); 0.5 without a crossing.
+__device__ float q_search(const Sdf& s, float x0, float y0, float z0, float dx, float dy, float dz, float L,
+                          int n_coarse, int n_bisect) {
   // torch.linspace(0, 1, n_coarse + 1) in FP32
   const int steps = n_coarse + 1, half = steps / 2;
   const float step = 1.0f / (float)(steps - 1);
-  const int64_t f = fluid_flat[l];
-  const float fi = (float)(f / ((int64_t)s.ny * s.nz)), fj = (float)((f / s.nz) % s.ny), fk = (float)(f % s.nz);
-  const float x0 = fi * s.dx + s.ox, y0 = fj * s.dx + s.oy, z0 = fk * s.dx + s.oz;
-  const int q = (int)dir_i[l];
-  const float cx = (float)CX(q), cy = (float)CY(q), cz = (float)CZ(q);
-  const float cn = sqrtf(cx * cx + cy * cy + cz * cz);
-  const float den = cn + 1e-30f;
-  const float dx = cx / den, dy = cy / den, dz = cz / den;
-  const float L = s.dx * cn;
   float t_prev = 0.f, p_prev = 0.f, lo = 0.f, hi = 0.f;
   bool found = false;
   for (int k = 0; k < steps; ++k) {
@@ -134,23 +124,80 @@ __global__ __launch_bounds__(kBlock) void link_q_kernel(Sdf s, const int64_t* __
     t_prev = t;
     p_prev = p;
   }
-  float qv = 0.5f;
-  if (found) {
-    for (int it = 0; it < n_bisect; ++it) {
-      const float mid = 0.5f * (lo + hi);
-      const float pm = sdf_tri(s, x0 + mid * dx, y0 + mid * dy, z0 + mid * dz);
-      if (pm > 0.0f) lo = mid; else hi = mid;
-    }
-    qv = hi / (L + 1e-30f);
+  if (!found) return 0.5f;
+  for (int it = 0; it < n_bisect; ++it) {
+    const float mid = 0.5f * (lo + hi);
+    const float pm = sdf_tri(s, x0 + mid * dx, y0 + mid * dy, z0 + mid * dz);
+    if (pm > 0.0f) lo = mid; else hi = mid;
   }
+  return hi / (L + 1e-30f);
+}
+
+// BFL weights (bfl_weights_from_q, sdf_tracing.py:187-205)
+__device__ __forceinline__ void bfl_weights(float qv, float& w1, float& w2, uint8_t& ge) {
+  const float a = qv >= 0.5f ? 1.0f / (2.0f * qv) : 2.0f * qv;
+  w1 = a;
+  w2 = 1.0f - a;
+  ge = qv >= 0.5f;
+}
+
+__global__ __launch_bounds__(kBlock) void link_q_kernel(Sdf s, const int64_t* __restrict__ fluid_flat,
+                                                        const int64_t* __restrict__ dir_i, int64_t n, int n_coarse,
+                                                        int n_bisect, float* __restrict__ q_out,
+                                                        float* __restrict__ w1, float* __restrict__ w2,
+                                                        uint8_t* __restrict__ ge) {
+  const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (l >= n) return;
+  const int64_t f = fluid_flat[l];
+  const float fi = (float)(f / ((int64_t)s.ny * s.nz)), fj = (float)((f / s.nz) % s.ny), fk = (float)(f % s.nz);
+  const float x0 = fi * s.dx + s.ox, y0 = fj * s.dx + s.oy, z0 = fk * s.dx + s.oz;
+  const int q = (int)dir_i[l];
+  const float cx = (float)CX(q), cy = (float)CY(q), cz = (float)CZ(q);
+  const float cn = sqrtf(cx * cx + cy * cy + cz * cz);
+  const float den = cn + 1e-30f;
+  const float dx = cx / den, dy = cy / den, dz = cz / den;
+  const float L = s.dx * cn;
+  const float qv = q_search(s, x0, y0, z0, dx, dy, dz, L, n_coarse, n_bisect);
   q_out[l] = qv;
-  if (qv >= 0.5f) {
-    const float a = 1.0f / (2.0f * qv);
-    w1[l] = a; w2[l] = 1.0f - a; ge[l] = 1;
-  } else {
-    const float a = 2.0f * qv;
-    w1[l] = a; w2[l] = 1.0f - a; ge[l] = 0;
-  }
+  bfl_weights(qv, w1[l], w2[l], ge[l]);
+}
+
+// sdf_trilinear (sdf_tracing.py:88-125) at arbitrary points x[n][3]
+__global__ __launch_bounds__(kBlock) void sdf_points_kernel(Sdf s, const float* __restrict__ x, int64_t n,
+                                                            float* __restrict__ out) {
+  const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (l >= n) return;
+  out[l] = sdf_tri(s, x[3 * l], x[3 * l + 1], x[3 * l + 2]);
+}
+
+// bfl_q_from_sdf for arbitrary rays: start points x0[n][3], unit directions d[n][3], lengths L[n]
+__global__ __launch_bounds__(kBlock) void q_rays_kernel(Sdf s, const float* __restrict__ x0,
+                                                        const float* __restrict__ d, const float* __restrict__ L,
+                                                        int64_t n, int n_coarse, int n_bisect,
+                                                        float* __restrict__ q_out) {
+  const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (l >= n) return;
+  q_out[l] = q_search(s, x0[3 * l], x0[3 * l + 1], x0[3 * l + 2], d[3 * l], d[3 * l + 1], d[3 * l + 2], L[l],
+                      n_coarse, n_bisect);
+}
+
+__global__ __launch_bounds__(kBlock) void weights_kernel(const float* __restrict__ q, int64_t n,
+                                                         uint8_t* __restrict__ ge, float* __restrict__ w1,
+                                                         float* __restrict__ w2) {
+  const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (l >= n) return;
+  bfl_weights(q[l], w1[l], w2[l], ge[l]);
+}
+
+int make_sdf(Sdf& s, const float* phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch, int px,
+             int py, int pz) {
+  FC_REQUIRE(nx > 1 && ny > 1 && nz > 1, "sdf: phi extents must be > 1");
+  FC_REQUIRE(pitch > 0.f, "sdf: pitch must be > 0");
+  FC_REQUIRE(phi, "sdf: null phi");
+  s.phi = phi; s.nx = nx; s.ny = ny; s.nz = nz; s.ox = ox; s.oy = oy; s.oz = oz; s.dx = pitch;
+  s.hx = (float)((double)nx - 1.001); s.hy = (float)((double)ny - 1.001); s.hz = (float)((double)nz - 1.001);
+  s.px = px != 0; s.py = py != 0; s.pz = pz != 0;
+  return 0;
 }
 
 }  // namespace
@@ -176,12 +223,42 @@ int fc_links_q(const float* phi, int nx, int ny, int nz, float ox, float oy, flo
   if (n == 0) return 0;
   FC_REQUIRE(phi && fluid_flat && dir_i && q && w1 && w2 && ge, "links_q: null pointer argument");
   Sdf s;
-  s.phi = phi; s.nx = nx; s.ny = ny; s.nz = nz; s.ox = ox; s.oy = oy; s.oz = oz; s.dx = pitch;
-  s.hx = (float)((double)nx - 1.001); s.hy = (float)((double)ny - 1.001); s.hz = (float)((double)nz - 1.001);
-  s.px = px != 0; s.py = py != 0; s.pz = pz != 0;
+  if (int rc = make_sdf(s, phi, nx, ny, nz, ox, oy, oz, pitch, px, py, pz)) return rc;
   hipLaunchKernelGGL(link_q_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, s, fluid_flat, dir_i, n,
                      n_coarse, n_bisect, q, w1, w2, ge);
   return check_launch("links_q");
+}
+
+int fc_sdf_trilinear(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch,
+                     const float* x, int64_t n, float* out, hipStream_t stream) {
+  FC_REQUIRE(n >= 0, "sdf_trilinear: n must be >= 0");
+  Sdf s;
+  if (int rc = make_sdf(s, phi, nx, ny, nz, ox, oy, oz, pitch, 0, 0, 0)) return rc;
+  if (n == 0) return 0;
+  FC_REQUIRE(x && out, "sdf_trilinear: null pointer argument");
+  hipLaunchKernelGGL(sdf_points_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, s, x, n, out);
+  return check_launch("sdf_trilinear");
+}
+
+int fc_bfl_q_rays(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch,
+                  const float* x0, const float* d, const float* L, int64_t n, int n_coarse, int n_bisect, float* q,
+                  hipStream_t stream) {
+  FC_REQUIRE(n >= 0 && n_coarse >= 1 && n_bisect >= 0, "bfl_q_rays: n >= 0, n_coarse >= 1, n_bisect >= 0 required");
+  Sdf s;
+  if (int rc = make_sdf(s, phi, nx, ny, nz, ox, oy, oz, pitch, 0, 0, 0)) return rc;
+  if (n == 0) return 0;
+  FC_REQUIRE(x0 && d && L && q, "bfl_q_rays: null pointer argument");
+  hipLaunchKernelGGL(q_rays_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, s, x0, d, L, n, n_coarse,
+                     n_bisect, q);
+  return check_launch("bfl_q_rays");
+}
+
+int fc_bfl_weights(const float* q, int64_t n, uint8_t* ge, float* w1, float* w2, hipStream_t stream) {
+  FC_REQUIRE(n >= 0, "bfl_weights: n must be >= 0");
+  if (n == 0) return 0;
+  FC_REQUIRE(q && ge && w1 && w2, "bfl_weights: null pointer argument");
+  hipLaunchKernelGGL(weights_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, q, n, ge, w1, w2);
+  return check_launch("bfl_weights");
 }
 
 }  // extern "C"

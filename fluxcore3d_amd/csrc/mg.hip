@@ -715,6 +715,72 @@ __global__ __launch_bounds__(kBlock) void residual_f32_kernel(const float* __res
   block_partials<1>(v, partial);
 }
 
+// ---- Chebyshev smoother (chebyshev_smooth, mg_pcg.py:195-247) ----------------
+// lambda_max = max(diagL * dt + 1): per-block maxima, then one folding block
+__global__ __launch_bounds__(kBlock) void cheb_lam_kernel(const float* __restrict__ diagL, int64_t n, float dt,
+                                                          float* __restrict__ partial) {
+  __shared__ float sh[kBlock / 64];
+  float m = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    m = fmaxf(m, diagL[i] * dt + 1.0f);
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float a = sh[0];
+    for (int w = 1; w < kBlock / 64; ++w) a = fmaxf(a, sh[w]);
+    partial[blockIdx.x] = a;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void max_fold_kernel(const float* __restrict__ partial, int nparts,
+                                                          float* __restrict__ out) {
+  __shared__ float sh[kBlock / 64];
+  float m = -INFINITY;
+  for (int p = threadIdx.x; p < nparts; p += kBlock) m = fmaxf(m, partial[p]);
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float a = sh[0];
+    for (int w = 1; w < kBlock / 64; ++w) a = fmaxf(a, sh[w]);
+    out[0] = a;
+  }
+}
+
+// first step: r = (b - Lo) mask, p = invD r, x_new = x + p
+__global__ __launch_bounds__(kBlock) void cheb_first_kernel(const float* __restrict__ x, const float* __restrict__ b,
+                                                            const float* __restrict__ Lo,
+                                                            const float* __restrict__ invD,
+                                                            const float* __restrict__ mask, float* __restrict__ p,
+                                                            float* __restrict__ x_new, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float r = (b[i] - Lo[i]) * mask[i];
+  const float pv = invD[i] * r;
+  p[i] = pv;
+  x_new[i] = x[i] + pv;
+}
+
+// later steps: p = rho_new (s (invD r) + rho_prev p), x_new += p; the last step
+// writes x = x_new mask
+__global__ __launch_bounds__(kBlock) void cheb_step_kernel(float* __restrict__ x_new, const float* __restrict__ b,
+                                                           const float* __restrict__ Lo,
+                                                           const float* __restrict__ invD,
+                                                           const float* __restrict__ mask, float* __restrict__ p,
+                                                           float rho_new, float s, float rho_prev,
+                                                           float* __restrict__ x_final, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float m = mask[i];
+  const float r = (b[i] - Lo[i]) * m;
+  const float pn = rho_new * (s * (invD[i] * r) + rho_prev * p[i]);
+  const float xn = x_new[i] + pn;
+  p[i] = pn;
+  x_new[i] = xn;
+  if (x_final) x_final[i] = xn * m;
+}
+
 int make_level(Level& g, int nx, int ny, int nz, int ax, int out_idx) {
   FC_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "mg: every extent must be >= 2 (got %d,%d,%d)", nx, ny, nz);
   // 32-bit cell indices, including the stride of the capped reduction grids
@@ -1002,5 +1068,34 @@ int fc_mg_residual_f32(const float* b, const float* Lo, const float* mask, float
 }
 
 int fc_mg_scalar_slots(void) { return S_N; }
+
+int fc_mg_cheb_lambda_max(const float* diagL, int64_t n, double dt, float* partial, float* out, hipStream_t stream) {
+  FC_REQUIRE(diagL && partial && out && n > 0, "cheb_lambda_max: bad arguments");
+  const unsigned gsz = reduce_grid(n);
+  hipLaunchKernelGGL(cheb_lam_kernel, dim3(gsz), dim3(kBlock), 0, stream, diagL, n, (float)dt, partial);
+  if (int rc = check_launch("cheb_lambda_max")) return rc;
+  hipLaunchKernelGGL(max_fold_kernel, dim3(1), dim3(kBlock), 0, stream, (const float*)partial, (int)gsz, out);
+  return check_launch("cheb_lambda_max/fold");
+}
+
+int fc_mg_cheb_first(const float* x, const float* b, const float* Lo, const float* invD, const float* mask, float* p,
+                     float* x_new, int64_t n, hipStream_t stream) {
+  FC_REQUIRE(x && b && Lo && invD && mask && p && x_new && n >= 0, "cheb_first: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(cheb_first_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x, b, Lo, invD, mask, p,
+                     x_new, n);
+  return check_launch("cheb_first");
+}
+
+int fc_mg_cheb_step(float* x_new, const float* b, const float* Lo, const float* invD, const float* mask, float* p,
+                    double rho_new, double s, double rho_prev, float* x_final, int64_t n, hipStream_t stream) {
+  FC_REQUIRE(x_new && b && Lo && invD && mask && p && n >= 0, "cheb_step: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(cheb_step_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x_new, b, Lo, invD, mask,
+                     p, (float)rho_new, (float)s, (float)rho_prev, x_final, n);
+  return check_launch("cheb_step");
+}
+
+int fc_mg_cheb_partials_count(int64_t n) { return (int)reduce_grid(n); }
 
 }  // extern "C"

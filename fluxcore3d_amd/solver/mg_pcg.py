@@ -220,17 +220,89 @@ def pcg_mg(mg_solver, b, x0=None, tol: float = 1e-3, max_it: int = 15):
     return x, {"converged": info["converged"], "it": info["it"], "res_rel": info["res_rel"]}
 
 
-def patch_mg_pcg(mg_solver, pe_threshold: float = 1.5):
+_lib.register({
+    "fc_mg_cheb_partials_count": (_lib.I32, [_lib.I64]),
+    "fc_mg_cheb_lambda_max": (_lib.I32, [_lib.P, _lib.I64, _lib.F64, _lib.P, _lib.P, _lib.P]),
+    "fc_mg_cheb_first": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.I64, _lib.P]),
+    "fc_mg_cheb_step": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.P, _lib.F64, _lib.F64, _lib.F64,
+                                   _lib.P, _lib.I64, _lib.P]),
+}, unchecked={"fc_mg_cheb_partials_count"})
+
+
+def chebyshev_smooth(mg_solver, level: int, n_steps: int = 2, lam_min_ratio: float = 0.1) -> None:
+    """Chebyshev smoothing of levels[level]['x'] against levels[level]['b_buf'], in
+    place (mg_pcg.py:195-247): lambda_max = max(diagL dt + 1), lambda_min =
+    lam_min_ratio lambda_max, d and c the centre and half-width of that interval;
+    x_1 = x + invD r, then n_steps - 1 recurrences p = rho (2 d / c invD r +
+    rho_prev p) with rho = 1 / (2 d / c - rho_prev); the result is masked (not for
+    n_steps = 1, as in the reference). Every array operation runs in HIP kernels."""
+    lv = mg_solver._levels[level]
+    n = int(lv["x"].numel())
+    stream = mg_solver._stream()
+    if "cheb_p" not in lv:
+        lv["cheb_p"] = torch.empty_like(lv["x"])
+        lv["cheb_x"] = torch.empty_like(lv["x"])
+        lv["cheb_partial"] = torch.empty(int(_lib.call("fc_mg_cheb_partials_count", n)) + 1, dtype=torch.float32,
+                                         device=lv["x"].device)
+    part = lv["cheb_partial"]
+    _lib.call("fc_mg_cheb_lambda_max", lv["diagL"].data_ptr(), n, float(mg_solver._dt_cache), part[1:].data_ptr(),
+              part.data_ptr(), stream)
+    lam_max = float(part[0].item())
+    lam_min = lam_min_ratio * lam_max
+    d = (lam_max + lam_min) / 2.0
+    c = (lam_max - lam_min) / 2.0
+    x, b, p, x_new = lv["x"], lv["b_buf"], lv["cheb_p"], lv["cheb_x"]
+    inv, mask = lv["inv_diagM"], lv["mask_f"]
+    mg_solver._Mop(x, level)
+    _lib.call("fc_mg_cheb_first", x.data_ptr(), b.data_ptr(), lv["Lo"].data_ptr(), inv.data_ptr(), mask.data_ptr(),
+              p.data_ptr(), x_new.data_ptr(), n, stream)
+    if n_steps == 1:
+        x.copy_(x_new)
+        return
+    rho_prev = 1.0
+    for k in range(1, n_steps):
+        rho_new = 1.0 / (2.0 * d / c - rho_prev)
+        mg_solver._Mop(x_new, level)
+        _lib.call("fc_mg_cheb_step", x_new.data_ptr(), b.data_ptr(), lv["Lo"].data_ptr(), inv.data_ptr(),
+                  mask.data_ptr(), p.data_ptr(), rho_new, 2.0 * d / c, rho_prev,
+                  x.data_ptr() if k == n_steps - 1 else None, n, stream)
+        rho_prev = rho_new
+
+
+def patch_mg_pcg(mg_solver, pe_threshold: float = 1.5, nonsymmetric_route: Optional[str] = None):
     """Route mg_solver.solve through PCG or BiCGSTAB (mg_pcg.py:250-275): with the
     reference's heuristic pe = 0.1 * max(diagL) of the finest level, above
-    ``pe_threshold`` bicgstab_mg, else pcg_mg."""
+    ``pe_threshold`` bicgstab_mg, else pcg_mg.
+
+    The heuristic looks at diagL only, so a non-symmetric system (advection, or a
+    varying rho*c_p) with a small diagL goes to (flexible) CG, which can diverge on it:
+    the 192x192x384 heat sink with advection reaches a relative residual of 4e24.
+    ``nonsymmetric_route`` = "gcr" or "fgmres" (default None: the reference's routing)
+    sends every system whose finest level keeps per-cell coefficients (not the
+    symmetric face form) to gcr_fp64 (6 directions) or fgmres_fp64 (16), which
+    minimise the residual. The info dict names the solver that ran ("route")."""
     import types
 
+    if nonsymmetric_route not in (None, "gcr", "fgmres"):
+        raise ValueError("nonsymmetric_route must be None, 'gcr' or 'fgmres'")
+
     def _solve_pcg(self, b, x0=None, max_cycles=15, tol=1e-3):
-        pe = float(self._levels[0]["diagL"].max()) * 0.1
+        lv0 = self._levels[0]
+        if nonsymmetric_route is not None and lv0.get("faces") is None:
+            if nonsymmetric_route == "gcr":
+                x, info = gcr_fp64(self, b, x0=x0, tol=tol, max_it=max_cycles, m=6)
+            else:
+                x, info = fgmres_fp64(self, b, x0=x0, tol=tol, max_it=max_cycles, m=16)
+            info["route"] = nonsymmetric_route
+            return x, info
+        pe = float(lv0["diagL"].max()) * 0.1
         if pe > pe_threshold:
-            return bicgstab_mg(self, b, x0, tol=tol, max_it=max_cycles)
-        return pcg_mg(self, b, x0, tol=tol, max_it=max_cycles)
+            x, info = bicgstab_mg(self, b, x0, tol=tol, max_it=max_cycles)
+            info["route"] = "bicgstab"
+        else:
+            x, info = pcg_mg(self, b, x0, tol=tol, max_it=max_cycles)
+            info["route"] = "pcg"
+        return x, info
 
     mg_solver.solve = types.MethodType(_solve_pcg, mg_solver)
     return mg_solver

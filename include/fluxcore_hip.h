@@ -156,6 +156,17 @@ int fc_links_mask(const uint8_t *solid, int nx, int ny, int nz, int px, int py, 
 int fc_links_q(const float *phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch, int px, int py,
                int pz, const int64_t *fluid_flat, const int64_t *dir_i, int64_t n, int n_coarse, int n_bisect,
                float *q, float *w1, float *w2, uint8_t *q_ge_half, hipStream_t stream);
+/* sdf_trilinear (sdf_tracing.py:88-125): phi at points x[n][3] (world units,
+ * origin (ox, oy, oz), spacing pitch), grid coordinates clamped to [0, n - 1.001] */
+int fc_sdf_trilinear(const float *phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch,
+                     const float *x, int64_t n, float *out, hipStream_t stream);
+/* bfl_q_from_sdf (sdf_tracing.py:127-184) for arbitrary rays: start points
+ * x0[n][3], unit directions d[n][3], lengths L[n] -> q[n] */
+int fc_bfl_q_rays(const float *phi, int nx, int ny, int nz, float ox, float oy, float oz, float pitch,
+                  const float *x0, const float *d, const float *L, int64_t n, int n_coarse, int n_bisect, float *q,
+                  hipStream_t stream);
+/* bfl_weights_from_q (sdf_tracing.py:187-205): q >= 1/2, w1, w2 */
+int fc_bfl_weights(const float *q, int64_t n, uint8_t *q_ge_half, float *w1, float *w2, hipStream_t stream);
 
 
 /* ---- geometric multigrid (mg_kernels.cu / nlevel_mg_cuda.py) ---- */
@@ -220,6 +231,17 @@ int fc_mg_residual_f32(const float *b, const float *Lo, const float *mask, float
  * 3 * fc_mg_partials_count(n) doubles, scal fc_mg_scalar_slots() doubles */
 int fc_mg_partials_count(int64_t n);
 int fc_mg_scalar_slots(void);
+/* chebyshev_smooth (mg_pcg.py:195-247) building blocks on one level:
+ * lambda_max = max(diagL dt + 1) into out[0] (partial: fc_mg_cheb_partials_count(n) floats);
+ * first step r = (b - Lo) mask, p = invD r, x_new = x + p;
+ * later steps p = rho_new (s invD r + rho_prev p), x_new += p, and x_final = x_new mask
+ * when x_final is not NULL (FP32, the scalars rounded to FP32 as the reference's tensor ops do) */
+int fc_mg_cheb_partials_count(int64_t n);
+int fc_mg_cheb_lambda_max(const float *diagL, int64_t n, double dt, float *partial, float *out, hipStream_t stream);
+int fc_mg_cheb_first(const float *x, const float *b, const float *Lo, const float *invD, const float *mask, float *p,
+                     float *x_new, int64_t n, hipStream_t stream);
+int fc_mg_cheb_step(float *x_new, const float *b, const float *Lo, const float *invD, const float *mask, float *p,
+                    double rho_new, double s, double rho_prev, float *x_final, int64_t n, hipStream_t stream);
 int fc_mg_apply_A64(const double *p, double *Ap, const double *r, const float *mask, const fc_mg_coef *coef,
                     double dt, int nx, int ny, int nz, int ax, int out_idx, double *partial, double *scal,
                     hipStream_t stream);

@@ -239,6 +239,40 @@ def bfl_q(phi, origin, pitch, fluid_flat, dir_i, n_coarse=6, n_bisect=10):
     return q, w1, w2, ge.astype(bool)
 
 
+def sdf_points(phi, origin, pitch, x):
+    """sdf_trilinear (sdf_tracing.py:88-125) at points x (..., 3), FP32 eager op order."""
+    phi = np.ascontiguousarray(phi, np.float32)
+    pts = np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1, 3))
+    out = np.empty(pts.shape[0], np.float32)
+    org = np.ascontiguousarray(np.asarray(origin, np.float32))
+    _lib(np.float32).orc_sdf_points(_p(phi), *[C.c_int(v) for v in phi.shape], _p(org), C.c_float(pitch), _p(pts),
+                                    C.c_int64(pts.shape[0]), _p(out))
+    return out.reshape(np.asarray(x).shape[:-1])
+
+
+def q_rays(phi, origin, pitch, x0, d, L, n_coarse=6, n_bisect=10):
+    """bfl_q_from_sdf (sdf_tracing.py:127-184) for arbitrary rays, FP32 eager op order."""
+    phi = np.ascontiguousarray(phi, np.float32)
+    x0 = np.ascontiguousarray(np.asarray(x0, np.float32).reshape(-1, 3))
+    d = np.ascontiguousarray(np.asarray(d, np.float32).reshape(-1, 3))
+    L = np.ascontiguousarray(np.asarray(L, np.float32).reshape(-1))
+    q = np.empty(L.size, np.float32)
+    org = np.ascontiguousarray(np.asarray(origin, np.float32))
+    _lib(np.float32).orc_q_rays(_p(phi), *[C.c_int(v) for v in phi.shape], _p(org), C.c_float(pitch), _p(x0), _p(d),
+                                _p(L), C.c_int64(L.size), C.c_int(n_coarse), C.c_int(n_bisect), _p(q))
+    return q
+
+
+def bfl_weights(q):
+    """bfl_weights_from_q (sdf_tracing.py:187-205): (q >= 1/2, w1, w2)."""
+    q = np.ascontiguousarray(np.asarray(q, np.float32).reshape(-1))
+    ge = np.empty(q.size, np.uint8)
+    w1 = np.empty(q.size, np.float32)
+    w2 = np.empty(q.size, np.float32)
+    _lib(np.float32).orc_bfl_weights(_p(q), C.c_int64(q.size), _p(ge), _p(w1), _p(w2))
+    return ge.astype(bool), w1, w2
+
+
 def linspace01(steps):
     out = np.empty(steps, np.float32)
     _lib(np.float32).orc_linspace01(C.c_int(steps), _p(out))

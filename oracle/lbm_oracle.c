@@ -363,6 +363,42 @@ void orc_linspace01(int steps, float *out) {
   for (int i = 0; i < steps; ++i) out[i] = i < half ? 0.0f + step * (float)i : 1.0f - step * (float)(steps - i - 1);
 }
 
+/* bfl_q_from_sdf for one ray (sdf_tracing.py:127-184): samples at
+ * linspace(0, 1) * L along x0 + t d, first + -> (not +) crossing, bisection
+ * keeping hi on the solid side, q = hi / (L + 1e-30); 0.5 without a crossing. */
+static float q_ray(const float *phi, int nx, int ny, int nz, const float origin[3], float pitch, const float x0[3],
+                   const float d[3], float L, int n_coarse, int n_bisect, const float *tv) {
+  float tt[64], ph[64];
+  for (int s = 0; s <= n_coarse; ++s) {
+    tt[s] = tv[s] * L;
+    float p[3];
+    for (int a = 0; a < 3; ++a) { float m = tt[s] * d[a]; p[a] = x0[a] + m; }
+    ph[s] = sdf_tri(phi, nx, ny, nz, origin, pitch, p[0], p[1], p[2]);
+  }
+  int idx = -1;
+  for (int s = 0; s < n_coarse; ++s)
+    if (ph[s] > 0.0f && !(ph[s + 1] > 0.0f)) { idx = s; break; }
+  if (idx < 0) return 0.5f;
+  float lo = tt[idx], hi = tt[idx + 1];
+  for (int it = 0; it < n_bisect; ++it) {
+    float mid = lo + hi;
+    mid = 0.5f * mid;
+    float p[3];
+    for (int a = 0; a < 3; ++a) { float m = mid * d[a]; p[a] = x0[a] + m; }
+    const float pm = sdf_tri(phi, nx, ny, nz, origin, pitch, p[0], p[1], p[2]);
+    if (pm > 0.0f) lo = mid; else hi = mid;
+  }
+  return hi / (L + 1e-30f);
+}
+
+/* bfl_weights_from_q (sdf_tracing.py:187-205) */
+static void bfl_w(float qv, float *w1, float *w2, uint8_t *ge) {
+  const int g = qv >= 0.5f;
+  *ge = (uint8_t)g;
+  if (g) { const float a = 1.0f / (2.0f * qv); *w1 = a; *w2 = 1.0f - a; }
+  else { const float a = 2.0f * qv; *w1 = a; *w2 = 1.0f - a; }
+}
+
 /* build_ibb_from_sdf_gpu + bfl_q_from_sdf + bfl_weights_from_q
  * (sdf_tracing.py:127-259): for each link, x0 = (i,j,k)*pitch + origin,
  * d = c/|c|, L = pitch*|c|, coarse samples then bisection, q = hi/(L+1e-30),
@@ -382,35 +418,28 @@ void orc_bfl_q(const float *phi, int nx, int ny, int nz, const float origin[3], 
     const float den = cn + 1e-30f;
     const float d[3] = {craw[0] / den, craw[1] / den, craw[2] / den};
     const float L = pitch * cn;
-    float tt[64], ph[64];
-    for (int s = 0; s <= n_coarse; ++s) {
-      tt[s] = tv[s] * L;
-      float p[3];
-      for (int a = 0; a < 3; ++a) { float m = tt[s] * d[a]; p[a] = x0[a] + m; }
-      ph[s] = sdf_tri(phi, nx, ny, nz, origin, pitch, p[0], p[1], p[2]);
-    }
-    int idx = -1;
-    for (int s = 0; s < n_coarse; ++s)
-      if (ph[s] > 0.0f && !(ph[s + 1] > 0.0f)) { idx = s; break; }
-    float qv = 0.5f;
-    if (idx >= 0) {
-      float lo = tt[idx], hi = tt[idx + 1];
-      for (int it = 0; it < n_bisect; ++it) {
-        float mid = lo + hi;
-        mid = 0.5f * mid;
-        float p[3];
-        for (int a = 0; a < 3; ++a) { float m = mid * d[a]; p[a] = x0[a] + m; }
-        const float pm = sdf_tri(phi, nx, ny, nz, origin, pitch, p[0], p[1], p[2]);
-        if (pm > 0.0f) lo = mid; else hi = mid;
-      }
-      qv = hi / (L + 1e-30f);
-    }
-    q_out[l] = qv;
-    const int g = qv >= 0.5f;
-    ge[l] = (uint8_t)g;
-    if (g) { const float a = 1.0f / (2.0f * qv); w1[l] = a; w2[l] = 1.0f - a; }
-    else { const float a = 2.0f * qv; w1[l] = a; w2[l] = 1.0f - a; }
+    q_out[l] = q_ray(phi, nx, ny, nz, origin, pitch, x0, d, L, n_coarse, n_bisect, tv);
+    bfl_w(q_out[l], &w1[l], &w2[l], &ge[l]);
   }
+}
+
+/* sdf_trilinear (sdf_tracing.py:88-125) at points x[n][3] */
+void orc_sdf_points(const float *phi, int nx, int ny, int nz, const float origin[3], float pitch, const float *x,
+                    int64_t n, float *out) {
+  for (int64_t l = 0; l < n; ++l) out[l] = sdf_tri(phi, nx, ny, nz, origin, pitch, x[3 * l], x[3 * l + 1], x[3 * l + 2]);
+}
+
+/* bfl_q_from_sdf (sdf_tracing.py:127-184) for arbitrary rays x0[n][3], d[n][3], L[n] */
+void orc_q_rays(const float *phi, int nx, int ny, int nz, const float origin[3], float pitch, const float *x0,
+                const float *d, const float *L, int64_t n, int n_coarse, int n_bisect, float *q_out) {
+  float tv[64];
+  orc_linspace01(n_coarse + 1, tv);
+  for (int64_t l = 0; l < n; ++l) q_out[l] = q_ray(phi, nx, ny, nz, origin, pitch, x0 + 3 * l, d + 3 * l, L[l], n_coarse, n_bisect, tv);
+}
+
+/* bfl_weights_from_q (sdf_tracing.py:187-205) */
+void orc_bfl_weights(const float *q, int64_t n, uint8_t *ge, float *w1, float *w2) {
+  for (int64_t l = 0; l < n; ++l) bfl_w(q[l], &w1[l], &w2[l], &ge[l]);
 }
 
 int orc_num_threads(void) {

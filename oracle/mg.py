@@ -325,6 +325,31 @@ def thermal_step_guarded(h: Hierarchy, T_old, T0, src, mask, dt, T_lo, T_hi, tol
     return reconstruct(T0, theta, h.ax, h.levels[0]["out_idx"]), hist
 
 
+def chebyshev_smooth(h: Hierarchy, level, x, b, n_steps=2, lam_min_ratio=0.1):
+    """chebyshev_smooth (mg_pcg.py:195-247) on level ``level`` of ``h`` (dtype of x);
+    returns the smoothed x."""
+    lv = h.levels[level]
+    d_ = x.dtype.type
+    lam_max = float((lv["co"][6] * d_(h.dt) + d_(1.0)).max())
+    lam_min = lam_min_ratio * lam_max
+    d = (lam_max + lam_min) / 2.0
+    c = (lam_max - lam_min) / 2.0
+    m, inv = lv["mask"], lv["invD"]
+    r = (b - h.mop(x, level)) * m
+    p = inv * r
+    x_new = x + p
+    if n_steps == 1:
+        return x_new
+    rho_prev = 1.0
+    for _ in range(1, n_steps):
+        rho_new = 1.0 / (2.0 * d / c - rho_prev)
+        r = (b - h.mop(x_new, level)) * m
+        p = d_(rho_new) * (d_(2.0 * d / c) * (inv * r) + d_(rho_prev) * p)
+        x_new = x_new + p
+        rho_prev = rho_new
+    return x_new * m
+
+
 def fcg_mg_fp64(h: Hierarchy, b, tol=1e-8, max_it=100, truncation=1):
     """Flexible CG (Notay 2000, truncation 1) with FP64 vectors and residual;
     the preconditioner (one FP32 V-cycle of ``h``) sees the residual cast to

@@ -497,3 +497,53 @@ def test_fgmres_exact_breakdown_with_identity_operator():
     assert np.isfinite(x64).all() and np.isfinite(info["hist"]).all()
     assert info["converged"] and info["it"] == 1, info
     np.testing.assert_array_equal(x64, b.astype(np.float64))
+
+
+@pytest.mark.parametrize("n_steps", [1, 2, 3])
+@pytest.mark.parametrize("level", [0, 1])
+def test_chebyshev_smooth_matches_oracle(level, n_steps):
+    """chebyshev_smooth (mg_pcg.py:195-247) on a level of the config-1 conduction
+    hierarchy against the oracle's FP32 restatement (1e-6 of the largest value)."""
+    from fluxcore3d_amd.solver.mg_pcg import chebyshev_smooth
+    c = thermal_cases.sphere_conduction(64, 32, 32, dt=10.0)
+    s, h = _solver_pair(c)
+    shape = s._levels[level]["shape"]
+    rng = np.random.default_rng(level * 10 + n_steps)
+    x0 = rng.standard_normal(shape).astype(np.float32)
+    b = rng.standard_normal(shape).astype(np.float32)
+    lv = s._levels[level]
+    lv["x"].copy_(T(x0).view_as(lv["x"]))
+    lv["b_buf"].copy_(T(b).view_as(lv["b_buf"]))
+    chebyshev_smooth(s, level, n_steps=n_steps)
+    want = mg.chebyshev_smooth(h, level, x0, b, n_steps)
+    got = lv["x"].cpu().numpy().reshape(shape)
+    assert rel(got, want) < 1e-6, rel(got, want)
+    assert np.abs(got - x0).max() > 1e-3  # it smoothed
+
+
+@pytest.mark.parametrize("route", ["gcr", "fgmres"])
+def test_patch_mg_pcg_nonsymmetric_route(route):
+    """patch_mg_pcg(nonsymmetric_route=...): the heat-sink system with advection (1/4
+    scale, per-cell coefficients) that the reference's heuristic sends to CG (here with
+    a threshold above its pe) goes to the minimal-residual solver and reaches 1e-8; a
+    symmetric conduction system (face form) keeps the reference's routing."""
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.solver.mg_pcg import patch_mg_pcg
+    from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
+    c = geometry.cht_problem(scale=0.25, device=DEV)
+    s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, c["ax"], c["sign"], c["s"], omega=0.8)
+    s.build_if_needed(c["k"], c["rcp"], c["u"], c["v"], c["w"], c["mask"], c["dt"])
+    assert s._levels[0].get("faces") is None
+    pe = float(s._levels[0]["diagL"].max()) * 0.1
+    patch_mg_pcg(s, pe_threshold=2.0 * pe)
+    _, info = s.solve(c["b"], max_cycles=40, tol=1e-8)
+    assert info["route"] == "pcg"
+    patch_mg_pcg(s, pe_threshold=2.0 * pe, nonsymmetric_route=route)
+    _, info = s.solve(c["b"], max_cycles=300, tol=1e-8)
+    assert info["route"] == route and info["converged"] and info["res_rel"] < 1e-8, info["res_rel"]
+    sym = thermal_cases.sphere_conduction(32, 16, 16, dt=1.0e3)
+    s2, _ = _solver_pair(sym, omega=0.8)
+    assert s2._levels[0].get("faces") is not None
+    patch_mg_pcg(s2, pe_threshold=1e30, nonsymmetric_route=route)
+    _, info2 = s2.solve(T(sym["b"]), max_cycles=100, tol=1e-8)
+    assert info2["route"] == "pcg" and info2["converged"]

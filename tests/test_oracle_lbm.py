@@ -434,3 +434,38 @@ def test_sponge_sigma_one_gives_feq_and_partial_sigma_interpolates():
     j = C.T.astype(np.float64) @ got
     j_want = (1 - sig) * (C.T.astype(np.float64) @ f) + sig * rho * u
     assert np.abs(j - j_want).max() < 1e-7 * np.abs(rho * u).max()
+
+
+def test_sdf_points_is_the_trilinear_interpolant():
+    """sdf_trilinear (sdf_tracing.py:88-125): the trilinear interpolant of phi at grid
+    coordinates clamped to [0, n - 1.001] (scipy map_coordinates, order 1, in FP64)."""
+    from scipy.ndimage import map_coordinates
+    rng = np.random.default_rng(12)
+    phi = rng.standard_normal((9, 7, 11)).astype(np.float32)
+    origin, pitch = np.array([0.3, -1.2, 2.0], np.float32), 0.5
+    g = rng.uniform(-2.0, 12.0, (500, 3))
+    x = (origin + pitch * g).astype(np.float32)
+    got = lbm.sdf_points(phi, origin, pitch, x)
+    gg = (x.astype(np.float64) - origin) / pitch
+    gg = np.clip(gg, 0.0, np.array(phi.shape) - 1.001)
+    want = map_coordinates(phi.astype(np.float64), gg.T, order=1, mode="nearest")
+    assert np.abs(got - want).max() < 1e-5 * np.abs(phi).max()
+
+
+def test_q_rays_reproduce_the_link_rays_and_weights_closed_form():
+    """The generic-ray q search gives the link q bit for bit for the link rays, and
+    the weights satisfy w1 = 1/(2q), w2 = 1 - w1 (q >= 1/2), w1 = 2q, w2 = 1 - 2q."""
+    phi = _sphere_phi()
+    links = lbm.ibb_from_sdf(phi < 0, phi)
+    x0 = np.stack(np.unravel_index(links["fluid_flat"], phi.shape), 1).astype(np.float32)
+    c = C[links["dir_i"]].astype(np.float32)
+    cn = np.sqrt((c * c).sum(1)).astype(np.float32)
+    d = (c / (cn + np.float32(1e-30))[:, None]).astype(np.float32)
+    q = lbm.q_rays(phi, (0, 0, 0), 1.0, x0, d, cn)
+    np.testing.assert_array_equal(q, links["q"])
+    qq = np.concatenate([links["q"], np.float32([0.5, 1.0, 0.25, 1e-3])])
+    ge, w1, w2 = lbm.bfl_weights(qq)
+    np.testing.assert_array_equal(ge, qq >= 0.5)
+    assert np.abs(np.where(ge, w1 * 2 * qq, w1 / (2 * qq)) - 1).max() < 1e-6
+    assert np.abs(w1 + w2 - 1).max() < 1e-6
+    assert w1[-4] == 1 and w2[-4] == 0

@@ -165,3 +165,41 @@ def test_reconstruct_matches_face_loop(ax, out_idx):
                 break
         want[c] = val[tuple(src)]
     np.testing.assert_array_equal(mg.reconstruct(T0, th, ax, out_idx), want)
+
+
+def _uniform_box(nx=6, ny=9, nz=5, kval=2.0):
+    k = np.full((nx, ny, nz), kval)
+    z = np.zeros_like(k)
+    return mg.Hierarchy(k, np.ones_like(k), z, z, z, np.ones_like(k, dtype=bool), 0, 1, 1.0, omega=0.8,
+                        min_coarse_cells=1)
+
+
+@pytest.mark.parametrize("n_steps", [1, 2, 3])
+def test_chebyshev_smooth_fixed_point_and_eigenmode(n_steps):
+    """chebyshev_smooth (mg_pcg.py:195-247), FP64: the exact solution of A x = b is a
+    fixed point, and on an eigenvector v of A (uniform conduction box, v = cos(pi a j /
+    (ny - 1)) across the transverse y axis, where the mirror ghosts make it exact) the
+    error is multiplied by the smoother's polynomial in the eigenvalue m of invD A,
+    m = omega (1 + 2 a_y dt (1 - cos theta)) / (1 + dt diagL)."""
+    h = _uniform_box()
+    h.set_dt(0.7)
+    nx, ny, nz = h.levels[0]["shape"]
+    rng = np.random.default_rng(1)
+    x_star = rng.standard_normal((nx, ny, nz))
+    b = h.mop(x_star, 0)
+    np.testing.assert_allclose(mg.chebyshev_smooth(h, 0, x_star, b, n_steps), x_star, atol=1e-12)
+    co = h.levels[0]["co"]
+    a_y, diag = float(co[2][0, 2, 0]), float(co[6][0, 2, 0])
+    theta = np.pi * 3 / (ny - 1)
+    v = np.broadcast_to(np.cos(theta * np.arange(ny))[None, :, None], (nx, ny, nz)).copy()
+    np.testing.assert_allclose(h.mop(v, 0), (1 + 0.7 * 2 * a_y * (1 - np.cos(theta))) * v, atol=1e-12)
+    m = 0.8 * (1 + 0.7 * 2 * a_y * (1 - np.cos(theta))) / (1 + 0.7 * diag)
+    lam_max = 1 + 0.7 * diag
+    d, c = 0.55 * lam_max, 0.45 * lam_max
+    e, p, rho_prev = 1 - m, -m, 1.0  # scalar error recurrence of the smoother
+    for _ in range(1, n_steps):
+        rho = 1.0 / (2 * d / c - rho_prev)
+        p = rho * (2 * d / c * (-m * e) + rho_prev * p)
+        e, rho_prev = e + p, rho
+    got = mg.chebyshev_smooth(h, 0, x_star + v, b, n_steps) - x_star
+    np.testing.assert_allclose(got, e * v, atol=1e-12)
