@@ -410,6 +410,12 @@ class AAEngine:
         """One time step; writes pre-collision macros if the four outputs are given.
         Solid nodes hold the constants (1, 0, 0, 0); they are written when the
         output buffers differ from the previous step's and skipped otherwise."""
+        self.step_fused(rho, u, v, w)
+        self.apply_bcs()
+
+    def step_fused(self, rho=None, u=None, v=None, w=None) -> None:
+        """The fused collide/stream/Bouzidi launch of step(), without the inlet/outlet
+        plane kernels (apply_bcs() must follow before the next step)."""
         stream = _lib.stream_handle(self.device)
         ptrs = (_lib.ptr(rho), _lib.ptr(u), _lib.ptr(v), _lib.ptr(w))
         solid_macros = int(ptrs != self._macro_ptrs)
@@ -417,7 +423,10 @@ class AAEngine:
         _lib.call("fc_aa_step", self.pop.data_ptr(), STORAGE_CODES[self.storage], *self._dims, self._tables,
                   self._collision, self.t & 1, int(self.t == self.t_loaded), *ptrs, solid_macros, stream)
         self.t += 1
-        self._apply_bcs(stream)
+
+    def apply_bcs(self) -> None:
+        """Inlet/outlet plane kernels on the inputs of the next step."""
+        self._apply_bcs(_lib.stream_handle(self.device))
 
     def _apply_bcs(self, stream) -> None:
         parity = self.t & 1

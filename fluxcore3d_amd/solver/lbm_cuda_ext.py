@@ -444,23 +444,58 @@ class LBMCUDAUpgrade:
 
 
 def add_step_profiler(sim, print_every: int = 200):
-    """Synchronising per-step timer around sim.step (lbm_cuda_ext.py:693-748)."""
-    sim._prof = {"total": 0.0, "n": 0}
-    inner = sim.step
+    """Synchronising per-phase step timer (lbm_cuda_ext.py:693-748), injected after
+    LBMCUDAUpgrade.patch. With the reference kernel sequence the phases are the
+    reference's (collide, sponge, stream, ibb, bc); the fused step has two launches
+    per step, the fused collide/stream/Bouzidi kernel ("fused") and the inlet/outlet
+    plane kernels ("bc"). Prints ms per phase and steps per second every
+    ``print_every`` steps."""
+    fused = "_fc_engine" in sim.__dict__
+    phases = ("fused", "bc") if fused else ("collide", "sponge", "stream", "ibb", "bc")
+    sim._prof = {k: 0.0 for k in phases + ("total",)}
+    sim._prof["n"] = 0
+
+    def _sync_time():
+        torch.cuda.synchronize()
+        return time.perf_counter()
 
     def _timed_step(self, do_thermal=True):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        out = inner(do_thermal)
-        torch.cuda.synchronize()
+        t = [_sync_time()]
+        if fused:
+            eng = self.__dict__["_fc_engine"]
+            eng.step_fused(self.rho.view(-1), self.u.view(-1), self.v.view(-1), self.wv.view(-1))
+            t.append(_sync_time())
+            eng.apply_bcs()
+            t.append(_sync_time())
+        else:
+            self._collide()
+            t.append(_sync_time())
+            if self._sponge_sigma_flat is not None:
+                self._apply_sponge_zone(self._f_post)
+            t.append(_sync_time())
+            self._stream()
+            t.append(_sync_time())
+            if getattr(self, "ibb_data", None) is not None and self.ibb_data.n_links > 0:
+                self._interp_bounce_back_from_post(self._f_post)
+            else:
+                self._bounce_back_post_stream()
+            t.append(_sync_time())
+            self._apply_flow_bcs()
+            t.append(_sync_time())
         p = self._prof
-        p["total"] += time.perf_counter() - t0
+        for k, a, b in zip(phases, t[:-1], t[1:]):
+            p[k] += b - a
+        p["total"] += t[-1] - t[0]
         p["n"] += 1
         if p["n"] % print_every == 0:
             n = p["n"]
-            mlups = n * self.nx * self.ny * self.nz / p["total"] / 1e6
-            print(f"[PROFILE] {n} steps  {p['total'] / n * 1e3:.3f} ms/step  {mlups:.1f} MLUPS")
-        return out
+            print(f"[PROFILE] {n} steps  (ms/step)")
+            for k in phases + ("total",):
+                print(f"  {k:10s}: {p[k] / n * 1e3:.3f} ms")
+            print(f"  it/s      : {n / p['total']:.1f}")
+        if not fused and do_thermal and callable(getattr(self, "_original_step_thermal", None)):
+            return self._original_step_thermal()
+        return (0, 0)
 
     sim.step = types.MethodType(_timed_step, sim)
     print(f"[PROFILER] Injected. Prints every {print_every} steps.")

@@ -351,3 +351,30 @@ def test_compact_and_dense_layouts_agree_bit_for_bit(case, kind, storage):
     for a, b in zip(out["dense"], out["compact"]):
         np.testing.assert_array_equal(a, b)
     assert np.abs(out["dense"][1]).max() > 1e-4
+
+
+@pytest.mark.parametrize("engine", ["aa", "reference"])
+def test_step_profiler_phases_same_state(engine, capsys):
+    """add_step_profiler (lbm_cuda_ext.py:693-748): per-phase times (the reference's
+    phases for the kernel sequence; fused kernel and plane kernels for the fused
+    step), printed every print_every steps, and the same state as the plain step."""
+    from fluxcore3d_amd.solver.lbm_cuda_ext import LBMCUDAUpgrade, add_step_profiler
+    c = cases.config1()
+    sims = []
+    for prof in (False, True):
+        sim = _sim_for(c, "bgk", engine)
+        LBMCUDAUpgrade.patch(sim)
+        if prof:
+            add_step_profiler(sim, print_every=5)
+        for _ in range(10):
+            sim.step(do_thermal=False)
+        sims.append(sim)
+    torch.cuda.synchronize()
+    for name in ("rho", "u", "v", "wv", "f"):
+        assert torch.equal(getattr(sims[0], name), getattr(sims[1], name)), name
+    p = sims[1]._prof
+    want = {"fused", "bc"} if engine == "aa" else {"collide", "sponge", "stream", "ibb", "bc"}
+    assert set(p) == want | {"total", "n"} and p["n"] == 10
+    assert all(p[k] >= 0 for k in want) and p["total"] >= sum(p[k] for k in want) * 0.999
+    out = capsys.readouterr().out
+    assert out.count("[PROFILE] ") == 2 and "it/s" in out
