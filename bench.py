@@ -24,6 +24,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_FLOPS = 2.5e15
 METRIC = "D3Q19 MLUPS (%HBM roofline) + thermal solve s to 1e-8 resid, NN-PCG vs GMG-PCG"
 
 
@@ -50,6 +51,74 @@ def parse():
     p.add_argument("--unet-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "fluxcore3d_amd",
                                                           "data", "unet_precond.npz"))
     return p.parse_args()
+
+
+# Algorithmic HBM bytes per cell of the multigrid and Krylov kernels (FP32 V-cycle,
+# FP64 Krylov vectors). Face form: 3 face-coefficient arrays (12 B) instead of the 6
+# coefficients + diagL (28 B) per cell.
+#   V-cycle level (not the coarsest): first two sweeps from x = 0 (b, invD, coefficients,
+#   write x), residual + restriction (x, b, mask, coefficients, 1/8 coarse write),
+#   prolongation + correction (1/8 coarse read, x, mask, write x), two sweeps (x,
+#   coefficients, b, invD, write): 24 + 24.5 + 12.5 + 2 * 28 = 117 B (face form),
+#   40 + 40.5 + 12.5 + 2 * 44 = 181 B (per-cell arrays); coarsest level: one pass of
+#   b, invD, coefficients and x (24 / 40 B), its 20 sweeps run in LDS.
+#   Krylov work per iteration on the finest level: FCG dot(z, Ap_old) 12 + new direction
+#   20 + A p with p.Ap, p.r 40 (56) + x, r update 52 = 124 (140) B; GCR(6) 280 (296) B;
+#   FGMRES(16) 184 (200) B averaged over a restart cycle. True residual: 44 (60) B, twice
+#   per solve.
+_VC_LEVEL_B = (117.0, 181.0)
+_VC_COARSE_B = (24.0, 40.0)
+_KRYLOV_B = {"fcg": (124.0, 140.0), "gcr6": (280.0, 296.0), "fgmres16": (184.0, 200.0)}
+_TRUE_RES_B = (44.0, 60.0)
+
+
+def _form(lv):
+    return 0 if lv.get("faces") is not None else 1
+
+
+def vcycle_bytes(levels) -> float:
+    """Algorithmic bytes of one V-cycle preconditioner application."""
+    total = 0.0
+    for i, lv in enumerate(levels):
+        n = float(np.prod(lv["shape"]))
+        total += n * (_VC_COARSE_B if i == len(levels) - 1 else _VC_LEVEL_B)[_form(lv)]
+    return total
+
+
+def solve_roofline(levels, iterations, solver, measured_s, network_flops=0.0, network_iterations=0):
+    """T_roof = (iterations x (V-cycle + Krylov bytes) + 2 true residuals) / 8 TB/s, plus
+    the U-Net's FLOPs / 2.5 PFLOP/s for iterations that apply the network instead of the
+    V-cycle; frac = T_roof / T_measured."""
+    n0 = float(np.prod(levels[0]["shape"]))
+    f0 = _form(levels[0])
+    vc = vcycle_bytes(levels)
+    kb = _KRYLOV_B[solver][f0] * n0
+    mg_it = iterations - network_iterations
+    bytes_ = mg_it * (vc + kb) + network_iterations * kb + 2 * _TRUE_RES_B[f0] * n0
+    t_roof = bytes_ / (HBM_PEAK_GBS * 1e9) + network_iterations * network_flops / MFMA_BF16_PEAK_FLOPS
+    return {"t_roof_s": round(t_roof, 5), "t_measured_s": round(measured_s, 5),
+            "frac": round(t_roof / max(measured_s, 1e-12), 4), "algorithmic_bytes": int(bytes_),
+            "vcycle_bytes": int(vc), "krylov_bytes_per_iteration": int(kb)}
+
+
+def thermal_cpu_baseline(n=64, seed=0):
+    """The oracle's FP64-outer flexible CG with the FP32 V-cycle (NumPy, one core) on the
+    n^3 version of the blob problem, to a relative residual of 1e-8."""
+    from fluxcore3d_amd import geometry
+    from oracle import mg as omg
+
+    c = geometry.conduction_problem(n, seed=seed, device="cpu")
+    f32 = lambda t: t.cpu().numpy().astype(np.float32)  # noqa: E731
+    h = omg.Hierarchy(f32(c["k"]), f32(c["rcp"]), f32(c["u"]), f32(c["v"]), f32(c["w"]), f32(c["mask"]) > 0,
+                      c["ax"], c["sign"], c["s"], omega=0.8)
+    h.set_dt(c["dt"])
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(1):  # one core: NumPy's BLAS calls (the dot products) included
+        t0 = time.perf_counter()
+        _, info = omg.fcg_mg_fp64(h, f32(c["b"]), tol=1e-8, max_it=500)
+        secs = time.perf_counter() - t0
+    return {"n": n, "s": secs, "iterations": info["it"], "res": info["hist"][-1]}
 
 
 def thermal_leg(n, device, seed=0, weights=None):
@@ -119,6 +188,14 @@ def thermal_leg(n, device, seed=0, weights=None):
            "gmg_bicgstab_s": round(bicg_secs, 4), "gmg_bicgstab_iterations": bi["it"],
            "gmg_bicgstab_converged": bool(bi["converged"]), "gmg_bicgstab_final_rel_residual": bi["res_rel"],
            "nn_pcg_s": None}
+    lv = mg._levels
+    out["roofline"] = {
+        "bound": "hbm", "peak_gbs": HBM_PEAK_GBS,
+        "gmg_pcg": solve_roofline(lv, info["it"], "fcg", secs),
+        "gmg_gcr6": solve_roofline(lv, gi["it"], "gcr6", gcr_secs),
+        "gmg_fgmres16": solve_roofline(lv, fi["it"], "fgmres16", fgmres_secs),
+        "model": "T_roof = sum over launches of algorithmic bytes / 8 TB/s (V-cycle 117 B per cell per level in "
+                 "face form, Krylov bytes per iteration as in bench.py); frac = T_roof / T_measured"}
     out["cht"] = cht_leg(device)
     if weights is None or not os.path.exists(weights):
         out["note"] = "no trained U-Net weights found; NN-PCG not run"
@@ -132,6 +209,9 @@ def thermal_leg(n, device, seed=0, weights=None):
     xn, ni = nn_fcg(mg, net, c["b"], c["k"], tol=1e-8, max_it=500)
     torch.cuda.synchronize()
     nn_secs = time.perf_counter() - t0
+    flops = float(net.flops(tuple(c["k"].shape))) if hasattr(net, "flops") else 0.0
+    out["roofline"]["nn_gcr"] = solve_roofline(lv, ni["it"], "gcr6", nn_secs, network_flops=flops,
+                                               network_iterations=int(ni.get("it_nn") or 0))
     out.update({"nn_pcg_s": round(nn_secs, 4), "nn_pcg_iterations": ni["it"],
                 "nn_iterations_with_network": ni.get("it_nn"), "nn_fallback_to_gmg": bool(ni["fallback"]),
                 "nn_final_rel_residual_fp64": ni["res_rel"], "nn_converged": bool(ni["converged"]),
@@ -208,22 +288,34 @@ def build_workload(n, seed, collision, storage, device, layout="dense"):
 
 
 def cpu_baseline(solid, links, S, omega, collision, steps):
-    """FP32 oracle step on the host (kind "port")."""
+    """The FP32 oracle step (C + OpenMP restatement of the reference step, with the same
+    Guo body force) on the host, on all cores and on one core (kind "port")."""
     from oracle import lbm as olbm
 
     n = solid.shape[0]
     N = solid.size
-    f = np.broadcast_to(olbm.W32[:, None], (19, N)).copy()
-    threads = int(olbm._lib(np.float32).orc_num_threads())
-    f, *_ = olbm.step(f, solid, n, n, n, (1, 1, 1), collision, omega, 1.2, S, 0.0, links)  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        f, *_ = olbm.step(f, solid, n, n, n, (1, 1, 1), collision, omega, 1.2, S, 0.0, links)
-    dt = time.perf_counter() - t0
     n_fluid = int((~solid).sum())
-    return {"value": round(n_fluid * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps (+1 warm-up) of the same {n}^3 {collision} workload, FP32 oracle "
-                      f"(C/OpenMP restatement of the reference step, no body force), {dt:.1f} s"}
+    threads = int(olbm._lib(np.float32).orc_num_threads())
+    out = {}
+    for label, nthr, k in (("all", threads, steps), ("one", 1, max(2, steps // 8))):
+        olbm.set_threads(nthr)
+        f = np.broadcast_to(olbm.W32[:, None], (19, N)).copy()
+        if label == "all":  # warm-up (the 1-thread sample follows on warm caches)
+            f, *_ = olbm.step(f, solid, n, n, n, (1, 1, 1), collision, omega, 1.2, S, 0.0, links,
+                              force=(1e-6, 0.0, 0.0))
+        t0 = time.perf_counter()
+        for _ in range(k):
+            f, *_ = olbm.step(f, solid, n, n, n, (1, 1, 1), collision, omega, 1.2, S, 0.0, links,
+                              force=(1e-6, 0.0, 0.0))
+        dt = time.perf_counter() - t0
+        out[label] = (n_fluid * k / dt / 1e6, k, dt)
+    olbm.set_threads(threads)
+    (v_all, k_all, t_all), (v_one, k_one, t_one) = out["all"], out["one"]
+    return {"value": round(v_all, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "value_1_thread": round(v_one, 3),
+            "sample": f"{k_all} steps (+1 warm-up) of the same {n}^3 {collision} workload with the body force, FP32 "
+                      f"oracle (C/OpenMP restatement of the reference step) on {threads} threads, {t_all:.1f} s; "
+                      f"{k_one} steps on 1 thread (no warm-up step), {t_one:.1f} s"}
 
 
 def traffic_per_launch(args):
@@ -319,6 +411,18 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(solid, links, S, omega, args.collision, args.cpu_steps)
+        if thermal is not None:
+            tc = thermal_cpu_baseline(64)
+            it_gpu = thermal["gmg_pcg_iterations"]
+            scale = (args.thermal_n / tc["n"]) ** 3 * it_gpu / tc["iterations"]
+            thermal["cpu_baseline"] = {
+                "value": round(tc["s"] * scale, 2), "unit": "s", "cores": 1, "kind": "port",
+                "extrapolated": True,
+                "sample": f"oracle FP64-outer flexible CG with the FP32 V-cycle (NumPy, one core) on the "
+                          f"{tc['n']}^3 blob problem: {tc['s']:.2f} s, {tc['iterations']} iterations to "
+                          f"{tc['res']:.2e}; value = that time x ({args.thermal_n}/{tc['n']})^3 cells x "
+                          f"{it_gpu}/{tc['iterations']} iterations (extrapolated to the {args.thermal_n}^3 solve)",
+                "measured_64cube_s": round(tc["s"], 3)}
     line = {
         "metric": METRIC,
         "value": round(mlups, 2),

@@ -59,9 +59,10 @@ def minv(dtype=np.float64) -> np.ndarray:
     return np.linalg.inv(moment_matrix().astype(np.float64)).astype(dtype)
 
 
-def macro_collide(kind, f, solid, om1=1.0, om2=1.0, S=None, cs_dx2=0.0):
+def macro_collide(kind, f, solid, om1=1.0, om2=1.0, S=None, cs_dx2=0.0, force=None):
     """One fused macro+collide pass (lbm_kernels.cu:258-436,1018-1117).
-    kind: 'bgk' | 'trt' | 'mrt' | 'mrt_smag'. Returns f_post, rho, u, v, w."""
+    kind: 'bgk' | 'trt' | 'mrt' | 'mrt_smag'. ``force`` (3 values, lattice units)
+    adds the Guo body force (not in the reference). Returns f_post, rho, u, v, w."""
     dt = f.dtype
     R = _real(dt)
     f = np.ascontiguousarray(f.reshape(19, -1))
@@ -72,8 +73,9 @@ def macro_collide(kind, f, solid, om1=1.0, om2=1.0, S=None, cs_dx2=0.0):
     S_arr = np.zeros(19, dt) if S is None else np.ascontiguousarray(np.asarray(S, dt))
     Mi = np.ascontiguousarray(minv(dt))
     k = {"bgk": 0, "trt": 1, "mrt": 2, "mrt_smag": 3}[kind]
+    F = None if force is None else np.ascontiguousarray(np.asarray(force, dt).reshape(3))
     _lib(dt).orc_macro_collide(C.c_int(k), _p(f), _p(f_post), _p(rho), _p(u), _p(v), _p(w), _p(solid), R(om1),
-                               R(om2), _p(S_arr), _p(Mi), R(cs_dx2), C.c_int64(N))
+                               R(om2), _p(S_arr), _p(Mi), R(cs_dx2), None if F is None else _p(F), C.c_int64(N))
     return f_post, rho, u, v, w
 
 
@@ -157,11 +159,12 @@ def feq(rho, u, v, w, dtype=np.float32):
 
 
 def step(f, solid_flow, nx, ny, nz, periodic, kind="bgk", om1=1.0, om2=1.0, S=None, cs_dx2=0.0, links=None,
-         sigma=None, bcs=()):
+         sigma=None, bcs=(), force=None):
     """One reference step (lbm_cuda_ext.py:445-517): collide -> sponge -> stream_pull
     -> IBB + reset_solid (or bounce-back at solid nodes) -> flow BCs.
-    Returns f_next and the pre-collision macros (rho, u, v, w) with u = 0 in solids."""
-    fp, rho, u, v, w = macro_collide(kind, f, solid_flow, om1, om2, S, cs_dx2)
+    Returns f_next and the pre-collision macros (rho, u, v, w) with u = 0 in solids.
+    ``force``: Guo body force in the collision (not in the reference)."""
+    fp, rho, u, v, w = macro_collide(kind, f, solid_flow, om1, om2, S, cs_dx2, force)
     sf = solid_flow.reshape(-1).astype(bool)
     u[sf] = 0; v[sf] = 0; w[sf] = 0
     if sigma is not None:
@@ -271,6 +274,12 @@ def bfl_weights(q):
     w2 = np.empty(q.size, np.float32)
     _lib(np.float32).orc_bfl_weights(_p(q), C.c_int64(q.size), _p(ge), _p(w1), _p(w2))
     return ge.astype(bool), w1, w2
+
+
+def set_threads(n: int) -> None:
+    """OpenMP thread count of both oracle libraries."""
+    for dt in (np.float32, np.float64):
+        _lib(dt).orc_set_threads(C.c_int(int(n)))
 
 
 def linspace01(steps):

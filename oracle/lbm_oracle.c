@@ -107,9 +107,20 @@ void orc_moment_matrix(int *out) { memcpy(out, M, sizeof(M)); }
  * flow_solver_upgrade.py:278-319 and the sparse Minv of :186-252),
  * 3 MRT + Smagorinsky (:1018-1117). Minv is numpy.linalg.inv(M)
  * (flow_solver_upgrade.py:123), passed in row-major by the caller. */
+/* Guo body-force source of direction q (Guo, Zheng & Shi, Phys. Rev. E 65, 046308,
+ * 2002): w_q [3 (c_q - u).F + 9 (c_q.u)(c_q.F)]. Not in the reference (a build
+ * extension for body-force-driven flow); F = NULL disables it. */
+static real guo_src(int q, real ux, real uy, real uz, const real *F) {
+  const real cx = (real)CX[q], cy = (real)CY[q], cz = (real)CZ[q];
+  const real cu = cx * ux + cy * uy + cz * uz;
+  const real cf = cx * F[0] + cy * F[1] + cz * F[2];
+  const real uf = ux * F[0] + uy * F[1] + uz * F[2];
+  return wq(q) * ((real)3 * (cf - uf) + (real)9 * cu * cf);
+}
+
 void orc_macro_collide(int kind, const real *f, real *f_post, real *rho_o, real *u_o, real *v_o, real *w_o,
                        const uint8_t *solid, real om1, real om2, const real *S, const real *Minv, real cs_dx2,
-                       int64_t N) {
+                       const real *F, int64_t N) {
 #pragma omp parallel for schedule(static)
   for (int64_t x = 0; x < N; ++x) {
     if (solid[x]) {
@@ -117,19 +128,35 @@ void orc_macro_collide(int kind, const real *f, real *f_post, real *rho_o, real 
       rho_o[x] = 1; u_o[x] = 0; v_o[x] = 0; w_o[x] = 0;
       continue;
     }
-    real fi[19], feq[19], rho, ux, uy, uz;
+    real fi[19], feq[19], src[19], rho, ux, uy, uz;
     for (int q = 0; q < 19; ++q) fi[q] = f[q * N + x];
     macros(fi, &rho, &ux, &uy, &uz);
+    if (F) {  /* Guo velocity u = (j + F/2) / rho */
+      const real inv = (real)1 / (rho + (real)1e-12f);
+      real jx = 0, jy = 0, jz = 0;
+      for (int q = 0; q < 19; ++q) { jx += (real)CX[q] * fi[q]; jy += (real)CY[q] * fi[q]; jz += (real)CZ[q] * fi[q]; }
+      ux = (jx + (real)0.5 * F[0]) * inv; uy = (jy + (real)0.5 * F[1]) * inv; uz = (jz + (real)0.5 * F[2]) * inv;
+      for (int q = 0; q < 19; ++q) src[q] = guo_src(q, ux, uy, uz, F);
+    } else {
+      for (int q = 0; q < 19; ++q) src[q] = 0;
+    }
     rho_o[x] = rho; u_o[x] = ux; v_o[x] = uy; w_o[x] = uz;
     feq19(rho, ux, uy, uz, feq);
     if (kind == 0) {
-      for (int q = 0; q < 19; ++q) f_post[q * N + x] = fi[q] + om1 * (feq[q] - fi[q]);
+      for (int q = 0; q < 19; ++q) {
+        f_post[q * N + x] = fi[q] + om1 * (feq[q] - fi[q]);
+        if (F) f_post[q * N + x] += ((real)1 - (real)0.5 * om1) * src[q];
+      }
     } else if (kind == 1) {
       for (int q = 0; q < 19; ++q) {
         const int o = OPP[q];
         const real fp = (real)0.5 * (fi[q] + fi[o]), fm = (real)0.5 * (fi[q] - fi[o]);
         const real ep = (real)0.5 * (feq[q] + feq[o]), em = (real)0.5 * (feq[q] - feq[o]);
         f_post[q * N + x] = (fp + om1 * (ep - fp)) + (fm + om2 * (em - fm));
+        if (F) {
+          const real sp = (real)0.5 * (src[q] + src[o]), sm = (real)0.5 * (src[q] - src[o]);
+          f_post[q * N + x] += ((real)1 - (real)0.5 * om1) * sp + ((real)1 - (real)0.5 * om2) * sm;
+        }
       }
     } else {
       real m[19], meq[19], dm[19], s[19];
@@ -161,6 +188,14 @@ void orc_macro_collide(int kind, const real *f, real *f_post, real *rho_o, real 
       }
       for (int k = 0; k < 19; ++k) dm[k] = s[k] * (m[k] - meq[k]);
       dm[0] = 0; dm[3] = 0; dm[5] = 0; dm[7] = 0;
+      if (F) {  /* + M^-1 (I - S/2) M src; the conserved rows (S = 0) pass the source whole */
+        for (int k = 0; k < 19; ++k) {
+          real a = 0;
+          for (int q = 0; q < 19; ++q) a += (real)M[k][q] * src[q];
+          const real sk = (k == 0 || k == 3 || k == 5 || k == 7) ? (real)0 : s[k];
+          dm[k] -= ((real)1 - (real)0.5 * sk) * a;
+        }
+      }
       for (int q = 0; q < 19; ++q) {
         real d = 0;
         for (int k = 0; k < 19; ++k) d += Minv[q * 19 + k] * dm[k];
@@ -440,6 +475,15 @@ void orc_q_rays(const float *phi, int nx, int ny, int nz, const float origin[3],
 /* bfl_weights_from_q (sdf_tracing.py:187-205) */
 void orc_bfl_weights(const float *q, int64_t n, uint8_t *ge, float *w1, float *w2) {
   for (int64_t l = 0; l < n; ++l) bfl_w(q[l], &w1[l], &w2[l], &ge[l]);
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  extern void omp_set_num_threads(int);
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
 }
 
 int orc_num_threads(void) {

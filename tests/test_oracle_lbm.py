@@ -469,3 +469,56 @@ def test_q_rays_reproduce_the_link_rays_and_weights_closed_form():
     assert np.abs(np.where(ge, w1 * 2 * qq, w1 / (2 * qq)) - 1).max() < 1e-6
     assert np.abs(w1 + w2 - 1).max() < 1e-6
     assert w1[-4] == 1 and w2[-4] == 0
+
+
+@pytest.mark.parametrize("kind", ["bgk", "trt", "mrt"])
+def test_guo_force_poiseuille_between_bouzidi_walls(kind):
+    """The oracle's Guo body force (a build extension: the reference has none) with
+    textbook Bouzidi walls at q = 0.3 and 0.7 (4 x 24 x 4 channel, periodic in x and z,
+    FP64, tau = 0.8, F = 1e-5 along x): the steady profile is the parabola
+    u(y) = F / (2 nu) (y - y1)(y2 - y) to the discretisation error of the walls
+    (5e-3 relative), and the momentum balance F * (fluid nodes) = wall friction holds."""
+    nx, ny, nz = 4, 24, 4
+    y1, y2 = 0.7, ny - 1 - 0.3
+    j = np.arange(ny, dtype=np.float64)
+    d = np.minimum(j - y1, y2 - j)
+    phi = np.broadcast_to(d[None, :, None], (nx, ny, nz)).astype(np.float32).copy()
+    solid = phi < 0
+    links = lbm.ibb_from_sdf(solid, phi, periodic=(True, True, True))
+    links = dict(links, dir_i=lbm.OPP[links["dir_i"]])  # textbook BFL: the unknown population
+    tau = 0.8
+    om, nu, F = 1.0 / tau, (tau - 0.5) / 3.0, 1e-5
+    om2 = 1.0 / (0.5 + 0.1875 / (tau - 0.5))
+    S = np.array([0, 1.19, 1.4, 0, 1.4, 0, 1.4, 0, 1.4, om, 1.4, om, 1.4, om, om, om, 1.98, 1.98, 1.98])
+    f = np.broadcast_to(lbm.W32[:, None].astype(np.float64), (19, nx * ny * nz)).copy()
+    for _ in range(6000):
+        f, rho, u, v, w = lbm.step(f, solid, nx, ny, nz, (1, 1, 1), kind, om, om2, S if kind == "mrt" else None,
+                                   0.0, links, force=(F, 0.0, 0.0))
+    prof = u.reshape(nx, ny, nz)[1, :, 1]
+    fluid = ~solid[1, :, 1]
+    exact = F / (2 * nu) * (j - y1) * (y2 - j)
+    err = np.linalg.norm(prof[fluid] - exact[fluid]) / np.linalg.norm(exact[fluid])
+    assert err < 5e-3, err
+    assert prof.max() > 0.5 * exact.max()  # the force drives the flow
+
+
+@pytest.mark.parametrize("om", [0.7, 1.3, 1.9])
+def test_guo_force_mrt_with_uniform_rates_is_bgk_and_adds_f_to_momentum(om):
+    """Forced collision (FP64): the post-collision momentum is j + F for BGK, TRT and
+    MRT, and MRT with every non-conserved rate equal to omega is forced BGK (the two
+    differ only through the weight literals, sum w = 1 + 5e-8)."""
+    f = _perturbed_eq(40, seed=11)
+    solid = np.zeros(40, np.uint8)
+    F = np.array([1e-3, -2e-3, 5e-4])
+    S = np.full(19, om)
+    S[[0, 3, 5, 7]] = 0.0
+    fb, rho, u, v, w = lbm.macro_collide("bgk", f, solid, om1=om, force=F)
+    ft, *_ = lbm.macro_collide("trt", f, solid, om1=om, om2=1.1, force=F)
+    fm, *_ = lbm.macro_collide("mrt", f, solid, S=S, force=F)
+    j = C.T.astype(np.float64) @ f
+    lit = 3e-7 * (np.abs(j).max() + np.abs(F).max())  # omega x the 5e-8 by which the weight literals miss
+    for g in (fb, ft, fm):
+        assert np.abs(C.T.astype(np.float64) @ g - (j + F[:, None])).max() < lit
+    # the macros are the Guo velocity (j + F/2) / rho
+    assert np.abs(np.stack([u, v, w]) - (j + 0.5 * F[:, None]) / f.sum(0)).max() < 1e-12
+    assert np.abs(fm - fb).max() < 1e-7
