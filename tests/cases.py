@@ -1,4 +1,5 @@
 """Shared workload builders for the tests (synthetic geometry, link sets from the oracle)."""
+import functools
 import types
 
 import numpy as np
@@ -65,6 +66,25 @@ def pin_fin_channel(nx=48, ny=48, nz=96):
     1/4 scale): plate 4 voxels, 4x4 pins r=2 h=30, flow +Z, inlet u=0.04,
     convective outlet, walls +-X +-Y, TRT with magic parameter 1/4."""
     phi, solid, _ = geometry.pin_fin_plate(nx, ny, nz, plate=4, pins=4, radius=2.0, height=30)
+    cfg = Cfg(flow_dir="+Z", domain_walls=["+X", "-X", "+Y", "-Y"], domain_outlets=[], transverse_walls=False,
+              outlet_bc_mode="convective", rho_out_lat=1.0, u_lat_max=0.12, collision="trt", sponge_length=0,
+              u_in_lat=0.04)
+    solid_flow = solid.copy()
+    solid_flow[0], solid_flow[-1] = True, True
+    solid_flow[:, 0], solid_flow[:, -1] = True, True
+    links = olbm.ibb_from_sdf(solid, phi)
+    tau = 0.6
+    return dict(nx=nx, ny=ny, nz=nz, phi=phi, solid=solid, solid_flow=solid_flow, cfg=cfg, links=links,
+                omega=1.0 / tau, omega_minus=1.0 / (0.5 + 0.25 / (tau - 0.5)))
+
+
+@functools.lru_cache(maxsize=1)
+def pin_fin_heat_sink():
+    """The coupled heat-sink workload at its full size: 192x192x384 channel, 16-voxel base
+    plate with an 8x8 grid of pins (radius 4, height 120), flow +Z, inlet u = 0.04,
+    convective outlet, walls +-X +-Y, TRT with magic parameter 1/4."""
+    phi, solid, _ = geometry.pin_fin_plate()
+    nx, ny, nz = solid.shape
     cfg = Cfg(flow_dir="+Z", domain_walls=["+X", "-X", "+Y", "-Y"], domain_outlets=[], transverse_walls=False,
               outlet_bc_mode="convective", rho_out_lat=1.0, u_lat_max=0.12, collision="trt", sponge_length=0,
               u_in_lat=0.04)
