@@ -477,7 +477,9 @@ __global__ __launch_bounds__(kBlock) void restrict_field_kernel(const float* __r
     s += src[((int64_t)i * ny + j) * nz + k];
   }
   const float m = s / 8.f;
-  dst[cidx] = threshold ? (m >= 0.999f ? 1.f : 0.f) : m;
+  // threshold 1: the reference's rule (unknown where every child is); 2: unknown where
+  // any child is (for systems whose masked cells are decoupled by zero conductance)
+  dst[cidx] = threshold == 1 ? (m >= 0.999f ? 1.f : 0.f) : (threshold == 2 ? (m > 0.f ? 1.f : 0.f) : m);
 }
 #pragma clang fp contract(on)
 
@@ -881,6 +883,7 @@ int fc_mg_inv_diag(const float* diagL, const float* mask, double dt, double omeg
 int fc_mg_restrict_field(const float* src, float* dst, int nx, int ny, int nz, int threshold, hipStream_t stream) {
   FC_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "restrict_field: extents must be >= 2");
   FC_REQUIRE(src && dst, "restrict_field: null pointer argument");
+  FC_REQUIRE(threshold >= 0 && threshold <= 2, "restrict_field: threshold must be 0, 1 or 2");
   const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
   hipLaunchKernelGGL(restrict_field_kernel, dim3(grid_for((int64_t)cx * cy * cz, kBlock)), dim3(kBlock), 0, stream,
                      src, dst, nx, ny, nz, cx, cy, cz, threshold);

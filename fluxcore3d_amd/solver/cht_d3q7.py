@@ -1,0 +1,292 @@
+"""Coupled conjugate heat transfer on the flow's lattice: D3Q7 fluid temperature
+every flow step, implicit conduction of the solid every ``couple_every`` steps.
+
+No counterpart in the reference. Its pipeline (UI_components/cht_worker.py:23-165)
+solves the flow to steady state, freezes it and then marches the temperature of
+fluid and solid together with the implicit finite-volume upwind operator
+(solver/nlevel_mg_cuda.py:418-507, LBM_CHT.py:2208-2359, driven by
+solve_thermal_steady_only, LBM_CHT.py:2485-2558). Here the fluid temperature is
+carried by the flow as it develops:
+
+* ``D3Q7Transport`` — one HIP launch per flow step (fc_d3q7_step): BGK D3Q7
+  advection-diffusion with the step's velocity, c_s^2 = 1/4, kappa_f = (1/omega -
+  1/2)/4; adiabatic walls bounce back, conducting solids enter through
+  anti-bounce-back at their temperature, inlet nodes hold the inlet temperature,
+  open faces have zero gradient. In the same launch every conducting node adds
+  the energy its fluid neighbours hand to it.
+* ``SolidConduction`` — every ``couple_every`` flow steps one implicit step of the
+  solid over that interval: rho_cp_s (T - T_old) + dt K T = acc + dt rho_cp_s src,
+  K the face Laplacian of conductance k_s between conducting cells (zero flux to
+  everything else), acc the accumulated interface energy, src the per-step
+  temperature source (a surface flux as in SurfaceFluxBC.src_step,
+  thermal_bc.py:170-196). Solved with flexible CG in FP64 preconditioned by the
+  FP32 geometric-multigrid V-cycle (A = I + dt L of NLevelGeometricMGSolver with
+  k = k_s on conducting cells and 0 elsewhere, so the harmonic face mean
+  decouples the solid), on a box one cell larger than the solid on every side so
+  that no conducting cell touches the multigrid's mirrored domain faces.
+* ``CoupledCHT`` — the loop around a drop-in flow simulation
+  (LBMCUDAUpgrade.patch): flow step -> D3Q7 step -> solid step every
+  ``couple_every`` steps, with the callbacks of run_simulation_blocking.
+
+The exchange is conservative by construction: the solid receives exactly the
+energy the fluid's anti-bounce-back inputs remove (same operands, same order),
+so fluid energy + accumulated energy changes only through the inlet, the outlet
+and the source. Temperatures are carried as deviations from the inlet
+temperature; ``temperature()`` adds it back.
+"""
+from __future__ import annotations
+
+from typing import Callable, Optional
+
+import torch
+
+from .. import _lib
+from . import mg_cuda_ext  # noqa: F401  (registers the multigrid entry points)
+from .mg_pcg import FCGWorkspace, fcg_fp64
+from .nlevel_mg_cuda import NLevelGeometricMGSolver
+
+Tensor = torch.Tensor
+P, I32, I64, F32, F64 = _lib.P, _lib.I32, _lib.I64, _lib.F32, _lib.F64
+
+_lib.register({
+    "fc_d3q7_step": (I32, [P, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, F32, F32, P]),
+    "fc_d3q7_init": (I32, [P, P, P, P, P, I64, P]),
+    "fc_cht_solid_rhs": (I32, [P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, F64, F64, P, P, P]),
+    "fc_cht_solid_store": (I32, [P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, P]),
+})
+
+FLUID, SOLID, WALL, INLET = 0, 1, 2, 16
+
+
+def _stream(t: Tensor):
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _ptr(t: Optional[Tensor]):
+    return None if t is None else t.data_ptr()
+
+
+def _f32(t: Tensor, n: int, name: str) -> Tensor:
+    if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != n or t.device.type != "cuda":
+        raise ValueError(f"{name}: expected a contiguous float32 HIP tensor of {n} elements")
+    return t
+
+
+def node_flags(conducting: Tensor, wall: Optional[Tensor] = None, inlet: Optional[Tensor] = None) -> Tensor:
+    """uint8 node kinds: 1 conducting solid, 2 adiabatic wall (non-conducting solid of
+    the flow), 0 fluid, | 16 on fluid nodes of the inlet plane."""
+    fl = torch.zeros(conducting.shape, dtype=torch.uint8, device=conducting.device)
+    if wall is not None:
+        fl[wall.bool()] = WALL
+    fl[conducting.bool()] = SOLID
+    if inlet is not None:
+        fl[inlet.bool() & (fl == FLUID)] = INLET
+    return fl.contiguous()
+
+
+class D3Q7Transport:
+    """Fluid temperature populations (two [7][N] FP32 buffers) and the step launch."""
+
+    def __init__(self, flag: Tensor, omega: float, periodic=(0, 0, 0), T_in: float = 0.0):
+        if flag.dtype != torch.uint8 or flag.dim() != 3 or flag.device.type != "cuda":
+            raise ValueError("flag: expected a 3-D uint8 HIP tensor")
+        if not 0.0 < float(omega) < 2.0:
+            raise ValueError("omega must be in (0, 2)")
+        err = _lib.load_error()
+        if err is not None:
+            raise RuntimeError(f"D3Q7Transport needs libfluxcore_hip.so: {err}")
+        self.flag = flag.contiguous()
+        self.shape = tuple(flag.shape)
+        self.N = int(flag.numel())
+        self.omega = float(omega)
+        self.periodic = tuple(int(bool(p)) for p in periodic)
+        self.T_in = float(T_in)
+        dev = flag.device
+        self.g = [torch.zeros((7, self.N), dtype=torch.float32, device=dev) for _ in range(2)]
+        self.T = torch.zeros(self.N, dtype=torch.float32, device=dev)
+        self.cur = 0
+
+    @property
+    def kappa(self) -> float:
+        return (1.0 / self.omega - 0.5) / 4.0
+
+    def init(self, T0: Tensor, u: Tensor, v: Tensor, w: Tensor) -> None:
+        """Populations at equilibrium with temperature T0 and velocity (u, v, w)."""
+        args = [_f32(t.reshape(-1), self.N, n) for t, n in ((T0, "T0"), (u, "u"), (v, "v"), (w, "w"))]
+        self.T.copy_(args[0])
+        _lib.call("fc_d3q7_init", self.g[self.cur].data_ptr(), *[a.data_ptr() for a in args], self.N,
+                  _stream(self.T))
+
+    def step(self, u: Tensor, v: Tensor, w: Tensor, T_solid: Tensor, acc: Optional[Tensor] = None,
+             Q: Optional[Tensor] = None) -> Tensor:
+        """One step with this flow step's velocity; returns T (fluid from D3Q7,
+        conducting nodes T_solid)."""
+        uu = [_f32(t.reshape(-1), self.N, n) for t, n in ((u, "u"), (v, "v"), (w, "w"))]
+        ts = _f32(T_solid.reshape(-1), self.N, "T_solid")
+        a = None if acc is None else _f32(acc.reshape(-1), self.N, "acc")
+        q = None if Q is None else _f32(Q.reshape(-1), self.N, "Q")
+        nx, ny, nz = self.shape
+        src, dst = self.g[self.cur], self.g[1 - self.cur]
+        _lib.call("fc_d3q7_step", src.data_ptr(), dst.data_ptr(), self.T.data_ptr(), *[t.data_ptr() for t in uu],
+                  self.flag.data_ptr(), ts.data_ptr(), _ptr(q), _ptr(a), nx, ny, nz, *self.periodic, self.omega,
+                  self.T_in, _stream(self.T))
+        self.cur = 1 - self.cur
+        return self.T
+
+    @property
+    def populations(self) -> Tensor:
+        """Post-collision populations of the last step ([7][N])."""
+        return self.g[self.cur]
+
+
+class SolidConduction:
+    """Implicit conduction step of the conducting nodes over ``dt`` flow steps."""
+
+    def __init__(self, flag: Tensor, k_s: float, rho_cp: float, dt: float, tol: float = 1e-8, max_it: int = 300,
+                 omega: float = 0.8, n_pre: int = 3, n_post: int = 3):
+        if k_s <= 0 or rho_cp <= 0 or dt <= 0:
+            raise ValueError("k_s, rho_cp and dt must be positive")
+        self.flag = flag.contiguous()
+        self.shape = tuple(flag.shape)
+        self.k_s, self.rho_cp, self.dt, self.tol, self.max_it = float(k_s), float(rho_cp), float(dt), tol, max_it
+        dev = flag.device
+        cond = (flag & 3) == SOLID
+        if not bool(cond.any()):
+            raise ValueError("no conducting nodes")
+        lo, hi = [], []
+        for ax in range(3):
+            other = tuple(a for a in range(3) if a != ax)
+            idx = torch.nonzero(cond.any(dim=other[1]).any(dim=other[0])).reshape(-1)
+            lo.append(int(idx.min()))
+            hi.append(int(idx.max()))
+        # one non-conducting cell on every side, extents rounded up to multiples of 8
+        self.origin = tuple(v - 1 for v in lo)
+        self.box = tuple(((h - l + 3) + 7) // 8 * 8 for l, h in zip(lo, hi))
+        box_cond = torch.zeros(self.box, dtype=torch.bool, device=dev)
+        g_sl, b_sl = [], []
+        for ax in range(3):
+            g0 = max(self.origin[ax], 0)
+            g1 = min(self.origin[ax] + self.box[ax], self.shape[ax])
+            g_sl.append(slice(g0, g1))
+            b_sl.append(slice(g0 - self.origin[ax], g1 - self.origin[ax]))
+        box_cond[tuple(b_sl)] = cond[tuple(g_sl)]
+        self.n_unknowns = int(box_cond.sum())
+        # conductance k_s inside the solid, 0 outside (the harmonic face mean then decouples
+        # the solid); heat capacity rho_cp everywhere, so partially solid coarse cells
+        # keep the solid's; coarse unknowns wherever any child is solid (thin pins stay
+        # on the coarse levels: 55 instead of 233 FCG iterations on the quarter-scale
+        # heat sink in the FP64 model of the same hierarchy)
+        k = torch.where(box_cond, torch.tensor(self.k_s, device=dev), torch.tensor(0.0, device=dev))
+        rcp = torch.full(self.box, self.rho_cp, device=dev)
+        z = torch.zeros(self.box, dtype=torch.float32, device=dev)
+        self.mg = NLevelGeometricMGSolver(dev, torch.float32, flow_axis=2, flow_sign=+1, thermal_dt_scale=1.0,
+                                          omega=omega, n_pre=n_pre, n_post=n_post, coarse_mask="any")
+        self.mg.build_if_needed(k.float(), rcp.float(), z, z, z, box_cond.float(), self.dt)
+        self.b = torch.zeros(self.box, dtype=torch.float64, device=dev)
+        self.x0 = torch.zeros(self.box, dtype=torch.float64, device=dev)
+        self.ws = FCGWorkspace(self.box, dev)
+
+    def _geom(self):
+        return (*self.shape, *self.origin, *self.box)
+
+    def step(self, T_solid: Tensor, acc: Tensor, src: Optional[Tensor] = None) -> dict:
+        """T_solid <- solution of the implicit step; acc restarts at zero."""
+        n = 1
+        for s in self.shape:
+            n *= s
+        ts = _f32(T_solid.reshape(-1), n, "T_solid")
+        a = _f32(acc.reshape(-1), n, "acc")
+        q = None if src is None else _f32(src.reshape(-1), n, "src")
+        stream = _stream(ts)
+        _lib.call("fc_cht_solid_rhs", ts.data_ptr(), a.data_ptr(), _ptr(q), self.flag.data_ptr(), *self._geom(),
+                  self.rho_cp, self.dt, self.b.data_ptr(), self.x0.data_ptr(), stream)
+        _, info = fcg_fp64(self.mg, self.b, x0=self.x0, tol=self.tol, max_it=self.max_it, ws=self.ws)
+        if not info["converged"]:
+            raise RuntimeError(f"solid conduction step did not reach {self.tol:g} in {info['it']} iterations "
+                               f"(relative residual {info['res_rel']:.3e})")
+        _lib.call("fc_cht_solid_store", info["x64"].data_ptr(), ts.data_ptr(), a.data_ptr(), self.flag.data_ptr(),
+                  *self._geom(), stream)
+        return {"it": info["it"], "res_rel": info["res_rel"]}
+
+
+def inlet_mask(sim) -> Optional[Tensor]:
+    """Fluid nodes of the velocity-inlet plane of the drop-in simulation's flow plan."""
+    from .lbm_cuda_ext import flow_bc_plan
+    m = None
+    for bc in flow_bc_plan(sim):
+        if bc["kind"] == "zou_he" and bc["bc_type"] == 0:
+            m = torch.zeros((sim.nx, sim.ny, sim.nz), dtype=torch.bool, device=sim.solid.device) if m is None else m
+            sl = [slice(None)] * 3
+            sl[bc["axis"]] = bc["plane"]
+            m[tuple(sl)] = True
+    return m
+
+
+class CoupledCHT:
+    """Flow step -> D3Q7 step -> (every ``couple_every`` steps) solid step.
+
+    ``sim``: a drop-in flow simulation (LBMCHT3D_Torch fields, LBMCUDAUpgrade.patch
+    applied) whose ``step(do_thermal=False)`` leaves the step's velocity in
+    ``sim.u``, ``sim.v``, ``sim.wv``. Conducting nodes are ``sim.solid``, adiabatic
+    walls the rest of ``sim.solid_flow``. Fluid rho c_p is the unit; the solid has
+    conductivity ``k_ratio`` kappa_f and heat capacity ``rho_cp_ratio``; kappa_f =
+    nu / ``prandtl`` with nu from the flow's relaxation rate. ``src``: per-step
+    temperature source of the solid (K per flow step, solid heat capacity)."""
+
+    def __init__(self, sim, src: Optional[Tensor] = None, k_ratio: float = 1000.0, rho_cp_ratio: float = 1.6,
+                 prandtl: float = 0.71, couple_every: int = 50, tol: float = 1e-8, T_in: float = 0.0,
+                 periodic=(0, 0, 0)):
+        self.sim = sim
+        self.every = int(couple_every)
+        if self.every < 1:
+            raise ValueError("couple_every must be >= 1")
+        nu = (1.0 / float(sim.omega_plus) - 0.5) / 3.0
+        self.kappa_f = nu / float(prandtl)
+        self.omega_g = 1.0 / (0.5 + 4.0 * self.kappa_f)
+        self.k_s = float(k_ratio) * self.kappa_f
+        self.rho_cp_s = float(rho_cp_ratio)
+        self.T_in = float(T_in)
+        wall = sim.solid_flow.bool() & ~sim.solid.bool()
+        self.flag = node_flags(sim.solid, wall, inlet_mask(sim))
+        self.transport = D3Q7Transport(self.flag, self.omega_g, periodic=periodic, T_in=0.0)
+        self.solid = SolidConduction(self.flag, self.k_s, self.rho_cp_s, float(self.every), tol=tol)
+        N = self.transport.N
+        dev = self.flag.device
+        self.T_solid = torch.zeros(N, dtype=torch.float32, device=dev)
+        self.acc = torch.zeros(N, dtype=torch.float32, device=dev)
+        self.src = None if src is None else src.to(device=dev, dtype=torch.float32).reshape(-1).contiguous()
+        z = torch.zeros(N, dtype=torch.float32, device=dev)
+        self.transport.init(z, z, z, z)
+        self.n = 0
+        self.history: list = []
+
+    def step(self) -> None:
+        sim = self.sim
+        sim.step(do_thermal=False)
+        self.transport.step(sim.u, sim.v, sim.wv, self.T_solid, acc=self.acc)
+        self.n += 1
+        if self.n % self.every == 0:
+            info = self.solid.step(self.T_solid, self.acc, self.src)
+            info["step"] = self.n
+            self.history.append(info)
+
+    def run(self, steps: int, log_fn: Optional[Callable] = None, progress_fn: Optional[Callable] = None,
+            abort_fn: Optional[Callable] = None, report_every: int = 1000) -> dict:
+        """``steps`` coupled steps with the callbacks of run_simulation_blocking
+        (log_fn(str), progress_fn(0..100), abort_fn() -> True to stop)."""
+        for s in range(int(steps)):
+            if abort_fn is not None and abort_fn():
+                raise RuntimeError("Aborted by user.")
+            self.step()
+            if progress_fn is not None and (s + 1) % max(1, steps // 100) == 0:
+                progress_fn(int(100 * (s + 1) / steps))
+            if log_fn is not None and (s + 1) % report_every == 0:
+                last = self.history[-1] if self.history else {}
+                log_fn(f"[CHT] step {self.n}: T_max {float(self.temperature().max()):.4f}, "
+                       f"solid solve {last.get('it', 0)} it, residual {last.get('res_rel', 0.0):.2e}")
+        return {"steps": self.n, "solves": len(self.history)}
+
+    def temperature(self) -> Tensor:
+        """(nx, ny, nz) temperature: fluid from D3Q7, conducting nodes from the solid."""
+        T = torch.where((self.flag.reshape(-1) & 3) == SOLID, self.T_solid, self.transport.T)
+        return (T + self.T_in).view(self.transport.shape)

@@ -39,7 +39,7 @@ class MGCoef(_C.Structure):
 class NLevelGeometricMGSolver:
     def __init__(self, device, dtype, flow_axis: int, flow_sign: int, thermal_dt_scale: float, omega: float = 0.8,
                  n_pre: int = 2, n_post: int = 2, n_coarse: int = 20, min_coarse_cells: int = 4,
-                 face_coefficients: bool = True, fused_first_sweeps: bool = True):
+                 face_coefficients: bool = True, fused_first_sweeps: bool = True, coarse_mask: str = "all"):
         # The reference falls through to its PyTorch operator for CPU or float64 input
         # (nlevel_mg_cuda.py:71-79); this solver runs the HIP kernels only.
         if torch.device(device).type != "cuda":
@@ -61,6 +61,13 @@ class NLevelGeometricMGSolver:
         self.min_coarse_cells = int(min_coarse_cells)
         self.face_coefficients = bool(face_coefficients)
         self.fused_first_sweeps = bool(fused_first_sweeps)
+        # "all": a coarse cell is an unknown where all 8 children are (>= 0.999, the
+        # reference's rule); "any": where any child is. "any" suits systems whose
+        # non-unknown cells carry zero conductance (a solid conducting on its own):
+        # thin parts then stay on the coarse levels instead of dropping out.
+        if coarse_mask not in ("all", "any"):
+            raise ValueError("coarse_mask must be 'all' or 'any'")
+        self.coarse_mask = coarse_mask
         self._ready = False
         self._dt_cache: Optional[float] = None
         self._shape_cache: Optional[tuple] = None
@@ -120,7 +127,8 @@ class NLevelGeometricMGSolver:
                 _lib.call("fc_mg_restrict_field", t.data_ptr(), c.data_ptr(), nx, ny, nz, 0, stream)
                 nf.append(c)
             cm = torch.empty(cshape, device=dev, dtype=torch.float32)
-            _lib.call("fc_mg_restrict_field", msk.data_ptr(), cm.data_ptr(), nx, ny, nz, 1, stream)
+            _lib.call("fc_mg_restrict_field", msk.data_ptr(), cm.data_ptr(), nx, ny, nz,
+                      1 if self.coarse_mask == "all" else 2, stream)
             fields, msk = nf, cm
         self._nlevels = len(self._levels)
         self._shape_cache = tuple(k.shape)

@@ -184,7 +184,9 @@ int fc_mg_build_coeffs(const float *k, const float *rho_cp, const float *u, cons
 /* _update_inv_diag (nlevel_mg_cuda.py:262-267) */
 int fc_mg_inv_diag(const float *diagL, const float *mask, double dt, double omega, float *inv_diagM, int64_t n,
                    hipStream_t stream);
-/* _restrict_2x (nlevel_mg_cuda.py:512-519); threshold=1 gives the coarse mask (>= 0.999) */
+/* _restrict_2x (nlevel_mg_cuda.py:512-519); threshold=1 gives the coarse mask (>= 0.999);
+ * threshold=2 (no counterpart in the reference) marks a coarse cell as unknown where
+ * any child is, for systems whose other cells are decoupled by zero conductance */
 int fc_mg_restrict_field(const float *src, float *dst, int nx, int ny, int nz, int threshold, hipStream_t stream);
 
 /* One level's coefficients. fx, fy, fz (all set or all null) hold the face form
@@ -368,6 +370,32 @@ int fc_bicgstab_s(double *s, const double *r, const double *v, const float *mask
 int fc_bicgstab_update(double *x, double *r, const double *s, const double *t, const float *p_hat,
                        const float *s_hat, const float *mask, double alpha, double omega, int64_t n,
                        hipStream_t stream);
+
+/* ---- D3Q7 fluid temperature and the solid conduction step (csrc/d3q7.hip) ----
+ * No counterpart in the reference: its temperature is an implicit finite-volume
+ * upwind operator solved after the flow is frozen (nlevel_mg_cuda.py:418-507,
+ * LBM_CHT.py:2208-2359, cht_worker.py:23-165). These carry the fluid temperature
+ * every flow step and hand the energy crossing the fluid-solid interface to the
+ * solid, whose conduction step is solved with the multigrid entry points above.
+ * flag: 0 fluid, 1 conducting solid, 2 adiabatic wall, | 16 inlet (held at T_in).
+ * g_in, g_out: [7][N] post-collision populations (0, +x, -x, +y, -y, +z, -z),
+ * N = nx ny nz < 2^31 in C order; Q (may be NULL): fluid heat source per step;
+ * acc (may be NULL): per-node energy the solid receives, accumulated. */
+int fc_d3q7_step(const float *g_in, float *g_out, float *T_out, const float *u, const float *v, const float *w,
+                 const uint8_t *flag, const float *T_solid, const float *Q, float *acc, int nx, int ny, int nz,
+                 int px, int py, int pz, float omega, float T_in, hipStream_t stream);
+/* g = w_i T (1 + 4 c_i.u) */
+int fc_d3q7_init(float *g, const float *T, const float *u, const float *v, const float *w, int64_t n,
+                 hipStream_t stream);
+/* right-hand side b = T_s + dt src + acc / rho_cp and start x0 = T_s of the solid's
+ * implicit step (I + dt L) T = b on a box (origin ox, oy, oz >= -1; extents bx, by,
+ * bz) of the nx x ny x nz grid; zero on box cells that are not conducting */
+int fc_cht_solid_rhs(const float *T_solid, const float *acc, const float *src, const uint8_t *flag, int nx, int ny,
+                     int nz, int ox, int oy, int oz, int bx, int by, int bz, double rho_cp, double dt, double *b,
+                     double *x0, hipStream_t stream);
+/* T_s = x on the box's conducting cells; their acc restarts at zero */
+int fc_cht_solid_store(const double *x, float *T_solid, float *acc, const uint8_t *flag, int nx, int ny, int nz,
+                       int ox, int oy, int oz, int bx, int by, int bz, hipStream_t stream);
 
 #ifdef __cplusplus
 }

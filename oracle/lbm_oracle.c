@@ -477,6 +477,103 @@ void orc_bfl_weights(const float *q, int64_t n, uint8_t *ge, float *w1, float *w
   for (int64_t l = 0; l < n; ++l) bfl_w(q[l], &w1[l], &w2[l], &ge[l]);
 }
 
+/* ---------------------------------------------------------------------------
+ * D3Q7 advection-diffusion of a temperature carried by the flow (no counterpart in
+ * the reference, whose temperature is a finite-volume operator, nlevel_mg_cuda.py:
+ * 418-507). Textbook BGK model (Chopard, Falcone & Latt 2009): directions 0, +-x,
+ * +-y, +-z, weights 1/4 and 1/8 (c_s^2 = 1/4), equilibrium w_i T (1 + 4 c_i.u),
+ * kappa = (1/omega - 1/2)/4. Pull streaming with bounce-back from adiabatic wall
+ * nodes, anti-bounce-back 2 w_i T_s - g*_opp from conducting solid nodes (the solid
+ * temperature on the face between the nodes), zero gradient across non-periodic
+ * domain faces, inlet nodes (flag bit 4) held at equilibrium at T_in. Solid nodes
+ * report T_s and, with acc, add the energy their fluid neighbours lose to them.
+ * Flags: 0 fluid, 1 conducting solid, 2 adiabatic wall, | 16 inlet.
+ * ------------------------------------------------------------------------- */
+static const int C7X[7] = {0, 1, -1, 0, 0, 0, 0};
+static const int C7Y[7] = {0, 0, 0, 1, -1, 0, 0};
+static const int C7Z[7] = {0, 0, 0, 0, 0, 1, -1};
+static const int O7[7] = {0, 2, 1, 4, 3, 6, 5};
+static real w7(int q) { return q == 0 ? (real)0.25 : (real)0.125; }
+
+static int nb_axis(int c, int d, int n, int per, int *out) {
+  int v = c + d;
+  if (v < 0 || v >= n) {
+    if (!per) return 0;
+    v = v < 0 ? v + n : v - n;
+  }
+  *out = v;
+  return 1;
+}
+
+void orc_d3q7_step(const real *gin, real *gout, real *T_out, const real *u, const real *v, const real *w,
+                   const uint8_t *flag, const real *T_solid, const real *Q, real *acc, int nx, int ny, int nz,
+                   int px, int py, int pz, real omega, real T_in) {
+  const int64_t N = (int64_t)nx * ny * nz;
+#pragma omp parallel for schedule(static)
+  for (int64_t x = 0; x < N; ++x) {
+    const int kind = flag[x] & 3;
+    if (kind == 2) continue;
+    const int i = (int)(x / ((int64_t)ny * nz)), j = (int)((x / nz) % ny), k = (int)(x % nz);
+    if (kind == 1) {
+      const real ts = T_solid[x];
+      T_out[x] = ts;
+      if (!acc) continue;
+      real e = 0;
+      for (int q = 1; q < 7; ++q) {
+        int a, b, c;
+        if (!(nb_axis(i, C7X[q], nx, px, &a) && nb_axis(j, C7Y[q], ny, py, &b) && nb_axis(k, C7Z[q], nz, pz, &c)))
+          continue;
+        const int64_t s = ((int64_t)a * ny + b) * nz + c;
+        if ((flag[s] & 3) != 0 || (flag[s] & 16)) continue;
+        const real out = gin[O7[q] * N + s];
+        const real two_wt = (real)2 * w7(q) * ts;
+        const real back = two_wt - out;
+        e = e + (out - back);
+      }
+      acc[x] = acc[x] + e;
+      continue;
+    }
+    const real ux = u[x], uy = v[x], uz = w[x];
+    real g[7];
+    if (flag[x] & 16) {
+      for (int q = 0; q < 7; ++q) {
+        const real cu = (real)C7X[q] * ux + (real)C7Y[q] * uy + (real)C7Z[q] * uz;
+        g[q] = w7(q) * T_in * ((real)1 + (real)4 * cu);
+        gout[q * N + x] = g[q];
+      }
+      T_out[x] = T_in;
+      continue;
+    }
+    for (int q = 0; q < 7; ++q) {
+      int a, b, c;
+      if (!(nb_axis(i, -C7X[q], nx, px, &a) && nb_axis(j, -C7Y[q], ny, py, &b) &&
+            nb_axis(k, -C7Z[q], nz, pz, &c))) {
+        g[q] = gin[q * N + x];
+        continue;
+      }
+      const int64_t s = ((int64_t)a * ny + b) * nz + c;
+      const int sk = flag[s] & 3;
+      if (sk == 0) g[q] = gin[q * N + s];
+      else if (sk == 2) g[q] = gin[O7[q] * N + x];
+      else {
+        const real two_wt = (real)2 * w7(q) * T_solid[s];
+        g[q] = two_wt - gin[O7[q] * N + x];
+      }
+    }
+    real T = g[0];
+    for (int q = 1; q < 7; ++q) T = T + g[q];
+    T_out[x] = T;
+    const real src = Q ? Q[x] : (real)0;
+    for (int q = 0; q < 7; ++q) {
+      const real cu = (real)C7X[q] * ux + (real)C7Y[q] * uy + (real)C7Z[q] * uz;
+      const real geq = w7(q) * T * ((real)1 + (real)4 * cu);
+      real o = g[q] + omega * (geq - g[q]);
+      if (Q) o = o + w7(q) * src;
+      gout[q * N + x] = o;
+    }
+  }
+}
+
 void orc_set_threads(int n) {
 #ifdef _OPENMP
   extern void omp_set_num_threads(int);

@@ -30,7 +30,7 @@ def test_python_binding_knows_every_declared_symbol():
     import fluxcore3d_amd.pre_post.sdf_tracing  # noqa: F401  (registers its entry points)
     import fluxcore3d_amd.solver.lbm_cuda_ext  # noqa: F401
     for mod in ("fluxcore3d_amd.solver.mg_cuda_ext", "fluxcore3d_amd.solver.thermal_cuda_ext", "fluxcore3d_amd.solver.mg_pcg",
-                "fluxcore3d_amd.solver.nn_precond"):
+                "fluxcore3d_amd.solver.nn_precond", "fluxcore3d_amd.solver.cht_d3q7"):
         try:
             __import__(mod)
         except ModuleNotFoundError:
