@@ -48,6 +48,9 @@ def parse():
                    help="run a 256 MiB device-to-device copy first (reference kernel for PMC byte counters)")
     p.add_argument("--thermal-n", type=int, default=256)
     p.add_argument("--no-thermal", action="store_true")
+    p.add_argument("--cht-steps", type=int, default=1000,
+                   help="coupled heat-transfer steps timed (config 3 specifies 20000)")
+    p.add_argument("--no-cht", action="store_true")
     p.add_argument("--unet-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "fluxcore3d_amd",
                                                           "data", "unet_precond.npz"))
     return p.parse_args()
@@ -258,6 +261,113 @@ def cht_leg(device):
     return out
 
 
+D3Q7_FLUID_BYTES = 7 * 4 * 2 + 3 * 4 + 1 + 4  # populations in and out, velocity, flag, temperature out
+D3Q7_SOLID_BYTES = 1 + 4 + 4  # flag, solid temperature in, temperature out
+D3Q7_INTERFACE_BYTES = 8  # energy accumulator read and written (solid nodes next to fluid)
+
+
+def coupled_leg(device, steps, warmup, every=50):
+    """Config 3's coupled conjugate heat transfer at full size: 192x192x384 pin-fin
+    heat sink, TRT (magic 1/4) flow from rest with Zou/He inlet u = 0.04 and convective
+    outlet, D3Q7 fluid temperature every step, implicit solid conduction (k_s/k_f =
+    1000, rho_cp ratio 1.6, base flux as 1e-3 K per step on the plate's bottom layer)
+    by GMG-preconditioned FCG to 1e-8 every ``every`` steps. Reports the whole loop's
+    time per step and, timed on their own, the flow step, the D3Q7 kernel (with its
+    HBM roofline) and the solid solves."""
+    import types
+
+    import torch
+
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.pre_post.sdf_tracing import build_ibb_from_sdf_gpu
+    from fluxcore3d_amd.solver.cht_d3q7 import FLUID, SOLID, WALL, CoupledCHT
+    from fluxcore3d_amd.solver.flow_state import FlowState
+    from fluxcore3d_amd.solver.lbm_cuda_ext import LBMCUDAUpgrade
+
+    phi, solid, solid_flow = geometry.pin_fin_plate()
+    solid_flow[:, 0] = True
+    nx, ny, nz = solid.shape
+    tau = 0.6
+    cfg = types.SimpleNamespace(flow_dir="+Z", domain_walls=["+X", "-X", "+Y", "-Y"], domain_outlets=[],
+                                transverse_walls=False, outlet_bc_mode="convective", rho_out_lat=1.0, u_lat_max=0.12,
+                                collision="trt", sponge_length=0, u_in_lat=0.04, fc_engine="aa")
+    sim = FlowState(nx, ny, nz, cfg, solid, solid_flow, phi, 1.0 / tau, device,
+                    omega_minus=1.0 / (0.5 + 0.25 / (tau - 0.5)))
+    build_ibb_from_sdf_gpu(sim, (0.0, 0.0, 0.0), 1.0, (0, 0, 0))
+    LBMCUDAUpgrade.patch(sim)
+    src = torch.zeros((nx, ny, nz), device=device)
+    src[:, 0, :] = 1e-3
+    src[~sim.solid] = 0.0
+    cht = CoupledCHT(sim, src=src, k_ratio=1000.0, rho_cp_ratio=1.6, couple_every=every)
+    for _ in range(warmup):
+        cht.step()
+    torch.cuda.synchronize()
+    solves0 = len(cht.history)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        cht.step()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1)
+    hist = cht.history[solves0:]
+    fl = cht.flag.reshape(-1)
+    kind = fl & 3
+    # the state after the coupled run (the timings below step the parts on their own)
+    T = cht.temperature()
+    temps = {"T_fluid_max": round(float(T.reshape(-1)[kind == FLUID].max()), 6),
+             "T_solid_max": round(float(cht.T_solid.max()), 6),
+             "T_outlet_mean": round(float(T[:, :, -1][(kind.view(nx, ny, nz)[:, :, -1] == FLUID)].mean()), 6)}
+    n_fluid = int((kind == FLUID).sum())
+    n_solid = int((kind == SOLID).sum())
+    n_wall = int((kind == WALL).sum())
+    # interface solid nodes (a fluid face neighbour): from the accumulator pattern of one step
+    cht.acc.zero_()
+    cht.transport.step(sim.u, sim.v, sim.wv, cht.T_solid, acc=cht.acc)
+    n_iface = int((cht.acc != 0).sum())
+    # flow step, D3Q7 kernel and solid solve on their own (events on torch's current stream,
+    # the stream every launch here uses)
+    k = 200
+    ev0.record()
+    for _ in range(k):
+        sim.step(do_thermal=False)
+    ev1.record()
+    torch.cuda.synchronize()
+    flow_ms = ev0.elapsed_time(ev1) / k
+    ev0.record()
+    for _ in range(k):
+        cht.transport.step(sim.u, sim.v, sim.wv, cht.T_solid, acc=cht.acc)
+    ev1.record()
+    torch.cuda.synchronize()
+    d3q7_ms = ev0.elapsed_time(ev1) / k
+    d3q7_bytes = (n_fluid * D3Q7_FLUID_BYTES + n_solid * D3Q7_SOLID_BYTES + n_iface * D3Q7_INTERFACE_BYTES
+                  + n_wall * 1)
+    ev0.record()
+    info = cht.solid.step(cht.T_solid, cht.acc, cht.src)
+    ev1.record()
+    torch.cuda.synchronize()
+    solve_ms = ev0.elapsed_time(ev1)
+    return {
+        "workload": f"pin-fin heat sink {nx}x{ny}x{nz}: TRT flow + D3Q7 temperature every step, solid conduction "
+                    f"(k_s/k_f 1000, rho_cp 1.6) by GMG-FCG to 1e-8 every {every} steps; base flux 1e-3 K/step",
+        "steps": steps, "warmup": warmup, "ms_per_step": round(ms / steps, 5),
+        "mlups_fluid": round(n_fluid * steps / (ms * 1e-3) / 1e6, 2), "wall_s_timed": round(wall, 3),
+        "fluid_nodes": n_fluid, "solid_nodes": n_solid, "interface_solid_nodes": n_iface,
+        "solid_solves": len(hist), "solid_solve_iterations": [h["it"] for h in hist][:10],
+        "solid_solve_it_mean": round(float(np.mean([h["it"] for h in hist])), 2) if hist else None,
+        "solid_solve_max_res": max(h["res_rel"] for h in hist) if hist else None,
+        "flow_ms_per_step": round(flow_ms, 5),
+        "d3q7": {"ms": round(d3q7_ms, 5), "algorithmic_bytes": d3q7_bytes,
+                 "achieved_gbs": round(d3q7_bytes / (d3q7_ms * 1e-3) / 1e9, 1),
+                 "roofline_frac": round(d3q7_bytes / (d3q7_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "solid_solve_ms": round(solve_ms, 3), "solid_solve_it": info["it"],
+        "box": list(cht.solid.box), "box_unknowns": cht.solid.n_unknowns, **temps,
+    }
+
+
 def build_workload(n, seed, collision, storage, device, layout="dense"):
     import torch
 
@@ -408,6 +518,11 @@ def main():
     thermal = None
     if not args.no_thermal:
         thermal = thermal_leg(args.thermal_n, dev, weights=args.unet_weights)
+    coupled = None
+    if not args.no_cht:
+        torch.cuda.empty_cache()
+        coupled = coupled_leg(dev, args.cht_steps, warmup=100)
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(solid, links, S, omega, args.collision, args.cpu_steps)
@@ -448,6 +563,7 @@ def main():
         "cpu_baseline": cpu,
         "half_storage": half,
         "thermal": thermal,
+        "coupled_cht": coupled,
         "mlups_all_nodes": round(N * args.steps / (ms * 1e-3) / 1e6, 2),
         "wall_s_timed": round(wall, 4),
         "setup_s": round(setup_s, 2),

@@ -10,12 +10,17 @@
 // One launch per step, per node x (pull): input i is g*_i(x - c_i) of the previous
 // step, except
 //   * x - c_i is an adiabatic wall node: bounce-back, g*_opp(i)(x) (zero flux);
-//   * x - c_i is a conducting solid node: anti-bounce-back at the solid's temperature,
+//   * x - c_i is a conducting solid node (kind 1): anti-bounce-back at its temperature,
 //     -g*_opp(i)(x) + 2 w_i T_solid(x - c_i) (Dirichlet on the face between the nodes);
 //   * x - c_i is outside the domain on a non-periodic axis: g*_i(x) (zero gradient);
 // then T = sum_i g_i (written), g*_i = g_i + omega (g_i^eq - g_i) + w_i Q(x) with an
 // optional heat source Q (temperature units per step). Nodes on the inlet plane
-// (flag bit 4) are held at T_in: g*_i = w_i T_in (1 + 4 c_i.u). Wall nodes are not
+// (flag bit 4) are held at T_in: g*_i = w_i T_in (1 + 4 c_i.u). Kind 3 marks a
+// conducting solid carried on the lattice (conjugate conduction without a separate
+// solve): it streams and collides like fluid, with u = 0 and its own rate omega_s,
+// so kappa_s = (1/omega_s - 1/2)/4; in steady state the face between two kinds
+// conducts with the harmonic mean of the two sides, as a finite-volume operator
+// with harmonic-mean face conductivities does. Wall nodes are not
 // touched; a solid node writes T_out = T_solid and, when acc is given, adds to acc the
 // energy its fluid neighbours (inlet nodes excepted) lose to it in this step: per link,
 // g*_opp(i)(x + c_i) minus the anti-bounce-back value that replaces it, so the solid
@@ -28,7 +33,7 @@ using namespace fc;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint8_t G_FLUID = 0, G_SOLID = 1, G_WALL = 2, G_KIND = 3, G_INLET = 16;
+constexpr uint8_t G_FLUID = 0, G_SOLID = 1, G_WALL = 2, G_CONJ = 3, G_KIND = 3, G_INLET = 16;
 
 __device__ __forceinline__ int cx7(int i) { return i == 1 ? 1 : (i == 2 ? -1 : 0); }
 __device__ __forceinline__ int cy7(int i) { return i == 3 ? 1 : (i == 4 ? -1 : 0); }
@@ -54,7 +59,8 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
                                                            const float* __restrict__ T_solid,
                                                            const float* __restrict__ Q, float* __restrict__ acc,
                                                            int nx, int ny, int nz,
-                                                           int px, int py, int pz, float omega, float T_in) {
+                                                           int px, int py, int pz, float omega, float omega_s,
+                                                           float T_in) {
   const int64_t N = (int64_t)nx * ny * nz;
   const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (x >= N) return;
@@ -88,7 +94,10 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
     acc[x] = acc[x] + e;
     return;
   }
-  const float ux = u[x], uy = v[x], uz = w[x];
+  // a conducting node carried on the lattice: no velocity, its own relaxation rate
+  const bool conj = kind == G_CONJ;
+  const float ux = conj ? 0.0f : u[x], uy = conj ? 0.0f : v[x], uz = conj ? 0.0f : w[x];
+  const float om = conj ? omega_s : omega;
   float g[7];
   if (fl & G_INLET) {
 #pragma unroll
@@ -110,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
     }
     const int64_t s = ((int64_t)a * ny + b) * nz + c;
     const uint8_t sk = flag[s] & G_KIND;
-    if (sk == G_FLUID) {
+    if (sk == G_FLUID || sk == G_CONJ) {
       g[q] = gin[q * N + s];
     } else if (sk == G_WALL) {
       g[q] = gin[opp7(q) * N + x];
@@ -128,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
   for (int q = 0; q < 7; ++q) {
     const float cu = (float)cx7(q) * ux + (float)cy7(q) * uy + (float)cz7(q) * uz;
     const float geq = w7(q) * T * (1.0f + 4.0f * cu);
-    float o = g[q] + omega * (geq - g[q]);
+    float o = g[q] + om * (geq - g[q]);
     if (Q) o = o + w7(q) * src;
     gout[q * N + x] = o;
   }
@@ -210,15 +219,16 @@ extern "C" {
 
 int fc_d3q7_step(const float* g_in, float* g_out, float* T_out, const float* u, const float* v, const float* w,
                  const uint8_t* flag, const float* T_solid, const float* Q, float* acc, int nx, int ny, int nz, int px, int py,
-                 int pz, float omega, float T_in, hipStream_t stream) {
+                 int pz, float omega, float omega_s, float T_in, hipStream_t stream) {
   FC_REQUIRE(nx > 0 && ny > 0 && nz > 0, "d3q7_step: grid extents must be positive");
   FC_REQUIRE(g_in && g_out && T_out && u && v && w && flag && T_solid, "d3q7_step: null pointer argument");
   FC_REQUIRE(g_in != g_out, "d3q7_step: input and output populations must be different buffers");
-  FC_REQUIRE(omega > 0.f && omega < 2.f, "d3q7_step: omega must be in (0, 2)");
+  FC_REQUIRE(omega > 0.f && omega < 2.f && omega_s > 0.f && omega_s < 2.f,
+             "d3q7_step: omega and omega_s must be in (0, 2)");
   const int64_t N = (int64_t)nx * ny * nz;
   FC_REQUIRE(N < ((int64_t)1 << 31), "d3q7_step: the grid must hold fewer than 2^31 nodes");
   hipLaunchKernelGGL(d3q7_step_kernel, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, stream, g_in, g_out, T_out, u, v,
-                     w, flag, T_solid, Q, acc, nx, ny, nz, px != 0, py != 0, pz != 0, omega, T_in);
+                     w, flag, T_solid, Q, acc, nx, ny, nz, px != 0, py != 0, pz != 0, omega, omega_s, T_in);
   return check_launch("d3q7_step");
 }
 

@@ -62,19 +62,56 @@ struct Sdf {
   int nx, ny, nz;
   float ox, oy, oz, dx;
   float hx, hy, hz;  // extent - 1.001 (clamp limits)
-  int px, py, pz;    // periodic axes wrap instead of clamping (build extension)
+  int px, py, pz;    // periodic axes wrap instead of clamping (link q search; build extension)
 };
 
-// grid coordinate along one axis: clamp (reference) or wrap (periodic axis)
-__device__ __forceinline__ void axis_coord(float g, float h, int n, int per, int& i0, int& i1, float& fr) {
+// grid coordinate along one axis, clamped to [0, n - 1.001] (the reference's rule)
+__device__ __forceinline__ void axis_coord(float g, float h, int n, int& i0, int& i1, float& fr) {
+  g = g < 0.0f ? 0.0f : (g > h ? h : g);
+  i0 = (int)floorf(g);
+  fr = g - (float)i0;
+  i1 = min(i0 + 1, n - 1);
+}
+
+// trilinear SDF at a world point (sdf_trilinear, sdf_tracing.py:88-125)
+__device__ float sdf_tri(const Sdf& s, float x, float y, float z) {
+  const float gx = (x - s.ox) / s.dx, gy = (y - s.oy) / s.dx, gz = (z - s.oz) / s.dx;
+  int i0, i1, j0, j1, k0, k1;
+  float fx, fy, fz;
+  axis_coord(gx, s.hx, s.nx, i0, i1, fx);
+  axis_coord(gy, s.hy, s.ny, j0, j1, fy);
+  axis_coord(gz, s.hz, s.nz, k0, k1, fz);
+  auto P = [&](int a, int b, int c) { return s.phi[((int64_t)a * s.ny + b) * s.nz + c]; };
+  const float c000 = P(i0, j0, k0), c100 = P(i1, j0, k0), c010 = P(i0, j1, k0), c110 = P(i1, j1, k0);
+  const float c001 = P(i0, j0, k1), c101 = P(i1, j0, k1), c011 = P(i0, j1, k1), c111 = P(i1, j1, k1);
+  const float cx00 = c000 + fx * (c100 - c000);
+  const float cx10 = c010 + fx * (c110 - c010);
+  const float cx01 = c001 + fx * (c101 - c001);
+  const float cx11 = c011 + fx * (c111 - c011);
+  const float cxy0 = cx00 + fy * (cx10 - cx00);
+  const float cxy1 = cx01 + fy * (cx11 - cx01);
+  return cxy0 + fz * (cxy1 - cxy0);
+}
+
+// Sample of the trilinear SDF at x0 + t d. Non-periodic axes follow the reference's
+// arithmetic (world point, grid coordinate clamped to [0, n - 1.001]). Periodic
+// axes (build extension) take the node's own index plus the offset t d / pitch,
+// wrapped: the cell and the fraction do not depend on where the node sits, so
+// every copy of a periodic geometry gets the same bits (a box of whole periods is
+// an exact sub-domain of a larger box).
+__device__ __forceinline__ void sample_axis(float x0, float m, float o, float pitch, float h, int n, int per, int node,
+                                            int& i0, int& i1, float& fr) {
   if (per) {
-    const float fl = floorf(g);
-    fr = g - fl;
-    int a = (int)fl % n;
+    const float off = m / pitch;
+    const float fl = floorf(off);
+    fr = off - fl;
+    int a = (node + (int)fl) % n;
     a = a < 0 ? a + n : a;
     i0 = a;
     i1 = a + 1 == n ? 0 : a + 1;
   } else {
+    const float x = x0 + m;
+    float g = (x - o) / pitch;
     g = g < 0.0f ? 0.0f : (g > h ? h : g);
     i0 = (int)floorf(g);
     fr = g - (float)i0;
@@ -82,13 +119,13 @@ __device__ __forceinline__ void axis_coord(float g, float h, int n, int per, int
   }
 }
 
-__device__ float sdf_tri(const Sdf& s, float x, float y, float z) {
-  const float gx = (x - s.ox) / s.dx, gy = (y - s.oy) / s.dx, gz = (z - s.oz) / s.dx;
+__device__ float sdf_sample(const Sdf& s, int ni, int nj, int nk, float x0, float y0, float z0, float t, float dx,
+                            float dy, float dz) {
   int i0, i1, j0, j1, k0, k1;
   float fx, fy, fz;
-  axis_coord(gx, s.hx, s.nx, s.px, i0, i1, fx);
-  axis_coord(gy, s.hy, s.ny, s.py, j0, j1, fy);
-  axis_coord(gz, s.hz, s.nz, s.pz, k0, k1, fz);
+  sample_axis(x0, t * dx, s.ox, s.dx, s.hx, s.nx, s.px, ni, i0, i1, fx);
+  sample_axis(y0, t * dy, s.oy, s.dx, s.hy, s.ny, s.py, nj, j0, j1, fy);
+  sample_axis(z0, t * dz, s.oz, s.dx, s.hz, s.nz, s.pz, nk, k0, k1, fz);
   auto P = [&](int a, int b, int c) { return s.phi[((int64_t)a * s.ny + b) * s.nz + c]; };
   const float c000 = P(i0, j0, k0), c100 = P(i1, j0, k0), c010 = P(i0, j1, k0), c110 = P(i1, j1, k0);
   const float c001 = P(i0, j0, k1), c101 = P(i1, j0, k1), c011 = P(i0, j1, k1), c111 = P(i1, j1, k1);
@@ -104,9 +141,10 @@ __device__ float sdf_tri(const Sdf& s, float x, float y, float z) {
 // q in (0, 1] along the segment x0 + t d, t in [0, L] (bfl_q_from_sdf,
 // sdf_tracing.py:127-184): n_coarse + 1 samples at torch.linspace(0, 1) * L, the
 // first + -> (not +) crossing, n_bisect halvings keeping the solid-side end hi,
-// q = hi / (L + 1e-30); 0.5 without a crossing.
-__device__ float q_search(const Sdf& s, float x0, float y0, float z0, float dx, float dy, float dz, float L,
-                          int n_coarse, int n_bisect) {
+// q = hi / (L + 1e-30); 0.5 without a crossing. (ni, nj, nk): the start node's
+// index, read on periodic axes only.
+__device__ float q_search(const Sdf& s, int ni, int nj, int nk, float x0, float y0, float z0, float dx, float dy,
+                          float dz, float L, int n_coarse, int n_bisect) {
   // torch.linspace(0, 1, n_coarse + 1) in FP32
   const int steps = n_coarse + 1, half = steps / 2;
   const float step = 1.0f / (float)(steps - 1);
@@ -115,7 +153,7 @@ __device__ float q_search(const Sdf& s, float x0, float y0, float z0, float dx, 
   for (int k = 0; k < steps; ++k) {
     const float tv = k < half ? 0.0f + step * (float)k : 1.0f - step * (float)(steps - k - 1);
     const float t = tv * L;
-    const float p = sdf_tri(s, x0 + t * dx, y0 + t * dy, z0 + t * dz);
+    const float p = sdf_sample(s, ni, nj, nk, x0, y0, z0, t, dx, dy, dz);
     if (k > 0 && !found && p_prev > 0.0f && !(p > 0.0f)) {
       found = true;
       lo = t_prev;
@@ -127,13 +165,12 @@ __device__ float q_search(const Sdf& s, float x0, float y0, float z0, float dx, 
   if (!found) return 0.5f;
   for (int it = 0; it < n_bisect; ++it) {
     const float mid = 0.5f * (lo + hi);
-    const float pm = sdf_tri(s, x0 + mid * dx, y0 + mid * dy, z0 + mid * dz);
+    const float pm = sdf_sample(s, ni, nj, nk, x0, y0, z0, mid, dx, dy, dz);
     if (pm > 0.0f) lo = mid; else hi = mid;
   }
   return hi / (L + 1e-30f);
 }
 
-// BFL weights (bfl_weights_from_q, sdf_tracing.py:187-205)
 __device__ __forceinline__ void bfl_weights(float qv, float& w1, float& w2, uint8_t& ge) {
   const float a = qv >= 0.5f ? 1.0f / (2.0f * qv) : 2.0f * qv;
   w1 = a;
@@ -149,7 +186,8 @@ __global__ __launch_bounds__(kBlock) void link_q_kernel(Sdf s, const int64_t* __
   const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (l >= n) return;
   const int64_t f = fluid_flat[l];
-  const float fi = (float)(f / ((int64_t)s.ny * s.nz)), fj = (float)((f / s.nz) % s.ny), fk = (float)(f % s.nz);
+  const int ni = (int)(f / ((int64_t)s.ny * s.nz)), nj = (int)((f / s.nz) % s.ny), nk = (int)(f % s.nz);
+  const float fi = (float)ni, fj = (float)nj, fk = (float)nk;
   const float x0 = fi * s.dx + s.ox, y0 = fj * s.dx + s.oy, z0 = fk * s.dx + s.oz;
   const int q = (int)dir_i[l];
   const float cx = (float)CX(q), cy = (float)CY(q), cz = (float)CZ(q);
@@ -157,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void link_q_kernel(Sdf s, const int64_t* __
   const float den = cn + 1e-30f;
   const float dx = cx / den, dy = cy / den, dz = cz / den;
   const float L = s.dx * cn;
-  const float qv = q_search(s, x0, y0, z0, dx, dy, dz, L, n_coarse, n_bisect);
+  const float qv = q_search(s, ni, nj, nk, x0, y0, z0, dx, dy, dz, L, n_coarse, n_bisect);
   q_out[l] = qv;
   bfl_weights(qv, w1[l], w2[l], ge[l]);
 }
@@ -177,8 +215,8 @@ __global__ __launch_bounds__(kBlock) void q_rays_kernel(Sdf s, const float* __re
                                                         float* __restrict__ q_out) {
   const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (l >= n) return;
-  q_out[l] = q_search(s, x0[3 * l], x0[3 * l + 1], x0[3 * l + 2], d[3 * l], d[3 * l + 1], d[3 * l + 2], L[l],
-                      n_coarse, n_bisect);
+  q_out[l] = q_search(s, 0, 0, 0, x0[3 * l], x0[3 * l + 1], x0[3 * l + 2], d[3 * l], d[3 * l + 1], d[3 * l + 2],
+                      L[l], n_coarse, n_bisect);
 }
 
 __global__ __launch_bounds__(kBlock) void weights_kernel(const float* __restrict__ q, int64_t n,

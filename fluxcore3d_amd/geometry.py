@@ -177,10 +177,16 @@ def gyroid(n=384, period=48.0, level=0.3, shape=None):
     """Gyroid solid g(x) < level, g = sin x cos y + sin y cos z + sin z cos x,
     x = 2 pi i / period. phi = (g - level) / |grad g| clipped to +-4 (a first-order
     distance in voxels, negative in solid), so Bouzidi q is the trilinear zero
-    crossing."""
+    crossing. With an integer period every period holds the same values bit for
+    bit, so a periodic box of whole periods is an exact sub-domain of a larger one."""
     nx, ny, nz = shape if shape is not None else (n, n, n)
     w = 2.0 * math.pi / period
-    i, j, k = np.meshgrid(np.arange(nx) * w, np.arange(ny) * w, np.arange(nz) * w, indexing="ij")
+
+    def coord(m):  # node index mod an integer period: every period gets the same bits
+        a = np.arange(m)
+        return (a % int(period) if float(period).is_integer() else a) * w
+
+    i, j, k = np.meshgrid(coord(nx), coord(ny), coord(nz), indexing="ij")
     g = np.sin(i) * np.cos(j) + np.sin(j) * np.cos(k) + np.sin(k) * np.cos(i)
     gx = np.cos(i) * np.cos(j) - np.sin(k) * np.sin(i)
     gy = -np.sin(i) * np.sin(j) + np.cos(j) * np.cos(k)

@@ -49,13 +49,13 @@ Tensor = torch.Tensor
 P, I32, I64, F32, F64 = _lib.P, _lib.I32, _lib.I64, _lib.F32, _lib.F64
 
 _lib.register({
-    "fc_d3q7_step": (I32, [P, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, F32, F32, P]),
+    "fc_d3q7_step": (I32, [P, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, F32, F32, F32, P]),
     "fc_d3q7_init": (I32, [P, P, P, P, P, I64, P]),
     "fc_cht_solid_rhs": (I32, [P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, F64, F64, P, P, P]),
     "fc_cht_solid_store": (I32, [P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, P]),
 })
 
-FLUID, SOLID, WALL, INLET = 0, 1, 2, 16
+FLUID, SOLID, WALL, CONJ, INLET = 0, 1, 2, 3, 16
 
 
 def _stream(t: Tensor):
@@ -85,13 +85,16 @@ def node_flags(conducting: Tensor, wall: Optional[Tensor] = None, inlet: Optiona
 
 
 class D3Q7Transport:
-    """Fluid temperature populations (two [7][N] FP32 buffers) and the step launch."""
+    """Fluid temperature populations (two [7][N] FP32 buffers) and the step launch.
+    ``omega_s`` relaxes kind-3 nodes (conducting solid carried on the lattice)."""
 
-    def __init__(self, flag: Tensor, omega: float, periodic=(0, 0, 0), T_in: float = 0.0):
+    def __init__(self, flag: Tensor, omega: float, periodic=(0, 0, 0), T_in: float = 0.0,
+                 omega_s: Optional[float] = None):
         if flag.dtype != torch.uint8 or flag.dim() != 3 or flag.device.type != "cuda":
             raise ValueError("flag: expected a 3-D uint8 HIP tensor")
-        if not 0.0 < float(omega) < 2.0:
-            raise ValueError("omega must be in (0, 2)")
+        omega_s = omega if omega_s is None else omega_s
+        if not (0.0 < float(omega) < 2.0 and 0.0 < float(omega_s) < 2.0):
+            raise ValueError("omega and omega_s must be in (0, 2)")
         err = _lib.load_error()
         if err is not None:
             raise RuntimeError(f"D3Q7Transport needs libfluxcore_hip.so: {err}")
@@ -99,6 +102,7 @@ class D3Q7Transport:
         self.shape = tuple(flag.shape)
         self.N = int(flag.numel())
         self.omega = float(omega)
+        self.omega_s = float(omega_s)
         self.periodic = tuple(int(bool(p)) for p in periodic)
         self.T_in = float(T_in)
         dev = flag.device
@@ -129,7 +133,7 @@ class D3Q7Transport:
         src, dst = self.g[self.cur], self.g[1 - self.cur]
         _lib.call("fc_d3q7_step", src.data_ptr(), dst.data_ptr(), self.T.data_ptr(), *[t.data_ptr() for t in uu],
                   self.flag.data_ptr(), ts.data_ptr(), _ptr(q), _ptr(a), nx, ny, nz, *self.periodic, self.omega,
-                  self.T_in, _stream(self.T))
+                  self.omega_s, self.T_in, _stream(self.T))
         self.cur = 1 - self.cur
         return self.T
 
@@ -290,3 +294,91 @@ class CoupledCHT:
         """(nx, ny, nz) temperature: fluid from D3Q7, conducting nodes from the solid."""
         T = torch.where((self.flag.reshape(-1) & 3) == SOLID, self.T_solid, self.transport.T)
         return (T + self.T_in).view(self.transport.shape)
+
+
+# ----------------------------------------------------------------------------
+# Conjugate heat on periodic porous media: the solid carried on the D3Q7 lattice
+# (kind 3, u = 0, its own relaxation rate), one launch per step for fluid and solid.
+# ----------------------------------------------------------------------------
+_C7 = ((0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))
+_OPP7 = (0, 2, 1, 4, 3, 6, 5)
+
+
+def _interface_masks(flag: Tensor, solid_kind: int):
+    kind = flag & 3
+    fluid = kind == FLUID
+    solid = kind == solid_kind
+    return [fluid & torch.roll(solid, shifts=_C7[q], dims=(0, 1, 2)) for q in range(7)]
+
+
+def interface_heat(g: Tensor, flag: Tensor, solid_kind: int = CONJ) -> float:
+    """Energy per step crossing from solid nodes into fluid nodes of a periodic
+    domain, from the post-collision populations g ([7][N]): over every fluid node x
+    and direction q with x - c_q solid, g*_q(x - c_q) - g*_opp(q)(x) (FP64 sums)."""
+    shape = tuple(flag.shape)
+    gv = g.view(7, *shape)
+    masks = _interface_masks(flag, solid_kind)
+    e = torch.zeros((), dtype=torch.float64, device=g.device)
+    for q in range(1, 7):
+        m = masks[q]
+        e += torch.roll(gv[q], shifts=_C7[q], dims=(0, 1, 2))[m].double().sum() - gv[_OPP7[q]][m].double().sum()
+    return float(e)
+
+
+def interface_links(flag: Tensor, solid_kind: int = CONJ) -> int:
+    """Fluid-solid faces of a periodic domain (links between face neighbours)."""
+    return int(sum(int(m.sum()) for m in _interface_masks(flag, solid_kind)[1:]))
+
+
+def nusselt(g: Tensor, T: Tensor, flag: Tensor, kappa_f: float, solid_kind: int = CONJ) -> dict:
+    """Interstitial Nusselt number of a periodic conjugate problem: h = Q / (A (mean
+    T_solid - mean T_fluid)), Nu = h d_h / k_f, Q the energy per step crossing the
+    interface, A the interface links, d_h = 4 V_fluid / A, k_f = kappa_f (fluid
+    rho c_p = 1)."""
+    kind = flag.reshape(-1) & 3
+    Tf64 = T.reshape(-1).double()
+    Q = interface_heat(g, flag, solid_kind)
+    A = interface_links(flag, solid_kind)
+    Ts = float(Tf64[kind == solid_kind].mean())
+    Tfl = float(Tf64[kind == FLUID].mean())
+    d_h = 4.0 * float((kind == FLUID).sum()) / A
+    h = Q / (A * (Ts - Tfl))
+    return {"Nu": h * d_h / kappa_f, "h": h, "Q": Q, "A": A, "d_h": d_h, "T_solid_mean": Ts, "T_fluid_mean": Tfl}
+
+
+class ConjugateHeat:
+    """Periodic flow (an AAEngine, one fused launch per step) and conjugate heat:
+    fluid temperature on D3Q7 with the step's velocity, the solid carried on the
+    same lattice with conductivity ``k_ratio`` kappa_f (equal rho c_p), a uniform
+    source ``q_solid`` per solid node per step and the balancing uniform sink in the
+    fluid (a statistically steady state without drift of the mean temperature)."""
+
+    def __init__(self, engine, solid: Tensor, kappa_f: float, k_ratio: float = 10.0, q_solid: float = 1e-4):
+        dev = solid.device
+        self.engine = engine
+        self.shape = tuple(solid.shape)
+        N = solid.numel()
+        self.flag = torch.where(solid.bool(), torch.tensor(CONJ, dtype=torch.uint8, device=dev),
+                                torch.tensor(FLUID, dtype=torch.uint8, device=dev)).contiguous()
+        self.kappa_f = float(kappa_f)
+        self.kappa_s = float(k_ratio) * self.kappa_f
+        omega_f = 1.0 / (0.5 + 4.0 * self.kappa_f)
+        omega_s = 1.0 / (0.5 + 4.0 * self.kappa_s)
+        self.transport = D3Q7Transport(self.flag, omega_f, periodic=(1, 1, 1), omega_s=omega_s)
+        n_s = int(solid.sum())
+        n_f = N - n_s
+        self.Q = torch.where(solid.reshape(-1).bool(), torch.tensor(float(q_solid), device=dev),
+                             torch.tensor(-float(q_solid) * n_s / n_f, device=dev)).contiguous()
+        self.macro = [torch.zeros(N, device=dev) for _ in range(4)]
+        self._zero = torch.zeros(N, device=dev)
+        self.transport.init(self._zero, self._zero, self._zero, self._zero)
+        self.n = 0
+
+    def step(self) -> None:
+        self.engine.step(*self.macro)
+        _, u, v, w = self.macro
+        self.transport.step(u, v, w, self._zero, Q=self.Q)
+        self.n += 1
+
+    def nusselt(self) -> dict:
+        return nusselt(self.transport.populations, self.transport.T, self.flag, self.kappa_f)

@@ -377,13 +377,15 @@ int fc_bicgstab_update(double *x, double *r, const double *s, const double *t, c
  * LBM_CHT.py:2208-2359, cht_worker.py:23-165). These carry the fluid temperature
  * every flow step and hand the energy crossing the fluid-solid interface to the
  * solid, whose conduction step is solved with the multigrid entry points above.
- * flag: 0 fluid, 1 conducting solid, 2 adiabatic wall, | 16 inlet (held at T_in).
+ * flag: 0 fluid, 1 conducting solid (temperature T_solid, solved apart), 2 adiabatic
+ * wall, 3 conducting solid carried on the lattice (u = 0, relaxation omega_s:
+ * conjugate conduction in the same launch), | 16 inlet (held at T_in).
  * g_in, g_out: [7][N] post-collision populations (0, +x, -x, +y, -y, +z, -z),
  * N = nx ny nz < 2^31 in C order; Q (may be NULL): fluid heat source per step;
  * acc (may be NULL): per-node energy the solid receives, accumulated. */
 int fc_d3q7_step(const float *g_in, float *g_out, float *T_out, const float *u, const float *v, const float *w,
                  const uint8_t *flag, const float *T_solid, const float *Q, float *acc, int nx, int ny, int nz,
-                 int px, int py, int pz, float omega, float T_in, hipStream_t stream);
+                 int px, int py, int pz, float omega, float omega_s, float T_in, hipStream_t stream);
 /* g = w_i T (1 + 4 c_i.u) */
 int fc_d3q7_init(float *g, const float *T, const float *u, const float *v, const float *w, int64_t n,
                  hipStream_t stream);

@@ -14,7 +14,10 @@ Pins (tests/test_oracle_d3q7.py): the analytic 1-D advection-diffusion of a
 Gaussian pulse (with the model's known diffusivity (1/omega - 1/2)(1/4 - u^2)
 along the flow), the linear conduction profile between two solids held at
 different temperatures, exact energy conservation with walls, solids and
-periodic faces, and the solid solve against the conduction balance it encodes.
+periodic faces, the solid solve against the conduction balance it encodes, the
+conjugate (carried-solid) steady state against the harmonic-mean finite-volume
+solution, exact tiling of periodic domains, and the Nusselt diagnostic against
+the steady periodic slab solved by finite volumes.
 """
 from __future__ import annotations
 
@@ -30,7 +33,7 @@ CY7 = np.array([0, 0, 0, 1, -1, 0, 0])
 CZ7 = np.array([0, 0, 0, 0, 0, 1, -1])
 W7 = np.array([0.25] + [0.125] * 6)
 
-FLUID, SOLID, WALL, INLET = 0, 1, 2, 16
+FLUID, SOLID, WALL, CONJ, INLET = 0, 1, 2, 3, 16
 
 
 def flags(conducting: np.ndarray, wall: Optional[np.ndarray] = None, inlet: Optional[np.ndarray] = None) -> np.ndarray:
@@ -55,9 +58,10 @@ def equilibrium(T, u, v, w, dtype=np.float64) -> np.ndarray:
     return out
 
 
-def step(g, fl, u, v, w, T_solid, shape, periodic=(0, 0, 0), omega=1.0, T_in=0.0, Q=None, acc=None):
+def step(g, fl, u, v, w, T_solid, shape, periodic=(0, 0, 0), omega=1.0, T_in=0.0, Q=None, acc=None, omega_s=None):
     """One D3Q7 step (orc_d3q7_step). Returns (g_next, T); ``acc`` (solid energy
-    accumulator, same dtype) is updated in place."""
+    accumulator, same dtype) is updated in place; ``omega_s`` relaxes kind-3 nodes
+    (default omega)."""
     dtype = g.dtype.type
     lib = olbm._lib(dtype)
     nx, ny, nz = shape
@@ -73,7 +77,8 @@ def step(g, fl, u, v, w, T_solid, shape, periodic=(0, 0, 0), omega=1.0, T_in=0.0
     R = olbm._real(dtype)
     lib.orc_d3q7_step(olbm._p(gin), olbm._p(gout), olbm._p(T), *[olbm._p(a) for a in arr[:3]], olbm._p(fl),
                       olbm._p(arr[3]), None if Qa is None else olbm._p(Qa), None if acc is None else olbm._p(acc),
-                      C.c_int(nx), C.c_int(ny), C.c_int(nz), *[C.c_int(int(p)) for p in periodic], R(omega), R(T_in))
+                      C.c_int(nx), C.c_int(ny), C.c_int(nz), *[C.c_int(int(p)) for p in periodic], R(omega),
+                      R(omega if omega_s is None else omega_s), R(T_in))
     return gout, T
 
 
@@ -166,3 +171,47 @@ class CoupledLoop:
             if on_step is not None:
                 on_step(self)
         return self
+
+
+def interface_heat(g, fl, shape, solid_kind=CONJ) -> float:
+    """Energy per step that crosses from carried-solid nodes into fluid nodes, from
+    the post-collision populations g ([7][N]) of a periodic domain: over every
+    fluid node x and direction q with x - c_q solid, g*_q(x - c_q) - g*_opp(q)(x)."""
+    nx, ny, nz = shape
+    g = np.asarray(g).reshape(7, nx, ny, nz)
+    kind = (np.asarray(fl).reshape(nx, ny, nz) & 3)
+    fluid = kind == FLUID
+    solid = kind == solid_kind
+    opp = [0, 2, 1, 4, 3, 6, 5]
+    e = 0.0
+    for q in range(1, 7):
+        c = (int(CX7[q]), int(CY7[q]), int(CZ7[q]))
+        m = fluid & np.roll(solid, shift=c, axis=(0, 1, 2))  # x fluid, x - c_q solid
+        e += float(np.roll(g[q], shift=c, axis=(0, 1, 2))[m].sum() - g[opp[q]][m].sum())
+    return e
+
+
+def interface_links(fl, shape, solid_kind=CONJ) -> int:
+    """Fluid-solid faces (lattice links between face neighbours) of a periodic domain."""
+    nx, ny, nz = shape
+    kind = np.asarray(fl).reshape(nx, ny, nz) & 3
+    fluid = kind == FLUID
+    solid = kind == solid_kind
+    return int(sum((fluid & np.roll(solid, shift=(int(CX7[q]), int(CY7[q]), int(CZ7[q])), axis=(0, 1, 2))).sum()
+                   for q in range(1, 7)))
+
+
+def nusselt(g, T, fl, shape, kappa_f, solid_kind=CONJ) -> dict:
+    """Interstitial Nusselt number of a periodic conjugate problem:
+    h = Q / (A (mean T_solid - mean T_fluid)), Nu = h d_h / k_f with Q the energy per
+    step crossing the interface (interface_heat), A the interface links, d_h =
+    4 V_fluid / A and k_f = kappa_f (fluid rho c_p = 1)."""
+    kind = np.asarray(fl).reshape(-1) & 3
+    T = np.asarray(T, np.float64).reshape(-1)
+    Q = interface_heat(g, fl, shape, solid_kind)
+    A = interface_links(fl, shape, solid_kind)
+    Ts = float(T[kind == solid_kind].mean())
+    Tf = float(T[kind == FLUID].mean())
+    d_h = 4.0 * float((kind == FLUID).sum()) / A
+    h = Q / (A * (Ts - Tf))
+    return {"Nu": h * d_h / kappa_f, "h": h, "Q": Q, "A": A, "d_h": d_h, "T_solid_mean": Ts, "T_fluid_mean": Tf}

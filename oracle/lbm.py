@@ -226,8 +226,10 @@ def build_links(solid: np.ndarray, periodic=(False, False, False)):
             "ff_flat": np.concatenate(fff_all).astype(np.int64), "ff_valid": np.concatenate(ok_all)}
 
 
-def bfl_q(phi, origin, pitch, fluid_flat, dir_i, n_coarse=6, n_bisect=10):
-    """q, w1, w2, q>=1/2 per link (sdf_tracing.py:127-259), FP32 eager op order."""
+def bfl_q(phi, origin, pitch, fluid_flat, dir_i, n_coarse=6, n_bisect=10, periodic=(False, False, False)):
+    """q, w1, w2, q>=1/2 per link (sdf_tracing.py:127-259), FP32 eager op order.
+    ``periodic`` (build extension): samples on those axes are the node's index plus
+    the offset t d / pitch, wrapped (the reference clamps every axis)."""
     phi = np.ascontiguousarray(phi, np.float32)
     nx, ny, nz = phi.shape
     n = int(fluid_flat.size)
@@ -236,9 +238,9 @@ def bfl_q(phi, origin, pitch, fluid_flat, dir_i, n_coarse=6, n_bisect=10):
     org = np.ascontiguousarray(np.asarray(origin, np.float32))
     ff = np.ascontiguousarray(fluid_flat, np.int64)
     di = np.ascontiguousarray(dir_i, np.int64)
-    _lib(np.float32).orc_bfl_q(_p(phi), C.c_int(nx), C.c_int(ny), C.c_int(nz), _p(org), C.c_float(pitch), _p(ff),
-                               _p(di), C.c_int64(n), C.c_int(n_coarse), C.c_int(n_bisect), _p(q), _p(w1), _p(w2),
-                               _p(ge))
+    _lib(np.float32).orc_bfl_q_per(_p(phi), C.c_int(nx), C.c_int(ny), C.c_int(nz), _p(org), C.c_float(pitch),
+                                   _p(ff), _p(di), C.c_int64(n), C.c_int(n_coarse), C.c_int(n_bisect),
+                                   *[C.c_int(int(bool(p))) for p in periodic], _p(q), _p(w1), _p(w2), _p(ge))
     return q, w1, w2, ge.astype(bool)
 
 
@@ -291,6 +293,6 @@ def linspace01(steps):
 def ibb_from_sdf(solid, phi, origin=(0.0, 0.0, 0.0), pitch=1.0, periodic=(False, False, False)):
     """Complete IBBData-equivalent dict for a geometry (links + q + weights)."""
     links = build_links(solid, periodic)
-    q, w1, w2, ge = bfl_q(phi, origin, pitch, links["fluid_flat"], links["dir_i"])
+    q, w1, w2, ge = bfl_q(phi, origin, pitch, links["fluid_flat"], links["dir_i"], periodic=periodic)
     links.update(q=q, w1=w1, w2=w2, q_ge_half=ge)
     return links

@@ -398,17 +398,70 @@ void orc_linspace01(int steps, float *out) {
   for (int i = 0; i < steps; ++i) out[i] = i < half ? 0.0f + step * (float)i : 1.0f - step * (float)(steps - i - 1);
 }
 
+/* One sample of the trilinear SDF at x0 + m (m = t d per axis). Non-periodic axes:
+ * the reference's world point and clamp (sdf_tri). Periodic axes (build extension):
+ * the start node's index plus the offset m / pitch, wrapped, so the cell and the
+ * fraction do not depend on where the node sits (exact tiling of periodic boxes). */
+static float sdf_sample(const float *phi, int nx, int ny, int nz, const float o[3], float pitch, const int per[3],
+                        const int node[3], const float x0[3], const float m[3]) {
+  if (!per[0] && !per[1] && !per[2]) {
+    float p[3];
+    for (int a = 0; a < 3; ++a) p[a] = x0[a] + m[a];
+    return sdf_tri(phi, nx, ny, nz, o, pitch, p[0], p[1], p[2]);
+  }
+  const int n[3] = {nx, ny, nz};
+  int64_t i0[3], i1[3];
+  float fr[3];
+  for (int a = 0; a < 3; ++a) {
+    if (per[a]) {
+      const float off = m[a] / pitch;
+      const float fl = floorf(off);
+      fr[a] = off - fl;
+      int c = (node[a] + (int)fl) % n[a];
+      c = c < 0 ? c + n[a] : c;
+      i0[a] = c;
+      i1[a] = c + 1 == n[a] ? 0 : c + 1;
+    } else {
+      const float h = (float)((double)n[a] - 1.001);
+      const float x = x0[a] + m[a];
+      float g = (x - o[a]) / pitch;
+      g = g < 0.0f ? 0.0f : (g > h ? h : g);
+      i0[a] = (int64_t)floorf(g);
+      fr[a] = g - (float)i0[a];
+      i1[a] = i0[a] + 1 < n[a] - 1 ? i0[a] + 1 : n[a] - 1;
+    }
+  }
+#define PHI(a, b, c) phi[((a) * (int64_t)ny + (b)) * nz + (c)]
+  const float c000 = PHI(i0[0], i0[1], i0[2]), c100 = PHI(i1[0], i0[1], i0[2]);
+  const float c010 = PHI(i0[0], i1[1], i0[2]), c110 = PHI(i1[0], i1[1], i0[2]);
+  const float c001 = PHI(i0[0], i0[1], i1[2]), c101 = PHI(i1[0], i0[1], i1[2]);
+  const float c011 = PHI(i0[0], i1[1], i1[2]), c111 = PHI(i1[0], i1[1], i1[2]);
+#undef PHI
+  const float fx = fr[0], fy = fr[1], fz = fr[2];
+  float t;
+  t = c100 - c000; t = fx * t; const float cx00 = c000 + t;
+  t = c110 - c010; t = fx * t; const float cx10 = c010 + t;
+  t = c101 - c001; t = fx * t; const float cx01 = c001 + t;
+  t = c111 - c011; t = fx * t; const float cx11 = c011 + t;
+  t = cx10 - cx00; t = fy * t; const float cxy0 = cx00 + t;
+  t = cx11 - cx01; t = fy * t; const float cxy1 = cx01 + t;
+  t = cxy1 - cxy0; t = fz * t;
+  return cxy0 + t;
+}
+
 /* bfl_q_from_sdf for one ray (sdf_tracing.py:127-184): samples at
  * linspace(0, 1) * L along x0 + t d, first + -> (not +) crossing, bisection
- * keeping hi on the solid side, q = hi / (L + 1e-30); 0.5 without a crossing. */
-static float q_ray(const float *phi, int nx, int ny, int nz, const float origin[3], float pitch, const float x0[3],
-                   const float d[3], float L, int n_coarse, int n_bisect, const float *tv) {
+ * keeping hi on the solid side, q = hi / (L + 1e-30); 0.5 without a crossing.
+ * node: the start node's index (read on periodic axes only). */
+static float q_ray(const float *phi, int nx, int ny, int nz, const float origin[3], float pitch, const int per[3],
+                   const int node[3], const float x0[3], const float d[3], float L, int n_coarse, int n_bisect,
+                   const float *tv) {
   float tt[64], ph[64];
   for (int s = 0; s <= n_coarse; ++s) {
     tt[s] = tv[s] * L;
-    float p[3];
-    for (int a = 0; a < 3; ++a) { float m = tt[s] * d[a]; p[a] = x0[a] + m; }
-    ph[s] = sdf_tri(phi, nx, ny, nz, origin, pitch, p[0], p[1], p[2]);
+    float m[3];
+    for (int a = 0; a < 3; ++a) m[a] = tt[s] * d[a];
+    ph[s] = sdf_sample(phi, nx, ny, nz, origin, pitch, per, node, x0, m);
   }
   int idx = -1;
   for (int s = 0; s < n_coarse; ++s)
@@ -418,9 +471,9 @@ static float q_ray(const float *phi, int nx, int ny, int nz, const float origin[
   for (int it = 0; it < n_bisect; ++it) {
     float mid = lo + hi;
     mid = 0.5f * mid;
-    float p[3];
-    for (int a = 0; a < 3; ++a) { float m = mid * d[a]; p[a] = x0[a] + m; }
-    const float pm = sdf_tri(phi, nx, ny, nz, origin, pitch, p[0], p[1], p[2]);
+    float m[3];
+    for (int a = 0; a < 3; ++a) m[a] = mid * d[a];
+    const float pm = sdf_sample(phi, nx, ny, nz, origin, pitch, per, node, x0, m);
     if (pm > 0.0f) lo = mid; else hi = mid;
   }
   return hi / (L + 1e-30f);
@@ -438,14 +491,16 @@ static void bfl_w(float qv, float *w1, float *w2, uint8_t *ge) {
  * (sdf_tracing.py:127-259): for each link, x0 = (i,j,k)*pitch + origin,
  * d = c/|c|, L = pitch*|c|, coarse samples then bisection, q = hi/(L+1e-30),
  * q = 0.5 without a crossing; weights w1, w2 and the q >= 1/2 class. */
-void orc_bfl_q(const float *phi, int nx, int ny, int nz, const float origin[3], float pitch, const int64_t *fluid_flat,
-               const int64_t *dir_i, int64_t n, int n_coarse, int n_bisect, float *q_out, float *w1, float *w2,
-               uint8_t *ge) {
+void orc_bfl_q_per(const float *phi, int nx, int ny, int nz, const float origin[3], float pitch,
+                   const int64_t *fluid_flat, const int64_t *dir_i, int64_t n, int n_coarse, int n_bisect, int px,
+                   int py, int pz, float *q_out, float *w1, float *w2, uint8_t *ge) {
   float tv[64];
   orc_linspace01(n_coarse + 1, tv);
+  const int per[3] = {px != 0, py != 0, pz != 0};
   for (int64_t l = 0; l < n; ++l) {
     const int64_t f = fluid_flat[l];
-    const float fi = (float)(f / ((int64_t)ny * nz)), fj = (float)((f / nz) % ny), fk = (float)(f % nz);
+    const int node[3] = {(int)(f / ((int64_t)ny * nz)), (int)((f / nz) % ny), (int)(f % nz)};
+    const float fi = (float)node[0], fj = (float)node[1], fk = (float)node[2];
     const float x0[3] = {fi * pitch + origin[0], fj * pitch + origin[1], fk * pitch + origin[2]};
     const int q = (int)dir_i[l];
     const float craw[3] = {(float)CX[q], (float)CY[q], (float)CZ[q]};
@@ -453,9 +508,15 @@ void orc_bfl_q(const float *phi, int nx, int ny, int nz, const float origin[3], 
     const float den = cn + 1e-30f;
     const float d[3] = {craw[0] / den, craw[1] / den, craw[2] / den};
     const float L = pitch * cn;
-    q_out[l] = q_ray(phi, nx, ny, nz, origin, pitch, x0, d, L, n_coarse, n_bisect, tv);
+    q_out[l] = q_ray(phi, nx, ny, nz, origin, pitch, per, node, x0, d, L, n_coarse, n_bisect, tv);
     bfl_w(q_out[l], &w1[l], &w2[l], &ge[l]);
   }
+}
+
+void orc_bfl_q(const float *phi, int nx, int ny, int nz, const float origin[3], float pitch, const int64_t *fluid_flat,
+               const int64_t *dir_i, int64_t n, int n_coarse, int n_bisect, float *q_out, float *w1, float *w2,
+               uint8_t *ge) {
+  orc_bfl_q_per(phi, nx, ny, nz, origin, pitch, fluid_flat, dir_i, n, n_coarse, n_bisect, 0, 0, 0, q_out, w1, w2, ge);
 }
 
 /* sdf_trilinear (sdf_tracing.py:88-125) at points x[n][3] */
@@ -469,7 +530,9 @@ void orc_q_rays(const float *phi, int nx, int ny, int nz, const float origin[3],
                 const float *d, const float *L, int64_t n, int n_coarse, int n_bisect, float *q_out) {
   float tv[64];
   orc_linspace01(n_coarse + 1, tv);
-  for (int64_t l = 0; l < n; ++l) q_out[l] = q_ray(phi, nx, ny, nz, origin, pitch, x0 + 3 * l, d + 3 * l, L[l], n_coarse, n_bisect, tv);
+  const int per[3] = {0, 0, 0}, node[3] = {0, 0, 0};
+  for (int64_t l = 0; l < n; ++l)
+    q_out[l] = q_ray(phi, nx, ny, nz, origin, pitch, per, node, x0 + 3 * l, d + 3 * l, L[l], n_coarse, n_bisect, tv);
 }
 
 /* bfl_weights_from_q (sdf_tracing.py:187-205) */
@@ -487,7 +550,10 @@ void orc_bfl_weights(const float *q, int64_t n, uint8_t *ge, float *w1, float *w
  * temperature on the face between the nodes), zero gradient across non-periodic
  * domain faces, inlet nodes (flag bit 4) held at equilibrium at T_in. Solid nodes
  * report T_s and, with acc, add the energy their fluid neighbours lose to them.
- * Flags: 0 fluid, 1 conducting solid, 2 adiabatic wall, | 16 inlet.
+ * Kind 3: conducting solid carried on the lattice (streams like fluid, u = 0, rate
+ * omega_s): conjugate conduction whose steady interface conductance is the harmonic
+ * mean of the two sides. Flags: 0 fluid, 1 conducting solid, 2 adiabatic wall,
+ * 3 carried solid, | 16 inlet.
  * ------------------------------------------------------------------------- */
 static const int C7X[7] = {0, 1, -1, 0, 0, 0, 0};
 static const int C7Y[7] = {0, 0, 0, 1, -1, 0, 0};
@@ -507,7 +573,7 @@ static int nb_axis(int c, int d, int n, int per, int *out) {
 
 void orc_d3q7_step(const real *gin, real *gout, real *T_out, const real *u, const real *v, const real *w,
                    const uint8_t *flag, const real *T_solid, const real *Q, real *acc, int nx, int ny, int nz,
-                   int px, int py, int pz, real omega, real T_in) {
+                   int px, int py, int pz, real omega, real omega_s, real T_in) {
   const int64_t N = (int64_t)nx * ny * nz;
 #pragma omp parallel for schedule(static)
   for (int64_t x = 0; x < N; ++x) {
@@ -533,7 +599,9 @@ void orc_d3q7_step(const real *gin, real *gout, real *T_out, const real *u, cons
       acc[x] = acc[x] + e;
       continue;
     }
-    const real ux = u[x], uy = v[x], uz = w[x];
+    const int conj = kind == 3;
+    const real ux = conj ? (real)0 : u[x], uy = conj ? (real)0 : v[x], uz = conj ? (real)0 : w[x];
+    const real om = conj ? omega_s : omega;
     real g[7];
     if (flag[x] & 16) {
       for (int q = 0; q < 7; ++q) {
@@ -553,7 +621,7 @@ void orc_d3q7_step(const real *gin, real *gout, real *T_out, const real *u, cons
       }
       const int64_t s = ((int64_t)a * ny + b) * nz + c;
       const int sk = flag[s] & 3;
-      if (sk == 0) g[q] = gin[q * N + s];
+      if (sk == 0 || sk == 3) g[q] = gin[q * N + s];
       else if (sk == 2) g[q] = gin[O7[q] * N + x];
       else {
         const real two_wt = (real)2 * w7(q) * T_solid[s];
@@ -567,7 +635,7 @@ void orc_d3q7_step(const real *gin, real *gout, real *T_out, const real *u, cons
     for (int q = 0; q < 7; ++q) {
       const real cu = (real)C7X[q] * ux + (real)C7Y[q] * uy + (real)C7Z[q] * uz;
       const real geq = w7(q) * T * ((real)1 + (real)4 * cu);
-      real o = g[q] + omega * (geq - g[q]);
+      real o = g[q] + om * (geq - g[q]);
       if (Q) o = o + w7(q) * src;
       gout[q * N + x] = o;
     }

@@ -5,7 +5,8 @@ import types
 import numpy as np
 
 from fluxcore3d_amd import geometry
-from fluxcore3d_amd.solver.d3q19 import W, mrt_rates
+from fluxcore3d_amd.solver.flow_state import FlowState
+from fluxcore3d_amd.solver.d3q19 import mrt_rates
 from oracle import lbm as olbm
 
 
@@ -13,40 +14,8 @@ class Cfg(types.SimpleNamespace):
     pass
 
 
-class MiniSim:
-    """The fields of LBMCHT3D_Torch (LBM_CHT.py:562-720) that the drop-in patch reads."""
-
-    def __init__(self, nx, ny, nz, cfg, solid, solid_flow, phi, omega, device, S=None, links=None, omega_minus=None):
-        import torch
-        self.cfg = cfg
-        self.nx, self.ny, self.nz = nx, ny, nz
-        self.device = torch.device(device)
-        self.dtype = torch.float32
-        self.solid = torch.as_tensor(solid, device=self.device)
-        self.solid_flow = torch.as_tensor(solid_flow, device=self.device)
-        self.phi = torch.as_tensor(phi, device=self.device)
-        self.omega_plus = torch.tensor(float(omega))
-        self.omega = self.omega_plus
-        if omega_minus is not None:
-            self.omega_minus = torch.tensor(float(omega_minus))
-        self.u_in_lat = float(getattr(cfg, "u_in_lat", 0.05))
-        self.f = torch.as_tensor(np.broadcast_to(W[:, None, None, None], (19, nx, ny, nz)).copy(), device=self.device)
-        self.rho = torch.ones((nx, ny, nz), device=self.device)
-        self.u = torch.zeros((nx, ny, nz), device=self.device)
-        self.v = torch.zeros_like(self.u)
-        self.wv = torch.zeros_like(self.u)
-        if S is not None:
-            self._mrt_S = torch.as_tensor(S)
-        self.ibb_data = links
-
-    def _bounce_back_post_stream(self):
-        s = self.solid_flow
-        if not s.any():
-            return
-        f_copy = self.f.clone()
-        opp = [0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15]
-        for i in range(19):
-            self.f[i, s] = f_copy[opp[i], s]
+# the minimal drop-in flow state lives in the package (the benchmark uses it too)
+MiniSim = FlowState
 
 
 def config1():
@@ -112,3 +81,14 @@ def periodic_pack(n=48, seed=3):
 
 def mrt_S(omega):
     return mrt_rates(omega)
+
+
+def gyroid_heat_params():
+    """Config 4's conjugate heat leg: MRT flow (tau 0.55, other rates 1.2) driven by a
+    body force 1e-5 along x through the period-48 gyroid from rest; fluid kappa =
+    nu / 0.71, solid conductivity 10x the fluid's (equal rho c_p), source 1e-4 per
+    solid node per step balanced by a uniform sink in the fluid; 5000 steps."""
+    tau = 0.55
+    om = 1.0 / tau
+    return dict(omega=om, S=mrt_rates(om, s_e=1.2, s_bulk=1.2, s_ghost=1.2), force=(1e-5, 0.0, 0.0),
+                kappa_f=(tau - 0.5) / 3.0 / 0.71, k_ratio=10.0, q_solid=1e-4, steps=5000)

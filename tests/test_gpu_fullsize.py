@@ -130,17 +130,11 @@ def test_pack_256_links_q_and_two_steps_match_oracle():
     assert ref["fluid_flat"].size > 10 ** 6
     for k in ("fluid_flat", "dir_i", "ff_flat", "ff_valid"):
         np.testing.assert_array_equal(links[k], ref[k], err_msg=k)
-    # q: the oracle clamps SDF samples at the faces as the reference does, the periodic
-    # build wraps them; on links whose samples stay inside the box (node 1..n-3 on
-    # every axis) both read the same eight values and must agree bit for bit
-    ff = links["fluid_flat"]
-    inner = np.ones(ff.size, bool)
-    for c in (ff // (n * n), (ff // n) % n, ff % n):
-        inner &= (c >= 1) & (c <= n - 3)
-    assert inner.mean() > 0.9
+    # q on every link, the ones that wrap across the periodic faces included: both
+    # sides sample on periodic axes at the node's index plus the offset, wrapped
     links["q"] = q.cpu().numpy()
     for k in ("q", "w1", "w2", "q_ge_half"):
-        np.testing.assert_array_equal(links[k][inner], ref[k][inner], err_msg=k)
+        np.testing.assert_array_equal(links[k], ref[k], err_msg=k)
     # the bench's MRT rates and body force, from a perturbed equilibrium
     om = 1.0 / 0.55
     S = np.full(19, 1.2, np.float32)

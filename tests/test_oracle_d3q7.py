@@ -236,3 +236,161 @@ def test_solid_step_eigenmode_fixed_point_and_balance():
     rhs = acc[cond].sum() + dt * rcp * src[cond].sum()
     assert abs(lhs - rhs) < 1e-10 * (abs(rhs) + 1)
     assert np.array_equal(T[~cond], T_old[~cond])
+
+
+def _fv_periodic_or_walled(kap, src, TL=None, TR=None):
+    """Steady 1-D finite volumes with harmonic-mean face conductances 1/(1/(2k_i) +
+    1/(2k_j)); walls at -1/2 and n - 1/2 held at TL, TR (conductance 2k), or a
+    periodic ring when TL is None (mean fixed at zero)."""
+    n = kap.size
+    A = np.zeros((n, n))
+    b = np.array(src, np.float64)
+    pairs = [(i, i + 1) for i in range(n - 1)] + ([(n - 1, 0)] if TL is None else [])
+    for i, j in pairs:
+        G = 1.0 / (1.0 / (2 * kap[i]) + 1.0 / (2 * kap[j]))
+        A[i, i] += G
+        A[j, j] += G
+        A[i, j] -= G
+        A[j, i] -= G
+    if TL is not None:
+        A[0, 0] += 2 * kap[0]
+        b[0] += 2 * kap[0] * TL
+        A[-1, -1] += 2 * kap[-1]
+        b[-1] += 2 * kap[-1] * TR
+        return np.linalg.solve(A, b)
+    A = np.vstack([A, np.ones(n)])
+    return np.linalg.lstsq(A, np.append(b, 0.0), rcond=None)[0]
+
+
+def test_conjugate_two_layer_steady_state_is_harmonic_finite_volumes():
+    """Fluid (kind 0, u = 0) and carried solid (kind 3, own rate) between two solids
+    held at 0 and 1: the steady D3Q7 profile equals finite volumes with harmonic-mean
+    face conductances (the face rule of the reference's operator)."""
+    n = 30
+    fl = np.zeros((n + 2, 1, 1), np.uint8)
+    fl[0] = fl[-1] = od.SOLID
+    fl[1 + 12:-1] = od.CONJ
+    Ts = np.zeros(n + 2)
+    Ts[-1] = 1.0
+    kf, ks = 0.05, 0.4
+    om_f, om_s = 1 / (0.5 + 4 * kf), 1 / (0.5 + 4 * ks)
+    N = n + 2
+    z = np.zeros(N)
+    g = od.equilibrium(np.zeros(N), z, z, z)
+    for _ in range(60000):
+        g, T = od.step(g, fl, z, z, z, Ts, (N, 1, 1), periodic=(0, 1, 1), omega=om_f, omega_s=om_s)
+    kap = np.where(fl[1:-1, 0, 0] == od.CONJ, ks, kf)
+    exact = _fv_periodic_or_walled(kap, np.zeros(n), 0.0, 1.0)
+    assert np.abs(T[1:-1] - exact).max() < 1e-11
+
+
+def test_conjugate_periodic_tiling_is_exact():
+    rng = np.random.default_rng(4)
+    base = (6, 5, 7)
+    Nb = int(np.prod(base))
+    kind = rng.choice([0, 0, 0, 2, 3, 3], size=base).astype(np.uint8)
+    fields = [0.05 * rng.standard_normal(base) for _ in range(3)] + [rng.random(base), 1e-3 * rng.standard_normal(base)]
+    tiled = [np.tile(a, (2, 2, 2)) for a in [kind] + fields]
+    out = []
+    for kd, (u, v, w, T0, Q) in ((kind, fields), (tiled[0], tiled[1:])):
+        shape = kd.shape
+        g = od.equilibrium(T0, u, v, w)
+        for _ in range(25):
+            g, T = od.step(g, kd, u, v, w, np.zeros(kd.size), shape, periodic=(1, 1, 1), omega=1.3, omega_s=0.7,
+                           Q=Q)
+        out.append(T.reshape(shape))
+    assert np.array_equal(np.tile(out[0], (2, 2, 2)), out[1])
+    assert Nb > 0
+
+
+def test_energy_with_carried_solid_and_sources():
+    rng = np.random.default_rng(9)
+    shape = (10, 9, 8)
+    N = int(np.prod(shape))
+    kind = rng.choice([0, 0, 2, 3], size=N).astype(np.uint8)
+    u, v, w = (0.04 * rng.standard_normal(N) for _ in range(3))
+    T0 = rng.random(N)
+    Q = 1e-3 * rng.standard_normal(N)
+    live = (kind == 0) | (kind == 3)
+    g = od.equilibrium(T0, u, v, w)
+    for s in range(1, 31):
+        g, T = od.step(g, kind, u, v, w, np.zeros(N), shape, periodic=(1, 1, 1), omega=1.2, omega_s=0.8, Q=Q)
+        # the source enters the populations after the temperature is read
+        assert abs(T[live].sum() - (T0[live].sum() + (s - 1) * Q[live].sum())) < 1e-11 * N
+
+
+def test_interface_heat_of_the_periodic_slab_equals_the_source():
+    """Periodic slab (carried solid a nodes, fluid b nodes), source q in the solid and
+    the balancing sink in the fluid: at steady state the energy crossing the two
+    interfaces per step, read from the populations, is q a."""
+    a, b = 7, 11
+    n = a + b
+    fl = np.zeros((n, 1, 1), np.uint8)
+    fl[:a] = od.CONJ
+    kf, ks, q = 0.05, 0.3, 1e-3
+    src = np.where(fl.reshape(-1) == od.CONJ, q, -q * a / b)
+    z = np.zeros(n)
+    g = od.equilibrium(np.zeros(n), z, z, z)
+    for _ in range(40000):
+        g, T = od.step(g, fl, z, z, z, z, (n, 1, 1), periodic=(1, 1, 1), omega=1 / (0.5 + 4 * kf),
+                       omega_s=1 / (0.5 + 4 * ks), Q=src)
+    assert abs(od.interface_heat(g, fl, (n, 1, 1)) - q * a) < 1e-12
+    assert od.interface_links(fl, (n, 1, 1)) == 2
+
+
+def test_nusselt_of_a_three_layer_slab_matches_finite_volumes():
+    """Solid held at 1 | carried solid (a nodes) | fluid (b nodes) | solid held at 0,
+    no sources: the steady state is harmonic-mean finite volumes, so the heat
+    through the carried-solid/fluid face, both layer means and the Nusselt number
+    J / (A (T_s - T_f)) d_h / k_f follow from the finite-volume solution."""
+    a, b = 9, 13
+    n = a + b
+    fl = np.zeros((n + 2, 1, 1), np.uint8)
+    fl[0] = fl[-1] = od.SOLID
+    fl[1:1 + a] = od.CONJ
+    Ts = np.zeros(n + 2)
+    Ts[0] = 1.0
+    kf, ks = 0.05, 0.35
+    N = n + 2
+    z = np.zeros(N)
+    g = od.equilibrium(np.zeros(N), z, z, z)
+    for _ in range(60000):
+        g, T = od.step(g, fl, z, z, z, Ts, (N, 1, 1), periodic=(0, 1, 1), omega=1 / (0.5 + 4 * kf),
+                       omega_s=1 / (0.5 + 4 * ks))
+    res = od.nusselt(g, T, fl, (N, 1, 1), kf)
+    kap = np.where(fl[1:-1, 0, 0] == od.CONJ, ks, kf)
+    Tfv = _fv_periodic_or_walled(kap, np.zeros(n), 1.0, 0.0)
+    J = (1.0 - Tfv[0]) * 2 * ks  # wall face of the first node
+    assert abs(res["Q"] - J) < 1e-11 and res["A"] == 1 and res["d_h"] == 4.0 * b
+    assert abs(res["T_solid_mean"] - Tfv[:a].mean()) < 1e-11 and abs(res["T_fluid_mean"] - Tfv[a:].mean()) < 1e-11
+    Nu = J / (Tfv[:a].mean() - Tfv[a:].mean()) * 4.0 * b / kf
+    assert abs(res["Nu"] - Nu) < 1e-9 * Nu
+
+
+def test_gyroid_flow_and_heat_on_96_cube_are_the_tiled_period():
+    """The 96^3 crop of config 4 is an exact sub-problem: the oracle's MRT flow step
+    (Bouzidi links, body force) and the conjugate D3Q7 step on the 96^3 periodic
+    gyroid equal the 48^3 period tiled, bit for bit."""
+    from tests import cases
+    p = cases.gyroid_heat_params()
+    out = {}
+    for n in (48, 96):
+        c = cases.gyroid_box(n=n, period=48.0)
+        N = n ** 3
+        solid = c["solid"].reshape(-1)
+        fl = np.where(solid, od.CONJ, od.FLUID).astype(np.uint8)
+        n_s = int(solid.sum())
+        Q = np.where(solid, p["q_solid"], -p["q_solid"] * n_s / (N - n_s))
+        f = np.broadcast_to(olbm.W32[:, None].astype(np.float64), (19, N)).copy()
+        z = np.zeros(N)
+        g = od.equilibrium(z, z, z, z)
+        kf = p["kappa_f"]
+        for _ in range(3):
+            f, rho, u, v, w = olbm.step(f, c["solid"], n, n, n, (1, 1, 1), "mrt", p["omega"], 1.0,
+                                        p["S"].astype(np.float64), 0.0, c["links"], force=p["force"])
+            g, T = od.step(g, fl, u, v, w, z, (n, n, n), periodic=(1, 1, 1), omega=1 / (0.5 + 4 * kf),
+                           omega_s=1 / (0.5 + 4 * p["k_ratio"] * kf), Q=Q)
+        out[n] = [a.reshape(n, n, n) for a in (u, v, w, T)] + [f.reshape(19, n, n, n)]
+    for a, b in zip(out[48][:4], out[96][:4]):
+        assert np.array_equal(np.tile(a, (2, 2, 2)), b)
+    assert np.array_equal(np.tile(out[48][4], (1, 2, 2, 2)), out[96][4])
