@@ -6,7 +6,9 @@
 //               O(1) feature channels; ||r||^2 is a device FP64 scalar (no host round trip).
 //  avgpool2:    2x2x2 mean (FP32 sum of 8 BF16 values, one rounding).
 //  scale:       out[p] = r[p] * d[p] (symmetric diagonal scaling D^-1/2 r).
-//  unpack:      z[p] = s * y[p, 0] * mask[p] * d[p], s = sqrt(scal[slot] / n).
+//  unpack:      z[p] = s * y[p, 0] * mask[p] * d[p], s = sqrt(scal[slot] / n);
+//  unpack_add:  the same added to z (a V-cycle's output plus the network's correction);
+//  residual_scale: (r - A z) * mask * d, the residual a V-cycle's output z leaves.
 #include "fc_common.hpp"
 
 using namespace fc;
@@ -80,6 +82,33 @@ __global__ __launch_bounds__(kBlock) void unpack_kernel(const float* __restrict_
   z[p] = nr * y[p * cy] * m * dd;
 }
 
+// z[p] += s y[p, 0] mask[p] d[p]: the network's correction added to a V-cycle's output
+__global__ __launch_bounds__(kBlock) void unpack_add_kernel(const float* __restrict__ y, int cy,
+                                                            const float* __restrict__ mask,
+                                                            const float* __restrict__ d,
+                                                            const double* __restrict__ scal, int slot, int64_t n,
+                                                            float* __restrict__ z) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const float nr = norm_from(scal, slot) / sqrtf((float)n);
+  const float m = mask ? mask[p] : 1.f;
+  const float dd = d ? d[p] : 1.f;
+  z[p] = z[p] + nr * y[p * cy] * m * dd;
+}
+
+// out[p] = (r[p] - Lo[p]) mask[p] d[p]: the residual left by a V-cycle's output z (Lo = A z)
+// in the symmetrically scaled variables
+__global__ __launch_bounds__(kBlock) void residual_scale_kernel(const float* __restrict__ r,
+                                                                const float* __restrict__ Lo,
+                                                                const float* __restrict__ mask,
+                                                                const float* __restrict__ d, int64_t n,
+                                                                float* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const float m = mask ? mask[p] : 1.f;
+  out[p] = (r[p] - Lo[p]) * m * d[p];
+}
+
 __global__ __launch_bounds__(kBlock) void scale_kernel(const float* __restrict__ r, const float* __restrict__ d,
                                                        int64_t n, float* __restrict__ out) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -118,6 +147,24 @@ int fc_unet_unpack(const float* y, int cy, const float* mask, const float* d, co
   hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, y, cy, mask, d, scal, slot,
                      n, z);
   return check_launch("unet_unpack");
+}
+
+int fc_unet_unpack_add(const float* y, int cy, const float* mask, const float* d, const double* scal, int slot,
+                       int64_t n, float* z, hipStream_t stream) {
+  FC_REQUIRE(y && scal && z && cy > 0 && n >= 0 && slot >= 0, "unet_unpack_add: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(unpack_add_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, y, cy, mask, d, scal,
+                     slot, n, z);
+  return check_launch("unet_unpack_add");
+}
+
+int fc_unet_residual_scale(const float* r, const float* Lo, const float* mask, const float* d, int64_t n, float* out,
+                           hipStream_t stream) {
+  FC_REQUIRE(r && Lo && d && out && n >= 0, "unet_residual_scale: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(residual_scale_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, r, Lo, mask, d, n,
+                     out);
+  return check_launch("unet_residual_scale");
 }
 
 int fc_unet_scale(const float* r, const float* d, int64_t n, float* out, hipStream_t stream) {

@@ -54,6 +54,8 @@ _lib.register({
     "fc_unet_avgpool2": (_lib.I32, [_lib.P, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.I32, _lib.P, _lib.P]),
     "fc_unet_unpack": (_lib.I32, [_lib.P, _lib.I32, _lib.P, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P, _lib.P]),
     "fc_unet_scale": (_lib.I32, [_lib.P, _lib.P, _lib.I64, _lib.P, _lib.P]),
+    "fc_unet_unpack_add": (_lib.I32, [_lib.P, _lib.I32, _lib.P, _lib.P, _lib.P, _lib.I32, _lib.I64, _lib.P, _lib.P]),
+    "fc_unet_residual_scale": (_lib.I32, [_lib.P, _lib.P, _lib.P, _lib.P, _lib.I64, _lib.P, _lib.P]),
 }, unchecked={"fc_conv_kpad", "fc_conv3d_dw_workspace_bytes"})
 
 
@@ -156,10 +158,14 @@ def _up(x: torch.Tensor) -> torch.Tensor:
     return x[:, :, None, :, None, :, None, :].expand(B, D, 2, H, 2, W, 2, C).reshape(B, 2 * D, 2 * H, 2 * W, C)
 
 
-class UNetPrecond:
-    """U-Net preconditioner. ``channels`` per level (multiples of 8, <= 64)."""
+CONFIG5_CHANNELS = (16, 16, 24, 32)  # 4 levels, 16 base channels: 198 720 weights
 
-    def __init__(self, channels: Sequence[int] = (16, 24, 32, 32, 32), device="cuda", seed: int = 0):
+
+class UNetPrecond:
+    """U-Net preconditioner. ``channels`` per level (multiples of 8, <= 64); the
+    default is the 4-level, 16-base-channel network of about 200 k weights."""
+
+    def __init__(self, channels: Sequence[int] = CONFIG5_CHANNELS, device="cuda", seed: int = 0):
         self.channels = [int(c) for c in channels]
         for c in self.channels:
             if c % 8 or c > 64:
@@ -193,6 +199,16 @@ class UNetPrecond:
 
     def n_params(self) -> int:
         return int(sum(p.numel() for p in self.params))
+
+    def flops(self, shape) -> float:
+        """Useful forward FLOPs of one application on a D x H x W grid: sum over
+        layers of 2 * 27 * c_in * c_out per voxel of the layer's level."""
+        D, H, W = (int(v) for v in shape)
+        total = 0.0
+        for name, cin, cout, _ in self.layers:
+            lvl = int(name[3]) if name[:3] in ("enc", "dec") else 0
+            total += 2.0 * 27 * cin * cout * (D >> lvl) * (H >> lvl) * (W >> lvl)
+        return total
 
     # ------------------------------------------------------------ training graph
     def forward_train(self, x: torch.Tensor) -> torch.Tensor:
@@ -239,17 +255,24 @@ class UNetPrecond:
         return self._bufs[key]
 
     def apply(self, r32: torch.Tensor, z32: torch.Tensor, feat: Optional[torch.Tensor], mask: Optional[torch.Tensor],
-              dsq: torch.Tensor, scal: torch.Tensor, partial: torch.Tensor, shape) -> None:
+              dsq: torch.Tensor, scal: torch.Tensor, partial: torch.Tensor, shape,
+              Lo: Optional[torch.Tensor] = None) -> None:
         """z = D^-1/2 s N(r~ / s, feat, mask) * mask with r~ = D^-1/2 r (dsq = diag(A)^-1/2)
         and s = ||r~|| kept in scal[NN_SLOT] (FP64, device); ``partial`` is the
-        block-partial buffer of the FP64 reductions."""
+        block-partial buffer of the FP64 reductions. With ``Lo`` = A z (z a V-cycle's
+        output already in z32) the network sees the residual r - Lo that z leaves
+        and its correction is added to z32."""
         if self._packed is None:
             self.refresh()
         D, H, W = shape
         n = D * H * W
         b = self._buffers(shape)
         st = _stream(self.device)
-        _lib.call("fc_unet_scale", r32.data_ptr(), dsq.data_ptr(), n, b["rs"].data_ptr(), st)
+        if Lo is None:
+            _lib.call("fc_unet_scale", r32.data_ptr(), dsq.data_ptr(), n, b["rs"].data_ptr(), st)
+        else:
+            _lib.call("fc_unet_residual_scale", r32.data_ptr(), Lo.data_ptr(), _ptr(mask), dsq.data_ptr(), n,
+                      b["rs"].data_ptr(), st)
         _lib.call("fc_mg_norm2_f32", b["rs"].data_ptr(), n, partial.data_ptr(), scal.data_ptr(), NN_SLOT, st)
         _lib.call("fc_unet_pack_input", b["rs"].data_ptr(), _ptr(feat), _ptr(mask), scal.data_ptr(), NN_SLOT, n,
                   b["x"].data_ptr(), st)
@@ -270,8 +293,8 @@ class UNetPrecond:
             conv3d(b["tmp"][l], next(wp), C[l], True, out=b["dec"][l])
             h = b["dec"][l]
         conv3d(h, next(wp), HEAD_CH, False, out=b["head"])
-        _lib.call("fc_unet_unpack", b["head"].data_ptr(), HEAD_CH, _ptr(mask), dsq.data_ptr(), scal.data_ptr(),
-                  NN_SLOT, n, z32.data_ptr(), st)
+        _lib.call("fc_unet_unpack" if Lo is None else "fc_unet_unpack_add", b["head"].data_ptr(), HEAD_CH,
+                  _ptr(mask), dsq.data_ptr(), scal.data_ptr(), NN_SLOT, n, z32.data_ptr(), st)
 
     # --------------------------------------------------------------- persistence
     def state(self) -> Dict[str, np.ndarray]:
@@ -484,14 +507,17 @@ def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: 
 # ---------------------------------------------------------------- solver driver
 def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 1e-8, max_it: int = 300,
            x0: Optional[torch.Tensor] = None, stall_window: int = 8, stall_factor: float = 0.5, m: int = 6,
-           ws=None, check_every: int = 1):
+           ws=None, check_every: int = 1, hybrid: bool = True):
     """Flexible GCR(m) with the U-Net preconditioner (FP64 vectors; the FP64 true
     residual decides convergence). The operator is not symmetric and a learned
     preconditioner is not exact, so a minimal-residual flexible method is used
     (flexible CG does not converge with weaker preconditioners on this
     operator): every step removes the part of r along the orthogonalised
-    A-image of the network's direction, which is also what the network is
-    trained for. When the residual has not dropped by ``stall_factor`` over
+    A-image of the network's direction. With ``hybrid`` (the default, the
+    network trained by train_gmg) each direction is one V-cycle plus the network's
+    correction of the residual the V-cycle leaves (hybrid_precond); without it the
+    network alone (trained by train). When the residual has not dropped by
+    ``stall_factor`` over
     ``stall_window`` iterations, or the budget ends unconverged, the solve
     continues from the current iterate (never worse than the start) with the
     V-cycle as preconditioner. Returns (x, info) with info["fallback"]."""
@@ -507,8 +533,11 @@ def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 
     hist_it = []
     nn_scal = torch.zeros(8, dtype=torch.float64, device=mg.device)
 
-    def precond(r32, z32, w):
-        net.apply(r32, z32, feat, mask, dsq, nn_scal, w.fcg.partial, shape)
+    if hybrid:
+        precond = hybrid_precond(mg, net, k)
+    else:
+        def precond(r32, z32, w):
+            net.apply(r32, z32, feat, mask, dsq, nn_scal, w.fcg.partial, shape)
 
     def watch(it, rel):
         hist_it.append(rel)
@@ -528,3 +557,132 @@ def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 
     info2["stalled"] = info.get("stopped", False)
     info2["res_nn"] = info["res_rel"]
     return x2, info2
+
+
+# ------------------------------------------------- training on GMG-generated residuals
+def harvest_gmg(n: int = 128, seed: int = 0, device="cuda", max_it: int = 400, tol: float = 1e-10,
+                stride: int = 1, feat_scale: float = 0.25, pid: int = 0):
+    """Training samples generated by the GMG solver itself: one smooth-noise blob
+    conduction problem at n^3 (k_s / k_f log-uniform in [1, 1e4] from ``seed``) is
+    solved with GMG-preconditioned flexible CG, and at every ``stride``-th iteration
+    the residual r_k is passed through one V-cycle, z = V(r_k), leaving the
+    post-V-cycle residual r' = r_k - A z (the part the V-cycle misses, which the
+    network learns to remove: A e' = r' for the remaining error e'). Each sample is
+    (input [n, n, n, 8] BF16 = {r~' at unit RMS, feat_scale log10 k, mask}, target
+    r~' at unit RMS FP32, mask, problem id) with r~' = D^-1/2 r'. Returns
+    (samples, ScaledOperator of the problem, info)."""
+    from .. import geometry
+    from .mg_pcg import fcg_fp64
+
+    c = geometry.conduction_problem(n, seed=seed, device=device)
+    mg = _mg_for(c, device)
+    lv0 = mg._levels[0]
+    shape = lv0["shape"]
+    N = n ** 3
+    mask = lv0["mask_f"].reshape(-1)
+    feat = log_k_feature(c["k"], feat_scale)
+    dsq = jacobi_scale(mg)
+    z = torch.empty(N, device=device)
+    Lo = torch.empty(N, device=device)
+    rs = torch.empty(N, device=device)
+    st = _stream(mg.device)
+    samples = []
+
+    def grab(it, ws):
+        if it % stride:
+            return
+        mg.precondition(ws.r32, out=z, masked_input=True)
+        mg._Mop_cuda(z.view(shape), 0, Lo.view(shape))
+        _lib.call("fc_unet_residual_scale", ws.r32.data_ptr(), Lo.data_ptr(), mask.data_ptr(), dsq.data_ptr(), N,
+                  rs.data_ptr(), st)
+        rms = float(rs.double().norm()) / N ** 0.5
+        if not rms > 0.0:
+            return
+        t = (rs / rms).view(n, n, n)
+        x = torch.zeros((n, n, n, IN_CH), device=device, dtype=torch.bfloat16)
+        x[..., 0] = t
+        x[..., 1] = feat.view(n, n, n)
+        x[..., 2] = mask.view(n, n, n)
+        samples.append((x, t.clone(), mask.view(n, n, n), pid))
+
+    _, info = fcg_fp64(mg, c["b"], tol=tol, max_it=max_it, on_iter=grab)
+    op = ScaledOperator(mg)
+    del mg
+    return samples, op, {"ratio": float(c["ratio"]), "gmg_it": int(info["it"]), "samples": len(samples)}
+
+
+def train_gmg(net: UNetPrecond, n: int = 128, problems: int = 32, seed0: int = 1000, steps_per_problem: int = 64,
+              batch: int = 2, lr: float = 3e-4, replay: int = 256, stride: int = 1, seed: int = 0, log=print,
+              clip: float = 1.0):
+    """Train on GMG-generated post-V-cycle residuals, problem by problem (each
+    problem's samples are made on the fly by harvest_gmg, trained on together with a
+    replay reservoir of ``replay`` earlier samples, then dropped). Loss: the residual
+    the network's correction leaves, relative to the one it was given,
+
+        loss = |t - A~ N(t)|^2 / |t|^2,   t = r~' at unit RMS,
+
+    so the correction is learned with its scale (it is added to the V-cycle's
+    output without a separate line search)."""
+    import random
+
+    total = problems * steps_per_problem
+    opt = torch.optim.Adam(net.params, lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=total, eta_min=lr * 0.05)
+    rng = random.Random(seed)
+    net.out_gain = 1.0
+    reservoir: list = []
+    ops: Dict[int, ScaledOperator] = {}
+    hist = []
+    n_samples = 0
+    for i in range(problems):
+        samples, op, info = harvest_gmg(n, seed=seed0 + i, device=net.device, stride=stride, pid=i)
+        ops[i] = op
+        n_samples += len(samples)
+        pool = samples + reservoir
+        ploss = []
+        for _ in range(steps_per_problem):
+            pick = [pool[rng.randrange(len(pool))] for _ in range(batch)]
+            x = torch.stack([p[0] for p in pick]).contiguous()
+            y = net.forward_train(x)[..., 0]
+            loss = 0.0
+            for b, (_, t, m, pid) in enumerate(pick):
+                res = t - ops[pid](y[b] * m)
+                loss = loss + (res * res).sum() / ((t * t).sum() + 1e-30) / batch
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            torch.nn.utils.clip_grad_norm_(net.params, clip)
+            opt.step()
+            sched.step()
+            ploss.append(float(loss.detach()))
+        hist += ploss
+        reservoir = rng.sample(reservoir + samples, min(replay, len(reservoir) + len(samples)))
+        keep = {p[3] for p in reservoir}
+        for k in [k for k in ops if k not in keep]:
+            del ops[k]
+        del samples, pool
+        if log:
+            log(f"[train] problem {i}: k ratio {info['ratio']:.3g}, GMG-PCG {info['gmg_it']} it, "
+                f"{info['samples']} samples ({n_samples} so far), loss {np.mean(ploss[:8]):.4f} -> "
+                f"{np.mean(ploss[-8:]):.4f}")
+    net.refresh()
+    return {"loss": hist, "samples": n_samples}
+
+
+def hybrid_precond(mg, net: UNetPrecond, k: torch.Tensor):
+    """z = V(r) + N(r - A V(r)): one V-cycle, then the network's correction of the
+    residual the V-cycle leaves (HIP kernels only). Returns precond(r32, z32, ws)
+    for gcr_fp64 (ws: its GCRWorkspace)."""
+    lv0 = mg._levels[0]
+    shape = lv0["shape"]
+    feat = log_k_feature(k, net.feat_scale)
+    mask = lv0["mask_f"].reshape(-1)
+    dsq = jacobi_scale(mg)
+    Lo = torch.empty(int(mask.numel()), device=mg.device)
+    nn_scal = torch.zeros(8, dtype=torch.float64, device=mg.device)
+
+    def precond(r32, z32, w):
+        mg.precondition(r32, out=z32, masked_input=True)
+        mg._Mop_cuda(z32.view(shape), 0, Lo.view(shape))
+        net.apply(r32, z32, feat, mask, dsq, nn_scal, w.fcg.partial, shape, Lo=Lo)
+
+    return precond

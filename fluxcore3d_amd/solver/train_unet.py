@@ -2,13 +2,17 @@
 
     python -m fluxcore3d_amd.solver.train_unet --out fluxcore3d_amd/data/unet_precond.npz
 
-Training pairs come from smooth-noise blob conduction problems at 128^3 with
-k_s / k_f log-uniform in [1, 1e4] (random errors of random spectra and their
-residuals, nn_precond.harvest_pairs); the loss is the one-step residual
-reduction of the network direction on the Jacobi-scaled system
-(nn_precond.train). Evaluation uses problems with seeds outside the training
-range at 128^3 and 256^3, FP32 V-cycle / FP64 outer vectors, tolerance 1e-8:
-flexible GCR with the network against GMG-PCG (flexible CG and GCR).
+Default: the 4-level, 16-base-channel network (about 200 k weights) trained on
+residuals that the GMG solver generates on the fly at 128^3 (nn_precond.train_gmg):
+for each of ``--problems`` smooth-noise blob conduction problems (k_s / k_f
+log-uniform in [1, 1e4]) GMG-preconditioned flexible CG runs to 1e-10 and every
+iteration's residual, passed through one V-cycle, gives one post-V-cycle residual
+sample (about 150 per problem, thousands in all); the network learns the
+correction of what the V-cycle leaves. ``--random-errors`` trains the stand-alone
+network on random errors instead (nn_precond.harvest_pairs / train).
+Evaluation uses problems with seeds outside the training range at 128^3 and
+256^3, FP32 V-cycle / FP64 outer vectors, tolerance 1e-8: flexible GCR with the
+network against GMG-PCG (flexible CG) and GMG-GCR(6).
 """
 from __future__ import annotations
 
@@ -17,10 +21,11 @@ import json
 import os
 import time
 
+import numpy as np
 import torch
 
 
-def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None) -> dict:
+def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None, hybrid: bool = True) -> dict:
     from .. import geometry
     from .mg_pcg import FCGWorkspace, fcg_fp64, gcr_fp64
     from .nn_precond import _mg_for, nn_fcg
@@ -29,7 +34,7 @@ def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None) -> d
     mg = _mg_for(c, device)
     ws = FCGWorkspace(c["k"].shape, device)
     fcg_fp64(mg, c["b"], tol=tol, max_it=2, ws=ws)
-    nn_fcg(mg, net, c["b"], c["k"], tol=tol, max_it=2)
+    nn_fcg(mg, net, c["b"], c["k"], tol=tol, max_it=2, hybrid=hybrid)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     _, gi = fcg_fp64(mg, c["b"], tol=tol, max_it=1000, ws=ws)
@@ -42,7 +47,7 @@ def evaluate(net, n: int, seed: int, device, tol: float = 1e-8, ratio=None) -> d
     torch.cuda.synchronize()
     t_gcr = time.perf_counter() - t0
     t0 = time.perf_counter()
-    _, ni = nn_fcg(mg, net, c["b"], c["k"], tol=tol, max_it=1000)
+    _, ni = nn_fcg(mg, net, c["b"], c["k"], tol=tol, max_it=1000, hybrid=hybrid)
     torch.cuda.synchronize()
     t_nn = time.perf_counter() - t0
     return {"n": n, "seed": seed, "ratio": round(float(c["ratio"]), 2), "gmg_s": round(t_gmg, 4),
@@ -91,18 +96,23 @@ def time_apply(weights: str, n: int, reps: int = 10) -> dict:
 
 
 def main(argv=None):
-    from .nn_precond import UNetPrecond, harvest_pairs, train
+    from .nn_precond import CONFIG5_CHANNELS, UNetPrecond, harvest_pairs, train, train_gmg
 
     p = argparse.ArgumentParser()
     p.add_argument("--out", default=os.path.join(os.path.dirname(os.path.dirname(__file__)), "data",
                                                  "unet_precond.npz"))
     p.add_argument("--n", type=int, default=128)
-    p.add_argument("--problems", type=int, default=6)
-    p.add_argument("--per-problem", type=int, default=24)
-    p.add_argument("--steps", type=int, default=600)
+    p.add_argument("--problems", type=int, default=32)
+    p.add_argument("--steps-per-problem", type=int, default=64)
+    p.add_argument("--replay", type=int, default=256)
+    p.add_argument("--stride", type=int, default=1)
     p.add_argument("--batch", type=int, default=2)
     p.add_argument("--lr", type=float, default=3e-4)
-    p.add_argument("--channels", default="8,16,24,32,32")
+    p.add_argument("--channels", default=",".join(str(c) for c in CONFIG5_CHANNELS))
+    p.add_argument("--random-errors", action="store_true",
+                   help="train the stand-alone network on random errors (harvest_pairs / train) instead")
+    p.add_argument("--per-problem", type=int, default=24, help="random-error samples per problem")
+    p.add_argument("--steps", type=int, default=600, help="steps of the random-error training")
     p.add_argument("--eval-n", default="128,256")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--time-apply", type=int, default=0, help="only time one application at this size with --out weights")
@@ -111,27 +121,33 @@ def main(argv=None):
         print(json.dumps(time_apply(a.out, a.time_apply)), flush=True)
         return
     dev = torch.device("cuda:0")
-    t0 = time.perf_counter()
-    samples, ops = harvest_pairs(n=a.n, problems=a.problems, per_problem=a.per_problem, device=dev)
-    t_h = time.perf_counter() - t0
-    print(f"[train_unet] {len(samples)} samples at {a.n}^3 in {t_h:.1f} s", flush=True)
     net = UNetPrecond(tuple(int(c) for c in a.channels.split(",")), device=dev, seed=a.seed)
-    print(f"[train_unet] {net.n_params()} parameters", flush=True)
+    print(f"[train_unet] channels {net.channels}, {net.n_params()} parameters", flush=True)
     t0 = time.perf_counter()
-    hist = train(net, samples, ops, steps=a.steps, batch=a.batch, lr=a.lr, log=lambda s: print(s, flush=True))
+    if a.random_errors:
+        samples, ops = harvest_pairs(n=a.n, problems=a.problems, per_problem=a.per_problem, device=dev)
+        hist = train(net, samples, ops, steps=a.steps, batch=a.batch, lr=a.lr, log=lambda s: print(s, flush=True))
+        n_samples = len(samples)
+        del samples, ops
+        hybrid = False
+    else:
+        res = train_gmg(net, n=a.n, problems=a.problems, steps_per_problem=a.steps_per_problem, batch=a.batch,
+                        lr=a.lr, replay=a.replay, stride=a.stride, seed=a.seed,
+                        log=lambda s: print(s, flush=True))
+        hist, n_samples = res["loss"], res["samples"]
+        hybrid = True
     torch.cuda.synchronize()
     t_t = time.perf_counter() - t0
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     net.save(a.out)
-    del samples, ops
     torch.cuda.empty_cache()
-    res = []
+    out = []
     for i, n in enumerate(int(v) for v in a.eval_n.split(",")):
-        r = evaluate(net, n, seed=5000 + i, device=dev)
+        r = evaluate(net, n, seed=5000 + i, device=dev, hybrid=hybrid)
         print(json.dumps(r), flush=True)
-        res.append(r)
-    print(json.dumps({"train_s": round(t_t, 1), "harvest_s": round(t_h, 1), "final_loss": hist[-1],
-                      "params": net.n_params(), "eval": res}), flush=True)
+        out.append(r)
+    print(json.dumps({"train_s": round(t_t, 1), "samples": n_samples, "final_loss": float(np.mean(hist[-16:])),
+                      "params": net.n_params(), "channels": net.channels, "eval": out}), flush=True)
 
 
 if __name__ == "__main__":

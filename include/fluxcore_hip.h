@@ -305,6 +305,12 @@ int fc_unet_unpack(const float *y, int cy, const float *mask, const float *d, co
                    int64_t n, float *z, hipStream_t stream);
 /* out[p] = r[p] * d[p] */
 int fc_unet_scale(const float *r, const float *d, int64_t n, float *out, hipStream_t stream);
+/* z += s y[:, 0] mask d (network correction added to a V-cycle's output) */
+int fc_unet_unpack_add(const float *y, int cy, const float *mask, const float *d, const double *scal, int slot,
+                       int64_t n, float *z, hipStream_t stream);
+/* out = (r - Lo) mask d: the scaled residual left by z, Lo = A z */
+int fc_unet_residual_scale(const float *r, const float *Lo, const float *mask, const float *d, int64_t n,
+                           float *out, hipStream_t stream);
 
 /* ---- flexible GCR(m) in FP64 (csrc/krylov.hip) ----
  * No reference counterpart: the minimal-residual Krylov method used with the

@@ -35,43 +35,109 @@ def test_training_operator_matches_hip_operator():
     assert abs(float(op(e0)[5, 6, 7]) - 1.0) < 1e-5
 
 
-def test_nn_fcg_reaches_tolerance_with_untrained_network():
-    """An untrained network is a poor preconditioner; the driver must still
-    reach the FP64 true-residual tolerance (through the GMG-PCG fallback)."""
+@pytest.mark.parametrize("hybrid", [True, False])
+def test_nn_fcg_reaches_tolerance_with_untrained_network(hybrid):
+    """An untrained network is a poor preconditioner (alone, or as the correction
+    after a V-cycle); the driver must still reach the FP64 true-residual tolerance
+    (through the GMG fallback if the network stalls)."""
     from fluxcore3d_amd.solver.mg_pcg import fcg_fp64
     from fluxcore3d_amd.solver.nn_precond import UNetPrecond, nn_fcg
     c, mg = _problem()
     net = UNetPrecond(channels=(8, 16, 16), device=DEV, seed=3)
-    x, info = nn_fcg(mg, net, c["b"], c["k"], tol=1e-8, max_it=300)
+    x, info = nn_fcg(mg, net, c["b"], c["k"], tol=1e-8, max_it=300, hybrid=hybrid)
     assert info["converged"] and info["res_rel"] < 1e-8
     xg, gi = fcg_fp64(mg, c["b"], tol=1e-10, max_it=300)
     assert (x - xg).norm() <= 1e-4 * xg.norm()
 
 
-def test_trained_network_one_step_reduction_on_held_out_problem():
-    """What training buys, on a problem outside the training seeds: for errors of
-    random spectra (the training distribution) the trained network's direction
-    removes most of the scaled residual in one minimal-residual step
-    (sin^2 between r~ and A~ z well below 1), the untrained network's almost none.
-    (On the solver's own residuals the trained network does not beat the V-cycle;
-    that is measured by bench.py's thermal leg.)"""
+def _weights():
     import os
-    from fluxcore3d_amd.solver import nn_precond as nn
-    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fluxcore3d_amd", "data",
+    return os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fluxcore3d_amd", "data",
                         "unet_precond.npz")
-    trained = nn.UNetPrecond.load(path, device=DEV)
-    untrained = nn.UNetPrecond(trained.channels, device=DEV, seed=11)
-    samples, ops = nn.harvest_pairs(n=64, problems=1, seed0=7777, per_problem=6, device=DEV, log=None)
 
-    def sin2(net):
+
+def test_shipped_network_is_the_config5_architecture():
+    from fluxcore3d_amd.solver import nn_precond as nn
+    net = nn.UNetPrecond.load(_weights(), device=DEV)
+    assert net.levels == 4 and net.channels[0] == 16
+    assert 150_000 <= net.n_params() <= 250_000, net.n_params()
+
+
+def test_trained_correction_on_gmg_residuals_of_a_held_out_problem():
+    """What training buys, on a problem outside the training seeds: on the
+    post-V-cycle residuals that GMG-PCG produces (the training distribution), the
+    trained network's correction removes most of the residual it is given (|t -
+    A~ N(t)|^2 / |t|^2 well below 1; 0.27 at the end of training), an untrained
+    network's correction does not (about 1 or more)."""
+    from fluxcore3d_amd.solver import nn_precond as nn
+    trained = nn.UNetPrecond.load(_weights(), device=DEV)
+    untrained = nn.UNetPrecond(trained.channels, device=DEV, seed=11)
+    samples, op, info = nn.harvest_gmg(n=64, seed=7777, device=DEV, stride=4)
+    assert len(samples) >= 8
+
+    def loss(net):
         out = []
         with torch.no_grad():
-            for x, e, m, pid, _ in samples:
-                Az = ops[pid](net.forward_train(x[None].contiguous())[0, ..., 0] * m)
-                r = ops[pid](e * m)
-                c = float((Az * r).sum())
-                out.append(1.0 - c * c / (float((Az * Az).sum()) * float((r * r).sum())))
+            for x, t, m, _ in samples:
+                res = t - op(net.forward_train(x[None].contiguous())[0, ..., 0] * m)
+                out.append(float((res * res).sum() / (t * t).sum()))
         return float(np.mean(out))
 
-    s_tr, s_un = sin2(trained), sin2(untrained)
-    assert s_tr < 0.5 and s_un > 0.9, (s_tr, s_un)
+    l_tr, l_un = loss(trained), loss(untrained)
+    print(f"[held-out 64^3, k ratio {info['ratio']:.3g}] trained {l_tr:.3f}, untrained {l_un:.3f}")
+    assert l_tr < 0.6 and l_un > 0.9, (l_tr, l_un)
+
+
+def test_nn_hybrid_256_converges_without_fallback_in_fewer_iterations_than_gmg_pcg():
+    """Config 5's evaluation size: the 256^3 blob problem (seed 0, k ratio 353) to an
+    FP64 relative residual of 1e-8. Flexible GCR(6) with the hybrid preconditioner
+    (V-cycle + the network's correction) converges with the network in every
+    iteration (no fallback) and in fewer iterations than GMG-PCG. GMG-GCR(6) with the
+    V-cycle alone is printed beside it: on this problem the network does not take the
+    iterations below that (measured 69 against 64; GMG-PCG 131)."""
+    from fluxcore3d_amd import geometry
+    from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, gcr_fp64
+    from fluxcore3d_amd.solver.nn_precond import UNetPrecond, _mg_for, nn_fcg
+    c = geometry.conduction_problem(256, seed=0, device=DEV)
+    mg = _mg_for(c, DEV)
+    net = UNetPrecond.load(_weights(), device=DEV)
+    xg, gi = fcg_fp64(mg, c["b"], tol=1e-8, max_it=500)
+    _, gg = gcr_fp64(mg, c["b"], tol=1e-8, max_it=500, m=6)
+    xn, ni = nn_fcg(mg, net, c["b"], c["k"], tol=1e-8, max_it=500)
+    print(f"[256^3 k ratio {c['ratio']:.3g}] GMG-PCG {gi['it']} it, GMG-GCR(6) {gg['it']} it, "
+          f"NN-GCR(6) {ni['it']} it (network in {ni['it_nn']}), fallback {ni['fallback']}")
+    assert gi["converged"] and ni["converged"] and ni["res_rel"] < 1e-8
+    assert not ni["fallback"] and ni["it_nn"] == ni["it"]
+    assert ni["it"] < gi["it"]
+    assert float((xn - xg).norm() / xg.norm()) < 1e-6
+
+
+def test_hybrid_apply_equals_vcycle_plus_network_on_the_left_residual():
+    """fc_unet_residual_scale and fc_unet_unpack_add: the hybrid application equals,
+    bit for bit, the V-cycle's output plus the plain network application on the
+    residual r - A z formed separately (same operations in the same order)."""
+    from fluxcore3d_amd.solver.mg_pcg import FCGWorkspace
+    from fluxcore3d_amd.solver.nn_precond import UNetPrecond, jacobi_scale, log_k_feature
+    c, mg = _problem(n=32)
+    lv0 = mg._levels[0]
+    shape = lv0["shape"]
+    N = int(np.prod(shape))
+    mask = lv0["mask_f"].reshape(-1)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    r = (torch.randn(N, generator=g).to(DEV) * mask).contiguous()
+    net = UNetPrecond.load(_weights(), device=DEV)
+    feat = log_k_feature(c["k"], net.feat_scale)
+    dsq = jacobi_scale(mg)
+    ws = FCGWorkspace(shape, DEV)
+    scal = torch.zeros(8, dtype=torch.float64, device=DEV)
+    z = torch.empty(N, device=DEV)
+    mg.precondition(r, out=z, masked_input=True)
+    zv = z.clone()
+    Lo = torch.empty(N, device=DEV)
+    mg._Mop_cuda(z.view(shape), 0, Lo.view(shape))
+    net.apply(r, z, feat, mask, dsq, scal, ws.partial, shape, Lo=Lo)
+    left = ((r - Lo) * mask).contiguous()
+    zn = torch.empty(N, device=DEV)
+    net.apply(left, zn, feat, mask, dsq, scal, ws.partial, shape)
+    assert torch.equal(z, zv + zn)
+    assert float(zn.abs().max()) > 0
