@@ -109,22 +109,26 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
     T_out[x] = T_in;
     return;
   }
+  // Every input is first read as a plain pull from x - c_q (or, outside a non-periodic
+  // face, from x itself): those addresses are valid whatever the neighbour is, so the
+  // loads issue together with the neighbours' flag bytes instead of after them. Inputs
+  // from walls and solids are then replaced (few nodes).
+  int64_t src_node[7];
+  uint8_t nk[7];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     int a = i, b = j, c = k;
     const bool in = nb7(i, -cx7(q), nx, px, a) && nb7(j, -cy7(q), ny, py, b) && nb7(k, -cz7(q), nz, pz, c);
-    if (!in) {
-      g[q] = gin[q * N + x];
-      continue;
-    }
-    const int64_t s = ((int64_t)a * ny + b) * nz + c;
-    const uint8_t sk = flag[s] & G_KIND;
-    if (sk == G_FLUID || sk == G_CONJ) {
-      g[q] = gin[q * N + s];
-    } else if (sk == G_WALL) {
+    src_node[q] = in ? ((int64_t)a * ny + b) * nz + c : x;
+    g[q] = __builtin_nontemporal_load(gin + q * N + src_node[q]);
+    nk[q] = in ? (uint8_t)(flag[src_node[q]] & G_KIND) : G_FLUID;
+  }
+#pragma unroll
+  for (int q = 1; q < 7; ++q) {
+    if (nk[q] == G_WALL) {
       g[q] = gin[opp7(q) * N + x];
-    } else {
-      const float two_wt = 2.0f * w7(q) * T_solid[s];
+    } else if (nk[q] == G_SOLID) {
+      const float two_wt = 2.0f * w7(q) * T_solid[src_node[q]];
       g[q] = two_wt - gin[opp7(q) * N + x];
     }
   }
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
     const float geq = w7(q) * T * (1.0f + 4.0f * cu);
     float o = g[q] + om * (geq - g[q]);
     if (Q) o = o + w7(q) * src;
-    gout[q * N + x] = o;
+    __builtin_nontemporal_store(o, gout + q * N + x);
   }
 }
 
@@ -173,9 +177,12 @@ __device__ __forceinline__ bool box_to_grid(const Box& b, int64_t t, int64_t& g)
   return true;
 }
 
-// b = T_s + dt src + acc / rho_cp and x0 = T_s on conducting cells: the right-hand side
-// of (I + dt L) T = b, L the solid conduction operator divided by rho_cp
+// b = T_s + dt src + acc / rho_cp on conducting cells: the right-hand side of
+// (I + dt L) T = b, L the solid conduction operator divided by rho_cp. Start x0 = T_s, or
+// with the previous interval's start temperature T_prev the linear extrapolation
+// 2 T_s - T_prev (the solid heats smoothly, so the solve starts closer to its answer)
 __global__ __launch_bounds__(kBlock) void solid_rhs_kernel(const float* __restrict__ T_solid,
+                                                           const float* __restrict__ T_prev,
                                                            const float* __restrict__ acc,
                                                            const float* __restrict__ src,
                                                            const uint8_t* __restrict__ flag, Box box, double inv_rcp,
@@ -185,9 +192,10 @@ __global__ __launch_bounds__(kBlock) void solid_rhs_kernel(const float* __restri
   int64_t g;
   double bv = 0.0, xv = 0.0;
   if (box_to_grid(box, t, g) && (flag[g] & G_KIND) == G_SOLID) {
-    xv = (double)T_solid[g];
+    const double ts = (double)T_solid[g];
     const double s = src ? dt * (double)src[g] : 0.0;
-    bv = (xv + s) + (double)acc[g] * inv_rcp;
+    bv = (ts + s) + (double)acc[g] * inv_rcp;
+    xv = T_prev ? 2.0 * ts - (double)T_prev[g] : ts;
   }
   b[t] = bv;
   x0[t] = xv;
@@ -240,16 +248,16 @@ int fc_d3q7_init(float* g, const float* T, const float* u, const float* v, const
   return check_launch("d3q7_init");
 }
 
-int fc_cht_solid_rhs(const float* T_solid, const float* acc, const float* src, const uint8_t* flag, int nx, int ny,
-                     int nz, int ox, int oy, int oz, int bx, int by, int bz, double rho_cp, double dt, double* b,
-                     double* x0, hipStream_t stream) {
+int fc_cht_solid_rhs(const float* T_solid, const float* T_prev, const float* acc, const float* src,
+                     const uint8_t* flag, int nx, int ny, int nz, int ox, int oy, int oz, int bx, int by, int bz,
+                     double rho_cp, double dt, double* b, double* x0, hipStream_t stream) {
   Box box;
   if (int rc = make_box(box, nx, ny, nz, ox, oy, oz, bx, by, bz)) return rc;
   FC_REQUIRE(T_solid && acc && flag && b && x0, "cht_solid_rhs: null pointer argument");
   FC_REQUIRE(rho_cp > 0.0 && dt > 0.0, "cht_solid_rhs: rho_cp and dt must be positive");
   const int64_t n = (int64_t)bx * by * bz;
-  hipLaunchKernelGGL(solid_rhs_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, T_solid, acc, src, flag,
-                     box, 1.0 / rho_cp, dt, b, x0);
+  hipLaunchKernelGGL(solid_rhs_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, T_solid, T_prev, acc,
+                     src, flag, box, 1.0 / rho_cp, dt, b, x0);
   return check_launch("cht_solid_rhs");
 }
 
