@@ -506,8 +506,8 @@ def train(net: UNetPrecond, samples, ops, steps: int = 600, batch: int = 2, lr: 
 
 # ---------------------------------------------------------------- solver driver
 def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 1e-8, max_it: int = 300,
-           x0: Optional[torch.Tensor] = None, stall_window: int = 8, stall_factor: float = 0.5, m: int = 6,
-           ws=None, check_every: int = 1, hybrid: bool = True):
+           x0: Optional[torch.Tensor] = None, stall_window: Optional[int] = None, stall_factor: float = 0.5,
+           m: int = 6, ws=None, check_every: int = 1, hybrid: bool = True):
     """Flexible GCR(m) with the U-Net preconditioner (FP64 vectors; the FP64 true
     residual decides convergence). The operator is not symmetric and a learned
     preconditioner is not exact, so a minimal-residual flexible method is used
@@ -516,8 +516,10 @@ def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 
     A-image of the network's direction. With ``hybrid`` (the default, the
     network trained by train_gmg) each direction is one V-cycle plus the network's
     correction of the residual the V-cycle leaves (hybrid_precond); without it the
-    network alone (trained by train). When the residual has not dropped by
-    ``stall_factor`` over
+    network alone (trained by train). ``stall_window`` defaults to 8 iterations for
+    the network alone and 24 for the hybrid, whose first iterations plateau like
+    GMG-GCR's own (at 512^3 the residual stays near 1 for about six). When the residual
+    has not dropped by ``stall_factor`` over
     ``stall_window`` iterations, or the budget ends unconverged, the solve
     continues from the current iterate (never worse than the start) with the
     V-cycle as preconditioner. Returns (x, info) with info["fallback"]."""
@@ -525,6 +527,8 @@ def nn_fcg(mg, net: UNetPrecond, b: torch.Tensor, k: torch.Tensor, tol: float = 
 
     lv0 = mg._levels[0]
     shape = lv0["shape"]
+    if stall_window is None:
+        stall_window = 24 if hybrid else 8
     if ws is None:
         ws = GCRWorkspace(shape, mg.device, m)
     feat = log_k_feature(k, net.feat_scale)

@@ -88,23 +88,25 @@ def test_trained_correction_on_gmg_residuals_of_a_held_out_problem():
     assert l_tr < 0.6 and l_un > 0.9, (l_tr, l_un)
 
 
-def test_nn_hybrid_256_converges_without_fallback_in_fewer_iterations_than_gmg_pcg():
-    """Config 5's evaluation size: the 256^3 blob problem (seed 0, k ratio 353) to an
-    FP64 relative residual of 1e-8. Flexible GCR(6) with the hybrid preconditioner
-    (V-cycle + the network's correction) converges with the network in every
-    iteration (no fallback) and in fewer iterations than GMG-PCG. GMG-GCR(6) with the
-    V-cycle alone is printed beside it: on this problem the network does not take the
-    iterations below that (measured 69 against 64; GMG-PCG 131)."""
+@pytest.mark.parametrize("n", [256, 512])
+def test_nn_hybrid_converges_without_fallback_in_fewer_iterations_than_gmg_pcg(n):
+    """Config 5's evaluation sizes: the 256^3 and 512^3 blob problems (seed 0, k ratio
+    353) to an FP64 relative residual of 1e-8. Flexible GCR(6) with the hybrid
+    preconditioner (V-cycle + the network's correction) converges with the network in
+    every iteration (no fallback) and in fewer iterations than GMG-PCG. GMG-GCR(6) with
+    the V-cycle alone is printed beside it: the network does not take the iterations
+    below that (measured at 256^3: 69 against 64, GMG-PCG 131; at 512^3: 85 against 86,
+    GMG-PCG 188)."""
     from fluxcore3d_amd import geometry
     from fluxcore3d_amd.solver.mg_pcg import fcg_fp64, gcr_fp64
     from fluxcore3d_amd.solver.nn_precond import UNetPrecond, _mg_for, nn_fcg
-    c = geometry.conduction_problem(256, seed=0, device=DEV)
+    c = geometry.conduction_problem(n, seed=0, device=DEV)
     mg = _mg_for(c, DEV)
     net = UNetPrecond.load(_weights(), device=DEV)
     xg, gi = fcg_fp64(mg, c["b"], tol=1e-8, max_it=500)
     _, gg = gcr_fp64(mg, c["b"], tol=1e-8, max_it=500, m=6)
     xn, ni = nn_fcg(mg, net, c["b"], c["k"], tol=1e-8, max_it=500)
-    print(f"[256^3 k ratio {c['ratio']:.3g}] GMG-PCG {gi['it']} it, GMG-GCR(6) {gg['it']} it, "
+    print(f"[{n}^3 k ratio {c['ratio']:.3g}] GMG-PCG {gi['it']} it, GMG-GCR(6) {gg['it']} it, "
           f"NN-GCR(6) {ni['it']} it (network in {ni['it_nn']}), fallback {ni['fallback']}")
     assert gi["converged"] and ni["converged"] and ni["res_rel"] < 1e-8
     assert not ni["fallback"] and ni["it_nn"] == ni["it"]
