@@ -91,7 +91,7 @@ def vcycle_bytes(levels) -> float:
 # one A z in FP32 (z read, face coefficients or the seven per-cell values, Lo written)
 # plus the scaled residual (r, Lo, mask, d read, one value written): the extra fine-level
 # work of a hybrid direction besides its V-cycle and the network
-_HYBRID_B = {"face": 4 + 12 + 4 + 20, "cell": 4 + 28 + 4 + 20}
+_HYBRID_B = (4.0 + 12 + 4 + 20, 4.0 + 28 + 4 + 20)  # face form, per-cell arrays
 
 
 def solve_roofline(levels, iterations, solver, measured_s, network_flops=0.0, network_iterations=0,
