@@ -37,8 +37,9 @@ def parse():
     p.add_argument("--collision", default="mrt", choices=["bgk", "trt", "mrt", "mrt_smag"])
     p.add_argument("--storage", default="fp32", choices=["fp32", "fp16", "bf16"])
     p.add_argument("--seed", type=int, default=0)
-    p.add_argument("--layout", default="dense", choices=["dense", "compact"],
-                   help="population layout of the fused step (compact: only fluid and ghost nodes own slots)")
+    p.add_argument("--layout", default="auto", choices=["auto", "dense", "compact"],
+                   help="population layout of the fused step (compact: only fluid and ghost nodes own slots, one "
+                        "lane per such node; auto: compact for FP32 storage on this pack, dense for 16-bit)")
     p.add_argument("--cpu-steps", type=int, default=16)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--macro-every-step", type=int, default=1)
@@ -385,7 +386,7 @@ def coupled_leg(device, steps, warmup, every=50):
     }
 
 
-def build_workload(n, seed, collision, storage, device, layout="dense"):
+def build_workload(n, seed, collision, storage, device, layout="auto"):
     import torch
 
     from fluxcore3d_amd import geometry
@@ -445,7 +446,7 @@ def cpu_baseline(solid, links, S, omega, collision, steps):
                       f"{k_one} steps on 1 thread (no warm-up step), {t_one:.1f} s"}
 
 
-def traffic_per_launch(args):
+def traffic_per_launch(args, layout):
     """HBM bytes per fused-step launch (FETCH_SIZE + WRITE_SIZE, each scaled by its
     reading for a copy of known size) from the committed rocprofv3 PMC summary of
     this workload, or None when no summary matches."""
@@ -454,7 +455,7 @@ def traffic_per_launch(args):
         return None
     with open(path) as fh:
         table = json.load(fh)
-    key = f"{args.n}^3 {args.collision} {args.storage}"
+    key = f"{args.n}^3 {args.collision} {args.storage}" + (" compact" if layout == "compact" else "")
     rec = table.get(key)
     return None if rec is None else int(rec["fetch_bytes"] + rec["write_bytes"])
 
@@ -505,6 +506,7 @@ def main():
     ok = bool(torch.isfinite(macro[1]).all()) and bool(torch.isfinite(macro[0]).all())
     ux_mean = float(macro[1][torch.as_tensor(~solid.reshape(-1), device=dev)].mean())
     n_links = int(eng.n_links)
+    layout, lanes = eng.layout, eng.lanes
     half = None
     if args.also_storage != "none" and args.storage == "fp32":
         del eng
@@ -572,9 +574,10 @@ def main():
                                f"sphere pack porosity {info['porosity']:.4f} (seed {args.seed}), body force 1e-6 x",
                    "grid": [args.n] * 3, "fluid_nodes": n_fluid, "n_links": n_links,
                    "storage": args.storage, "collision": args.collision, "ibb": "reference Bouzidi (periodic links)",
+                   "layout": layout, "lanes": lanes,
                    "macro_every_step": args.macro_every_step},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(args),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(args, layout),
                      "algorithmic_bytes_per_update": bpu,
                      "algorithmic_bytes_per_launch": bpu * n_fluid},
         "cpu_baseline": cpu,

@@ -443,10 +443,11 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
   if (gw > last_w) wm1 = wm0;
   const int64_t x = live ? (t.lane_nodes ? (int64_t)t.lane_nodes[r] : r) : 0;
 
-  // Populations first. Solid lanes (one lane per node) load their slots too and
-  // discard them: with every lane of a wavefront active, each load instruction
-  // covers whole 128-byte lines, which measured faster than the same lines
-  // requested by the fluid lanes alone.
+  // Populations first. Dense storage, one lane per node: solid lanes load their
+  // slots too and discard them: with every lane of a wavefront active, each load
+  // instruction covers whole 128-byte lines, which measured faster than the same
+  // lines requested by the fluid lanes alone. Compact storage: the lanes are the
+  // nodes owning slots, so a wavefront's own slots are 64 consecutive ones.
   int i = 0, j = 0, k = 0;
   uint32_t self = 0;
   float f[19];
@@ -983,7 +984,6 @@ static int pack(Grid& g, Tables& t, const fc_aa_tables* tb, int nx, int ny, int 
   t.lane_nodes = tb->lane_nodes; t.n_lanes = tb->n_lanes;
   FC_REQUIRE(t.n_lanes >= 0 && t.n_lanes <= g.N, "aa: lane count out of range");
   FC_REQUIRE(t.lane_nodes || t.n_lanes == g.N, "aa: lane list required unless there is one lane per node");
-  FC_REQUIRE(!tb->row_meta || !t.lane_nodes, "aa: compact storage runs with one lane per node");
   t.sp_mask = (const int4*)tb->sp_mask; t.sp_base = (const int4*)tb->sp_base;
   t.w1 = tb->w1; t.w2 = tb->w2; t.A = tb->A; t.B = tb->B; t.ff_idx = tb->ff_idx; t.n_v = tb->n_v;
   t.row_meta = (const uint2*)tb->row_meta;

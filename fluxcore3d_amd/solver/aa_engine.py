@@ -108,8 +108,13 @@ def build_storage(nx, ny, nz, solid: np.ndarray, periodic, bcs=()) -> dict:
 
 
 def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="reference", bcs=(),
-                 compact: bool = True) -> dict:
-    """Node/link tables of the fused step (pure NumPy, node order)."""
+                 compact: bool = True, lane_mask: np.ndarray | None = None) -> dict:
+    """Node/link tables of the fused step (pure NumPy, node order).
+
+    Lanes: one per node (``compact=False``), one per fluid node (``compact=True``,
+    dense storage: inputs from solid nodes are substituted from the source list), or
+    one per node of ``lane_mask`` (compact storage: the nodes that own slots; an
+    input from a node without slots reads as w_q, so there is no source list)."""
     N = nx * ny * nz
     solid = np.asarray(solid, bool).reshape(nx, ny, nz)
     flat_solid = solid.reshape(-1)
@@ -203,7 +208,15 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
     #            slots their neighbours read, so the list is empty)
     #   publish: lane | p << 6                     (post-collision f_p goes to B[ff_idx])
     # override = the node is not on an inlet/outlet plane (plane kernels rebuild those inputs).
-    lane_nodes = np.nonzero(~flat_solid)[0].astype(np.int64) if compact else np.arange(N, dtype=np.int64)
+    if lane_mask is not None:
+        lane_mask = np.asarray(lane_mask, bool).reshape(-1)
+        if np.any(~lane_mask & ~flat_solid):
+            raise ValueError("every fluid node needs a lane")
+        lane_nodes = np.nonzero(lane_mask)[0].astype(np.int64)
+    elif compact:
+        lane_nodes = np.nonzero(~flat_solid)[0].astype(np.int64)
+    else:
+        lane_nodes = np.arange(N, dtype=np.int64)
     n_lanes = int(lane_nodes.size)
     rank = np.full(N, -1, np.int64)
     rank[lane_nodes] = np.arange(n_lanes)
@@ -232,7 +245,7 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
     lk_code = ((rank[link_node] % 64) | (link_dir << 6) | (OPP_NP[link_dir] << 11)
                | (link_v.astype(np.int64) << 16) | (ovr_node[link_node].astype(np.int64) << 17)
                | (np.where(link_v, v_rank - v_base[rank[link_node] // 64], 0) << 18)).astype(np.int32)
-    if compact:
+    if compact and lane_mask is None:
         s_row, s_q = np.nonzero(((Smask[sp_idx][:, None] >> np.arange(19, dtype=np.uint32)) & 1).astype(bool))
         s_node = sp_idx[s_row]
     else:
@@ -267,31 +280,38 @@ class AAEngine:
         if ibb_mode not in ("reference", "textbook"):
             raise ValueError("ibb_mode must be 'reference' or 'textbook'")
         self.storage = storage
-        # Lane mapping of the fused step: "fluid" (one lane per fluid node) or "all"
-        # (one lane per node). With "all" every load instruction covers whole cache
-        # lines; on a porosity-0.6 256^3 sphere pack it is 1.8x (FP32) and 1.5x (FP16)
-        # faster than "fluid", whose wavefronts gather scattered nodes. "auto" = "all".
+        solid = np.ascontiguousarray(np.asarray(solid_flow, dtype=bool).reshape(self.nx, self.ny, self.nz))
+        # Population layout: "dense" (every node of the padded grid owns 19 slots) or
+        # "compact" (only fluid, ghost and plane-feeding solid nodes own slots,
+        # build_storage). "auto": compact for FP32 populations when at least 5 % of
+        # the nodes are solid (256^3 sphere pack, 40 % solid: 0.565 against 0.626 ms
+        # per step; 192x192x384 heat sink, 12.5 % solid: 0.718 against 0.848 ms),
+        # dense for 16-bit populations (the pack: 0.519 dense against 0.732 ms compact,
+        # half the bytes per lane leave too few in flight per wavefront).
+        if layout == "auto":
+            layout = "compact" if storage == "fp32" and solid.mean() >= 0.05 else "dense"
+        if layout not in ("dense", "compact"):
+            raise ValueError("layout must be 'auto', 'dense' or 'compact'")
+        self.layout = layout
+        # Lane mapping of the fused step: "fluid" or "all" (one lane per node).
+        # Dense storage: "fluid" is one lane per fluid node; "all" (the default) is
+        # 1.8x (FP32) and 1.5x (FP16) faster on a porosity-0.6 256^3 sphere pack,
+        # because each load instruction covers whole cache lines instead of gathering
+        # scattered nodes. Compact storage: "fluid" (the default) is one lane per node
+        # that owns slots (fluid nodes and the solid nodes feeding inlet/outlet planes).
+        # These lanes read consecutive slots, and every lane of a wavefront has
+        # populations to move, so each wavefront keeps 64 lanes x 19 loads in flight;
+        # with "all" a third of the lanes idle on solid nodes.
         if lanes == "auto":
-            lanes = "all"
+            lanes = "fluid" if layout == "compact" else "all"
         if lanes not in ("fluid", "all"):
             raise ValueError("lanes must be 'auto', 'fluid' or 'all'")
         self.lanes = lanes
-        # Population layout: "dense" (every node of the padded grid owns 19 slots) or
-        # "compact" (only fluid, ghost and plane-feeding solid nodes own slots,
-        # build_storage; one lane per node). "auto" = "dense".
-        if layout == "auto":
-            layout = "dense"
-        if layout not in ("dense", "compact"):
-            raise ValueError("layout must be 'auto', 'dense' or 'compact'")
-        if layout == "compact" and lanes != "all":
-            raise ValueError("compact layout runs with one lane per node (lanes='all')")
-        self.layout = layout
         self.device = torch.device(device)
         self.ibb_mode = ibb_mode
         self.bcs = [dict(b) for b in bcs]
         if len(self.bcs) > 6:
             raise ValueError("at most 6 inlet/outlet planes")
-        solid = np.ascontiguousarray(np.asarray(solid_flow, dtype=bool).reshape(self.nx, self.ny, self.nz))
         self._build_tables(solid, links)
         self._collision = self._make_collision(collision, omega, omega_minus, S, cs_dx2, force, sponge, sponge_axis)
         if self.layout == "compact":
@@ -307,11 +327,17 @@ class AAEngine:
 
     # ------------------------------------------------------------------ tables
     def _build_tables(self, solid: np.ndarray, links) -> None:
-        tb = build_tables(self.nx, self.ny, self.nz, solid, links, self.periodic, self.ibb_mode, self.bcs,
-                          compact=self.lanes == "fluid")
-        self.n_links, self.n_v, self.n_special = tb["n_links"], tb["n_v"], tb["n_special"]
+        sb = None
+        lane_mask = None
         if self.layout == "compact":
             sb = build_storage(self.nx, self.ny, self.nz, solid, self.periodic, self.bcs)
+            if self.lanes == "fluid":
+                g = sb["gpad"]
+                lane_mask = sb["present"][g[0]:g[0] + self.nx, g[1]:g[1] + self.ny, g[2]:g[2] + self.nz]
+        tb = build_tables(self.nx, self.ny, self.nz, solid, links, self.periodic, self.ibb_mode, self.bcs,
+                          compact=self.lanes == "fluid", lane_mask=lane_mask)
+        self.n_links, self.n_v, self.n_special = tb["n_links"], tb["n_v"], tb["n_special"]
+        if sb is not None:
             tb.update(row_meta=sb["row_meta"], wpr=sb["wpr"], n_store=sb["n_store"])
             self._n_store = sb["n_store"]
         else:

@@ -326,7 +326,8 @@ class LBMCUDAUpgrade:
                        cs_dx2=sim._cuda_Cs_dx2 or 0.0, force=force, sponge=sponge, sponge_axis=sponge_axis,
                        links=_links_numpy(getattr(sim, "ibb_data", None)),
                        ibb_mode=getattr(cfg, "ibb_mode", "reference"), bcs=flow_bc_plan(sim),
-                       storage=getattr(cfg, "population_storage", "fp32"), device=sim.device)
+                       storage=getattr(cfg, "population_storage", "fp32"), device=sim.device,
+                       layout=getattr(cfg, "population_layout", "auto"))
         eng.load(sim.f.reshape(19, -1).float())
         for name in ("rho", "u", "v", "wv"):
             t = getattr(sim, name)

@@ -90,8 +90,9 @@ typedef struct fc_aa_tables {
                                | override << 17 | second-node index in the wavefront << 18 */
   const int32_t *s_code;    /* [n_src] lane | q << 6 | override << 17 (input w_q after step 1) */
   const int32_t *f_code;    /* [n_publish] lane | p << 6 (post-collision f_p -> B[ff_idx]) */
-  const int32_t *lane_nodes; /* [n_lanes] node of each lane, 64 lanes per wavefront (the fluid nodes, or
-                               NULL for one lane per node); lanes on solid nodes idle */
+  const int32_t *lane_nodes; /* [n_lanes] node of each lane, 64 lanes per wavefront: the fluid nodes
+                               (dense storage), the nodes owning slots (compact storage), or NULL for
+                               one lane per node */
   int64_t n_lanes;
   const int32_t *sp_mask;   /* [n_special][4] link, solid-source, second-node, publish masks */
   const int32_t *sp_base;   /* [n_special][4] first link, first second-node link, first publish entry */
@@ -105,7 +106,7 @@ typedef struct fc_aa_tables {
    * R = I * PY + J and word w = K / 32: {mask, start}, bit b of mask set when node
    * K = 32 w + b owns slots, start = nodes owning slots before the word (padded C
    * order); storage index = start + popcount(mask & ((1 << b) - 1)). Nodes without
-   * slots stream w_q. One lane per node only. */
+   * slots stream w_q. */
   const int32_t *row_meta;  /* [PX * PY * wpr][2] */
   int wpr;                  /* words per row, (PZ + 31) / 32 */
   int64_t n_store;          /* nodes owning slots: the population arrays are [19][n_store] */

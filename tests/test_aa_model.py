@@ -48,8 +48,13 @@ def _locate(sb, I, J, K):
 
 
 def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="reference", compact=True,
-              storage="dense"):
-    tb = build_tables(nx, ny, nz, solid, links, periodic, ibb_mode, compact=compact)
+              storage="dense", slot_lanes=False):
+    lane_mask = None
+    if storage == "compact" and slot_lanes:  # one lane per node owning slots
+        sb = build_storage(nx, ny, nz, solid, periodic)
+        g = sb["gpad"]
+        lane_mask = sb["present"][g[0]:g[0] + nx, g[1]:g[1] + ny, g[2]:g[2] + nz]
+    tb = build_tables(nx, ny, nz, solid, links, periodic, ibb_mode, compact=compact, lane_mask=lane_mask)
     if storage == "compact":
         return _model_run_compact(nx, ny, nz, solid, links, periodic, f0, steps, om, tb)
     N = nx * ny * nz
@@ -121,9 +126,10 @@ def model_run(nx, ny, nz, solid, links, periodic, f0, steps, om, ibb_mode="refer
 
 
 def _model_run_compact(nx, ny, nz, solid, links, periodic, f0, steps, om, tb):
-    """One lane per node over compact storage: lanes whose node has no slots do
-    nothing; an input whose writer x - c_q has no slots is w_q (after the first
-    step); a store to x + c_q happens only where that node has slots."""
+    """Compact storage, one lane per node or per node owning slots (decoded through
+    the lane list, as the kernel does): lanes whose node has no slots do nothing; an
+    input whose writer x - c_q has no slots is w_q (after the first step); a store
+    to x + c_q happens only where that node has slots."""
     sb = build_storage(nx, ny, nz, solid, periodic)
     N = nx * ny * nz
     gx, gy, gz = sb["gpad"]
@@ -143,11 +149,13 @@ def _model_run_compact(nx, ny, nz, solid, links, periodic, f0, steps, om, tb):
 
     wm = tb["wave_meta"].astype(np.int64)
     lk, fc = tb["lk_code"].astype(np.int64), tb["f_code"].astype(np.int64)
-    l_node = _group_of(wm[:, 0], lk.size) * 64 + (lk & 63)
+    assert tb["n_src"] == 0  # inputs from nodes without slots read as w_q: no source list
+    fl = tb["lane_nodes"].astype(np.int64)
+    l_node = fl[_group_of(wm[:, 0], lk.size) * 64 + (lk & 63)]
     l_t, l_opp = (lk >> 6) & 31, (lk >> 11) & 31
     l_hasv, l_ovr = ((lk >> 16) & 1).astype(bool), ((lk >> 17) & 1).astype(bool)
     l_v = wm[_group_of(wm[:, 0], lk.size), 1] + (lk >> 18)
-    f_node = _group_of(wm[:, 2], fc.size) * 64 + (fc & 63)
+    f_node = fl[_group_of(wm[:, 2], fc.size) * 64 + (fc & 63)]
     f_p = (fc >> 6) & 31
     self_p, self_i = loc((0, 0, 0))
     up = [loc(-C[q]) for q in range(19)]  # writer x - c_q of input q
@@ -212,8 +220,9 @@ def test_storage_records_count_present_nodes():
     assert np.array_equal(idx[p], np.arange(sb["n_store"]))
 
 
+@pytest.mark.parametrize("slot_lanes", [False, True])
 @pytest.mark.parametrize("case", ["periodic_pack", "walled_box"])
-def test_compact_storage_model_matches_oracle(case):
+def test_compact_storage_model_matches_oracle(case, slot_lanes):
     if case == "periodic_pack":
         c = cases.periodic_pack(n=24, seed=3)
         per, solid = (1, 1, 1), c["solid"]
@@ -226,7 +235,8 @@ def test_compact_storage_model_matches_oracle(case):
     f0 = olbm.feq(1 + 0.01 * rng.random(N), *(0.02 * rng.standard_normal((3, N))), np.float64)
     om = 1.0 / 0.55
     steps = 7
-    mm = model_run(nx, ny, nz, solid, c["links"], per, f0, steps, om, compact=False, storage="compact")
+    mm = model_run(nx, ny, nz, solid, c["links"], per, f0, steps, om, compact=False, storage="compact",
+                   slot_lanes=slot_lanes)
     f = f0.copy()
     for _ in range(steps):
         f, *mo = olbm.step(f, solid, nx, ny, nz, per, "bgk", om, 1.0, None, 0.0, c["links"])
@@ -300,6 +310,36 @@ def test_entry_codes_match_node_masks(compact):
     for col, arr in enumerate((L, S, V, F)):
         assert np.array_equal(arr[sp], masks[:, col]), col
         assert not np.any(np.delete(arr, sp))
+
+
+def test_slot_lane_tables():
+    """Compact storage with one lane per node owning slots: the lanes are the fluid
+    nodes and the solid nodes feeding the inlet/outlet planes, in node order; there
+    is no source list, and the link and publication entries decode to the same nodes
+    and directions as with one lane per node."""
+    c = cases.config1()
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    bcs = [dict(axis=0, plane=0), dict(axis=0, plane=nx - 1)]
+    sb = build_storage(nx, ny, nz, c["solid_flow"], (0, 0, 0), bcs)
+    mask = sb["present"][1:-1, 1:-1, 1:-1]
+    tb = build_tables(nx, ny, nz, c["solid_flow"], c["links"], (0, 0, 0), bcs=bcs, compact=True, lane_mask=mask)
+    ref = build_tables(nx, ny, nz, c["solid_flow"], c["links"], (0, 0, 0), bcs=bcs, compact=False)
+    fl = tb["lane_nodes"].astype(np.int64)
+    assert np.array_equal(fl, np.nonzero(mask.reshape(-1))[0])
+    assert (c["solid_flow"].reshape(-1)[fl]).any()  # plane-feeding solid nodes have lanes
+    assert tb["n_src"] == 0 and tb["s_code"].size == 0
+    for t in (tb, ref):
+        wm = t["wave_meta"].astype(np.int64)
+        lk = t["lk_code"].astype(np.int64)
+        lanes = t["lane_nodes"].astype(np.int64)
+        t["_nodes"] = lanes[_group_of(wm[:, 0], lk.size) * 64 + (lk & 63)]
+        t["_dirs"] = (lk >> 6) & 31
+    assert np.array_equal(tb["_nodes"], tb["link_node"])
+    key = lambda t: np.sort(t["_nodes"] * 19 + t["_dirs"])  # noqa: E731
+    assert np.array_equal(key(tb), key(ref))
+    with pytest.raises(ValueError):
+        build_tables(nx, ny, nz, c["solid_flow"], c["links"], (0, 0, 0), compact=True,
+                     lane_mask=np.zeros_like(mask))
 
 
 @pytest.mark.parametrize("compact", [True, False])

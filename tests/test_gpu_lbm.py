@@ -318,42 +318,56 @@ def test_aa_mrt_smagorinsky_with_sponge_and_planes():
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 @pytest.mark.parametrize("kind", ["bgk", "trt", "mrt"])
-@pytest.mark.parametrize("case", ["config1", "pack"])
+@pytest.mark.parametrize("case", ["config1", "pack", "pack64", "pin_fin_z128"])
 def test_compact_and_dense_layouts_agree_bit_for_bit(case, kind, storage):
     """The compact population layout (only fluid, ghost and plane-feeding solid nodes
     own slots; inputs from other solid nodes are w_q) gives the same bits as the dense
     layout: the 64x32x32 sphere channel with Zou/He inlet, convective outlet, walls and
-    Bouzidi links, and the periodic 48^3 sphere pack with a body force; 41 steps
-    (both parities), macros and materialised populations."""
+    Bouzidi links, the periodic 48^3 and 64^3 sphere packs with a body force, and a
+    32x32x128 pin-fin channel along z (inlet and outlet planes on z, ghost layers on
+    every face); 41 steps (both parities), macros and materialised populations.
+    Compact storage runs with one lane per node owning slots (the default: fluid nodes
+    and the solid nodes feeding the planes) and with one lane per node."""
     from fluxcore3d_amd.solver.aa_engine import AAEngine
+    force = None
     if case == "config1":
         c = cases.config1()
         per = (0, 0, 0)
         bcs = [dict(kind="zou_he", axis=0, plane=0, interior=1, bc_type=0, u_in=0.05, rho_out=1.0, sign=1),
                dict(kind="convective", axis=0, plane=c["nx"] - 1, interior=c["nx"] - 2, unknown_cax=-1,
                     rho_out=1.0)]
-        force = None
+    elif case == "pin_fin_z128":
+        c = cases.pin_fin_channel(32, 32, 128)
+        per = (0, 0, 0)
+        nz = c["nz"]
+        bcs = [dict(kind="zou_he", axis=2, plane=0, interior=1, bc_type=0, u_in=0.04, rho_out=1.0, sign=1),
+               dict(kind="convective", axis=2, plane=nz - 1, interior=nz - 2, unknown_cax=-1, rho_out=1.0)]
     else:
-        c = cases.periodic_pack()
+        c = cases.periodic_pack() if case == "pack" else cases.periodic_pack(n=64, seed=1)
         per, bcs, force = (1, 1, 1), (), (1e-5, 0.0, 0.0)
     nx, ny, nz = c["nx"], c["ny"], c["nz"]
     om = 1.0 / 0.6
     S = cases.mrt_S(om) if kind == "mrt" else None
     out, slots = {}, {}
-    for layout in ("dense", "compact"):
+    layouts = [("dense", "dense", "auto"), ("compact", "compact", "auto"), ("compact_all", "compact", "all")]
+    for name, layout, lanes in layouts:
         eng = AAEngine(nx, ny, nz, c["solid_flow"], periodic=per, collision=kind, omega=om, omega_minus=1.3, S=S,
-                       force=force, links=c["links"], bcs=bcs, storage=storage, device=DEV, layout=layout)
+                       force=force, links=c["links"], bcs=bcs, storage=storage, device=DEV, layout=layout,
+                       lanes=lanes)
+        assert (eng.lane_nodes is not None) == (name == "compact")
         N = nx * ny * nz
         eng.load(torch.as_tensor(np.broadcast_to(olbm.W32[:, None], (19, N)).copy(), device=DEV))
         macro = [torch.empty(N, device=DEV) for _ in range(4)]
         for _ in range(41):
             eng.step(*macro)
-        out[layout] = [m.cpu().numpy() for m in macro] + [eng.materialize().cpu().numpy()]
-        slots[layout] = eng.Ns
+        out[name] = [m.cpu().numpy() for m in macro] + [eng.materialize().cpu().numpy()]
+        slots[name] = eng.Ns
+        del eng
     assert slots["compact"] < slots["dense"]  # solid nodes away from the planes own no slots
-    for a, b in zip(out["dense"], out["compact"]):
-        np.testing.assert_array_equal(a, b)
-    assert np.abs(out["dense"][1]).max() > 1e-4
+    for name, _, _ in layouts[1:]:
+        for a, b in zip(out["dense"], out[name]):
+            np.testing.assert_array_equal(a, b)
+    assert np.abs(out["dense"][1 if case != "pin_fin_z128" else 3]).max() > 1e-4
 
 
 @pytest.mark.parametrize("engine", ["aa", "reference"])
