@@ -373,6 +373,23 @@ __global__ void fold_chunks_kernel(const float* __restrict__ partial, int n_chun
 
 }  // namespace
 
+// Dynamic LDS the staged kernel may take: the device's per-block limit less the static
+// K table, at most 80 KB (two blocks per CU on a 160 KB CU; a third would not fit).
+// Cached per device; 60 KB where the attribute is unavailable.
+static size_t lds_cap() {
+  static size_t cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 61440;
+  if (!cache[dev]) {
+    int v = 0;
+    size_t cap = 61440;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 4096)
+      cap = std::min<size_t>((size_t)v - sizeof(int2) * kLdsMaxK, 81920);
+    cache[dev] = cap;
+  }
+  return cache[dev];
+}
+
 extern "C" {
 
 int fc_conv_kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
@@ -388,10 +405,12 @@ int fc_conv3d_fwd(const void* x0, int c0, int up0, const void* x1, int c1, const
   FC_REQUIRE(y_bf16 || y_f32, "conv3d_fwd: no output");
   ConvShape s{B, D, H, W, c0, c1, up0, cout, fc_conv_kpad(c0 + c1)};
   const int mt = (cout + 15) / 16;
-  // LDS-staged kernel where rows are long enough and the three input planes fit in 60 KB
+  // LDS-staged kernel where rows are long enough and the three input planes fit in
+  // lds_cap(): up to 4 channel chunks (76 KB with the K table, 2 blocks per CU on
+  // gfx950's 160 KB) where the device allows it
   const size_t lds = (size_t)((c0 + c1) / 8) * 3 * kLdsPlane * sizeof(bf16x8);
   const int64_t elems = (int64_t)B * D * H * W * (c0 + c1 > cout ? c0 + c1 : cout);
-  if (W >= kLdsTW && lds <= 61440 && elems < ((int64_t)1 << 31) && fc_conv_kpad(c0 + c1) / 8 <= kLdsMaxK) {
+  if (W >= kLdsTW && lds <= lds_cap() && elems < ((int64_t)1 << 31) && fc_conv_kpad(c0 + c1) / 8 <= kLdsMaxK) {
     // output planes per block: up to kLdsDZ (fewer halo reloads) while keeping >= 2048 blocks
     const int64_t per_plane = (int64_t)B * ((H + kLdsTH - 1) / kLdsTH) * ((W + kLdsTW - 1) / kLdsTW);
     const int dz_block = (int)std::max<int64_t>(1, std::min<int64_t>(kLdsDZ, per_plane * D / 2048));

@@ -32,13 +32,14 @@ def _up(x):
 
 
 # W >= 64 runs the LDS-staged kernel (input tile of 3 x 6 x 66 voxels) when the tile
-# fits in 64 KB: partial h and w tiles, 1-3 channel chunks; cin = 40 at W = 96 falls
-# back to the cached kernel.
+# fits the device's LDS, at most 80 KB: partial h and w tiles, 1-4 channel chunks;
+# cin = 40 at W = 96 (95 KB) falls back to the cached kernel.
 @pytest.mark.parametrize("B,D,H,W,cin,cout,act", [(2, 6, 5, 19, 8, 12, False), (1, 4, 7, 33, 16, 16, True),
                                                    (1, 3, 3, 16, 24, 48, True), (1, 5, 4, 20, 40, 64, False),
                                                    (1, 3, 2, 70, 16, 24, True), (2, 2, 3, 40, 8, 32, False),
                                                    (1, 5, 6, 64, 8, 8, True), (2, 3, 5, 130, 24, 16, False),
-                                                   (1, 2, 9, 96, 40, 64, True)])
+                                                   (1, 2, 9, 96, 40, 64, True), (1, 4, 6, 64, 32, 16, True),
+                                                   (1, 3, 5, 130, 32, 8, False)])
 def test_forward_single_source(B, D, H, W, cin, cout, act):
     from fluxcore3d_amd.solver import nn_precond as nn
     x = _bf16((B, D, H, W, cin), 1)
@@ -55,13 +56,16 @@ def test_forward_single_source(B, D, H, W, cin, cout, act):
     assert np.all(np.abs(y16 - y) <= 2.0 ** -8 * np.abs(y) + 1e-30)
 
 
-@pytest.mark.parametrize("W", [16, 64])
-def test_forward_two_sources_with_upsample(W):
+@pytest.mark.parametrize("W,cs", [(16, 8), (64, 8), (64, 16), (128, 16)])
+def test_forward_two_sources_with_upsample(W, cs):
+    """Decoder convolution: the half-resolution source read through nearest
+    upsampling and the skip source read directly; 16 + 16 channels at W >= 64 is the
+    level-0 decoder layer's shape (four channel chunks in the LDS-staged kernel)."""
     from fluxcore3d_amd.solver import nn_precond as nn
     xc = _bf16((1, 4, 3, W // 2, 16), 3)  # half-resolution source
-    xs = _bf16((1, 8, 6, W, 8), 4)  # skip source at full resolution
+    xs = _bf16((1, 8, 6, W, cs), 4)  # skip source at full resolution
     cout = 24
-    w = _bf16((cout, 27, 24), 5).float() * 0.2
+    w = _bf16((cout, 27, 16 + cs), 5).float() * 0.2
     wpk = nn.pack_weights(w.to(DEV))
     y = nn.conv3d(xc.to(DEV), wpk, cout, True, x1=xs.to(DEV), up0=True, out_f32=True).cpu().numpy()
     xcat = np.concatenate([_up(_np(xc)), _np(xs)], axis=-1)
