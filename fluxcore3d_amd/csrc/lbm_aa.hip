@@ -65,10 +65,10 @@ struct Tables {
   const int4* sp_mask;       // [n_special] {L, S, V, F}
   const int4* sp_base;       // [n_special] {first plain link, v_base, ff_base, first second-node link}
   const float* w1;           // [n_links]
-  const float* w2;           // [n_links]
+  const float* w2;           // [n_links], or null: w2 = 1 - w1 (the reference's weights, exactly)
   float* A;                  // [n_links]
   float* B;                  // [2][n_v]
-  const int32_t* ff_idx;     // [n_ff]
+  const int32_t* ff_idx;     // [n_ff], or null: f_code bits 11.. hold the slot relative to the group's v_lo
   int64_t n_v;
   // compact storage (null: dense padded storage): per row R = I * PY + J and word
   // w = K / 32 of the padded grid, {presence mask, present nodes before the word}
@@ -180,6 +180,10 @@ __device__ __forceinline__ void store_plain(ST* pop, const Grid& g, const Tables
   Store<ST>::st(pop + (int64_t)dir * g.Ns + s, v, dir);
 }
 
+// Second Bouzidi weight of link l: stored, or 1 - w1 (how the reference forms it in
+// both branches, sdf_tracing.py:188-205, and in the fallback w1 = 1, w2 = 0).
+__device__ __forceinline__ float w2_of(const Tables& t, int64_t l, float w1) { return t.w2 ? t.w2[l] : 1.f - w1; }
+
 // All 19 inputs of a node: plain slots, rest weights for solid sources and
 // Bouzidi values for links. `first` marks the step right after loading: A then
 // holds the loaded populations themselves and solid sources have not streamed
@@ -197,7 +201,7 @@ __device__ __forceinline__ void compose_inputs(const ST* __restrict__ pop, const
       float v = t.A[l];
       if (!first && isv) {
         const int64_t mv = b.y + __popc(V & ((1u << q) - 1u));
-        v = v + t.w2[l] * t.B[(int64_t)(parity ^ 1) * t.n_v + mv];
+        v = v + w2_of(t, l, t.w1[l]) * t.B[(int64_t)(parity ^ 1) * t.n_v + mv];
       }
       f[q] = v;
     } else if (special && !first && ((S >> q) & 1u)) {
@@ -543,20 +547,20 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
     c0 = t.lk_code[l_lo + lane];
     a0 = t.A[l_lo + lane];
     w10 = t.w1[l_lo + lane];
-    w20 = t.w2[l_lo + lane];
+    if (t.w2) w20 = t.w2[l_lo + lane];  // else 1 - w10, formed where it is used
   }
   if (hasB0) b0 = Bprev[lane];
   if (hasL1) {
     c1 = t.lk_code[l_lo + lane + 64];
     a1 = t.A[l_lo + lane + 64];
     w11 = t.w1[l_lo + lane + 64];
-    w21 = t.w2[l_lo + lane + 64];
+    if (t.w2) w21 = t.w2[l_lo + lane + 64];
   }
   if (hasB1) b1 = Bprev[lane + 64];
   if (hasS0) sc0 = t.s_code[s_lo + lane];
   if (hasF0) {
     fc0 = t.f_code[f_lo + lane];
-    ff0 = (uint32_t)t.ff_idx[f_lo + lane];
+    if (t.ff_idx) ff0 = (uint32_t)t.ff_idx[f_lo + lane];  // else from fc0, formed at the store
   }
   const uint8_t nfl = t.nflag[x];  // x = 0 on dead lanes: no branch around the load
   const bool fluid = live && !(nfl & F_SOLID);
@@ -566,12 +570,14 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
 #pragma unroll
     for (int q = 0; q < 19; ++q) tile[wv][q][lane] = f[q];
     wave_lds_sync();
-    if (hasL0 && (c0 & kCodeOvr)) tile[wv][(c0 >> 6) & 31][c0 & 63] = hasB0 ? a0 + w20 * b0 : a0;
-    if (hasL1 && (c1 & kCodeOvr)) tile[wv][(c1 >> 6) & 31][c1 & 63] = hasB1 ? a1 + w21 * b1 : a1;
+    if (hasL0 && (c0 & kCodeOvr))
+      tile[wv][(c0 >> 6) & 31][c0 & 63] = hasB0 ? a0 + (t.w2 ? w20 : 1.f - w10) * b0 : a0;
+    if (hasL1 && (c1 & kCodeOvr))
+      tile[wv][(c1 >> 6) & 31][c1 & 63] = hasB1 ? a1 + (t.w2 ? w21 : 1.f - w11) * b1 : a1;
     for (int e = lane + 128; e < nL; e += 64) {
       const int code = t.lk_code[l_lo + e];
       float v = t.A[l_lo + e];
-      if (!first && e < nV) v = v + t.w2[l_lo + e] * Bprev[e];
+      if (!first && e < nV) v = v + w2_of(t, l_lo + e, t.w2 ? 0.f : t.w1[l_lo + e]) * Bprev[e];
       if (code & kCodeOvr) tile[wv][(code >> 6) & 31][code & 63] = v;
     }
     if (!first) {
@@ -647,27 +653,28 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
     if (hasL0) {
       const int n = c0 & 63;
       float a = w10 * tile[wv][(c0 >> 11) & 31][n];
-      if (!(c0 & kCodeV)) a = a + w20 * tile[wv][(c0 >> 6) & 31][n];
+      if (!(c0 & kCodeV)) a = a + (t.w2 ? w20 : 1.f - w10) * tile[wv][(c0 >> 6) & 31][n];
       t.A[l_lo + lane] = a;
     }
     if (hasL1) {
       const int n = c1 & 63;
       float a = w11 * tile[wv][(c1 >> 11) & 31][n];
-      if (!(c1 & kCodeV)) a = a + w21 * tile[wv][(c1 >> 6) & 31][n];
+      if (!(c1 & kCodeV)) a = a + (t.w2 ? w21 : 1.f - w11) * tile[wv][(c1 >> 6) & 31][n];
       t.A[l_lo + lane + 64] = a;
     }
     for (int e = lane + 128; e < nL; e += 64) {
       const int code = t.lk_code[l_lo + e];
       const int n = code & 63;
-      float a = t.w1[l_lo + e] * tile[wv][(code >> 11) & 31][n];
-      if (!(code & kCodeV)) a = a + t.w2[l_lo + e] * tile[wv][(code >> 6) & 31][n];
+      const float w1e = t.w1[l_lo + e];
+      float a = w1e * tile[wv][(code >> 11) & 31][n];
+      if (!(code & kCodeV)) a = a + w2_of(t, l_lo + e, w1e) * tile[wv][(code >> 6) & 31][n];
       t.A[l_lo + e] = a;
     }
     float* Bw = t.B + (int64_t)parity * t.n_v;
-    if (hasF0) Bw[(size_t)ff0] = tile[wv][(fc0 >> 6) & 31][fc0 & 63];
+    if (hasF0) Bw[t.ff_idx ? (size_t)ff0 : (size_t)(v_lo + (fc0 >> 11))] = tile[wv][(fc0 >> 6) & 31][fc0 & 63];
     for (int e = lane + 64; e < nF; e += 64) {
       const int code = t.f_code[f_lo + e];
-      Bw[t.ff_idx[f_lo + e]] = tile[wv][(code >> 6) & 31][code & 63];
+      Bw[t.ff_idx ? t.ff_idx[f_lo + e] : v_lo + (code >> 11)] = tile[wv][(code >> 6) & 31][code & 63];
     }
   }
 }

@@ -251,14 +251,22 @@ def build_tables(nx, ny, nz, solid: np.ndarray, links, periodic, ibb_mode="refer
     else:
         s_q = s_node = np.zeros(0, np.int64)
     s_code = ((rank[s_node] % 64) | (s_q << 6) | (ovr_node[s_node].astype(np.int64) << 17)).astype(np.int32)
-    f_code = ((rank[ff_node] % 64) | (ff_dir << 6)).astype(np.int32)
+    f_code = ((rank[ff_node] % 64) | (ff_dir << 6)).astype(np.int64)
     # per wavefront (+ totals): first link, second-node link, publish entry and source entry
     wave_meta = np.stack([np.searchsorted(link_wave, np.arange(waves + 1)), v_base,
                           np.searchsorted(ff_node, starts), np.searchsorted(s_node, starts)], 1).astype(np.int32)
     assert wave_meta.shape[0] == waves + 1 or n_lanes == 0
+    # A publication's B slot relative to its group's first second-node slot fits in the
+    # code's upper 21 bits on any grid whose x-planes hold under about a million links:
+    # the publishing node is a neighbour of the consuming one. Otherwise ff_idx is kept.
+    rel = ff_slot.astype(np.int64) - wave_meta[rank[ff_node] // 64, 1] if ff_node.size else np.zeros(0, np.int64)
+    ff_packed = bool(rel.size == 0 or (rel.min() >= -(1 << 20) and rel.max() < (1 << 20)))
+    if ff_packed:
+        f_code = f_code | (rel << 11)
+    f_code = f_code.astype(np.int32)  # in range: lane | p << 6 < 2^11, rel << 11 within 32 bits
     assert np.all((pos - wave_meta[link_wave, 0] < np.diff(wave_meta[:, 1])[link_wave]) == link_v)
     return dict(nflag=nflag, wave_base=wave_base, wave_meta=wave_meta, lk_code=lk_code, s_code=s_code,
-                f_code=f_code, n_src=int(s_node.size), lane_nodes=lane_nodes.astype(np.int32), n_lanes=n_lanes, sp_mask=sp_mask, sp_base=sp_base, w1=w1s, w2=w2s,
+                f_code=f_code, ff_packed=ff_packed, n_src=int(s_node.size), lane_nodes=lane_nodes.astype(np.int32), n_lanes=n_lanes, sp_mask=sp_mask, sp_base=sp_base, w1=w1s, w2=w2s,
                 link_node=link_node, link_dir=link_dir, B_init=B_init, ff_idx=ff_slot, ff_node=ff_node,
                 n_links=int(link_node.size), n_v=int(v_node.size), n_special=int(sp_idx.size), sp_idx=sp_idx)
 
@@ -269,7 +277,7 @@ class AAEngine:
     def __init__(self, nx, ny, nz, solid_flow, *, periodic=(False, False, False), collision="bgk",
                  omega=1.0, omega_minus=1.0, S=None, cs_dx2=0.0, force=None, sponge=None, sponge_axis=0,
                  links=None, ibb_mode="reference", bcs=(), storage="fp32", device="cuda", lanes="auto",
-                 layout="auto"):
+                 layout="auto", compress_links=True):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.N = self.nx * self.ny * self.nz
         self.periodic = tuple(bool(p) for p in periodic)
@@ -293,6 +301,10 @@ class AAEngine:
         if layout not in ("dense", "compact"):
             raise ValueError("layout must be 'auto', 'dense' or 'compact'")
         self.layout = layout
+        # compress_links: the kernel forms w2 = 1 - w1 where that holds bit for bit and
+        # reads each publication's B slot from its code where it fits (4 + 4 B less per
+        # link and publication); False keeps the explicit w2 and ff_idx arrays.
+        self._compress_links = bool(compress_links)
         # Lane mapping of the fused step: "fluid" or "all" (one lane per node).
         # Dense storage: "fluid" is one lane per fluid node; "all" (the default) is
         # 1.8x (FP32) and 1.5x (FP16) faster on a porosity-0.6 256^3 sphere pack,
@@ -372,12 +384,18 @@ class AAEngine:
         self.B_init = dt(np.concatenate([B_init, B_init]) if self.n_v else np.zeros(2, np.float32), torch.float32)
         self.B = self.B_init.clone()
         self.ff_idx = dt(tb["ff_idx"], torch.int32)
+        # tables the kernel can do without: w2 where it is 1 - w1 bit for bit (the
+        # reference's weights), ff_idx where the publication codes carry the slot
+        w1, w2 = np.asarray(tb["w1"], np.float32), np.asarray(tb["w2"], np.float32)
+        self.w2_derived = self._compress_links and bool(np.array_equal(w2, np.float32(1.0) - w1))
+        self.ff_packed = self._compress_links and bool(tb["ff_packed"])
         self.row_meta = None if tb["row_meta"] is None else dt(tb["row_meta"], torch.int32)
         self._tables = _lib.AATables(
             self.nflag.data_ptr(), self.wave_base.data_ptr(), self.wave_meta.data_ptr(), self.lk_code.data_ptr(),
             self.s_code.data_ptr(), self.f_code.data_ptr(), _lib.ptr(self.lane_nodes), self.n_lanes,
             self.sp_mask.data_ptr(), self.sp_base.data_ptr(),
-            self.w1.data_ptr(), self.w2.data_ptr(), self.A.data_ptr(), self.B.data_ptr(), self.ff_idx.data_ptr(),
+            self.w1.data_ptr(), 0 if self.w2_derived else self.w2.data_ptr(), self.A.data_ptr(), self.B.data_ptr(),
+            0 if self.ff_packed else self.ff_idx.data_ptr(),
             max(self.n_v, 1), _lib.ptr(self.row_meta), int(tb["wpr"]), int(tb["n_store"]))
 
     def _make_collision(self, collision, omega, omega_minus, S, cs_dx2, force, sponge, sponge_axis):

@@ -97,10 +97,11 @@ typedef struct fc_aa_tables {
   const int32_t *sp_mask;   /* [n_special][4] link, solid-source, second-node, publish masks */
   const int32_t *sp_base;   /* [n_special][4] first link, first second-node link, first publish entry */
   const float *w1;          /* [n_links] Bouzidi weights */
-  const float *w2;          /* [n_links] */
+  const float *w2;          /* [n_links], or NULL: w2 = 1 - w1 (the reference's weights, exactly) */
   float *A;                 /* [n_links] own-node part of each link value */
   float *B;                 /* [2][n_v] second-node part, by step parity */
-  const int32_t *ff_idx;    /* [n_publish] slot in B written by each publishing node */
+  const int32_t *ff_idx;    /* [n_publish] slot in B written by each publishing node, or NULL: the slot
+                               minus the group's first second-node slot is f_code >> 11 (signed) */
   int64_t n_v;
   /* Compact storage (NULL: dense, every padded node owns slots). Per padded row
    * R = I * PY + J and word w = K / 32: {mask, start}, bit b of mask set when node

@@ -304,6 +304,11 @@ def test_entry_codes_match_node_masks(compact):
     fc = tb["f_code"].astype(np.int64)
     f_node = fl[_group_of(wm[:, 2], fc.size) * 64 + (fc & 63)]
     assert np.array_equal(f_node, tb["ff_node"])
+    # the publication's B slot rides in the code's upper bits, relative to its group's first slot
+    assert tb["ff_packed"]
+    assert np.array_equal(wm[_group_of(wm[:, 2], fc.size), 1] + (fc >> 11), tb["ff_idx"])
+    # w2 = 1 - w1 bit for bit (the kernel forms it instead of loading it)
+    assert np.array_equal(tb["w2"], np.float32(1.0) - tb["w1"])
     np.bitwise_or.at(F, f_node, 1 << ((fc >> 6) & 31))
     sp = tb["sp_idx"]
     masks = tb["sp_mask"].view(np.uint32).astype(np.int64)

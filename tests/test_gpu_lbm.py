@@ -430,3 +430,63 @@ def test_guo_force_fused_step_matches_oracle(kind):
         worst = max(worst, max(np.abs(macro[a].cpu().numpy() - mw[a]).max() for a in (1, 2, 3)) / speed)
     assert worst <= 1e-5, worst
     assert rel(eng.materialize().cpu().numpy().reshape(19, -1), f) <= 1e-5
+
+
+@pytest.mark.parametrize("layout", ["dense", "compact"])
+@pytest.mark.parametrize("case", ["config1", "pack"])
+def test_compressed_link_records_same_bits(case, layout):
+    """The step forms w2 = 1 - w1 (exact for the reference's weights) and reads each
+    publication's B slot from its code; the explicit w2 and ff_idx arrays give the
+    same bits (MRT, 41 steps, macros and populations)."""
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    force, bcs, per = None, (), (1, 1, 1)
+    if case == "config1":
+        c = cases.config1()
+        per = (0, 0, 0)
+        bcs = [dict(kind="zou_he", axis=0, plane=0, interior=1, bc_type=0, u_in=0.05, rho_out=1.0, sign=1),
+               dict(kind="convective", axis=0, plane=c["nx"] - 1, interior=c["nx"] - 2, unknown_cax=-1,
+                    rho_out=1.0)]
+    else:
+        c = cases.periodic_pack()
+        force = (1e-5, 0.0, 0.0)
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    N = nx * ny * nz
+    om = 1.0 / 0.6
+    out = {}
+    for compress in (True, False):
+        eng = AAEngine(nx, ny, nz, c["solid_flow"], periodic=per, collision="mrt", omega=om, S=cases.mrt_S(om),
+                       force=force, links=c["links"], bcs=bcs, device=DEV, layout=layout, compress_links=compress)
+        assert eng.w2_derived == compress and eng.ff_packed == compress
+        eng.load(torch.as_tensor(np.broadcast_to(olbm.W32[:, None], (19, N)).copy(), device=DEV))
+        macro = [torch.empty(N, device=DEV) for _ in range(4)]
+        for _ in range(41):
+            eng.step(*macro)
+        out[compress] = [m.cpu().numpy() for m in macro] + [eng.materialize().cpu().numpy()]
+        del eng
+    for a, b in zip(out[True], out[False]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_stored_second_weight_against_oracle():
+    """Bouzidi weights with w2 != 1 - w1 (each w2 scaled by 0.999) keep the explicit w2
+    array; the fused step follows the FP32 oracle with the same weights."""
+    from fluxcore3d_amd.solver.aa_engine import AAEngine
+    c = cases.periodic_pack()
+    links = dict(c["links"])
+    links["w2"] = (np.asarray(links["w2"], np.float32) * np.float32(0.999)).astype(np.float32)
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    N = nx * ny * nz
+    om = 1.0 / 0.6
+    eng = AAEngine(nx, ny, nz, c["solid_flow"], periodic=(1, 1, 1), collision="bgk", omega=om,
+                   force=(1e-5, 0.0, 0.0), links=links, device=DEV)
+    assert not eng.w2_derived
+    eng.load(torch.as_tensor(np.broadcast_to(olbm.W32[:, None], (19, N)).copy(), device=DEV))
+    macro = [torch.empty(N, device=DEV) for _ in range(4)]
+    f = np.broadcast_to(olbm.W32[:, None], (19, N)).copy()
+    for _ in range(30):
+        eng.step(*macro)
+        f, rho, u, v, w = olbm.step(f, c["solid_flow"], nx, ny, nz, (1, 1, 1), "bgk", om, 1.0, None, 0.0, links,
+                                    force=(1e-5, 0.0, 0.0))
+    assert rel(eng.materialize().cpu().numpy().reshape(19, -1), f) <= 1e-5
+    assert np.abs(macro[1].cpu().numpy() - u).max() / np.abs(u).max() <= 1e-5
+    assert np.abs(u).max() > 1e-4
