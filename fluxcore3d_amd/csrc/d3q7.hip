@@ -64,14 +64,30 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
   const int64_t N = (int64_t)nx * ny * nz;
   const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (x >= N) return;
-  const uint8_t fl = flag[x];
-  const uint8_t kind = fl & G_KIND;
-  if (kind == G_WALL) return;
   // N < 2^31 (checked by the entry point): 32-bit index arithmetic
   const unsigned nynz = (unsigned)ny * (unsigned)nz, ux_ = (unsigned)x;
   const int i = (int)(ux_ / nynz);
   const unsigned rem = ux_ - (unsigned)i * nynz;
   const int j = (int)(rem / (unsigned)nz), k = (int)(rem - (unsigned)j * (unsigned)nz);
+  // The node's own flag, its seven pulls from x - c_q (or, outside a non-periodic face,
+  // from x itself), the neighbours' flag bytes and the velocity are all valid addresses
+  // whatever the node is, so they issue together before the first wait; the node's kind
+  // then decides what is used (walls and solids discard the pulls, few nodes).
+  const uint8_t fl = flag[x];
+  int64_t src_node[7];
+  uint8_t nk[7];
+  float g[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    int a = i, b = j, c = k;
+    const bool in = nb7(i, -cx7(q), nx, px, a) && nb7(j, -cy7(q), ny, py, b) && nb7(k, -cz7(q), nz, pz, c);
+    src_node[q] = in ? ((int64_t)a * ny + b) * nz + c : x;
+    g[q] = __builtin_nontemporal_load(gin + q * N + src_node[q]);
+    nk[q] = in ? (uint8_t)(flag[src_node[q]] & G_KIND) : G_FLUID;
+  }
+  const float u0 = u[x], v0 = v[x], w0 = w[x];
+  const uint8_t kind = fl & G_KIND;
+  if (kind == G_WALL) return;
   if (kind == G_SOLID) {
     // energy the fluid neighbours hand to this solid node in this step, formed from the
     // same operands and in the same order as the neighbours' anti-bounce-back inputs
@@ -96,33 +112,19 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
   }
   // a conducting node carried on the lattice: no velocity, its own relaxation rate
   const bool conj = kind == G_CONJ;
-  const float ux = conj ? 0.0f : u[x], uy = conj ? 0.0f : v[x], uz = conj ? 0.0f : w[x];
+  const float ux = conj ? 0.0f : u0, uy = conj ? 0.0f : v0, uz = conj ? 0.0f : w0;
   const float om = conj ? omega_s : omega;
-  float g[7];
   if (fl & G_INLET) {
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
       const float cu = (float)cx7(q) * ux + (float)cy7(q) * uy + (float)cz7(q) * uz;
-      g[q] = w7(q) * T_in * (1.0f + 4.0f * cu);
-      gout[q * N + x] = g[q];
+      const float gi = w7(q) * T_in * (1.0f + 4.0f * cu);
+      gout[q * N + x] = gi;
     }
     T_out[x] = T_in;
     return;
   }
-  // Every input is first read as a plain pull from x - c_q (or, outside a non-periodic
-  // face, from x itself): those addresses are valid whatever the neighbour is, so the
-  // loads issue together with the neighbours' flag bytes instead of after them. Inputs
-  // from walls and solids are then replaced (few nodes).
-  int64_t src_node[7];
-  uint8_t nk[7];
-#pragma unroll
-  for (int q = 0; q < 7; ++q) {
-    int a = i, b = j, c = k;
-    const bool in = nb7(i, -cx7(q), nx, px, a) && nb7(j, -cy7(q), ny, py, b) && nb7(k, -cz7(q), nz, pz, c);
-    src_node[q] = in ? ((int64_t)a * ny + b) * nz + c : x;
-    g[q] = __builtin_nontemporal_load(gin + q * N + src_node[q]);
-    nk[q] = in ? (uint8_t)(flag[src_node[q]] & G_KIND) : G_FLUID;
-  }
+  // inputs from walls and solids are replaced
 #pragma unroll
   for (int q = 1; q < 7; ++q) {
     if (nk[q] == G_WALL) {
