@@ -419,6 +419,9 @@ __device__ __forceinline__ void collide_fast(const Collision& c, float f[19], fl
 // constants overwrite tile inputs; new A values and second-node publications
 // are read from the post-collision tile. No per-lane loop over directions.
 constexpr int kWaves = kBlock / 64;
+// Plane kernels (inlet/outlet conditions) run one wavefront per block: a plane holds
+// tens of thousands of nodes, and single-wavefront blocks spread them over every CU.
+constexpr int kPlaneBlock = 64;
 constexpr int kCodeV = 1 << 16, kCodeOvr = 1 << 17;
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -690,8 +693,18 @@ __global__ __launch_bounds__(kBlock) void aa_solid_macros_kernel(const uint8_t* 
 
 // ---- helpers for kernels that visit arbitrary nodes -------------------------
 __device__ __forceinline__ int64_t special_index_scan(const Tables& t, int64_t x) {
+  // special nodes before x in its 64-node group: the group's whole 8-byte words of
+  // flags below x (every byte of them lies below x, inside the table) by a popcount
+  // of their special bits, the rest byte by byte
   int64_t s = t.wave_base[x >> 6];
-  for (int64_t p = x & ~(int64_t)63; p < x; ++p) s += (t.nflag[p] & F_SPECIAL) != 0;
+  const int64_t base = x & ~(int64_t)63;
+  const int n = (int)(x - base);
+  int p = 0;
+  if ((reinterpret_cast<uintptr_t>(t.nflag) & 7u) == 0u) {
+    const uint64_t* wd = reinterpret_cast<const uint64_t*>(t.nflag + base);
+    for (; p + 8 <= n; p += 8) s += __popcll((wd[p >> 3] >> 1) & 0x0101010101010101ull);  // F_SPECIAL = bit 1
+  }
+  for (; p < n; ++p) s += (t.nflag[base + p] & F_SPECIAL) != 0;
   return s;
 }
 
@@ -797,11 +810,11 @@ __device__ __forceinline__ float qdiag(int c) { return c != 0 ? 0.66666667f : -0
 // Regularised Zou/He on the AA state (same node update as zou_he_kernel in
 // lbm_ref.hip, lbm_kernels.cu:791-969).
 template <typename ST>
-__global__ __launch_bounds__(kBlock) void aa_zou_he_kernel(ST* __restrict__ pop, Grid g, Tables t, int parity,
-                                                           int first, int bc, int axis, int plane, int interior,
-                                                           int bc_type, float u_in, float rho_out, int sign, int n1,
-                                                           int n2) {
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kPlaneBlock) void aa_zou_he_kernel(ST* __restrict__ pop, Grid g, Tables t, int parity,
+                                                                int first, int bc, int axis, int plane, int interior,
+                                                                int bc_type, float u_in, float rho_out, int sign,
+                                                                int n1, int n2) {
+  const int tid = blockIdx.x * kPlaneBlock + threadIdx.x;
   if (tid >= n1 * n2) return;
   const int64_t fb = plane_flat(g, axis, plane, tid / n2, tid % n2);
   if (t.nflag[fb] & F_SOLID) return;
@@ -860,12 +873,11 @@ __global__ __launch_bounds__(kBlock) void aa_zou_he_kernel(ST* __restrict__ pop,
 // Convective outlet on the AA state, pass 1 (copy + partial sums) and pass 2
 // (mass correction); same semantics as convective_pass1/2 in lbm_ref.hip.
 template <typename ST>
-__global__ __launch_bounds__(kBlock) void aa_convective_pass1(ST* __restrict__ pop, Grid g, Tables t, int parity,
-                                                              int first, int bc, int axis, int outlet, int interior,
-                                                              int unknown_cax, int n1, int n2,
-                                                              double* __restrict__ partial) {
-  __shared__ double s_r[kBlock / 64], s_c[kBlock / 64];
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass1(ST* __restrict__ pop, Grid g, Tables t,
+                                                                   int parity, int first, int bc, int axis,
+                                                                   int outlet, int interior, int unknown_cax, int n1,
+                                                                   int n2, double* __restrict__ partial) {
+  const int tid = blockIdx.x * kPlaneBlock + threadIdx.x;
   double r = 0.0, cnt = 0.0;
   if (tid < n1 * n2) {
     const int64_t fo = plane_flat(g, axis, outlet, tid / n2, tid % n2);
@@ -888,32 +900,31 @@ __global__ __launch_bounds__(kBlock) void aa_convective_pass1(ST* __restrict__ p
     r += __shfl_down(r, off, 64);
     cnt += __shfl_down(cnt, off, 64);
   }
-  if ((threadIdx.x & 63) == 0) { s_r[threadIdx.x >> 6] = r; s_c[threadIdx.x >> 6] = cnt; }
-  __syncthreads();
   if (threadIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < kBlock / 64; ++w) { a += s_r[w]; b += s_c[w]; }
-    partial[2 * blockIdx.x] = a;
-    partial[2 * blockIdx.x + 1] = b;
+    partial[2 * blockIdx.x] = r;
+    partial[2 * blockIdx.x + 1] = cnt;
   }
 }
 
 template <typename ST>
-__global__ __launch_bounds__(kBlock) void aa_convective_pass2(ST* __restrict__ pop, Grid g, Tables t, int parity,
-                                                              int axis, int outlet, int n1, int n2, float rho_out,
-                                                              const double* __restrict__ partial, int nparts) {
-  __shared__ float s_corr;
-  __shared__ int s_skip;
-  if (threadIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < nparts; ++w) { a += partial[2 * w]; b += partial[2 * w + 1]; }
-    const float cnt = (float)b, sum = (float)a;
-    s_skip = (cnt < 0.5f || sum < 1e-10f);
-    s_corr = s_skip ? 1.f : rho_out * cnt / sum;
+__global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass2(ST* __restrict__ pop, Grid g, Tables t,
+                                                                   int parity, int axis, int outlet, int n1, int n2,
+                                                                   float rho_out, const double* __restrict__ partial,
+                                                                   int nparts) {
+  // every block folds the per-block partials the same way (lane l sums partials
+  // l, l + 64, ..., then a fixed shuffle tree), so all blocks apply the same factor
+  double a = 0.0, b = 0.0;
+  for (int w = threadIdx.x; w < nparts; w += 64) { a += partial[2 * w]; b += partial[2 * w + 1]; }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
   }
-  __syncthreads();
-  if (s_skip) return;
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  a = __shfl(a, 0, 64);
+  b = __shfl(b, 0, 64);
+  const float cnt = (float)b, sum = (float)a;
+  if (cnt < 0.5f || sum < 1e-10f) return;
+  const float corr = rho_out * cnt / sum;
+  const int tid = blockIdx.x * kPlaneBlock + threadIdx.x;
   if (tid >= n1 * n2) return;
   const int64_t fo = plane_flat(g, axis, outlet, tid / n2, tid % n2);
   if (t.nflag[fo] & F_SOLID) return;
@@ -921,7 +932,7 @@ __global__ __launch_bounds__(kBlock) void aa_convective_pass2(ST* __restrict__ p
   const int i = (int)(fo / nynz), j = (int)((fo / g.nz) % g.ny), k = (int)(fo % g.nz);
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const float v = load_plain(pop, g, t, parity, i, j, k, q) * s_corr;
+    const float v = load_plain(pop, g, t, parity, i, j, k, q) * corr;
     store_plain(pop, g, t, parity, i, j, k, q, v);
   }
 }
@@ -1101,9 +1112,9 @@ int fc_aa_zou_he(void* pop, int storage, int nx, int ny, int nz, int px, int py,
   FC_REQUIRE(plane_idx >= 0 && plane_idx < n_ax && interior_idx >= 0 && interior_idx < n_ax,
              "aa_zou_he: plane index out of range");
   const int n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
-  const unsigned grid = grid_for((int64_t)n1 * n2, kBlock);
+  const unsigned grid = grid_for((int64_t)n1 * n2, kPlaneBlock);
 #define FC_ZH(T)                                                                                                  \
-  hipLaunchKernelGGL(aa_zou_he_kernel<T>, dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, parity, first,      \
+  hipLaunchKernelGGL(aa_zou_he_kernel<T>, dim3(grid), dim3(kPlaneBlock), 0, stream, (T*)pop, g, t, parity, first,      \
                      bc_order, axis, plane_idx, interior_idx, bc_type, u_in_lat, rho_out_lat, sign, n1, n2)
   switch (storage) {
     case 0: FC_ZH(float); break;
@@ -1117,7 +1128,7 @@ int fc_aa_zou_he(void* pop, int storage, int nx, int ny, int nz, int px, int py,
 
 size_t fc_aa_convective_workspace_bytes(int n1, int n2) {
   if (n1 <= 0 || n2 <= 0) return 0;
-  return sizeof(double) * 2 * (size_t)grid_for((int64_t)n1 * n2, kBlock);
+  return sizeof(double) * 2 * (size_t)grid_for((int64_t)n1 * n2, kPlaneBlock);
 }
 
 int fc_aa_convective(void* pop, int storage, int nx, int ny, int nz, int px, int py, int pz,
@@ -1133,12 +1144,12 @@ int fc_aa_convective(void* pop, int storage, int nx, int ny, int nz, int px, int
   const int n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
   const size_t need = fc_aa_convective_workspace_bytes(n1, n2);
   FC_REQUIRE(workspace && ws_bytes >= need, "aa_convective: workspace of %zu bytes required", need);
-  const unsigned grid = grid_for((int64_t)n1 * n2, kBlock);
+  const unsigned grid = grid_for((int64_t)n1 * n2, kPlaneBlock);
 #define FC_CV(T)                                                                                                 \
-  hipLaunchKernelGGL(aa_convective_pass1<T>, dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, parity, first,  \
+  hipLaunchKernelGGL(aa_convective_pass1<T>, dim3(grid), dim3(kPlaneBlock), 0, stream, (T*)pop, g, t, parity, first,  \
                      bc_order, axis, outlet_idx, interior_idx, unknown_cax, n1, n2, (double*)workspace);          \
   if (int rc = check_launch("aa_convective/pass1")) return rc;                                                    \
-  hipLaunchKernelGGL(aa_convective_pass2<T>, dim3(grid), dim3(kBlock), 0, stream, (T*)pop, g, t, parity, axis,    \
+  hipLaunchKernelGGL(aa_convective_pass2<T>, dim3(grid), dim3(kPlaneBlock), 0, stream, (T*)pop, g, t, parity, axis,    \
                      outlet_idx, n1, n2, rho_out_lat, (const double*)workspace, (int)grid)
   switch (storage) {
     case 0: FC_CV(float); break;
