@@ -279,8 +279,8 @@ def cht_leg(device):
     return out
 
 
-D3Q7_FLUID_BYTES = 7 * 4 * 2 + 3 * 4 + 1 + 4  # populations in and out, velocity, flag, temperature out
-D3Q7_SOLID_BYTES = 1 + 4 + 4  # flag, solid temperature in, temperature out
+D3Q7_FLUID_BYTES = 7 * 4 * 2 + 3 * 4 + 2 + 4  # populations in and out, velocity, flag and neighbour mask, T out
+D3Q7_SOLID_BYTES = 2 + 4 + 4  # flag and neighbour mask, solid temperature in, temperature out
 D3Q7_INTERFACE_BYTES = 8  # energy accumulator read and written (solid nodes next to fluid)
 
 
@@ -362,7 +362,7 @@ def coupled_leg(device, steps, warmup, every=50):
     torch.cuda.synchronize()
     d3q7_ms = ev0.elapsed_time(ev1) / k
     d3q7_bytes = (n_fluid * D3Q7_FLUID_BYTES + n_solid * D3Q7_SOLID_BYTES + n_iface * D3Q7_INTERFACE_BYTES
-                  + n_wall * 1)
+                  + n_wall * 2)
     ev0.record()
     info = cht.solid.step(cht.T_solid, cht.acc, cht.src)
     ev1.record()

@@ -56,6 +56,7 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
                                                            float* __restrict__ T_out, const float* __restrict__ u,
                                                            const float* __restrict__ v, const float* __restrict__ w,
                                                            const uint8_t* __restrict__ flag,
+                                                           const uint8_t* __restrict__ nbmask,
                                                            const float* __restrict__ T_solid,
                                                            const float* __restrict__ Q, float* __restrict__ acc,
                                                            int nx, int ny, int nz,
@@ -74,6 +75,9 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
   // whatever the node is, so they issue together before the first wait; the node's kind
   // then decides what is used (walls and solids discard the pulls, few nodes).
   const uint8_t fl = flag[x];
+  // bit q - 1 of nbmask: x - c_q is a wall or a solid node inside the domain; only
+  // those neighbours' flags are read (without the mask, all six are)
+  const uint32_t nbm = nbmask ? (uint32_t)nbmask[x] : 0x3Fu;
   int64_t src_node[7];
   uint8_t nk[7];
   float g[7];
@@ -83,7 +87,8 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
     const bool in = nb7(i, -cx7(q), nx, px, a) && nb7(j, -cy7(q), ny, py, b) && nb7(k, -cz7(q), nz, pz, c);
     src_node[q] = in ? ((int64_t)a * ny + b) * nz + c : x;
     g[q] = __builtin_nontemporal_load(gin + q * N + src_node[q]);
-    nk[q] = in ? (uint8_t)(flag[src_node[q]] & G_KIND) : G_FLUID;
+    nk[q] = G_FLUID;
+    if (q > 0 && in && ((nbm >> (q - 1)) & 1u)) nk[q] = (uint8_t)(flag[src_node[q]] & G_KIND);
   }
   const float u0 = u[x], v0 = v[x], w0 = w[x];
   const uint8_t kind = fl & G_KIND;
@@ -228,8 +233,8 @@ int make_box(Box& b, int nx, int ny, int nz, int ox, int oy, int oz, int bx, int
 extern "C" {
 
 int fc_d3q7_step(const float* g_in, float* g_out, float* T_out, const float* u, const float* v, const float* w,
-                 const uint8_t* flag, const float* T_solid, const float* Q, float* acc, int nx, int ny, int nz, int px, int py,
-                 int pz, float omega, float omega_s, float T_in, hipStream_t stream) {
+                 const uint8_t* flag, const uint8_t* nbmask, const float* T_solid, const float* Q, float* acc, int nx,
+                 int ny, int nz, int px, int py, int pz, float omega, float omega_s, float T_in, hipStream_t stream) {
   FC_REQUIRE(nx > 0 && ny > 0 && nz > 0, "d3q7_step: grid extents must be positive");
   FC_REQUIRE(g_in && g_out && T_out && u && v && w && flag && T_solid, "d3q7_step: null pointer argument");
   FC_REQUIRE(g_in != g_out, "d3q7_step: input and output populations must be different buffers");
@@ -238,7 +243,7 @@ int fc_d3q7_step(const float* g_in, float* g_out, float* T_out, const float* u, 
   const int64_t N = (int64_t)nx * ny * nz;
   FC_REQUIRE(N < ((int64_t)1 << 31), "d3q7_step: the grid must hold fewer than 2^31 nodes");
   hipLaunchKernelGGL(d3q7_step_kernel, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, stream, g_in, g_out, T_out, u, v,
-                     w, flag, T_solid, Q, acc, nx, ny, nz, px != 0, py != 0, pz != 0, omega, omega_s, T_in);
+                     w, flag, nbmask, T_solid, Q, acc, nx, ny, nz, px != 0, py != 0, pz != 0, omega, omega_s, T_in);
   return check_launch("d3q7_step");
 }
 

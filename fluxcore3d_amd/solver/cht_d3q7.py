@@ -49,13 +49,31 @@ Tensor = torch.Tensor
 P, I32, I64, F32, F64 = _lib.P, _lib.I32, _lib.I64, _lib.F32, _lib.F64
 
 _lib.register({
-    "fc_d3q7_step": (I32, [P, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, F32, F32, F32, P]),
+    "fc_d3q7_step": (I32, [P, P, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, F32, F32, F32, P]),
     "fc_d3q7_init": (I32, [P, P, P, P, P, I64, P]),
     "fc_cht_solid_rhs": (I32, [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, F64, F64, P, P, P]),
     "fc_cht_solid_store": (I32, [P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, P]),
 })
 
 FLUID, SOLID, WALL, CONJ, INLET = 0, 1, 2, 3, 16
+
+
+def neighbour_mask(flag: Tensor, periodic) -> Tensor:
+    """uint8 per node: bit q - 1 set when x - c_q (q = 1..6: +x, -x, +y, -y, +z, -z)
+    lies inside the domain (wrapped on periodic axes) and is a wall or a conducting
+    solid solved apart. The D3Q7 step reads only those neighbours' flags."""
+    kind = flag.to(torch.int32) & 3
+    sw = ((kind == SOLID) | (kind == WALL)).to(torch.int32)
+    out = torch.zeros_like(sw)
+    for q, (ax, c) in enumerate(((0, 1), (0, -1), (1, 1), (1, -1), (2, 1), (2, -1))):
+        # value at x of the neighbour x - c_q: shift the field by +c_q along the axis
+        nb = torch.roll(sw, shifts=c, dims=ax)
+        if not periodic[ax]:
+            idx = [slice(None)] * 3
+            idx[ax] = 0 if c > 0 else -1  # x - c_q leaves the domain there
+            nb[tuple(idx)] = 0
+        out |= nb << q
+    return out.to(torch.uint8).contiguous()
 
 
 def _stream(t: Tensor):
@@ -86,7 +104,8 @@ def node_flags(conducting: Tensor, wall: Optional[Tensor] = None, inlet: Optiona
 
 class D3Q7Transport:
     """Fluid temperature populations (two [7][N] FP32 buffers) and the step launch.
-    ``omega_s`` relaxes kind-3 nodes (conducting solid carried on the lattice)."""
+    ``omega_s`` relaxes kind-3 nodes (conducting solid carried on the lattice). The node
+    flags are fixed for the transport's life: the neighbour mask is built from them once."""
 
     def __init__(self, flag: Tensor, omega: float, periodic=(0, 0, 0), T_in: float = 0.0,
                  omega_s: Optional[float] = None):
@@ -106,6 +125,7 @@ class D3Q7Transport:
         self.periodic = tuple(int(bool(p)) for p in periodic)
         self.T_in = float(T_in)
         dev = flag.device
+        self.nbmask = neighbour_mask(self.flag, self.periodic)
         self.g = [torch.zeros((7, self.N), dtype=torch.float32, device=dev) for _ in range(2)]
         self.T = torch.zeros(self.N, dtype=torch.float32, device=dev)
         self.cur = 0
@@ -132,7 +152,8 @@ class D3Q7Transport:
         nx, ny, nz = self.shape
         src, dst = self.g[self.cur], self.g[1 - self.cur]
         _lib.call("fc_d3q7_step", src.data_ptr(), dst.data_ptr(), self.T.data_ptr(), *[t.data_ptr() for t in uu],
-                  self.flag.data_ptr(), ts.data_ptr(), _ptr(q), _ptr(a), nx, ny, nz, *self.periodic, self.omega,
+                  self.flag.data_ptr(), self.nbmask.data_ptr(), ts.data_ptr(), _ptr(q), _ptr(a), nx, ny, nz,
+                  *self.periodic, self.omega,
                   self.omega_s, self.T_in, _stream(self.T))
         self.cur = 1 - self.cur
         return self.T

@@ -394,3 +394,32 @@ def test_gyroid_flow_and_heat_on_96_cube_are_the_tiled_period():
     for a, b in zip(out[48][:4], out[96][:4]):
         assert np.array_equal(np.tile(a, (2, 2, 2)), b)
     assert np.array_equal(np.tile(out[48][4], (1, 2, 2, 2)), out[96][4])
+
+
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 0, 1), (1, 1, 1)])
+def test_neighbour_mask_against_brute_force(periodic):
+    """The per-node mask the D3Q7 step reads (bit q - 1: x - c_q inside the domain and a
+    wall or a conducting solid) against a loop over every node and direction."""
+    import torch
+    from fluxcore3d_amd.solver.cht_d3q7 import neighbour_mask, FLUID, SOLID, WALL, CONJ, INLET
+
+    rng = np.random.default_rng(7)
+    shape = (5, 6, 7)
+    fl = rng.choice(np.array([FLUID, SOLID, WALL, CONJ, INLET], np.uint8), size=shape)
+    got = neighbour_mask(torch.from_numpy(fl), periodic).numpy()
+    c = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    want = np.zeros(shape, np.uint8)
+    for i, j, k in np.ndindex(*shape):
+        for q, d in enumerate(c):
+            p = [i - d[0], j - d[1], k - d[2]]
+            inside = True
+            for a in range(3):
+                if not 0 <= p[a] < shape[a]:
+                    if periodic[a]:
+                        p[a] %= shape[a]
+                    else:
+                        inside = False
+            if inside and (fl[tuple(p)] & 3) in (SOLID, WALL):
+                want[i, j, k] |= 1 << q
+    assert got.dtype == np.uint8
+    np.testing.assert_array_equal(got, want)
