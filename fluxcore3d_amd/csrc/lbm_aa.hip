@@ -482,6 +482,7 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
   // wrote input q (q = 0: the node itself); x + c_q is x - c_opp(q)
   uint32_t pres = 0;
   uint32_t nb[19];
+  const bool own_lines = (g.gx | g.gy | g.gz) == 0 && t.lane_nodes != nullptr;  // lane r owns slot r
 #pragma unroll
   for (int q = 0; q < 19; ++q) nb[q] = 0;
   if (live) {
@@ -518,17 +519,22 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
       }
       // Nodes with slots load their inputs (an input written by a node without
       // slots is w_q after the first step); the others have nothing to load.
-      // Temporal accesses: a wavefront's slots start anywhere in a cache line, so
-      // lines are shared with the neighbouring wavefronts and are merged in L2.
-      if (pres & 1u) {
+      // Temporal accesses where a wavefront's slots can start anywhere in a cache
+      // line (odd steps; even steps with ghost slots among the lanes' slots): such
+      // lines are shared with the neighbouring wavefronts and merged in L2. Without
+      // a ghost layer lane r owns slot r, so on even steps a wavefront's own slots
+      // are whole lines that no other wavefront touches: non-temporal.
+      if ((pres & 1u) && parity == 0) {
+        if (own_lines) {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) {
-          const bool from = ((pres >> q) & 1u) != 0u;
-          float v = WQ(q);
-          if (from || (first && parity == 0))
-            v = Store<ST>::ld(at(q + parity * (OPP(q) - q), parity ? nb[q] : nb[0]), q);
-          f[q] = v;
+          for (int q = 0; q < 19; ++q) f[q] = (((pres >> q) & 1u) || first) ? Store<ST>::ld_nt(at(q, nb[0]), q) : WQ(q);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 19; ++q) f[q] = (((pres >> q) & 1u) || first) ? Store<ST>::ld(at(q, nb[0]), q) : WQ(q);
         }
+      } else if (pres & 1u) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) f[q] = ((pres >> q) & 1u) ? Store<ST>::ld(at(OPP(q), nb[q]), q) : WQ(q);
       }
     }
   }
@@ -619,12 +625,20 @@ __global__ __launch_bounds__(kBlock, (KIND == 3 ? 1 : 4)) void aa_step_kernel(ST
 #pragma unroll
       for (int q = 0; q < 19; ++q)
         Store<ST>::st_nt(at(OPP(q) + parity * (q - OPP(q)), pslot(CX(q), CY(q), CZ(q))), f[q], q);
+    } else if ((pres & 1u) && parity == 0) {
+      // even: the node's own slots
+      if (own_lines) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) Store<ST>::st_nt(at(OPP(q), nb[0]), f[q], q);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) Store<ST>::st(at(OPP(q), nb[0]), f[q], q);
+      }
     } else if (pres & 1u) {
-      // even: the node's own slots; odd: slots of x + c_q, where that node has slots
+      // odd: slots of x + c_q, where that node has slots
 #pragma unroll
       for (int q = 0; q < 19; ++q)
-        if (parity == 0 || ((pres >> OPP(q)) & 1u))
-          Store<ST>::st(at(OPP(q) + parity * (q - OPP(q)), parity ? nb[OPP(q)] : nb[0]), f[q], q);
+        if ((pres >> OPP(q)) & 1u) Store<ST>::st(at(q, nb[OPP(q)]), f[q], q);
     }
     // copy-self on non-periodic faces through the ghost layer
     if (fluid && ((g.gx && (i == 0 || i == g.nx - 1)) || (g.gy && (j == 0 || j == g.ny - 1)) ||

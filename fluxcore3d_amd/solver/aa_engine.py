@@ -31,6 +31,10 @@ from .d3q19 import C as C_NP, OPP as OPP_NP, W as W_NP
 STORAGE_CODES = {"fp32": 0, "fp16": 1, "bf16": 2}
 STORAGE_DTYPES = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}
 COLLISION_CODES = {"bgk": 0, "trt": 1, "mrt": 2, "mrt_smag": 3}
+# Compact storage: the stride between the 19 direction arrays is the slot count rounded
+# up to this many slots, so that every direction array starts on a 256-byte boundary
+# (FP32) and a wavefront's 64 consecutive own slots fill whole 128-byte lines.
+SLOT_ALIGN = 64
 
 F_SOLID, F_SPECIAL, F_BC_SHIFT = 1, 2, 3
 
@@ -350,8 +354,12 @@ class AAEngine:
                           compact=self.lanes == "fluid", lane_mask=lane_mask)
         self.n_links, self.n_v, self.n_special = tb["n_links"], tb["n_v"], tb["n_special"]
         if sb is not None:
-            tb.update(row_meta=sb["row_meta"], wpr=sb["wpr"], n_store=sb["n_store"])
-            self._n_store = sb["n_store"]
+            stride = -(-sb["n_store"] // SLOT_ALIGN) * SLOT_ALIGN
+            dense = int(_lib.call("fc_aa_slot_count", self.nx, self.ny, self.nz, *[int(p) for p in self.periodic]))
+            if stride > dense:  # tiny grids: no room to round up
+                stride = sb["n_store"]
+            tb.update(row_meta=sb["row_meta"], wpr=sb["wpr"], n_store=stride)
+            self._n_store = stride
         else:
             tb.update(row_meta=None, wpr=0, n_store=0)
         self._upload(tb)

@@ -110,7 +110,8 @@ typedef struct fc_aa_tables {
    * slots stream w_q. */
   const int32_t *row_meta;  /* [PX * PY * wpr][2] */
   int wpr;                  /* words per row, (PZ + 31) / 32 */
-  int64_t n_store;          /* nodes owning slots: the population arrays are [19][n_store] */
+  int64_t n_store;          /* stride of the 19 population arrays ([19][n_store]): at least the number of
+                               nodes owning slots, at most the dense slot count */
 } fc_aa_tables;
 
 typedef struct fc_aa_collision {
