@@ -40,7 +40,16 @@ struct Coef {
   // face form (fc_common.hpp face_hi/face_lo); the kernels templated on FACE
   // read these three arrays instead of the six per-cell ones and diagL
   const float *fx, *fy, *fz;
+  // optional [nx][ny] row table: 0 where every cell of the z-row (i, j) has zero
+  // coefficients and a zero right-hand side, so that its iterate, residual and
+  // search directions stay exactly 0; the stencil and vector kernels skip such rows
+  const uint8_t* rows;
 };
+
+// false for a skipped row (its outputs keep the zeros they were allocated with)
+__device__ __forceinline__ bool row_live(const Coef& c, int ny, int i, int j) {
+  return c.rows == nullptr || c.rows[i * ny + j] != 0;
+}
 
 __device__ __forceinline__ int lo_ghost(const Level& g, int this_ax) {
   return (g.ax != this_ax || g.out_idx == 0) ? 1 : 0;
@@ -161,12 +170,13 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float* __restrict__
                                                        int zero_guess) {
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= g.nx * g.ny * g.nz) return;
+  int i, j, k;
+  unflat(g, idx, i, j, k);
+  if (!row_live(c, g.ny, i, j)) return;
   if (zero_guess) {  // x = 0: A x = 0
     x_out[idx] = zero_sweep(c, b, idx);
     return;
   }
-  int i, j, k;
-  unflat(g, idx, i, j, k);
   const float lo = stencil_lo<FACE>(x, c, g, dt, idx, i, j, k);
   st_once(x_out + idx, sweep_update(x[idx], ld_once(c.invD + idx), ld_once(b + idx), lo));
 }
@@ -182,6 +192,7 @@ __global__ __launch_bounds__(kBlock) void sweep2_from_zero_kernel(float* __restr
   if (idx >= g.nx * g.ny * g.nz) return;
   int i, j, k;
   unflat(g, idx, i, j, k);
+  if (!row_live(c, g.ny, i, j)) return;
   const Nbr o = neighbours(g, i, j, k);
   const C7 a = coef_at<FACE>(c, g, idx, i, j, k);
   const float xc = zero_sweep(c, b, idx);
@@ -207,6 +218,11 @@ __global__ __launch_bounds__(kBlock) void residual_restrict_kernel(const float* 
   const int row = blockIdx.x / kchunks;  // coarse (I, J)
   const int K0 = (blockIdx.x - row * kchunks) * 32;
   const int I = row / cy, J = row - I * cy;
+  // the whole block leaves when its four fine rows are skipped rows: the coarse row's
+  // right-hand side is then 0, which the buffer already holds
+  if (!(row_live(c, g.ny, 2 * I, 2 * J) || row_live(c, g.ny, 2 * I, 2 * J + 1) ||
+        row_live(c, g.ny, 2 * I + 1, 2 * J) || row_live(c, g.ny, 2 * I + 1, 2 * J + 1)))
+    return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = 2 * I + (wave >> 1), j = 2 * J + (wave & 1), k = 2 * K0 + lane;
   float r = 0.f;
@@ -262,13 +278,15 @@ __device__ __forceinline__ float corrected(float x, float e, float alpha, float 
 __global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __restrict__ xc, float* __restrict__ x,
                                                                  const float* __restrict__ mask, int nx, int ny,
                                                                  int nz, int cx, int cy, int cz, float sx, float sy,
-                                                                 float sz, float alpha) {
+                                                                 float sz, float alpha,
+                                                                 const uint8_t* __restrict__ rows) {
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= nx * ny * nz) return;
   const unsigned nynz = (unsigned)ny * (unsigned)nz, u = (unsigned)idx;
   const int i = (int)(u / nynz);
   const unsigned rem = u - (unsigned)i * nynz;
   const int j = (int)(rem / (unsigned)nz), k = (int)(rem - (unsigned)j * (unsigned)nz);
+  if (rows && !rows[i * ny + j]) return;
   auto X = [&](int a, int b2, int c2) { return xc[(a * cy + b2) * cz + c2]; };
   x[idx] = corrected(x[idx], prolong_at(X, i, j, k, cx, cy, cz, sx, sy, sz), alpha, mask[idx]);
 }
@@ -278,7 +296,8 @@ __global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __
 __global__ __launch_bounds__(kBlock) void prolong_correct2_kernel(const float* __restrict__ xc, float* __restrict__ x,
                                                                   const float* __restrict__ mask, int nx, int ny,
                                                                   int nz, int cx, int cy, int cz, float sx, float sy,
-                                                                  float sz, float alpha) {
+                                                                  float sz, float alpha,
+                                                                  const uint8_t* __restrict__ rows) {
   const int hz = nz >> 1;
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= nx * ny * hz) return;
@@ -286,6 +305,7 @@ __global__ __launch_bounds__(kBlock) void prolong_correct2_kernel(const float* _
   const int i = (int)(u / nyhz);
   const unsigned rem = u - (unsigned)i * nyhz;
   const int j = (int)(rem / (unsigned)hz), k = 2 * (int)(rem - (unsigned)j * (unsigned)hz);
+  if (rows && !rows[i * ny + j]) return;
   const int idx = (i * ny + j) * nz + k;
   auto X = [&](int a, int b2, int c2) { return xc[(a * cy + b2) * cz + c2]; };
   typedef float f2 __attribute__((ext_vector_type(2)));
@@ -562,9 +582,9 @@ __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restr
                                                            Coef c, Level g, double dt, double* __restrict__ partial) {
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (idx < g.nx * g.ny * g.nz) {
-    int i, j, k;
-    unflat(g, idx, i, j, k);
+  int i = 0, j = 0, k = 0;
+  if (idx < g.nx * g.ny * g.nz) unflat(g, idx, i, j, k);
+  if (idx < g.nx * g.ny * g.nz && row_live(c, g.ny, i, j)) {
     const Nbr o = neighbours(g, i, j, k);
     const C7 q = coef_at<FACE>(c, g, idx, i, j, k);
     const double pc = p[idx];
@@ -608,12 +628,14 @@ __global__ __launch_bounds__(kBlock) void update_xr_kernel(double* __restrict__ 
                                                            float* __restrict__ r32, const double* __restrict__ p,
                                                            const double* __restrict__ Ap,
                                                            const double* __restrict__ scal, int64_t n,
+                                                           const uint8_t* __restrict__ rows, unsigned row_len,
                                                            double* __restrict__ partial) {
   double v[1] = {0.0};
   const double alpha = scal[S_ALPHA];
   // a stopped solve (scal[S_STOP] set) keeps x and r and re-forms the same r.r
   const bool stop = scal[S_STOP] != 0.0;
   for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    if (rows && !rows[(unsigned)idx / row_len]) continue;  // n < 2^31 (checked by the entry point)
     if (stop) {
       const double rn = r[idx];
       v[0] += rn * rn;
@@ -630,18 +652,21 @@ __global__ __launch_bounds__(kBlock) void update_xr_kernel(double* __restrict__ 
 
 // partial of z.Ap_old (z is the FP32 V-cycle output)
 __global__ __launch_bounds__(kBlock) void dot_z_kernel(const float* __restrict__ z, const double* __restrict__ Ap_old,
-                                                       int64_t n, double* __restrict__ partial) {
+                                                       int64_t n, const uint8_t* __restrict__ rows,
+                                                       unsigned row_len, double* __restrict__ partial) {
   double v[1] = {0.0};
   for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock)
-    v[0] += (double)z[idx] * Ap_old[idx];
+    if (!rows || rows[(unsigned)idx / row_len]) v[0] += (double)z[idx] * Ap_old[idx];
   block_partials<1>(v, partial);
 }
 
 // p = z - beta p_old (in place on p); p is kept once the solve has stopped
 __global__ __launch_bounds__(kBlock) void new_dir_kernel(double* __restrict__ p, const float* __restrict__ z,
-                                                         const double* __restrict__ scal, int64_t n, int first) {
+                                                         const double* __restrict__ scal, int64_t n, int first,
+                                                         const uint8_t* __restrict__ rows, unsigned row_len) {
   const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= n || scal[S_STOP] != 0.0) return;
+  if (rows && !rows[(unsigned)idx / row_len]) return;
   p[idx] = first ? (double)z[idx] : (double)z[idx] - scal[S_BETA] * p[idx];
 }
 
@@ -676,6 +701,7 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
   for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
     int i, j, k;
     unflat(g, idx, i, j, k);
+    if (!row_live(c, g.ny, i, j)) continue;
     const Nbr o = neighbours(g, i, j, k);
     const C7 q = coef_at<FACE>(c, g, idx, i, j, k);
     const double xc = x[idx];
@@ -800,6 +826,7 @@ extern "C" {
 struct fc_mg_coef {
   const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *invD;
   const float *fx, *fy, *fz;
+  const uint8_t* rows;
 };
 
 static Coef to_coef(const fc_mg_coef* c) {
@@ -807,6 +834,7 @@ static Coef to_coef(const fc_mg_coef* c) {
   o.aE = c->aE; o.aW = c->aW; o.aN = c->aN; o.aS = c->aS; o.aU = c->aU; o.aD = c->aD; o.diagL = c->diagL;
   o.invD = c->invD;
   o.fx = c->fx; o.fy = c->fy; o.fz = c->fz;
+  o.rows = c->rows;
   return o;
 }
 
@@ -930,7 +958,7 @@ int fc_mg_residual_restrict(const float* x, const float* b, const float* mask, c
 }
 
 int fc_mg_prolong_correct(const float* x_coarse, float* x, const float* mask, int nx, int ny, int nz, int cx, int cy,
-                          int cz, float alpha, hipStream_t stream) {
+                          int cz, float alpha, const uint8_t* rows, hipStream_t stream) {
   FC_REQUIRE(nx > 0 && ny > 0 && nz > 0 && cx > 0 && cy > 0 && cz > 0, "prolong_correct: extents must be positive");
   FC_REQUIRE(x_coarse && x && mask, "prolong_correct: null pointer argument");
   const int64_t n = (int64_t)nx * ny * nz;
@@ -939,10 +967,10 @@ int fc_mg_prolong_correct(const float* x_coarse, float* x, const float* mask, in
   const float sx = (float)cx / (float)nx, sy = (float)cy / (float)ny, sz = (float)cz / (float)nz;
   if (nz % 2 == 0 && (reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(mask)) % 8 == 0)
     hipLaunchKernelGGL(prolong_correct2_kernel, dim3(grid_for(n / 2, kBlock)), dim3(kBlock), 0, stream, x_coarse, x,
-                       mask, nx, ny, nz, cx, cy, cz, sx, sy, sz, alpha);
+                       mask, nx, ny, nz, cx, cy, cz, sx, sy, sz, alpha, rows);
   else
     hipLaunchKernelGGL(prolong_correct_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, x_coarse, x, mask,
-                       nx, ny, nz, cx, cy, cz, sx, sy, sz, alpha);
+                       nx, ny, nz, cx, cy, cz, sx, sy, sz, alpha, rows);
   return check_launch("prolong_correct");
 }
 
@@ -997,11 +1025,14 @@ int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* m
   return check_launch("apply_A64/fold");
 }
 
-int fc_mg_update_xr(double* x, double* r, float* r32, const double* p, const double* Ap, int64_t n, double* partial,
-                    double* scal, hipStream_t stream) {
+int fc_mg_update_xr(double* x, double* r, float* r32, const double* p, const double* Ap, int64_t n,
+                    const uint8_t* rows, int row_len, double* partial, double* scal, hipStream_t stream) {
   FC_REQUIRE(x && r && r32 && p && Ap && partial && scal, "update_xr: null pointer argument");
+  FC_REQUIRE(!rows || (row_len > 0 && n % row_len == 0 && n < ((int64_t)1 << 31)),
+             "update_xr: rows need a positive row length dividing n < 2^31");
   const unsigned gsz = reduce_grid(n);
-  hipLaunchKernelGGL(update_xr_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, r, r32, p, Ap, scal, n, partial);
+  hipLaunchKernelGGL(update_xr_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, r, r32, p, Ap, scal, n, rows,
+                     (unsigned)(rows ? row_len : 1), partial);
   if (int rc = check_launch("update_xr")) return rc;
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, (int)S_RR, 0);
   return check_launch("update_xr/fold");
@@ -1018,18 +1049,22 @@ int fc_solver_stop_check(double* scal, int rr_slot, int stop_slot, double bnorm,
 
 int fc_mg_stop_slot(void) { return S_STOP; }
 
-int fc_mg_new_direction(double* p, const float* z, const double* Ap_old, int64_t n, int first, double* partial,
-                        double* scal, hipStream_t stream) {
+int fc_mg_new_direction(double* p, const float* z, const double* Ap_old, int64_t n, int first, const uint8_t* rows,
+                        int row_len, double* partial, double* scal, hipStream_t stream) {
   FC_REQUIRE(p && z && partial && scal, "new_direction: null pointer argument");
+  FC_REQUIRE(!rows || (row_len > 0 && n % row_len == 0 && n < ((int64_t)1 << 31)),
+             "new_direction: rows need a positive row length dividing n < 2^31");
+  const unsigned rl = (unsigned)(rows ? row_len : 1);
   const unsigned gsz = reduce_grid(n);
   if (!first) {
     FC_REQUIRE(Ap_old, "new_direction: Ap_old required after the first iteration");
-    hipLaunchKernelGGL(dot_z_kernel, dim3(gsz), dim3(kBlock), 0, stream, z, Ap_old, n, partial);
+    hipLaunchKernelGGL(dot_z_kernel, dim3(gsz), dim3(kBlock), 0, stream, z, Ap_old, n, rows, rl, partial);
     if (int rc = check_launch("new_direction/dot")) return rc;
     hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (int)gsz, 1, scal, (int)S_ZAP, 2);
     if (int rc = check_launch("new_direction/fold")) return rc;
   }
-  hipLaunchKernelGGL(new_dir_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, p, z, scal, n, first);
+  hipLaunchKernelGGL(new_dir_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, p, z, scal, n, first, rows,
+                     rl);
   return check_launch("new_direction");
 }
 

@@ -168,7 +168,7 @@ class SolidConduction:
     """Implicit conduction step of the conducting nodes over ``dt`` flow steps."""
 
     def __init__(self, flag: Tensor, k_s: float, rho_cp: float, dt: float, tol: float = 1e-8, max_it: int = 300,
-                 omega: float = 0.8, n_pre: int = 3, n_post: int = 3):
+                 omega: float = 0.8, n_pre: int = 3, n_post: int = 3, skip_empty_rows: bool = True):
         if k_s <= 0 or rho_cp <= 0 or dt <= 0:
             raise ValueError("k_s, rho_cp and dt must be positive")
         self.flag = flag.contiguous()
@@ -205,7 +205,8 @@ class SolidConduction:
         rcp = torch.full(self.box, self.rho_cp, device=dev)
         z = torch.zeros(self.box, dtype=torch.float32, device=dev)
         self.mg = NLevelGeometricMGSolver(dev, torch.float32, flow_axis=2, flow_sign=+1, thermal_dt_scale=1.0,
-                                          omega=omega, n_pre=n_pre, n_post=n_post, coarse_mask="any")
+                                          omega=omega, n_pre=n_pre, n_post=n_post, coarse_mask="any",
+                                          skip_empty_rows=skip_empty_rows)
         self.mg.build_if_needed(k.float(), rcp.float(), z, z, z, box_cond.float(), self.dt)
         self.b = torch.zeros(self.box, dtype=torch.float64, device=dev)
         self.x0 = torch.zeros(self.box, dtype=torch.float64, device=dev)

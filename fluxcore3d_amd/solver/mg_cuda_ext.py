@@ -25,12 +25,12 @@ _lib.register({
     "fc_mg_sweep": (I32, [P, P, P, P, F64, I32, I32, I32, I32, I32, I32, P]),
     "fc_mg_sweep2_from_zero": (I32, [P, P, P, F64, I32, I32, I32, I32, I32, P]),
     "fc_mg_residual_restrict": (I32, [P, P, P, P, F64, I32, I32, I32, I32, I32, P, P]),
-    "fc_mg_prolong_correct": (I32, [P, P, P, I32, I32, I32, I32, I32, I32, F32, P]),
+    "fc_mg_prolong_correct": (I32, [P, P, P, I32, I32, I32, I32, I32, I32, F32, P, P]),
     "fc_mg_coarse_solve": (I32, [P, P, P, P, F64, I32, I32, I32, I32, I32, I32, I32, P]),
     "fc_mg_partials_count": (I32, [I64]),
     "fc_mg_apply_A64": (I32, [P, P, P, P, P, F64, I32, I32, I32, I32, I32, P, P, P]),
-    "fc_mg_update_xr": (I32, [P, P, P, P, P, I64, P, P, P]),
-    "fc_mg_new_direction": (I32, [P, P, P, I64, I32, P, P, P]),
+    "fc_mg_update_xr": (I32, [P, P, P, P, P, I64, P, I32, P, P, P]),
+    "fc_mg_new_direction": (I32, [P, P, P, I64, I32, P, I32, P, P, P]),
     "fc_mg_true_residual": (I32, [P, P, P, P, P, P, F64, I32, I32, I32, I32, I32, P, P, P]),
     "fc_mg_norm2_f32": (I32, [P, I64, P, P, I32, P]),
     "fc_mg_residual_f32": (I32, [P, P, P, P, I64, P, P, I32, P]),

@@ -33,13 +33,15 @@ Tensor = torch.Tensor
 
 
 class MGCoef(_C.Structure):
-    _fields_ = [(n, _C.c_void_p) for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL", "invD", "fx", "fy", "fz")]
+    _fields_ = [(n, _C.c_void_p) for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL", "invD", "fx", "fy", "fz",
+                                           "rows")]
 
 
 class NLevelGeometricMGSolver:
     def __init__(self, device, dtype, flow_axis: int, flow_sign: int, thermal_dt_scale: float, omega: float = 0.8,
                  n_pre: int = 2, n_post: int = 2, n_coarse: int = 20, min_coarse_cells: int = 4,
-                 face_coefficients: bool = True, fused_first_sweeps: bool = True, coarse_mask: str = "all"):
+                 face_coefficients: bool = True, fused_first_sweeps: bool = True, coarse_mask: str = "all",
+                 skip_empty_rows: bool = False):
         # The reference falls through to its PyTorch operator for CPU or float64 input
         # (nlevel_mg_cuda.py:71-79); this solver runs the HIP kernels only.
         if torch.device(device).type != "cuda":
@@ -68,6 +70,12 @@ class NLevelGeometricMGSolver:
         if coarse_mask not in ("all", "any"):
             raise ValueError("coarse_mask must be 'all' or 'any'")
         self.coarse_mask = coarse_mask
+        # Skip z-rows without unknowns in the stencil kernels. Valid when every cell of
+        # such a row is decoupled (its six coefficients and diagL are 0, which the build
+        # checks) and the right-hand sides and start vectors handed to the solver vanish
+        # there: those cells then stay exactly 0, and skipping them leaves every other
+        # value, sum and iteration count bit for bit as they were.
+        self.skip_empty_rows = bool(skip_empty_rows)
         self._ready = False
         self._dt_cache: Optional[float] = None
         self._shape_cache: Optional[tuple] = None
@@ -109,14 +117,19 @@ class NLevelGeometricMGSolver:
             dx = float(2 ** len(self._levels))
             shape = (nx, ny, nz)
             lv = {"shape": shape, "out_idx": out_idx, "mask_f": msk}
-            for name in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL", "inv_diagM", "x", "r", "b_buf", "Lo", "tmp"):
+            for name in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL", "inv_diagM"):
                 lv[name] = torch.empty(shape, device=dev, dtype=torch.float32)
+            # zeros: with skipped rows no kernel writes those rows of the work vectors
+            for name in ("x", "r", "b_buf", "Lo", "tmp"):
+                lv[name] = torch.zeros(shape, device=dev, dtype=torch.float32)
             _lib.call("fc_mg_build_coeffs", *[t.data_ptr() for t in fields], nx, ny, nz, self.ax, out_idx, dx, self.s,
                       *[lv[n].data_ptr() for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL")], stream)
             lv["coef_struct"] = MGCoef(*[lv[n].data_ptr() for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL",
                                                                     "inv_diagM")])
             lv["coef"] = _C.addressof(lv["coef_struct"])  # pointer to the struct, kept alive by the level
             lv["faces"] = self._build_faces(lv) if self.face_coefficients else None
+            if self.skip_empty_rows:
+                self._build_rows(lv)
             self._levels.append(lv)
             if min(nx // 2, ny // 2, nz // 2) <= self.min_coarse_cells:
                 break
@@ -149,6 +162,20 @@ class NLevelGeometricMGSolver:
         st = lv["coef_struct"]
         st.fx, st.fy, st.fz = (f.data_ptr() for f in faces)
         return faces
+
+    def _build_rows(self, lv) -> None:
+        """Row table of one level: 1 where the z-row holds an unknown (mask != 0)."""
+        nx, ny, nz = lv["shape"]
+        live = (lv["mask_f"] != 0).reshape(nx * ny, nz).any(dim=1)
+        dead = ~live
+        if bool(dead.any()):
+            coupled = torch.zeros(nx * ny, dtype=torch.bool, device=self.device)
+            for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL"):
+                coupled |= (lv[n].reshape(nx * ny, nz) != 0).any(dim=1)
+            if bool((coupled & dead).any()):
+                raise ValueError("skip_empty_rows: a row without unknowns has non-zero coefficients")
+        lv["rows"] = live.to(torch.uint8).contiguous()
+        lv["coef_struct"].rows = lv["rows"].data_ptr()
 
     def _update_inv_diag(self, dt: float) -> None:
         for lv in self._levels:
@@ -208,7 +235,7 @@ class NLevelGeometricMGSolver:
         self._vcycle(level + 1, x_is_zero=True)
         cx, cy, cz = lc["shape"]
         _lib.call("fc_mg_prolong_correct", lc["x"].data_ptr(), lv["x"].data_ptr(), lv["mask_f"].data_ptr(), nx, ny,
-                  nz, cx, cy, cz, 0.75, stream)
+                  nz, cx, cy, cz, 0.75, _lib.ptr(lv.get("rows")), stream)
         self._sweeps(level, self.n_post, False)
 
     # ------------------------------------------------------------------ API

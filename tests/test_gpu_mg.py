@@ -275,7 +275,7 @@ def test_prolongation_two_cells_per_thread_same_bits():
             xv.copy_(T(x0).view(-1))
             mv.copy_(T(m0).view(-1))
             _lib.call("fc_mg_prolong_correct", xc.data_ptr(), xv.data_ptr(), mv.data_ptr(), nx, ny, nz, cx, cy, cz,
-                      0.75, torch.cuda.current_stream().cuda_stream)
+                      0.75, None, torch.cuda.current_stream().cuda_stream)
             outs.append(xv.clone())
         assert torch.equal(outs[0], outs[1]), shape
         want = torch.nn.functional.interpolate(xc[None, None], size=shape, mode="trilinear", align_corners=False)[0, 0]
