@@ -185,11 +185,14 @@ __device__ __forceinline__ bool box_to_grid(const Box& b, int64_t t, int64_t& g)
 }
 
 // b = T_s + dt src + acc / rho_cp on conducting cells: the right-hand side of
-// (I + dt L) T = b, L the solid conduction operator divided by rho_cp. Start x0 = T_s, or
+// (I + dt L) T = b, L the solid conduction operator divided by rho_cp. Start x0 = T_s;
 // with the previous interval's start temperature T_prev the linear extrapolation
-// 2 T_s - T_prev (the solid heats smoothly, so the solve starts closer to its answer)
+// 2 T_s - T_prev, and with the one before, T_prev2, the quadratic one
+// 3 (T_s - T_prev) + T_prev2 (the solid heats smoothly, so the solve starts closer to
+// its answer)
 __global__ __launch_bounds__(kBlock) void solid_rhs_kernel(const float* __restrict__ T_solid,
                                                            const float* __restrict__ T_prev,
+                                                           const float* __restrict__ T_prev2,
                                                            const float* __restrict__ acc,
                                                            const float* __restrict__ src,
                                                            const uint8_t* __restrict__ flag, Box box, double inv_rcp,
@@ -202,7 +205,10 @@ __global__ __launch_bounds__(kBlock) void solid_rhs_kernel(const float* __restri
     const double ts = (double)T_solid[g];
     const double s = src ? dt * (double)src[g] : 0.0;
     bv = (ts + s) + (double)acc[g] * inv_rcp;
-    xv = T_prev ? 2.0 * ts - (double)T_prev[g] : ts;
+    if (T_prev && T_prev2)
+      xv = 3.0 * (ts - (double)T_prev[g]) + (double)T_prev2[g];
+    else
+      xv = T_prev ? 2.0 * ts - (double)T_prev[g] : ts;
   }
   b[t] = bv;
   x0[t] = xv;
@@ -255,16 +261,17 @@ int fc_d3q7_init(float* g, const float* T, const float* u, const float* v, const
   return check_launch("d3q7_init");
 }
 
-int fc_cht_solid_rhs(const float* T_solid, const float* T_prev, const float* acc, const float* src,
-                     const uint8_t* flag, int nx, int ny, int nz, int ox, int oy, int oz, int bx, int by, int bz,
-                     double rho_cp, double dt, double* b, double* x0, hipStream_t stream) {
+int fc_cht_solid_rhs(const float* T_solid, const float* T_prev, const float* T_prev2, const float* acc,
+                     const float* src, const uint8_t* flag, int nx, int ny, int nz, int ox, int oy, int oz, int bx,
+                     int by, int bz, double rho_cp, double dt, double* b, double* x0, hipStream_t stream) {
   Box box;
   if (int rc = make_box(box, nx, ny, nz, ox, oy, oz, bx, by, bz)) return rc;
   FC_REQUIRE(T_solid && acc && flag && b && x0, "cht_solid_rhs: null pointer argument");
   FC_REQUIRE(rho_cp > 0.0 && dt > 0.0, "cht_solid_rhs: rho_cp and dt must be positive");
+  FC_REQUIRE(!T_prev2 || T_prev, "cht_solid_rhs: T_prev2 needs T_prev");
   const int64_t n = (int64_t)bx * by * bz;
-  hipLaunchKernelGGL(solid_rhs_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, T_solid, T_prev, acc,
-                     src, flag, box, 1.0 / rho_cp, dt, b, x0);
+  hipLaunchKernelGGL(solid_rhs_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, T_solid, T_prev, T_prev2,
+                     acc, src, flag, box, 1.0 / rho_cp, dt, b, x0);
   return check_launch("cht_solid_rhs");
 }
 

@@ -409,13 +409,14 @@ int fc_d3q7_step(const float *g_in, float *g_out, float *T_out, const float *u, 
 /* g = w_i T (1 + 4 c_i.u) */
 int fc_d3q7_init(float *g, const float *T, const float *u, const float *v, const float *w, int64_t n,
                  hipStream_t stream);
-/* right-hand side b = T_s + dt src + acc / rho_cp and start x0 = T_s (or 2 T_s - T_prev
- * when T_prev, the previous interval's start temperature, is given) of the solid's
- * implicit step (I + dt L) T = b on a box (origin ox, oy, oz >= -1; extents bx, by,
- * bz) of the nx x ny x nz grid; zero on box cells that are not conducting */
-int fc_cht_solid_rhs(const float *T_solid, const float *T_prev, const float *acc, const float *src,
-                     const uint8_t *flag, int nx, int ny, int nz, int ox, int oy, int oz, int bx, int by, int bz,
-                     double rho_cp, double dt, double *b, double *x0, hipStream_t stream);
+/* right-hand side b = T_s + dt src + acc / rho_cp and start x0 = T_s (2 T_s - T_prev
+ * when T_prev, the previous interval's start temperature, is given; 3 (T_s - T_prev) +
+ * T_prev2 when the one before, T_prev2, is given too) of the solid's implicit step
+ * (I + dt L) T = b on a box (origin ox, oy, oz >= -1; extents bx, by, bz) of the
+ * nx x ny x nz grid; zero on box cells that are not conducting */
+int fc_cht_solid_rhs(const float *T_solid, const float *T_prev, const float *T_prev2, const float *acc,
+                     const float *src, const uint8_t *flag, int nx, int ny, int nz, int ox, int oy, int oz, int bx,
+                     int by, int bz, double rho_cp, double dt, double *b, double *x0, hipStream_t stream);
 /* T_s = x on the box's conducting cells; their acc restarts at zero */
 int fc_cht_solid_store(const double *x, float *T_solid, float *acc, const uint8_t *flag, int nx, int ny, int nz,
                        int ox, int oy, int oz, int bx, int by, int bz, hipStream_t stream);
