@@ -96,6 +96,8 @@ template <> struct Store<float> {
   static __device__ __forceinline__ void st(float* p, float v, int) { *p = v; }
   static __device__ __forceinline__ float ld_nt(const float* p, int) { return __builtin_nontemporal_load(p); }
   static __device__ __forceinline__ void st_nt(float* p, float v, int) { __builtin_nontemporal_store(v, p); }
+  // the value a store of v followed by a load returns
+  static __device__ __forceinline__ float rt(float v, int) { return v; }
 };
 template <> struct Store<__half> {
   static __device__ __forceinline__ float ld(const __half* p, int q) { return __half2float(*p) + WQ(q); }
@@ -107,6 +109,7 @@ template <> struct Store<__half> {
   static __device__ __forceinline__ void st_nt(__half* p, float v, int q) {
     __builtin_nontemporal_store(__half_as_ushort(__float2half(v - WQ(q))), reinterpret_cast<unsigned short*>(p));
   }
+  static __device__ __forceinline__ float rt(float v, int q) { return __half2float(__float2half(v - WQ(q))) + WQ(q); }
 };
 template <> struct Store<__hip_bfloat16> {
   static __device__ __forceinline__ float ld(const __hip_bfloat16* p, int q) {
@@ -122,6 +125,9 @@ template <> struct Store<__hip_bfloat16> {
   static __device__ __forceinline__ void st_nt(__hip_bfloat16* p, float v, int q) {
     __builtin_nontemporal_store(__bfloat16_as_ushort(__float2bfloat16(v - WQ(q))),
                                 reinterpret_cast<unsigned short*>(p));
+  }
+  static __device__ __forceinline__ float rt(float v, int q) {
+    return __bfloat162float(__float2bfloat16(v - WQ(q))) + WQ(q);
   }
 };
 
@@ -885,15 +891,24 @@ __global__ __launch_bounds__(kPlaneBlock) void aa_zou_he_kernel(ST* __restrict__
 }
 
 // Convective outlet on the AA state, pass 1 (copy + partial sums) and pass 2
-// (mass correction); same semantics as convective_pass1/2 in lbm_ref.hip.
+// (mass correction); same semantics as convective_pass1/2 in lbm_ref.hip. A plane
+// across z puts every input of every outlet node in its own cache line, so the copied
+// inputs do not go back to their slots in pass 1: pass 1 writes them, as the storage
+// type would return them, and their slot indices to a plane buffer ([19][plane] each,
+// coalesced), and pass 2 scales them from there into the slots. The slots are then
+// touched twice per step (pass 1 loads, pass 2 stores) instead of four times.
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 template <typename ST>
-__global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass1(ST* __restrict__ pop, Grid g, Tables t,
+__global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass1(const ST* __restrict__ pop, Grid g, Tables t,
                                                                    int parity, int first, int bc, int axis,
                                                                    int outlet, int interior, int unknown_cax, int n1,
-                                                                   int n2, double* __restrict__ partial) {
+                                                                   int n2, double* __restrict__ partial,
+                                                                   float* __restrict__ vals,
+                                                                   uint32_t* __restrict__ slots) {
   const int tid = blockIdx.x * kPlaneBlock + threadIdx.x;
+  const int P = n1 * n2;
   double r = 0.0, cnt = 0.0;
-  if (tid < n1 * n2) {
+  if (tid < P) {
     const int64_t fo = plane_flat(g, axis, outlet, tid / n2, tid % n2);
     if (!(t.nflag[fo] & F_SOLID)) {
       const int64_t fi = plane_flat(g, axis, interior, tid / n2, tid % n2);
@@ -906,8 +921,20 @@ __global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass1(ST* __restric
         if (C_AX(q, axis) == unknown_cax) a[q] = b2[q];
         rho += a[q];
       }
-      store_inputs(pop, g, t, parity, fo, a);
+      const int64_t nynz = (int64_t)g.ny * g.nz;
+      const int i = (int)(fo / nynz), j = (int)((fo / g.nz) % g.ny), k = (int)(fo % g.nz);
+      const int d = parity == 0 ? 0 : -1;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        int64_t s;
+        const bool has = node_storage(g, t, i, j, k, d * CX(q), d * CY(q), d * CZ(q), s);
+        slots[q * P + tid] = has ? (uint32_t)s : kNoSlot;
+        vals[q * P + tid] = Store<ST>::rt(a[q], q);
+      }
       r = rho; cnt = 1.0;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) slots[q * P + tid] = kNoSlot;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -921,10 +948,11 @@ __global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass1(ST* __restric
 }
 
 template <typename ST>
-__global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass2(ST* __restrict__ pop, Grid g, Tables t,
-                                                                   int parity, int axis, int outlet, int n1, int n2,
-                                                                   float rho_out, const double* __restrict__ partial,
-                                                                   int nparts) {
+__global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass2(ST* __restrict__ pop, Grid g, int parity, int n1,
+                                                                   int n2, float rho_out,
+                                                                   const double* __restrict__ partial, int nparts,
+                                                                   const float* __restrict__ vals,
+                                                                   const uint32_t* __restrict__ slots) {
   // every block folds the per-block partials the same way (lane l sums partials
   // l, l + 64, ..., then a fixed shuffle tree), so all blocks apply the same factor
   double a = 0.0, b = 0.0;
@@ -939,15 +967,14 @@ __global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass2(ST* __restric
   if (cnt < 0.5f || sum < 1e-10f) return;
   const float corr = rho_out * cnt / sum;
   const int tid = blockIdx.x * kPlaneBlock + threadIdx.x;
-  if (tid >= n1 * n2) return;
-  const int64_t fo = plane_flat(g, axis, outlet, tid / n2, tid % n2);
-  if (t.nflag[fo] & F_SOLID) return;
-  const int64_t nynz = (int64_t)g.ny * g.nz;
-  const int i = (int)(fo / nynz), j = (int)((fo / g.nz) % g.ny), k = (int)(fo % g.nz);
+  const int P = n1 * n2;
+  if (tid >= P) return;
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const float v = load_plain(pop, g, t, parity, i, j, k, q) * corr;
-    store_plain(pop, g, t, parity, i, j, k, q, v);
+    const uint32_t s = slots[q * P + tid];
+    if (s == kNoSlot) continue;
+    const int dir = parity == 0 ? q : OPP(q);
+    Store<ST>::st(pop + (int64_t)dir * g.Ns + s, vals[q * P + tid] * corr, dir);
   }
 }
 
@@ -1140,9 +1167,12 @@ int fc_aa_zou_he(void* pop, int storage, int nx, int ny, int nz, int px, int py,
   return check_launch("aa_zou_he");
 }
 
+// [2 per block] FP64 partial sums, then the plane buffer: [19][n1 n2] FP32 values and
+// [19][n1 n2] slot indices
 size_t fc_aa_convective_workspace_bytes(int n1, int n2) {
   if (n1 <= 0 || n2 <= 0) return 0;
-  return sizeof(double) * 2 * (size_t)grid_for((int64_t)n1 * n2, kPlaneBlock);
+  const size_t P = (size_t)n1 * (size_t)n2;
+  return sizeof(double) * 2 * (size_t)grid_for((int64_t)P, kPlaneBlock) + 19 * P * (sizeof(float) + sizeof(uint32_t));
 }
 
 int fc_aa_convective(void* pop, int storage, int nx, int ny, int nz, int px, int py, int pz,
@@ -1158,13 +1188,17 @@ int fc_aa_convective(void* pop, int storage, int nx, int ny, int nz, int px, int
   const int n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
   const size_t need = fc_aa_convective_workspace_bytes(n1, n2);
   FC_REQUIRE(workspace && ws_bytes >= need, "aa_convective: workspace of %zu bytes required", need);
+  FC_REQUIRE(g.Ns < (int64_t)kNoSlot, "aa_convective: more than 2^32 - 2 slots per direction");
   const unsigned grid = grid_for((int64_t)n1 * n2, kPlaneBlock);
+  double* partial = (double*)workspace;
+  float* vals = (float*)(partial + 2 * (size_t)grid);
+  uint32_t* slots = (uint32_t*)(vals + 19 * (size_t)n1 * n2);
 #define FC_CV(T)                                                                                                 \
-  hipLaunchKernelGGL(aa_convective_pass1<T>, dim3(grid), dim3(kPlaneBlock), 0, stream, (T*)pop, g, t, parity, first,  \
-                     bc_order, axis, outlet_idx, interior_idx, unknown_cax, n1, n2, (double*)workspace);          \
+  hipLaunchKernelGGL(aa_convective_pass1<T>, dim3(grid), dim3(kPlaneBlock), 0, stream, (const T*)pop, g, t, parity,  \
+                     first, bc_order, axis, outlet_idx, interior_idx, unknown_cax, n1, n2, partial, vals, slots);  \
   if (int rc = check_launch("aa_convective/pass1")) return rc;                                                    \
-  hipLaunchKernelGGL(aa_convective_pass2<T>, dim3(grid), dim3(kPlaneBlock), 0, stream, (T*)pop, g, t, parity, axis,    \
-                     outlet_idx, n1, n2, rho_out_lat, (const double*)workspace, (int)grid)
+  hipLaunchKernelGGL(aa_convective_pass2<T>, dim3(grid), dim3(kPlaneBlock), 0, stream, (T*)pop, g, parity, n1, n2,    \
+                     rho_out_lat, (const double*)partial, (int)grid, (const float*)vals, (const uint32_t*)slots)
   switch (storage) {
     case 0: FC_CV(float); break;
     case 1: FC_CV(__half); break;

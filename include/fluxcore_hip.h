@@ -142,6 +142,9 @@ int fc_aa_load(void *pop, int storage, int nx, int ny, int nz, int px, int py, i
 int fc_aa_zou_he(void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz, const fc_aa_tables *tables,
                  int parity, int first, int bc_order, int axis, int plane_idx, int interior_idx, int bc_type,
                  float u_in_lat, float rho_out_lat, int sign, hipStream_t stream);
+/* device workspace of fc_aa_convective for an n1 x n2 outlet plane: per-block partial
+ * sums and a plane buffer of the outlet inputs and their slot indices (about 152 bytes
+ * per plane node) */
 size_t fc_aa_convective_workspace_bytes(int n1, int n2);
 int fc_aa_convective(void *pop, int storage, int nx, int ny, int nz, int px, int py, int pz,
                      const fc_aa_tables *tables, int parity, int first, int bc_order, int axis, int outlet_idx,
