@@ -149,6 +149,7 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(const __bf16* __restrict_
 constexpr int kLdsTH = 4, kLdsTW = 64, kLdsPH = kLdsTH + 2, kLdsPW = kLdsTW + 2, kLdsDZ = 8;  // max planes per block
 constexpr int kLdsPlane = kLdsPH * kLdsPW;  // voxels of one input plane
 constexpr int kLdsMaxK = 128;               // K-step table entries (kpad / 8 <= 27 * 3 + 3)
+constexpr int kLoadBatch = 8;               // 16-byte input loads a thread has in flight while staging
 
 // tap = kk / nch for kk < 256 and nch <= 8 by a multiply-shift with the host's
 // magic = ceil(2^16 / nch) (checked on the host for every kk used)
@@ -181,29 +182,40 @@ __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(const __bf16* __restr
     const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
     ktab[kk] = make_int2((chunk * kLdsPH + ty) * kLdsPW + tx, tz);
   }
-  // input plane dd (absolute) into ring slot dd mod 3
+  // item `it` of input plane dd (absolute): 8 channels of one voxel, zero outside the volume
+  auto fetch = [&](int dd, int it) -> bf16x8 {
+    const int wx = it % kLdsPW, r1 = it / kLdsPW;
+    const int hy = r1 % kLdsPH, ch = r1 / kLdsPH;
+    const int hh = h0 + hy - 1, ww = w0 + wx - 1;
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
+    if ((unsigned)dd < (unsigned)s.D && (unsigned)hh < (unsigned)s.H && (unsigned)ww < (unsigned)s.W) {
+      if (ch < nch0) {
+        const unsigned pos = s.up0 ? ((unsigned)(b * (s.D >> 1) + (dd >> 1)) * (unsigned)(s.H >> 1) + (unsigned)(hh >> 1)) *
+                                             (unsigned)(s.W >> 1) + (unsigned)(ww >> 1)
+                                   : ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
+        v = *(const bf16x8*)(x0 + (pos * (unsigned)s.c0 + (unsigned)(ch << 3)));
+      } else {
+        const unsigned pos = ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
+        v = *(const bf16x8*)(x1 + (pos * (unsigned)s.c1 + (unsigned)((ch - nch0) << 3)));
+      }
+    }
+    return v;
+  };
+  // input plane dd (absolute) into ring slot dd mod 3; a thread's loads are all issued
+  // before its first LDS store (one memory latency per batch of kLoadBatch, not per item)
   auto load_plane = [&](int dd) {
     bf16x8* dst = ring + ((dd + 3) % 3) * slab;
     const int n_items = nch * kLdsPlane;
-    for (int it = threadIdx.x; it < n_items; it += 256) {
-      const int wx = it % kLdsPW, r1 = it / kLdsPW;
-      const int hy = r1 % kLdsPH, ch = r1 / kLdsPH;
-      const int hh = h0 + hy - 1, ww = w0 + wx - 1;
-      bf16x8 v;
+    for (int it0 = threadIdx.x; it0 < n_items; it0 += 256 * kLoadBatch) {
+      bf16x8 v[kLoadBatch];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
-      if ((unsigned)dd < (unsigned)s.D && (unsigned)hh < (unsigned)s.H && (unsigned)ww < (unsigned)s.W) {
-        if (ch < nch0) {
-          const unsigned pos = s.up0 ? ((unsigned)(b * (s.D >> 1) + (dd >> 1)) * (unsigned)(s.H >> 1) + (unsigned)(hh >> 1)) *
-                                               (unsigned)(s.W >> 1) + (unsigned)(ww >> 1)
-                                     : ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
-          v = *(const bf16x8*)(x0 + (pos * (unsigned)s.c0 + (unsigned)(ch << 3)));
-        } else {
-          const unsigned pos = ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
-          v = *(const bf16x8*)(x1 + (pos * (unsigned)s.c1 + (unsigned)((ch - nch0) << 3)));
-        }
-      }
-      dst[it] = v;
+      for (int u = 0; u < kLoadBatch; ++u)
+        if (it0 + 256 * u < n_items) v[u] = fetch(dd, it0 + 256 * u);
+#pragma unroll
+      for (int u = 0; u < kLoadBatch; ++u)
+        if (it0 + 256 * u < n_items) dst[it0 + 256 * u] = v[u];
     }
   };
   load_plane(d0 - 1);
@@ -279,6 +291,155 @@ __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(const __bf16* __restr
       }
     }
     __syncthreads();  // every wavefront is done with plane d - 1 before it is overwritten
+  }
+}
+
+// The LDS-staged convolution for R output planes per wavefront (inputs of up to 16
+// channels, up to 16 output channels). With one output plane per wavefront every B
+// fragment read from LDS feeds MT MFMAs (one for 16 output channels), and the layer is
+// bound by LDS reads. Here a wavefront computes planes d .. d+R-1 of its row: the K
+// steps run over (ty, tx, channel chunk) for each tz, and the B fragment of input
+// plane d-1+e serves every output plane r with tz = e - r in [0, 2]: R+2 fragment reads
+// feed 3R MFMAs (R = 4: half the LDS reads per MFMA). The block's ring holds the R+2
+// input planes; each step of R output planes loads R new ones.
+template <int MT, int R>
+__global__ __launch_bounds__(256) void conv_fwd_lds_r_kernel(const __bf16* __restrict__ x0,
+                                                             const __bf16* __restrict__ wt, ConvShape s, int act,
+                                                             float slope, int dz_block, __bf16* __restrict__ y16,
+                                                             float* __restrict__ y32) {
+  extern __shared__ bf16x8 ring[];  // [plane slot R+2][chunk of 8 channels][hy 6][wx 66]
+  __shared__ int2 ktab[kLdsMaxK];   // per 8-wide group of a tz block: {tile offset, weight offset or -1}
+  constexpr int NT = kLdsTW / 16, NS = R + 2;
+  const int nch = s.c0 >> 3, cin = s.c0;
+  const int slab = nch * kLdsPlane;
+  const int tiles_w = (s.W + kLdsTW - 1) / kLdsTW, tiles_h = (s.H + kLdsTH - 1) / kLdsTH;
+  const int tiles_d = (s.D + dz_block - 1) / dz_block;
+  unsigned bid = blockIdx.x;
+  const int tw = (int)(bid % (unsigned)tiles_w); bid /= (unsigned)tiles_w;
+  const int th = (int)(bid % (unsigned)tiles_h); bid /= (unsigned)tiles_h;
+  const int td = (int)(bid % (unsigned)tiles_d);
+  const int b = (int)(bid / (unsigned)tiles_d);
+  const int w0 = tw * kLdsTW, h0 = th * kLdsTH, d0 = td * dz_block, d1 = min(d0 + dz_block, s.D);
+  const int ngroups = 9 * nch;            // 8-wide K groups of one tz block
+  const int jsteps = (ngroups + 3) >> 2;  // 32-wide K steps per tz block
+  for (int kg = threadIdx.x; kg < 4 * jsteps; kg += 256) {
+    int2 e = make_int2(0, -1);  // past the block's groups: zero weights, any finite tile value
+    if (kg < ngroups) {
+      const int t9 = kg / nch, ch = kg - t9 * nch;
+      e = make_int2((ch * kLdsPH + t9 / 3) * kLdsPW + t9 % 3, t9 * cin + (ch << 3));
+    }
+    ktab[kg] = e;
+  }
+  // input planes [da, db] (absolute) into ring slots (dd - d0 + 1) mod NS; a thread's
+  // loads are all issued before its first LDS store (one memory latency per batch of
+  // kLoadBatch items, not per item)
+  auto load_planes = [&](int da, int db) {
+    const int per = nch * kLdsPlane, n_items = (db - da + 1) * per;
+    for (int it0 = threadIdx.x; it0 < n_items; it0 += 256 * kLoadBatch) {
+      bf16x8 v[kLoadBatch];
+#pragma unroll
+      for (int u = 0; u < kLoadBatch; ++u) {
+        const int it = it0 + 256 * u;
+        if (it >= n_items) continue;
+        const int pl = it / per, rem = it - pl * per, dd = da + pl;
+        const int wx = rem % kLdsPW, r1 = rem / kLdsPW;
+        const int hy = r1 % kLdsPH, ch = r1 / kLdsPH;
+        const int hh = h0 + hy - 1, ww = w0 + wx - 1;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[u][j] = (__bf16)0.0f;
+        if ((unsigned)dd < (unsigned)s.D && (unsigned)hh < (unsigned)s.H && (unsigned)ww < (unsigned)s.W) {
+          const unsigned pos = ((unsigned)(b * s.D + dd) * (unsigned)s.H + (unsigned)hh) * (unsigned)s.W + (unsigned)ww;
+          v[u] = *(const bf16x8*)(x0 + (pos * (unsigned)cin + (unsigned)(ch << 3)));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kLoadBatch; ++u) {
+        const int it = it0 + 256 * u;
+        if (it >= n_items) continue;
+        const int pl = it / per;
+        ring[((da + pl - d0 + 1) % NS) * slab + (it - pl * per)] = v[u];
+      }
+    }
+  };
+  load_planes(d0 - 1, min(d0 + R, d1));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int h = h0 + wv;
+  const int col = lane & 15, grp = lane >> 4;
+  const __bf16* wrow[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)  // rows past cout read the last row (valid memory); results dropped
+    wrow[m] = wt + (unsigned)min(m * 16 + col, s.cout - 1) * (unsigned)s.kpad;
+  const int lane_off = wv * kLdsPW + col;
+  const int tzk = 9 * cin;  // weight offset between tz blocks
+  bf16x8 zero8;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) zero8[j] = (__bf16)0.0f;
+  for (int d = d0; d < d1; d += R) {
+    if (d > d0 && d + 1 <= d1) load_planes(d + 1, min(d + R, d1));  // d-1 and d are in the ring already
+    __syncthreads();  // planes d-1 .. d+R and the K table are in LDS
+    f32x4 acc[R][MT][NT];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[r][m][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < jsteps; ++j) {
+      const int2 e = ktab[(j << 2) + grp];
+      bf16x8 av[3][MT];
+#pragma unroll
+      for (int tz = 0; tz < 3; ++tz)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) av[tz][m] = e.y >= 0 ? *(const bf16x8*)(wrow[m] + tz * tzk + e.y) : zero8;
+#pragma unroll
+      for (int pe = 0; pe < NS; ++pe) {
+        const bf16x8* row = ring + ((d - d0 + pe) % NS) * slab + e.x + lane_off;  // input plane d-1+pe
+        bf16x8 bv[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) bv[n] = row[16 * n];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int tz = pe - r;
+          if (tz < 0 || tz > 2) continue;
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+              acc[r][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[tz][m], bv[n], acc[r][m][n], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (d + r >= d1 || h >= s.H) continue;
+      const unsigned prow = ((unsigned)(b * s.D + d + r) * (unsigned)s.H + (unsigned)h) * (unsigned)s.W;
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        const int w = w0 + 16 * n + col;
+        if (w >= s.W) continue;
+        const unsigned p = prow + (unsigned)w;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int co = m * 16 + grp * 4;
+          if (co >= s.cout) continue;
+          float v[4];
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            float a = acc[r][m][n][rr];
+            if (act == 1 && a < 0.f) a *= slope;
+            v[rr] = a;
+          }
+          if (y32) *(f32x4*)(y32 + (p * (unsigned)s.cout + (unsigned)co)) = (f32x4){v[0], v[1], v[2], v[3]};
+          if (y16) {
+            bf16x4 o;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) o[rr] = (__bf16)v[rr];
+            *(bf16x4*)(y16 + (p * (unsigned)s.cout + (unsigned)co)) = o;
+          }
+        }
+      }
+    }
+    __syncthreads();  // every wavefront is done with planes d-1 .. d+R-2 before they are overwritten
   }
 }
 
@@ -410,6 +571,18 @@ int fc_conv3d_fwd(const void* x0, int c0, int up0, const void* x1, int c1, const
   // gfx950's 160 KB) where the device allows it
   const size_t lds = (size_t)((c0 + c1) / 8) * 3 * kLdsPlane * sizeof(bf16x8);
   const int64_t elems = (int64_t)B * D * H * W * (c0 + c1 > cout ? c0 + c1 : cout);
+  // R = 4 output planes per wavefront: one source of up to 16 channels, up to 16 outputs
+  constexpr int kR = 4;
+  const size_t lds_r = (size_t)(c0 / 8) * (kR + 2) * kLdsPlane * sizeof(bf16x8);
+  if (c1 == 0 && !up0 && c0 <= 16 && cout <= 16 && W >= kLdsTW && lds_r <= lds_cap() && elems < ((int64_t)1 << 31)) {
+    // output planes per block: a multiple of R up to 8 while keeping >= 2048 blocks
+    const int64_t per_plane = (int64_t)B * ((H + kLdsTH - 1) / kLdsTH) * ((W + kLdsTW - 1) / kLdsTW);
+    const int dz_block = per_plane * D >= 2048 * 8 ? 8 : kR;
+    const int64_t blocks = per_plane * ((D + dz_block - 1) / dz_block);
+    hipLaunchKernelGGL((conv_fwd_lds_r_kernel<1, kR>), dim3((unsigned)blocks), dim3(256), lds_r, stream,
+                       (const __bf16*)x0, (const __bf16*)w, s, act, slope, dz_block, (__bf16*)y_bf16, y_f32);
+    return check_launch("conv3d_fwd/lds_r");
+  }
   if (W >= kLdsTW && lds <= lds_cap() && elems < ((int64_t)1 << 31) && fc_conv_kpad(c0 + c1) / 8 <= kLdsMaxK) {
     // output planes per block: up to kLdsDZ (fewer halo reloads) while keeping >= 2048 blocks
     const int64_t per_plane = (int64_t)B * ((H + kLdsTH - 1) / kLdsTH) * ((W + kLdsTW - 1) / kLdsTW);

@@ -33,13 +33,17 @@ def _up(x):
 
 # W >= 64 runs the LDS-staged kernel (input tile of 3 x 6 x 66 voxels) when the tile
 # fits the device's LDS, at most 80 KB: partial h and w tiles, 1-4 channel chunks;
-# cin = 40 at W = 96 (95 KB) falls back to the cached kernel.
+# cin = 40 at W = 96 (95 KB) falls back to the cached kernel. Up to 16 input and 16
+# output channels run the four-planes-per-wavefront form: depths that are not a multiple
+# of four, a single plane, blocks of 4 and 8 planes.
 @pytest.mark.parametrize("B,D,H,W,cin,cout,act", [(2, 6, 5, 19, 8, 12, False), (1, 4, 7, 33, 16, 16, True),
                                                    (1, 3, 3, 16, 24, 48, True), (1, 5, 4, 20, 40, 64, False),
                                                    (1, 3, 2, 70, 16, 24, True), (2, 2, 3, 40, 8, 32, False),
                                                    (1, 5, 6, 64, 8, 8, True), (2, 3, 5, 130, 24, 16, False),
                                                    (1, 2, 9, 96, 40, 64, True), (1, 4, 6, 64, 32, 16, True),
-                                                   (1, 3, 5, 130, 32, 8, False)])
+                                                   (1, 3, 5, 130, 32, 8, False), (1, 7, 6, 64, 16, 16, True),
+                                                   (2, 9, 5, 130, 16, 12, False), (1, 1, 4, 64, 16, 16, False),
+                                                   (1, 10, 3, 70, 8, 16, True), (1, 16, 8, 128, 16, 16, True)])
 def test_forward_single_source(B, D, H, W, cin, cout, act):
     from fluxcore3d_amd.solver import nn_precond as nn
     x = _bf16((B, D, H, W, cin), 1)
@@ -54,6 +58,23 @@ def test_forward_single_source(B, D, H, W, cin, cout, act):
     # BF16 output: the FP32 result rounded once
     y16 = nn.conv3d(x.to(DEV), wpk, cout, act).float().cpu().numpy()
     assert np.all(np.abs(y16 - y) <= 2.0 ** -8 * np.abs(y) + 1e-30)
+
+
+def test_forward_planes_per_block_against_torch_fp32():
+    """A shape large enough for blocks of eight output planes (two steps of four planes
+    through the block's ring of input planes): against torch's FP32 conv3d of the same
+    BF16 values, to 1e-5 of the convolution of the absolute values."""
+    from fluxcore3d_amd.solver import nn_precond as nn
+    B, D, H, W, cin, cout = 4, 128, 128, 64, 16, 16
+    g = torch.Generator(device="cpu").manual_seed(11)
+    x = torch.randn((B, D, H, W, cin), generator=g).to(torch.bfloat16).to(DEV)
+    w = (torch.randn((cout, 27, cin), generator=g) * 0.2).to(torch.bfloat16).float().to(DEV)
+    y = nn.conv3d(x, nn.pack_weights(w), cout, False, out_f32=True)
+    xt = x.float().permute(0, 4, 1, 2, 3).contiguous()
+    wt = w.reshape(cout, 3, 3, 3, cin).permute(0, 4, 1, 2, 3).contiguous()
+    ref = torch.nn.functional.conv3d(xt, wt, padding=1).permute(0, 2, 3, 4, 1)
+    bound = torch.nn.functional.conv3d(xt.abs(), wt.abs(), padding=1).permute(0, 2, 3, 4, 1)
+    assert bool(((y - ref).abs() <= 1e-5 * bound + 1e-30).all())
 
 
 @pytest.mark.parametrize("W,cs", [(16, 8), (64, 8), (64, 16), (128, 16)])
