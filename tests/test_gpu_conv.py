@@ -77,16 +77,17 @@ def test_forward_planes_per_block_against_torch_fp32():
     assert bool(((y - ref).abs() <= 1e-5 * bound + 1e-30).all())
 
 
-@pytest.mark.parametrize("W,cs", [(16, 8), (64, 8), (64, 16), (128, 16)])
-def test_forward_two_sources_with_upsample(W, cs):
+@pytest.mark.parametrize("W,c0,cs,cout", [(16, 16, 8, 24), (64, 16, 8, 24), (64, 16, 16, 24), (128, 16, 16, 24),
+                                           (64, 16, 16, 16), (128, 8, 8, 12), (64, 8, 8, 16)])
+def test_forward_two_sources_with_upsample(W, c0, cs, cout):
     """Decoder convolution: the half-resolution source read through nearest
     upsampling and the skip source read directly; 16 + 16 channels at W >= 64 is the
-    level-0 decoder layer's shape (four channel chunks in the LDS-staged kernel)."""
+    level-0 decoder layer's shape (four channel chunks in the LDS-staged kernel); 8 + 8
+    channels into up to 16 outputs run the four-planes-per-wavefront form."""
     from fluxcore3d_amd.solver import nn_precond as nn
-    xc = _bf16((1, 4, 3, W // 2, 16), 3)  # half-resolution source
+    xc = _bf16((1, 4, 3, W // 2, c0), 3)  # half-resolution source
     xs = _bf16((1, 8, 6, W, cs), 4)  # skip source at full resolution
-    cout = 24
-    w = _bf16((cout, 27, 16 + cs), 5).float() * 0.2
+    w = _bf16((cout, 27, c0 + cs), 5).float() * 0.2
     wpk = nn.pack_weights(w.to(DEV))
     y = nn.conv3d(xc.to(DEV), wpk, cout, True, x1=xs.to(DEV), up0=True, out_f32=True).cpu().numpy()
     xcat = np.concatenate([_up(_np(xc)), _np(xs)], axis=-1)
