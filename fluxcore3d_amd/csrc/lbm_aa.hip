@@ -197,11 +197,13 @@ __device__ __forceinline__ float w2_of(const Tables& t, int64_t l, float w1) { r
 template <typename ST>
 __device__ __forceinline__ void compose_inputs(const ST* __restrict__ pop, const Grid& g, const Tables& t,
                                                int parity, int first, int i, int j, int k, bool special,
-                                               bool plain, int4 m, int4 b, float f[19]) {
+                                               bool plain, int4 m, int4 b, float f[19], uint32_t want = 0x7FFFFu) {
   const uint32_t L = plain ? 0u : (uint32_t)m.x, S = plain ? 0u : (uint32_t)m.y, V = (uint32_t)m.z;
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    if (special && ((L >> q) & 1u)) {
+    if (!((want >> q) & 1u)) {  // an input the caller does not use
+      f[q] = 0.f;
+    } else if (special && ((L >> q) & 1u)) {
       const bool isv = (V >> q) & 1u;
       const int64_t l = isv ? b.w + __popc(V & ((1u << q) - 1u)) : b.x + __popc(L & ~V & ((1u << q) - 1u));
       float v = t.A[l];
@@ -732,7 +734,7 @@ __device__ __forceinline__ int64_t special_index_scan(const Tables& t, int64_t x
 // nodes already rebuilt by an earlier BC of the same step hold plain inputs.
 template <typename ST>
 __device__ __forceinline__ void node_inputs(const ST* pop, const Grid& g, const Tables& t, int parity, int first,
-                                            int64_t x, int bc, float f[19]) {
+                                            int64_t x, int bc, float f[19], uint32_t want = 0x7FFFFu) {
   const uint8_t fl = t.nflag[x];
   const int64_t nynz = (int64_t)g.ny * g.nz;
   const int i = (int)(x / nynz), j = (int)((x / g.nz) % g.ny), k = (int)(x % g.nz);
@@ -745,7 +747,7 @@ __device__ __forceinline__ void node_inputs(const ST* pop, const Grid& g, const 
     m = t.sp_mask[s];
     b = t.sp_base[s];
   }
-  compose_inputs(pop, g, t, parity, first, i, j, k, special, plain, m, b, f);
+  compose_inputs(pop, g, t, parity, first, i, j, k, special, plain, m, b, f, want);
 }
 
 template <typename ST>
@@ -838,9 +840,13 @@ __global__ __launch_bounds__(kPlaneBlock) void aa_zou_he_kernel(ST* __restrict__
   if (tid >= n1 * n2) return;
   const int64_t fb = plane_flat(g, axis, plane, tid / n2, tid % n2);
   if (t.nflag[fb] & F_SOLID) return;
-  float fi[19];
-  node_inputs(pop, g, t, parity, first, fb, bc, fi);
   const int unknown = plane == 0 ? 1 : -1, known = -unknown;
+  // the unknown inputs are rebuilt below before anything reads them: not loaded
+  uint32_t rebuilt = 0;
+#pragma unroll
+  for (int q = 0; q < 19; ++q) rebuilt |= (C_AX(q, axis) == unknown ? 1u : 0u) << q;
+  float fi[19];
+  node_inputs(pop, g, t, parity, first, fb, bc, fi, 0x7FFFFu & ~rebuilt);
   float s0 = 0.f, sk = 0.f;
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
@@ -912,9 +918,13 @@ __global__ __launch_bounds__(kPlaneBlock) void aa_convective_pass1(const ST* __r
     const int64_t fo = plane_flat(g, axis, outlet, tid / n2, tid % n2);
     if (!(t.nflag[fo] & F_SOLID)) {
       const int64_t fi = plane_flat(g, axis, interior, tid / n2, tid % n2);
+      // of the interior node only the five inputs copied to the outlet are read
+      uint32_t unknown = 0;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) unknown |= (C_AX(q, axis) == unknown_cax ? 1u : 0u) << q;
       float a[19], b2[19];
       node_inputs(pop, g, t, parity, first, fo, bc, a);
-      node_inputs(pop, g, t, parity, first, fi, bc, b2);
+      node_inputs(pop, g, t, parity, first, fi, bc, b2, unknown);
       float rho = 0.f;
 #pragma unroll
       for (int q = 0; q < 19; ++q) {

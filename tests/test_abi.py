@@ -78,3 +78,23 @@ print('OK')
     res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0 and "OK" in res.stdout, res.stdout + res.stderr
     assert "libfluxcore_hip.so not loaded" in res.stderr
+
+
+def test_workspace_sizes_without_gpu():
+    """Host-side size queries: the convective outlet's workspace holds two FP64 partial
+    sums per 64-node block and the plane buffer of 19 FP32 values and 19 slot indices per
+    outlet node; the compact slot count of a periodic grid is its node count."""
+    from fluxcore3d_amd import _lib
+    lib = _lib.lib()
+    f = lib.fc_aa_convective_workspace_bytes
+    f.restype = ctypes.c_size_t
+    f.argtypes = [ctypes.c_int, ctypes.c_int]
+    for n1, n2 in ((192, 192), (32, 128), (7, 5)):
+        P = n1 * n2
+        assert f(n1, n2) == 16 * ((P + 63) // 64) + 19 * P * 8
+    assert f(0, 5) == 0
+    g = lib.fc_aa_slot_count
+    g.restype = ctypes.c_int64
+    g.argtypes = [ctypes.c_int] * 6
+    assert g(4, 5, 6, 1, 1, 1) == 120
+    assert g(4, 5, 6, 0, 0, 0) == 6 * 7 * 8
