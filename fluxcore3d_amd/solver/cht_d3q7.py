@@ -125,7 +125,7 @@ class D3Q7Transport:
         self.periodic = tuple(int(bool(p)) for p in periodic)
         self.T_in = float(T_in)
         dev = flag.device
-        self.nbmask = neighbour_mask(self.flag, self.periodic)
+        self.nbmask: Optional[Tensor] = neighbour_mask(self.flag, self.periodic)  # None: every flag read
         self.g = [torch.zeros((7, self.N), dtype=torch.float32, device=dev) for _ in range(2)]
         self.T = torch.zeros(self.N, dtype=torch.float32, device=dev)
         self.cur = 0
@@ -152,7 +152,7 @@ class D3Q7Transport:
         nx, ny, nz = self.shape
         src, dst = self.g[self.cur], self.g[1 - self.cur]
         _lib.call("fc_d3q7_step", src.data_ptr(), dst.data_ptr(), self.T.data_ptr(), *[t.data_ptr() for t in uu],
-                  self.flag.data_ptr(), self.nbmask.data_ptr(), ts.data_ptr(), _ptr(q), _ptr(a), nx, ny, nz,
+                  self.flag.data_ptr(), _ptr(self.nbmask), ts.data_ptr(), _ptr(q), _ptr(a), nx, ny, nz,
                   *self.periodic, self.omega,
                   self.omega_s, self.T_in, _stream(self.T))
         self.cur = 1 - self.cur

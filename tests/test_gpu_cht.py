@@ -31,8 +31,9 @@ def _t(a, dtype=torch.float32):
     return torch.as_tensor(np.ascontiguousarray(a), device=DEV).to(dtype)
 
 
-@pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 0, 1)])
-def test_d3q7_step_bit_exact_against_fp32_oracle(periodic):
+@pytest.mark.parametrize("periodic,mask", [((0, 0, 0), True), ((1, 0, 1), True), ((1, 0, 1), False)])
+def test_d3q7_step_bit_exact_against_fp32_oracle(periodic, mask):
+    """With the per-node neighbour mask, and without it (every neighbour's flag read)."""
     from fluxcore3d_amd.solver.cht_d3q7 import D3Q7Transport
     rng = np.random.default_rng(11)
     shape = (20, 18, 22)
@@ -45,6 +46,8 @@ def test_d3q7_step_bit_exact_against_fp32_oracle(periodic):
     Q = (1e-3 * rng.standard_normal(N)).astype(np.float32)
     om, om_s, T_in = 1.37, 0.71, 0.4
     tr = D3Q7Transport(torch.as_tensor(kind, device=DEV), om, periodic=periodic, T_in=T_in, omega_s=om_s)
+    if not mask:
+        tr.nbmask = None
     tr.init(_t(T0), _t(u), _t(v), _t(w))
     g = od.equilibrium(T0, u, v, w, np.float32)
     np.testing.assert_array_equal(tr.populations.cpu().numpy(), g)
