@@ -76,7 +76,8 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
   // then decides what is used (walls and solids discard the pulls, few nodes).
   const uint8_t fl = flag[x];
   // bit q - 1 of nbmask: x - c_q is a wall or a solid node inside the domain; only
-  // those neighbours' flags are read (without the mask, all six are)
+  // those neighbours' flags are read (without the mask, all six are). On a solved solid
+  // node the bits mean instead: x + c_q is a fluid node off the inlet (see below).
   const uint32_t nbm = nbmask ? (uint32_t)nbmask[x] : 0x3Fu;
   int64_t src_node[7];
   uint8_t nk[7];
@@ -99,14 +100,21 @@ __global__ __launch_bounds__(kBlock) void d3q7_step_kernel(const float* __restri
     const float ts = T_solid[x];
     T_out[x] = ts;
     if (!acc) return;
+    // with the mask, its bits name the fluid neighbours off the inlet, so neither their
+    // flags nor (for solid nodes without fluid neighbours) anything else is read
+    if (nbmask && nbm == 0u) return;
     float e = 0.0f;
 #pragma unroll
     for (int q = 1; q < 7; ++q) {
       int a = i, b = j, c = k;
       if (!(nb7(i, cx7(q), nx, px, a) && nb7(j, cy7(q), ny, py, b) && nb7(k, cz7(q), nz, pz, c))) continue;
       const int64_t s = ((int64_t)a * ny + b) * nz + c;
-      const uint8_t fs = flag[s];
-      if ((fs & G_KIND) != G_FLUID || (fs & G_INLET)) continue;
+      if (nbmask) {
+        if (!((nbm >> (q - 1)) & 1u)) continue;
+      } else {
+        const uint8_t fs = flag[s];
+        if ((fs & G_KIND) != G_FLUID || (fs & G_INLET)) continue;
+      }
       const float out = gin[opp7(q) * N + s];
       const float two_wt = 2.0f * w7(q) * ts;
       const float back = two_wt - out;

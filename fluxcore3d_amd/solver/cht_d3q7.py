@@ -58,22 +58,32 @@ _lib.register({
 FLUID, SOLID, WALL, CONJ, INLET = 0, 1, 2, 3, 16
 
 
+def _shifted(field: Tensor, ax: int, c: int, periodic) -> Tensor:
+    """Value at x of the neighbour x - c e_ax (0 where that leaves a non-periodic axis)."""
+    nb = torch.roll(field, shifts=c, dims=ax)
+    if not periodic[ax]:
+        idx = [slice(None)] * 3
+        idx[ax] = 0 if c > 0 else -1
+        nb[tuple(idx)] = 0
+    return nb
+
+
 def neighbour_mask(flag: Tensor, periodic) -> Tensor:
-    """uint8 per node: bit q - 1 set when x - c_q (q = 1..6: +x, -x, +y, -y, +z, -z)
-    lies inside the domain (wrapped on periodic axes) and is a wall or a conducting
-    solid solved apart. The D3Q7 step reads only those neighbours' flags."""
+    """uint8 per node, bits q - 1 for q = 1..6 (+x, -x, +y, -y, +z, -z); neighbours inside
+    the domain, wrapped on periodic axes.
+    * Solid nodes solved apart (kind 1): x + c_q is a fluid node off the inlet (the
+      neighbours whose energy the solid node receives).
+    * Every other node: x - c_q is a wall or a solid node solved apart (the inputs the
+      D3Q7 step replaces; it reads only those neighbours' flags)."""
     kind = flag.to(torch.int32) & 3
     sw = ((kind == SOLID) | (kind == WALL)).to(torch.int32)
-    out = torch.zeros_like(sw)
+    fl_open = ((kind == FLUID) & ((flag.to(torch.int32) & INLET) == 0)).to(torch.int32)
+    pull = torch.zeros_like(sw)
+    give = torch.zeros_like(sw)
     for q, (ax, c) in enumerate(((0, 1), (0, -1), (1, 1), (1, -1), (2, 1), (2, -1))):
-        # value at x of the neighbour x - c_q: shift the field by +c_q along the axis
-        nb = torch.roll(sw, shifts=c, dims=ax)
-        if not periodic[ax]:
-            idx = [slice(None)] * 3
-            idx[ax] = 0 if c > 0 else -1  # x - c_q leaves the domain there
-            nb[tuple(idx)] = 0
-        out |= nb << q
-    return out.to(torch.uint8).contiguous()
+        pull |= _shifted(sw, ax, c, periodic) << q         # x - c_q
+        give |= _shifted(fl_open, ax, -c, periodic) << q   # x + c_q
+    return torch.where(kind == SOLID, give, pull).to(torch.uint8).contiguous()
 
 
 def _stream(t: Tensor):

@@ -403,8 +403,9 @@ int fc_bicgstab_update(double *x, double *r, const double *s, const double *t, c
  * g_in, g_out: [7][N] post-collision populations (0, +x, -x, +y, -y, +z, -z),
  * N = nx ny nz < 2^31 in C order; Q (may be NULL): fluid heat source per step;
  * acc (may be NULL): per-node energy the solid receives, accumulated. nbmask (may be NULL,
- * static geometry): bit q - 1 set when node x - c_q (q = 1..6) lies inside the domain and
- * is a wall or a conducting solid (flag 1 or 2); only those neighbours' flags are read. */
+ * static geometry), bits q - 1 for q = 1..6 and neighbours inside the domain: on a node of
+ * flag kind 1, x + c_q is a fluid node off the inlet; on every other node, x - c_q is a
+ * wall or a conducting solid (flag 1 or 2). Only the neighbours it names are read. */
 int fc_d3q7_step(const float *g_in, float *g_out, float *T_out, const float *u, const float *v, const float *w,
                  const uint8_t *flag, const uint8_t *nbmask, const float *T_solid, const float *Q, float *acc,
                  int nx, int ny, int nz, int px, int py, int pz, float omega, float omega_s, float T_in,

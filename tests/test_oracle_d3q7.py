@@ -398,8 +398,9 @@ def test_gyroid_flow_and_heat_on_96_cube_are_the_tiled_period():
 
 @pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 0, 1), (1, 1, 1)])
 def test_neighbour_mask_against_brute_force(periodic):
-    """The per-node mask the D3Q7 step reads (bit q - 1: x - c_q inside the domain and a
-    wall or a conducting solid) against a loop over every node and direction."""
+    """The per-node mask the D3Q7 step reads against a loop over every node and direction:
+    on solid nodes solved apart, bit q - 1 when x + c_q is inside the domain and a fluid
+    node off the inlet; elsewhere, when x - c_q is inside and a wall or such a solid."""
     import torch
     from fluxcore3d_amd.solver.cht_d3q7 import neighbour_mask, FLUID, SOLID, WALL, CONJ, INLET
 
@@ -410,8 +411,10 @@ def test_neighbour_mask_against_brute_force(periodic):
     c = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
     want = np.zeros(shape, np.uint8)
     for i, j, k in np.ndindex(*shape):
+        solid = (fl[i, j, k] & 3) == SOLID
         for q, d in enumerate(c):
-            p = [i - d[0], j - d[1], k - d[2]]
+            sgn = 1 if solid else -1  # a solid looks at x + c_q, every other node at x - c_q
+            p = [i + sgn * d[0], j + sgn * d[1], k + sgn * d[2]]
             inside = True
             for a in range(3):
                 if not 0 <= p[a] < shape[a]:
@@ -419,7 +422,12 @@ def test_neighbour_mask_against_brute_force(periodic):
                         p[a] %= shape[a]
                     else:
                         inside = False
-            if inside and (fl[tuple(p)] & 3) in (SOLID, WALL):
+            if not inside:
+                continue
+            nb = fl[tuple(p)]
+            if solid and (nb & 3) == FLUID and not (nb & INLET):
+                want[i, j, k] |= 1 << q
+            if not solid and (nb & 3) in (SOLID, WALL):
                 want[i, j, k] |= 1 << q
     assert got.dtype == np.uint8
     np.testing.assert_array_equal(got, want)
