@@ -49,8 +49,8 @@ def parse():
                    help="run a 256 MiB device-to-device copy first (reference kernel for PMC byte counters)")
     p.add_argument("--thermal-n", type=int, default=256)
     p.add_argument("--no-thermal", action="store_true")
-    p.add_argument("--cht-steps", type=int, default=1000,
-                   help="coupled heat-transfer steps timed (config 3 specifies 20000)")
+    p.add_argument("--cht-steps", type=int, default=20000,
+                   help="coupled heat-transfer steps timed (config 3's length; about 23 s)")
     p.add_argument("--no-cht", action="store_true")
     p.add_argument("--unet-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "fluxcore3d_amd",
                                                           "data", "unet_precond.npz"))
