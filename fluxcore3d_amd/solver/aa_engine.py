@@ -22,6 +22,8 @@ Tables (built once, in node order):
 """
 from __future__ import annotations
 
+from typing import Optional
+
 import numpy as np
 import torch
 
@@ -458,19 +460,21 @@ class AAEngine:
                   _lib.stream_handle(self.device))
         return out
 
-    def step(self, rho=None, u=None, v=None, w=None) -> None:
+    def step(self, rho=None, u=None, v=None, w=None, solid_macros: Optional[bool] = None) -> None:
         """One time step; writes pre-collision macros if the four outputs are given.
         Solid nodes hold the constants (1, 0, 0, 0); they are written when the
-        output buffers differ from the previous step's and skipped otherwise."""
-        self.step_fused(rho, u, v, w)
+        output buffers differ from the previous step's and skipped otherwise, or as
+        ``solid_macros`` says (a caller that alternates between buffer sets whose solid
+        entries it has already written passes False)."""
+        self.step_fused(rho, u, v, w, solid_macros)
         self.apply_bcs()
 
-    def step_fused(self, rho=None, u=None, v=None, w=None) -> None:
+    def step_fused(self, rho=None, u=None, v=None, w=None, solid_macros: Optional[bool] = None) -> None:
         """The fused collide/stream/Bouzidi launch of step(), without the inlet/outlet
         plane kernels (apply_bcs() must follow before the next step)."""
         stream = _lib.stream_handle(self.device)
         ptrs = (_lib.ptr(rho), _lib.ptr(u), _lib.ptr(v), _lib.ptr(w))
-        solid_macros = int(ptrs != self._macro_ptrs)
+        solid_macros = int(ptrs != self._macro_ptrs) if solid_macros is None else int(bool(solid_macros))
         self._macro_ptrs = ptrs
         _lib.call("fc_aa_step", self.pop.data_ptr(), STORAGE_CODES[self.storage], *self._dims, self._tables,
                   self._collision, self.t & 1, int(self.t == self.t_loaded), *ptrs, solid_macros, stream)

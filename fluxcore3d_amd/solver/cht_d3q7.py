@@ -36,6 +36,8 @@ temperature; ``temperature()`` adds it back.
 """
 from __future__ import annotations
 
+import threading
+from collections import deque
 from typing import Callable, Optional
 
 import torch
@@ -285,11 +287,21 @@ class CoupledCHT:
     walls the rest of ``sim.solid_flow``. Fluid rho c_p is the unit; the solid has
     conductivity ``k_ratio`` kappa_f and heat capacity ``rho_cp_ratio``; kappa_f =
     nu / ``prandtl`` with nu from the flow's relaxation rate. ``src``: per-step
-    temperature source of the solid (K per flow step, solid heat capacity)."""
+    temperature source of the solid (K per flow step, solid heat capacity).
+
+    ``overlap`` (fused AA engine with its plain step only): the temperature does not act on the flow, so the
+    D3Q7 and solid steps run on a second stream, and the solid solve (small kernels, the
+    host waiting on each iteration's residual) runs from a host thread while the flow steps
+    go on. The velocity goes through a ring of ``ring`` buffer sets (``sim.u``, ``sim.v``,
+    ``sim.wv`` the first); a flow step waits only for the D3Q7 step that read its set
+    ``ring`` steps earlier, and D3Q7 steps due during a solve are queued behind it. Same
+    kernels and inputs in the same order on each stream, so the same bits as the
+    sequential loop; ``sync()`` (called by ``temperature()`` and ``run()``) joins the
+    thread and the streams and leaves the last step's velocity in the ``sim`` fields."""
 
     def __init__(self, sim, src: Optional[Tensor] = None, k_ratio: float = 1000.0, rho_cp_ratio: float = 1.6,
                  prandtl: float = 0.71, couple_every: int = 50, tol: float = 1e-8, T_in: float = 0.0,
-                 periodic=(0, 0, 0)):
+                 periodic=(0, 0, 0), overlap: bool = True, ring: int = 24):
         self.sim = sim
         self.every = int(couple_every)
         if self.every < 1:
@@ -313,8 +325,32 @@ class CoupledCHT:
         self.transport.init(z, z, z, z)
         self.n = 0
         self.history: list = []
+        self._eng = sim.__dict__.get("_fc_engine")
+        from .lbm_cuda_ext import LBMCUDAUpgrade
+        plain = getattr(sim.step, "__func__", None) is LBMCUDAUpgrade._step_aa  # not a profiling wrapper
+        self.overlap = (bool(overlap) and self._eng is not None and plain
+                        and not getattr(sim, "_cuda_debug_nan", False))
+        if self.overlap:
+            if int(ring) < 2:
+                raise ValueError("ring must be >= 2")
+            self._ring = int(ring)
+            self._tstream = torch.cuda.Stream(device=dev)  # default priority, like every stream here
+            self._vel = [tuple(t.view(-1) for t in (sim.u, sim.v, sim.wv))]
+            self._vel += [tuple(torch.zeros(N, dtype=torch.float32, device=dev) for _ in range(3))
+                          for _ in range(self._ring - 1)]
+            self._set_ready = [False] * self._ring  # solid entries written into the set
+            self._read_done: list = [None] * self._ring  # event: the thermal stream is done with the set
+            self._flow_done: list = [None] * self._ring  # event: the flow step that wrote the set
+            self._pending: deque = deque()  # flow steps whose D3Q7 step is not queued yet
+            self._n_thermal = 0
+            self._solver: Optional[threading.Thread] = None
+            self._solver_error: Optional[BaseException] = None
+            self._last = -1
 
     def step(self) -> None:
+        if self.overlap:
+            self._step_overlapped()
+            return
         sim = self.sim
         sim.step(do_thermal=False)
         self.transport.step(sim.u, sim.v, sim.wv, self.T_solid, acc=self.acc)
@@ -323,6 +359,82 @@ class CoupledCHT:
             info = self.solid.step(self.T_solid, self.acc, self.src)
             info["step"] = self.n
             self.history.append(info)
+
+    def _step_overlapped(self) -> None:
+        k = self.n % self._ring
+        while self._pending and self._pending[0] <= self.n - self._ring:
+            self._join_solver()  # set k is still to be read by a D3Q7 step not queued yet
+            self._flush()
+        flow = torch.cuda.current_stream(self.T_solid.device)
+        if self._read_done[k] is not None:  # the D3Q7 step that last read set k is done with it
+            flow.wait_event(self._read_done[k])
+        u, v, w = self._vel[k]
+        self._eng.step(self.sim.rho.view(-1), u, v, w, solid_macros=not self._set_ready[k])
+        self._set_ready[k] = True
+        done = torch.cuda.Event()
+        done.record(flow)
+        self._flow_done[k] = done
+        self._pending.append(self.n)
+        self._last = k
+        self.n += 1
+        self._flush()
+
+    def _flush(self) -> None:
+        """Queue the pending D3Q7 steps on the thermal stream, in order, unless a solid
+        solve is being queued there; a step that completes a coupling interval starts its
+        solve in a host thread."""
+        if self._solver is not None and self._solver.is_alive():
+            return
+        self._join_solver()
+        while self._pending:
+            m = self._pending.popleft()
+            k = m % self._ring
+            self._tstream.wait_event(self._flow_done[k])
+            u, v, w = self._vel[k]
+            with torch.cuda.stream(self._tstream):
+                self.transport.step(u, v, w, self.T_solid, acc=self.acc)
+                done = torch.cuda.Event()
+                done.record(self._tstream)
+            self._read_done[k] = done
+            self._n_thermal += 1
+            if self._n_thermal % self.every == 0:
+                self._solver = threading.Thread(target=self._solve, args=(self._n_thermal,), daemon=True)
+                self._solver.start()
+                return
+
+    def _solve(self, step: int) -> None:
+        try:
+            with torch.cuda.stream(self._tstream):
+                info = self.solid.step(self.T_solid, self.acc, self.src)
+            info["step"] = step
+            self.history.append(info)
+        except BaseException as e:  # re-raised in the calling thread by _join_solver
+            self._solver_error = e
+
+    def _join_solver(self) -> None:
+        if self._solver is not None:
+            self._solver.join()
+            self._solver = None
+        if self._solver_error is not None:
+            e, self._solver_error = self._solver_error, None
+            raise e
+
+    def sync(self) -> None:
+        """Finish queuing every step, join the thermal stream into the current one, and
+        leave the last step's velocity in ``sim.u``, ``sim.v``, ``sim.wv`` (no-op for the
+        sequential loop)."""
+        if not self.overlap or self._last < 0:
+            return
+        while True:
+            self._join_solver()
+            self._flush()
+            if not self._pending and self._solver is None:
+                break
+        self._join_solver()
+        torch.cuda.current_stream(self.T_solid.device).wait_stream(self._tstream)
+        if self._last != 0:
+            for dst, src in zip(self._vel[0], self._vel[self._last]):
+                dst.copy_(src)
 
     def run(self, steps: int, log_fn: Optional[Callable] = None, progress_fn: Optional[Callable] = None,
             abort_fn: Optional[Callable] = None, report_every: int = 1000) -> dict:
@@ -335,13 +447,16 @@ class CoupledCHT:
             if progress_fn is not None and (s + 1) % max(1, steps // 100) == 0:
                 progress_fn(int(100 * (s + 1) / steps))
             if log_fn is not None and (s + 1) % report_every == 0:
+                self.sync()
                 last = self.history[-1] if self.history else {}
                 log_fn(f"[CHT] step {self.n}: T_max {float(self.temperature().max()):.4f}, "
                        f"solid solve {last.get('it', 0)} it, residual {last.get('res_rel', 0.0):.2e}")
+        self.sync()
         return {"steps": self.n, "solves": len(self.history)}
 
     def temperature(self) -> Tensor:
         """(nx, ny, nz) temperature: fluid from D3Q7, conducting nodes from the solid."""
+        self.sync()
         T = torch.where((self.flag.reshape(-1) & 3) == SOLID, self.T_solid, self.transport.T)
         return (T + self.T_in).view(self.transport.shape)
 

@@ -289,3 +289,29 @@ def test_skip_empty_rows_rejects_coupled_rows():
     s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, 2, 1, 1.0, skip_empty_rows=True)
     with pytest.raises(ValueError, match="non-zero coefficients"):
         s.build_if_needed(k, torch.ones(shape, device=DEV), z, z, z, mask, 1.0)
+
+
+@pytest.mark.parametrize("ring", [24, 3])
+def test_overlapped_loop_same_bits_as_sequential(ring):
+    """The coupled loop with the D3Q7 and solid steps on a second stream and the solid
+    solves run from a host thread while the flow goes on equals the sequential loop bit for
+    bit on the quarter-scale heat sink: temperatures, the solid's energy accumulator, the
+    velocity left in the simulation's fields (126 steps: it ends in a set other than the
+    first) and the solid solves' iteration counts. A ring of three sets makes the flow
+    wait for the queued D3Q7 steps during every solve."""
+    from fluxcore3d_amd.solver.cht_d3q7 import CoupledCHT
+    out = {}
+    for overlap in (False, True):
+        c, sim = _quarter()
+        cht = CoupledCHT(sim, src=_t(_base_source(c, 1e-3)), k_ratio=1000.0, rho_cp_ratio=1.6, overlap=overlap,
+                         ring=ring)
+        assert cht.overlap == overlap
+        for _ in range(126):
+            cht.step()
+        T = cht.temperature()
+        out[overlap] = ([T.clone(), cht.T_solid.clone(), cht.acc.clone(), sim.u.clone(), sim.v.clone(),
+                         sim.wv.clone()], [h["it"] for h in cht.history])
+    assert out[False][1] == out[True][1] and len(out[True][1]) == 2
+    for a, b in zip(out[False][0], out[True][0]):
+        assert torch.equal(a, b)
+    assert float(out[True][0][3].abs().max()) > 0.0
