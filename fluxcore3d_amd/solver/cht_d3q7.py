@@ -442,6 +442,7 @@ class CoupledCHT:
         (log_fn(str), progress_fn(0..100), abort_fn() -> True to stop)."""
         for s in range(int(steps)):
             if abort_fn is not None and abort_fn():
+                self.sync()  # the steps taken so far complete, the solve thread is joined
                 raise RuntimeError("Aborted by user.")
             self.step()
             if progress_fn is not None and (s + 1) % max(1, steps // 100) == 0:

@@ -315,3 +315,32 @@ def test_overlapped_loop_same_bits_as_sequential(ring):
     for a, b in zip(out[False][0], out[True][0]):
         assert torch.equal(a, b)
     assert float(out[True][0][3].abs().max()) > 0.0
+
+
+def test_overlapped_run_with_callbacks_and_abort():
+    """run() on the overlapped loop with the worker's callbacks: a log line every 25 steps
+    (each one joins the streams and reads the temperature), progress in percent, and an
+    abort after 60 steps. The abort leaves the 60 steps complete, bit for bit the state of
+    60 sequential steps, with the solve thread joined."""
+    from fluxcore3d_amd.solver.cht_d3q7 import CoupledCHT
+    out = {}
+    for overlap in (False, True):
+        c, sim = _quarter()
+        cht = CoupledCHT(sim, src=_t(_base_source(c, 1e-3)), k_ratio=1000.0, rho_cp_ratio=1.6, overlap=overlap)
+        assert cht.overlap == overlap
+        if overlap:
+            logs, pct = [], []
+            with pytest.raises(RuntimeError, match="Aborted by user"):
+                cht.run(100, log_fn=logs.append, progress_fn=pct.append, abort_fn=lambda: cht.n >= 60,
+                        report_every=25)
+            assert cht.n == 60 and len(logs) == 2 and logs[0].startswith("[CHT] step 25")
+            assert pct[:2] == [1, 2] and pct[-1] == 60
+            assert cht._solver is None and not cht._pending
+        else:
+            for _ in range(60):
+                cht.step()
+        out[overlap] = ([cht.temperature().clone(), cht.acc.clone(), sim.u.clone()],
+                        [h["it"] for h in cht.history])
+    assert out[False][1] == out[True][1] and len(out[True][1]) == 1
+    for a, b in zip(out[False][0], out[True][0]):
+        assert torch.equal(a, b)
