@@ -47,8 +47,10 @@ struct Coef {
 };
 
 // false for a skipped row (its outputs keep the zeros they were allocated with)
-__device__ __forceinline__ bool row_live(const Coef& c, int ny, int i, int j) {
-  return c.rows == nullptr || c.rows[i * ny + j] != 0;
+// The skip table holds one entry per run of 64 consecutive cells (flat index >> 6).
+constexpr int kSkipShift = 6;
+__device__ __forceinline__ bool cell_live(const Coef& c, int idx) {
+  return c.rows == nullptr || c.rows[(unsigned)idx >> kSkipShift] != 0;
 }
 
 __device__ __forceinline__ int lo_ghost(const Level& g, int this_ax) {
@@ -171,8 +173,8 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float* __restrict__
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= g.nx * g.ny * g.nz) return;
   int i, j, k;
+  if (!cell_live(c, idx)) return;
   unflat(g, idx, i, j, k);
-  if (!row_live(c, g.ny, i, j)) return;
   if (zero_guess) {  // x = 0: A x = 0
     x_out[idx] = zero_sweep(c, b, idx);
     return;
@@ -191,8 +193,8 @@ __global__ __launch_bounds__(kBlock) void sweep2_from_zero_kernel(float* __restr
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= g.nx * g.ny * g.nz) return;
   int i, j, k;
+  if (!cell_live(c, idx)) return;
   unflat(g, idx, i, j, k);
-  if (!row_live(c, g.ny, i, j)) return;
   const Nbr o = neighbours(g, i, j, k);
   const C7 a = coef_at<FACE>(c, g, idx, i, j, k);
   const float xc = zero_sweep(c, b, idx);
@@ -218,11 +220,17 @@ __global__ __launch_bounds__(kBlock) void residual_restrict_kernel(const float* 
   const int row = blockIdx.x / kchunks;  // coarse (I, J)
   const int K0 = (blockIdx.x - row * kchunks) * 32;
   const int I = row / cy, J = row - I * cy;
-  // the whole block leaves when its four fine rows are skipped rows: the coarse row's
-  // right-hand side is then 0, which the buffer already holds
-  if (!(row_live(c, g.ny, 2 * I, 2 * J) || row_live(c, g.ny, 2 * I, 2 * J + 1) ||
-        row_live(c, g.ny, 2 * I + 1, 2 * J) || row_live(c, g.ny, 2 * I + 1, 2 * J + 1)))
-    return;
+  // the whole block leaves when the 64-cell runs holding its fine cells are all skipped:
+  // the right-hand side of its coarse cells is then 0, which the buffer already holds
+  if (c.rows != nullptr) {
+    const int klo = 2 * K0, khi = min(2 * K0 + 63, 2 * cz - 1);
+    bool live = false;
+    for (int d = 0; d < 4; ++d) {
+      const int base = ((2 * I + (d >> 1)) * g.ny + 2 * J + (d & 1)) * g.nz;
+      for (int s = (base + klo) >> kSkipShift; s <= (base + khi) >> kSkipShift; ++s) live |= c.rows[s] != 0;
+    }
+    if (!live) return;
+  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = 2 * I + (wave >> 1), j = 2 * J + (wave & 1), k = 2 * K0 + lane;
   float r = 0.f;
@@ -286,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void prolong_correct_kernel(const float* __
   const int i = (int)(u / nynz);
   const unsigned rem = u - (unsigned)i * nynz;
   const int j = (int)(rem / (unsigned)nz), k = (int)(rem - (unsigned)j * (unsigned)nz);
-  if (rows && !rows[i * ny + j]) return;
+  if (rows && !rows[u >> kSkipShift]) return;
   auto X = [&](int a, int b2, int c2) { return xc[(a * cy + b2) * cz + c2]; };
   x[idx] = corrected(x[idx], prolong_at(X, i, j, k, cx, cy, cz, sx, sy, sz), alpha, mask[idx]);
 }
@@ -305,8 +313,8 @@ __global__ __launch_bounds__(kBlock) void prolong_correct2_kernel(const float* _
   const int i = (int)(u / nyhz);
   const unsigned rem = u - (unsigned)i * nyhz;
   const int j = (int)(rem / (unsigned)hz), k = 2 * (int)(rem - (unsigned)j * (unsigned)hz);
-  if (rows && !rows[i * ny + j]) return;
-  const int idx = (i * ny + j) * nz + k;
+  const int idx = (i * ny + j) * nz + k;  // even (nz is), so idx and idx + 1 share a 64-cell run
+  if (rows && !rows[(unsigned)idx >> kSkipShift]) return;
   auto X = [&](int a, int b2, int c2) { return xc[(a * cy + b2) * cz + c2]; };
   typedef float f2 __attribute__((ext_vector_type(2)));
   const f2 xv = __builtin_nontemporal_load(reinterpret_cast<const f2*>(x + idx));
@@ -584,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void apply_A64_kernel(const double* __restr
   double v[2] = {0.0, 0.0};
   int i = 0, j = 0, k = 0;
   if (idx < g.nx * g.ny * g.nz) unflat(g, idx, i, j, k);
-  if (idx < g.nx * g.ny * g.nz && row_live(c, g.ny, i, j)) {
+  if (idx < g.nx * g.ny * g.nz && cell_live(c, idx)) {
     const Nbr o = neighbours(g, i, j, k);
     const C7 q = coef_at<FACE>(c, g, idx, i, j, k);
     const double pc = p[idx];
@@ -699,9 +707,9 @@ __global__ __launch_bounds__(kBlock) void true_residual_kernel(const double* __r
   // cells in index order
 #pragma unroll 4
   for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
+    if (!cell_live(c, idx)) continue;
     int i, j, k;
     unflat(g, idx, i, j, k);
-    if (!row_live(c, g.ny, i, j)) continue;
     const Nbr o = neighbours(g, i, j, k);
     const C7 q = coef_at<FACE>(c, g, idx, i, j, k);
     const double xc = x[idx];
@@ -1028,8 +1036,8 @@ int fc_mg_apply_A64(const double* p, double* Ap, const double* r, const float* m
 int fc_mg_update_xr(double* x, double* r, float* r32, const double* p, const double* Ap, int64_t n,
                     const uint8_t* rows, int row_len, double* partial, double* scal, hipStream_t stream) {
   FC_REQUIRE(x && r && r32 && p && Ap && partial && scal, "update_xr: null pointer argument");
-  FC_REQUIRE(!rows || (row_len > 0 && n % row_len == 0 && n < ((int64_t)1 << 31)),
-             "update_xr: rows need a positive row length dividing n < 2^31");
+  FC_REQUIRE(!rows || (row_len > 0 && n < ((int64_t)1 << 31)),
+             "update_xr: rows need a positive run length and n < 2^31");
   const unsigned gsz = reduce_grid(n);
   hipLaunchKernelGGL(update_xr_kernel, dim3(gsz), dim3(kBlock), 0, stream, x, r, r32, p, Ap, scal, n, rows,
                      (unsigned)(rows ? row_len : 1), partial);
@@ -1052,8 +1060,8 @@ int fc_mg_stop_slot(void) { return S_STOP; }
 int fc_mg_new_direction(double* p, const float* z, const double* Ap_old, int64_t n, int first, const uint8_t* rows,
                         int row_len, double* partial, double* scal, hipStream_t stream) {
   FC_REQUIRE(p && z && partial && scal, "new_direction: null pointer argument");
-  FC_REQUIRE(!rows || (row_len > 0 && n % row_len == 0 && n < ((int64_t)1 << 31)),
-             "new_direction: rows need a positive row length dividing n < 2^31");
+  FC_REQUIRE(!rows || (row_len > 0 && n < ((int64_t)1 << 31)),
+             "new_direction: rows need a positive run length and n < 2^31");
   const unsigned rl = (unsigned)(rows ? row_len : 1);
   const unsigned gsz = reduce_grid(n);
   if (!first) {

@@ -28,6 +28,7 @@ import torch
 
 from .. import _lib
 from . import mg_cuda_ext  # noqa: F401
+from .nlevel_mg_cuda import SKIP_RUN
 
 S_PAP, S_PR, S_RR, S_BB = 0, 1, 2, 3
 
@@ -165,7 +166,7 @@ def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
     dt = float(mg_solver._dt_cache)
     ax, oi = mg_solver.ax, lv0["out_idx"]
     mask = lv0["mask_f"]
-    rows = _lib.ptr(lv0.get("rows"))  # z-rows without unknowns, skipped (NULL: none)
+    rows = _lib.ptr(lv0.get("rows"))  # runs of SKIP_RUN cells without unknowns, skipped (NULL: none)
 
     def true_residual():
         _lib.call("fc_mg_true_residual", ws.x.data_ptr(), ws.b.data_ptr(), ws.r.data_ptr(), ws.r32.data_ptr(),
@@ -177,11 +178,11 @@ def fcg_fp64(mg_solver, b: torch.Tensor, x0: Optional[torch.Tensor] = None, tol:
     def step(j):
         precond(ws.r32, ws.z32)
         _lib.call("fc_mg_new_direction", ws.p.data_ptr(), ws.z32.data_ptr(), ws.Ap.data_ptr(), ws.n, int(j == 0),
-                  rows, nz, ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
+                  rows, SKIP_RUN, ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
         _lib.call("fc_mg_apply_A64", ws.p.data_ptr(), ws.Ap.data_ptr(), ws.r.data_ptr(), mask.data_ptr(),
                   lv0["coef"], dt, nx, ny, nz, ax, oi, ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
         _lib.call("fc_mg_update_xr", ws.x.data_ptr(), ws.r.data_ptr(), ws.r32.data_ptr(), ws.p.data_ptr(),
-                  ws.Ap.data_ptr(), ws.n, rows, nz, ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
+                  ws.Ap.data_ptr(), ws.n, rows, SKIP_RUN, ws.partial.data_ptr(), ws.scal.data_ptr(), stream)
 
     S_STOP = _stop_slot("mg")
     _clear_stop(ws.scal, S_RR, S_STOP, stream)  # the workspace may hold a finished solve's flag

@@ -30,6 +30,7 @@ from .. import _lib
 from . import mg_cuda_ext  # noqa: F401  (registers the MG entry points)
 
 Tensor = torch.Tensor
+SKIP_RUN = 64  # cells per entry of a level's skip table (the kernels index it by flat index >> 6)
 
 
 class MGCoef(_C.Structure):
@@ -70,8 +71,8 @@ class NLevelGeometricMGSolver:
         if coarse_mask not in ("all", "any"):
             raise ValueError("coarse_mask must be 'all' or 'any'")
         self.coarse_mask = coarse_mask
-        # Skip z-rows without unknowns in the stencil kernels. Valid when every cell of
-        # such a row is decoupled (its six coefficients and diagL are 0, which the build
+        # Skip runs of SKIP_RUN consecutive cells (flat index) without unknowns in the
+        # stencil kernels. Valid when every cell of such a run is decoupled (its six coefficients and diagL are 0, which the build
         # checks) and the right-hand sides and start vectors handed to the solver vanish
         # there: those cells then stay exactly 0, and skipping them leaves every other
         # value, sum and iteration count bit for bit as they were.
@@ -164,16 +165,25 @@ class NLevelGeometricMGSolver:
         return faces
 
     def _build_rows(self, lv) -> None:
-        """Row table of one level: 1 where the z-row holds an unknown (mask != 0)."""
-        nx, ny, nz = lv["shape"]
-        live = (lv["mask_f"] != 0).reshape(nx * ny, nz).any(dim=1)
+        """Skip table of one level: entry s is 1 where cells SKIP_RUN s .. SKIP_RUN s +
+        SKIP_RUN - 1 (flat index) hold an unknown (mask != 0)."""
+        n = lv["mask_f"].numel()
+        pad = (-n) % SKIP_RUN
+
+        def runs_any(t: Tensor) -> Tensor:
+            flat = t.reshape(-1)
+            if pad:
+                flat = torch.cat([flat, flat.new_zeros(pad)])
+            return flat.reshape(-1, SKIP_RUN).any(dim=1)
+
+        live = runs_any(lv["mask_f"] != 0)
         dead = ~live
         if bool(dead.any()):
-            coupled = torch.zeros(nx * ny, dtype=torch.bool, device=self.device)
-            for n in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL"):
-                coupled |= (lv[n].reshape(nx * ny, nz) != 0).any(dim=1)
+            coupled = torch.zeros_like(live)
+            for name in ("aE", "aW", "aN", "aS", "aU", "aD", "diagL"):
+                coupled |= runs_any(lv[name] != 0)
             if bool((coupled & dead).any()):
-                raise ValueError("skip_empty_rows: a row without unknowns has non-zero coefficients")
+                raise ValueError("skip_empty_rows: a run of cells without unknowns has non-zero coefficients")
         lv["rows"] = live.to(torch.uint8).contiguous()
         lv["coef_struct"].rows = lv["rows"].data_ptr()
 

@@ -203,9 +203,10 @@ int fc_mg_restrict_field(const float *src, float *dst, int nx, int ny, int nz, i
 typedef struct fc_mg_coef {
   const float *aE, *aW, *aN, *aS, *aU, *aD, *diagL, *invD;
   const float *fx, *fy, *fz;
-  /* NULL, or uint8 [nx][ny]: 0 for a z-row whose cells all have zero coefficients and
-   * whose right-hand side and iterate are 0. The sweeps, the residual restriction,
-   * fc_mg_apply_A64 and fc_mg_true_residual skip such rows and leave their outputs
+  /* NULL, or uint8 [ceil(nx ny nz / 64)]: entry s covers the 64 cells of flat index
+   * 64 s .. 64 s + 63; 0 where those cells all have zero coefficients and their
+   * right-hand side and iterate are 0. The sweeps, the residual restriction,
+   * fc_mg_apply_A64 and fc_mg_true_residual skip such runs and leave their outputs
    * there as they are (zeros), which gives the same bits as visiting them. */
   const uint8_t *rows;
 } fc_mg_coef;
@@ -230,7 +231,7 @@ int fc_mg_sweep2_from_zero(float *x_out, const float *b, const fc_mg_coef *coef,
 int fc_mg_residual_restrict(const float *x, const float *b, const float *mask, const fc_mg_coef *coef, double dt,
                             int nx, int ny, int nz, int ax, int out_idx, float *b_coarse, hipStream_t stream);
 /* _prolong_2x + correction of _vcycle (nlevel_mg_cuda.py:300-303,531-537); rows (NULL: none)
- * is the fine level's table of skipped z-rows, as in fc_mg_coef */
+ * is the fine level's table of skipped 64-cell runs, as in fc_mg_coef */
 int fc_mg_prolong_correct(const float *x_coarse, float *x, const float *mask, int nx, int ny, int nz, int cx, int cy,
                           int cz, float alpha, const uint8_t *rows, hipStream_t stream);
 /* coarsest-level sweeps of _vcycle (nlevel_mg_cuda.py:279-283) in one workgroup */
@@ -259,8 +260,9 @@ int fc_mg_cheb_step(float *x_new, const float *b, const float *Lo, const float *
 int fc_mg_apply_A64(const double *p, double *Ap, const double *r, const float *mask, const fc_mg_coef *coef,
                     double dt, int nx, int ny, int nz, int ax, int out_idx, double *partial, double *scal,
                     hipStream_t stream);
-/* rows (NULL: none): table of skipped rows of row_len consecutive cells (the finest
- * level's fc_mg_coef rows, row_len = nz); those cells are left as they are */
+/* rows (NULL: none): one entry per run of row_len consecutive cells, ceil(n / row_len)
+ * entries (the finest level's fc_mg_coef rows with row_len = 64); the cells of runs
+ * whose entry is 0 are left as they are */
 int fc_mg_update_xr(double *x, double *r, float *r32, const double *p, const double *Ap, int64_t n,
                     const uint8_t *rows, int row_len, double *partial, double *scal, hipStream_t stream);
 int fc_mg_new_direction(double *p, const float *z, const double *Ap_old, int64_t n, int first, const uint8_t *rows,

@@ -264,7 +264,7 @@ def test_solid_step_skipping_empty_rows_same_bits():
         sc = SolidConduction(flag, k_s, 1.6, 50.0, skip_empty_rows=skip)
         if skip:
             rows = sc.mg._levels[0]["rows"]
-            print(f"[solid rows] finest level: {float(rows.float().mean()):.3f} of the z-rows hold unknowns")
+            print(f"[solid rows] finest level: {float(rows.float().mean()):.3f} of the 64-cell runs hold unknowns")
             assert float(rows.float().mean()) < 0.5
         Ts = Ts0.clone()
         its = []
@@ -279,13 +279,14 @@ def test_solid_step_skipping_empty_rows_same_bits():
 
 
 def test_skip_empty_rows_rejects_coupled_rows():
-    """A row without unknowns whose cells still carry conductance cannot be skipped."""
+    """A run of 64 cells without unknowns (here four z-rows of 16) whose cells still carry
+    conductance cannot be skipped."""
     from fluxcore3d_amd.solver.nlevel_mg_cuda import NLevelGeometricMGSolver
     shape = (8, 8, 16)
     k = torch.ones(shape, device=DEV)
     z = torch.zeros(shape, device=DEV)
     mask = torch.ones(shape, device=DEV)
-    mask[0, 0, :] = 0.0
+    mask[0, 0:4, :] = 0.0
     s = NLevelGeometricMGSolver(torch.device(DEV), torch.float32, 2, 1, 1.0, skip_empty_rows=True)
     with pytest.raises(ValueError, match="non-zero coefficients"):
         s.build_if_needed(k, torch.ones(shape, device=DEV), z, z, z, mask, 1.0)
