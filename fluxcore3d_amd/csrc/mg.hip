@@ -220,22 +220,15 @@ __global__ __launch_bounds__(kBlock) void residual_restrict_kernel(const float* 
   const int row = blockIdx.x / kchunks;  // coarse (I, J)
   const int K0 = (blockIdx.x - row * kchunks) * 32;
   const int I = row / cy, J = row - I * cy;
-  // the whole block leaves when the 64-cell runs holding its fine cells are all skipped:
-  // the right-hand side of its coarse cells is then 0, which the buffer already holds
-  if (c.rows != nullptr) {
-    const int klo = 2 * K0, khi = min(2 * K0 + 63, 2 * cz - 1);
-    bool live = false;
-    for (int d = 0; d < 4; ++d) {
-      const int base = ((2 * I + (d >> 1)) * g.ny + 2 * J + (d & 1)) * g.nz;
-      for (int s = (base + klo) >> kSkipShift; s <= (base + khi) >> kSkipShift; ++s) live |= c.rows[s] != 0;
-    }
-    if (!live) return;
-  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = 2 * I + (wave >> 1), j = 2 * J + (wave & 1), k = 2 * K0 + lane;
+  const int idx = (i * g.ny + j) * g.nz + k;
+  // the whole block leaves when the 64-cell runs holding its fine cells are all skipped
+  // (each lane tests its own cell's run, in parallel): the right-hand side of its coarse
+  // cells is then 0, which the buffer already holds
+  if (c.rows != nullptr && !__syncthreads_or(k < 2 * cz && c.rows[(unsigned)idx >> kSkipShift] != 0)) return;
   float r = 0.f;
   if (k < 2 * cz) {
-    const int idx = (i * g.ny + j) * g.nz + k;
     const float lo = stencil_lo<FACE>(x, c, g, dt, idx, i, j, k);
     r = (ld_once(b + idx) - lo) * ld_once(mask + idx);
   }
