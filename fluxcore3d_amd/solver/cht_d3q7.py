@@ -301,7 +301,7 @@ class CoupledCHT:
 
     def __init__(self, sim, src: Optional[Tensor] = None, k_ratio: float = 1000.0, rho_cp_ratio: float = 1.6,
                  prandtl: float = 0.71, couple_every: int = 50, tol: float = 1e-8, T_in: float = 0.0,
-                 periodic=(0, 0, 0), overlap: bool = True, ring: int = 24):
+                 periodic=(0, 0, 0), overlap: bool = True, ring: int = 48):
         self.sim = sim
         self.every = int(couple_every)
         if self.every < 1:

@@ -292,7 +292,7 @@ def test_skip_empty_rows_rejects_coupled_rows():
         s.build_if_needed(k, torch.ones(shape, device=DEV), z, z, z, mask, 1.0)
 
 
-@pytest.mark.parametrize("ring", [24, 3])
+@pytest.mark.parametrize("ring", [48, 3])
 def test_overlapped_loop_same_bits_as_sequential(ring):
     """The coupled loop with the D3Q7 and solid steps on a second stream and the solid
     solves run from a host thread while the flow goes on equals the sequential loop bit for
